@@ -1,0 +1,452 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * CPU restatement of GERMAINE's matrix-assembly and shot-kernel arithmetic.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * this file's library; the product (germaine_amd/) never links or calls it.
+ *
+ * Every function follows the reference's arithmetic literally: the same C
+ * complex types, the same promotion order (float complex operands meeting
+ * double literals), the same evaluation order, so that gcc on x86-64 (no FMA,
+ * -ffp-contract=off) reproduces the reference's rounding.  Complex division
+ * goes through libgcc's __divsc3/__divdc3 exactly as in the reference.
+ *
+ * Grids are nx*ny INCLUDING the PML.  Material arrays are row-major float
+ * [ny][nx] (index (j-1)*nx + (i-1) for the reference's 1-based (i,j)).
+ * The row index of grid point (i,j) is k = (i-1) + (j-1)*nx
+ * (src/AC/init_A_AC_9p_pml.c:52-53,215).
+ *
+ * Pinning (see tests/test_oracle_assembly.py): a Kronecker-product
+ * construction of the same operator (independent of the per-point formula),
+ * the closed-form Hustedt interior coefficients, the closed-form nonzero count,
+ * a double-precision closed form of the PML profiles, and (TE) Smith complex
+ * division checked against libgcc on random operands.
+ */
+#include <complex.h>
+#include <math.h>
+#include <string.h>
+#include <stdlib.h>
+
+typedef float complex cf;
+typedef double complex cd;
+
+/* ---- src/calc_nonzero.c:10-52: number of 9-point entries ------------------ */
+int orc_nonzero(int nx, int ny)
+{
+    int nz = 0;
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            for (int dj = -1; dj <= 1; dj++)
+                for (int di = -1; di <= 1; di++) {
+                    int ii = i + di, jj = j + dj;
+                    if (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny) nz++;
+                }
+        }
+    return nz;
+}
+
+/* ---- src/pml_pro.c:20-160: classical PML profiles -------------------------
+ * out_x: 4 arrays of nx+2 floats (r, i, half r, half i), out_y: 4 of ny+2.   */
+void orc_pml_profiles(int nx, int ny, int npml, float dh, float a0_pml,
+                      float freq, float s, int free_surf, float *out_x, float *out_y)
+{
+    float *xr = out_x, *xi = out_x + (nx + 2), *xhr = out_x + 2 * (nx + 2), *xhi = out_x + 3 * (nx + 2);
+    float *yr = out_y, *yi = out_y + (ny + 2), *yhr = out_y + 2 * (ny + 2), *yhi = out_y + 3 * (ny + 2);
+    float pih = M_PI / 2.0;                                  /* :39 */
+    cf ci = 0.0 + 1.0 * I;                                   /* :42 */
+    float lpml = npml * dh;                                  /* :45 */
+    cf omega_pml = 2.0 * M_PI * (freq - I * s);              /* :49 */
+
+    for (int i = 0; i <= nx + 1; i++) { xr[i] = 1.0; xi[i] = 0.0; xhr[i] = 1.0; xhi[i] = 0.0; }
+    for (int i = 0; i <= ny + 1; i++) { yr[i] = 1.0; yi[i] = 0.0; yhr[i] = 1.0; yhi[i] = 0.0; }
+
+    for (int i = 1; i <= npml; i++) {                        /* :66-131 */
+        float x = i * dh;
+        float xh = x + 0.5 * dh;
+        float eps = a0_pml * (1.0 - cos((lpml - x) * pih / lpml));
+        float epsh = a0_pml * (1.0 - cos((lpml - xh) * pih / lpml));
+        xr[i] = creal(1.0 / (1.0 + eps / (ci * omega_pml)));
+        xi[i] = cimag(1.0 / (1.0 + eps / (ci * omega_pml)));
+        xhr[i] = creal(1.0 / (1.0 + epsh / (ci * omega_pml)));
+        xhi[i] = cimag(1.0 / (1.0 + epsh / (ci * omega_pml)));
+        yr[i] = xr[i]; yi[i] = xi[i]; yhr[i] = xhr[i]; yhi[i] = xhi[i];
+        xr[nx - i + 1] = xr[i]; xi[nx - i + 1] = xi[i]; xhr[nx - i + 1] = xhr[i]; xhi[nx - i + 1] = xhi[i];
+        yr[ny - i + 1] = xr[i]; yi[ny - i + 1] = xi[i]; yhr[ny - i + 1] = xhr[i]; yhi[ny - i + 1] = xhi[i];
+        if (free_surf == 1) { yr[i] = 1.0; yi[i] = 0.0; yhr[i] = 1.0; yhi[i] = 0.0; }
+    }
+    xr[0] = xr[1]; xi[0] = xi[1]; xhr[0] = xhr[1]; xhi[0] = xhi[1];          /* :134-152 */
+    xr[nx + 1] = xr[nx]; xi[nx + 1] = xi[nx]; xhr[nx + 1] = xhr[nx]; xhi[nx + 1] = xhi[nx];
+    yr[0] = yr[1]; yi[0] = yi[1]; yhr[0] = yhr[1]; yhi[0] = yhi[1];
+    yr[ny + 1] = yr[ny]; yi[ny + 1] = yi[ny]; yhr[ny + 1] = yhr[ny]; yhi[ny + 1] = yhi[ny];
+}
+
+/* ---- squared complex angular frequencies ---------------------------------
+ * AC: src/AC/init_A_AC_9p_pml.c:50  cpowf(2 pi (f - iS), 2)
+ * SH: src/SH/init_A_SH_9p_pml.c:50  cpowf(2 pi (f + iS), 2)
+ * TE: src/TE/init_A_TE_9p_pml.c:54-55 (also Omega itself)                   */
+void orc_omega2(int physics, float freq, float s, float *o2, float *om)
+{
+    cf w2, w;
+    if (physics == 1) {
+        w2 = cpowf(2.0 * M_PI * (freq - s * I), 2.0);
+        w = 2.0 * M_PI * (freq - s * I);
+    } else {
+        w2 = cpowf(2.0 * M_PI * (freq + s * I), 2.0);
+        w = 2.0 * M_PI * (freq + s * I);
+    }
+    o2[0] = crealf(w2); o2[1] = cimagf(w2);
+    om[0] = crealf(w); om[1] = cimagf(w);
+}
+
+/* Mixed-grid weights, src/AC/init_A_AC_9p_pml.c:34-42 (identical in TE/SH). */
+static void fd_weights(float dh, float *wa, float *wb, float *wc, float *wd, float *we, float *idh2)
+{
+    float a, b, c, d, e;
+    a = 0.6287326;
+    b = 0.3712667;
+    c = 1 - a - b;
+    b = 0.25 * b;
+    c = 0.25 * c;
+    d = 0.4382634;
+    e = 1 - d;
+    *wa = a; *wb = b; *wc = c; *wd = d; *we = e;
+    *idh2 = 1.0 / (dh * dh);
+}
+
+/* The nine Laplacian weights the stencil multiplies the PML factors with
+ * (struct matAC bpm..b0m, include/fd.h:55-58). */
+typedef struct { float bpm, bmp, bpp, bmm, b00, bp0, bm0, b0p, b0m; } lapw;
+
+/*
+ * One stencil row (grid point i,j), written in the reference's entry order
+ * NW, N, NE, W, C, E, SW, S, SE (ascending column).  mass[9] are the complex
+ * float mass factors of the nine points in that order (Omega2*ivp2 for AC,
+ * Omega2*rho for SH, Omega2*mu0*epse for TE).  Each expression mirrors
+ * src/AC/init_A_AC_9p_pml.c:68-213 term by term.
+ */
+static int stencil_row(int i, int j, int nx, int ny, int k, const cf *mass, lapw w,
+                       cf dx0, cf dy0, cf dxp, cf dxm, cf dyp, cf dym,
+                       float a, float b, float c, float d, float e, float idh2, float damp,
+                       int *irow, int *icol, double *ar, double *ai)
+{
+    int n = 0;
+    cf tmp;
+#define EMIT(col) do { ar[n] = creal(tmp); ai[n] = cimag(tmp); irow[n] = k; icol[n] = (col); n++; } while (0)
+    if ((i > 1) && (j > 1)) {                                                   /* NW :68-81 */
+        tmp = c * mass[0] + d * (0.25 * idh2 * dy0 * w.bmm * dym + 0.25 * idh2 * dx0 * w.bmm * dxm);
+        EMIT(k - 1 - nx);
+    }
+    if (j > 1) {                                                                /* N :84-100 */
+        tmp = b * mass[1]
+            + d * (0.25 * idh2 * dy0 * (w.bpm * dym + w.bmm * dym) + 0.25 * idh2 * dx0 * (-w.bpm * dxp - w.bmm * dxm))
+            + e * (dy0 * idh2 * w.bm0 * dym);
+        tmp += -damp * tmp;
+        EMIT(k - nx);
+    }
+    if ((i < nx) && (j > 1)) {                                                  /* NE :103-116 */
+        tmp = c * mass[2] + d * (0.25 * idh2 * dy0 * w.bpm * dym + 0.25 * idh2 * dx0 * w.bpm * dxp);
+        EMIT(k + 1 - nx);
+    }
+    if (i > 1) {                                                                /* W :119-135 */
+        tmp = b * mass[3]
+            + d * (0.25 * idh2 * dy0 * (-w.bmp * dyp - w.bmm * dym) + 0.25 * idh2 * dx0 * (w.bmp * dxm + w.bmm * dxm))
+            + e * (dx0 * idh2 * w.bm0 * dxm);
+        tmp += -damp * tmp;
+        EMIT(k - 1);
+    }
+    {                                                                           /* C :138-151 */
+        tmp = a * mass[4]
+            + d * (-0.25 * idh2 * dy0 * (w.bmp * dyp + w.bpm * dym + w.bpp * dyp + w.bmm * dym)
+                   - 0.25 * idh2 * dx0 * (w.bmp * dxm + w.bpm * dxp + w.bpp * dxp + w.bmm * dxm))
+            + e * (-dy0 * idh2 * (dyp * w.b0p + dym * w.b0m) - dx0 * idh2 * (dxp * w.bp0 + dxm * w.bm0));
+        tmp += damp * 4.0 * tmp;
+        EMIT(k);
+    }
+    if (i < nx) {                                                               /* E :154-170 */
+        tmp = b * mass[5]
+            + d * (0.25 * idh2 * dy0 * (-w.bpm * dym - w.bpp * dyp) + 0.25 * idh2 * dx0 * (w.bpm * dxp + w.bpp * dxp))
+            + e * (dx0 * idh2 * w.bp0 * dxp);
+        tmp += -damp * tmp;
+        EMIT(k + 1);
+    }
+    if ((j < ny) && (i > 1)) {                                                  /* SW :173-186 */
+        tmp = c * mass[6] + d * (0.25 * idh2 * dy0 * w.bmp * dyp + 0.25 * idh2 * dx0 * w.bmp * dxm);
+        EMIT(k - 1 + nx);
+    }
+    if (j < ny) {                                                               /* S :189-205 */
+        tmp = b * mass[7]
+            + d * (0.25 * idh2 * dy0 * (w.bmp * dyp + w.bpp * dyp) + 0.25 * idh2 * dx0 * (-w.bmp * dxm - w.bpp * dxp))
+            + e * (dy0 * idh2 * w.b0p * dyp);
+        tmp += -damp * tmp;
+        EMIT(k + nx);
+    }
+    if ((i < nx) && (j < ny)) {                                                 /* SE :208-213 */
+        tmp = c * mass[8] + d * (0.25 * idh2 * dy0 * w.bpp * dyp + 0.25 * idh2 * dx0 * w.bpp * dxp);
+        EMIT(k + 1 + nx);
+    }
+#undef EMIT
+    return n;
+}
+
+#define IDX(i, j) (((j) - 1) * nx + ((i) - 1))
+static const int NB_DI[9] = {-1, 0, 1, -1, 0, 1, -1, 0, 1};
+static const int NB_DJ[9] = {-1, -1, -1, 0, 0, 0, 1, 1, 1};
+
+/* PML factors of grid point (i,j), src/AC/init_A_AC_9p_pml.c:59-66 */
+static void pml_at(int i, int j, int nx, int ny, const float *px, const float *py,
+                   cf *dx0, cf *dy0, cf *dxp, cf *dxm, cf *dyp, cf *dym)
+{
+    const float *xr = px, *xi = px + (nx + 2), *xhr = px + 2 * (nx + 2), *xhi = px + 3 * (nx + 2);
+    const float *yr = py, *yi = py + (ny + 2), *yhr = py + 2 * (ny + 2), *yhi = py + 3 * (ny + 2);
+    *dy0 = yr[j] + yi[j] * I;
+    *dx0 = xr[i] + xi[i] * I;
+    *dyp = yhr[j] + yhi[j] * I;
+    *dym = yhr[j - 1] + yhi[j - 1] * I;
+    *dxp = xhr[i] + xhi[i] * I;
+    *dxm = xhr[i - 1] + xhi[i - 1] * I;
+}
+
+/* ---- AC: src/AC/init_mat_AC.c:17-26, src/av_rho.c:32-48,
+ *          src/AC/init_A_AC_9p_pml.c:17-220 ------------------------------------ */
+int orc_assemble_ac(int nx, int ny, float dh, float freq, float s,
+                    const float *vp, const float *rho, const float *px, const float *py,
+                    int *irow, int *icol, double *ar, double *ai)
+{
+    float a, b, c, d, e, idh2, damp = 0.0;
+    fd_weights(dh, &a, &b, &c, &d, &e, &idh2);
+    cf omega2 = cpowf(2.0 * M_PI * (freq - s * I), 2.0);
+    size_t N = (size_t)nx * ny;
+    float *ivp2 = malloc(N * sizeof(float)), *bb = malloc(N * sizeof(float));
+    for (size_t q = 0; q < N; q++) {
+        ivp2[q] = 1.0 / (rho[q] * vp[q] * vp[q]);
+        bb[q] = 1.0 / rho[q];
+    }
+    int count = 0, k = 0;
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            float bl = bb[IDX(i, j)];
+            lapw w = {bl, bl, bl, bl, bl, bl, bl, bl, bl};
+            cf mass[9];
+            for (int t = 0; t < 9; t++) {
+                int ii = i + NB_DI[t], jj = j + NB_DJ[t];
+                mass[t] = (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny) ? omega2 * ivp2[IDX(ii, jj)] : 0;
+            }
+            cf dx0, dy0, dxp, dxm, dyp, dym;
+            pml_at(i, j, nx, ny, px, py, &dx0, &dy0, &dxp, &dxm, &dyp, &dym);
+            count += stencil_row(i, j, nx, ny, k, mass, w, dx0, dy0, dxp, dxm, dyp, dym,
+                                 a, b, c, d, e, idh2, damp,
+                                 irow + count, icol + count, ar + count, ai + count);
+            k++;
+        }
+    free(ivp2); free(bb);
+    return count;
+}
+
+/* ---- SH: src/SH/init_mat_SH.c:17-32, src/SH/av_mu.c:12-28,
+ *          src/SH/init_A_SH_9p_pml.c:17-222 ------------------------------------ */
+int orc_assemble_sh(int nx, int ny, float dh, float freq, float s,
+                    const float *vs, const float *rho, const float *px, const float *py,
+                    int *irow, int *icol, double *ar, double *ai)
+{
+    float a, b, c, d, e, idh2, damp = 0.0;
+    fd_weights(dh, &a, &b, &c, &d, &e, &idh2);
+    cf omega2 = cpowf(2.0 * M_PI * (freq + s * I), 2.0);
+    /* mu with a one-cell halo: [0..ny+1][0..nx+1] */
+    int hx = nx + 2;
+    float *mu = malloc((size_t)(nx + 2) * (ny + 2) * sizeof(float));
+#define MU(i, j) mu[(size_t)(j) * hx + (i)]
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) MU(i, j) = rho[IDX(i, j)] * vs[IDX(i, j)] * vs[IDX(i, j)];
+    for (int j = 1; j <= ny; j++) { MU(0, j) = MU(1, j); MU(nx + 1, j) = MU(nx, j); }
+    for (int i = 0; i <= nx + 1; i++) { MU(i, 0) = MU(i, 1); MU(i, ny + 1) = MU(i, ny); }
+    int count = 0, k = 0;
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            lapw w;
+            w.bpm = 0.25 * (MU(i, j) + MU(i + 1, j) + MU(i, j - 1) + MU(i + 1, j - 1));
+            w.bmp = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i, j + 1) + MU(i - 1, j + 1));
+            w.bpp = 0.25 * (MU(i, j) + MU(i + 1, j) + MU(i, j + 1) + MU(i + 1, j + 1));
+            w.bmm = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i, j - 1) + MU(i - 1, j - 1));
+            w.b00 = MU(i, j);
+            w.bp0 = 0.5 * (MU(i, j) + MU(i + 1, j));
+            w.bm0 = 0.5 * (MU(i, j) + MU(i - 1, j));
+            w.b0p = 0.5 * (MU(i, j) + MU(i, j + 1));
+            w.b0m = 0.5 * (MU(i, j) + MU(i, j - 1));
+            cf mass[9];
+            for (int t = 0; t < 9; t++) {
+                int ii = i + NB_DI[t], jj = j + NB_DJ[t];
+                mass[t] = (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny) ? omega2 * rho[IDX(ii, jj)] : 0;
+            }
+            cf dx0, dy0, dxp, dxm, dyp, dym;
+            pml_at(i, j, nx, ny, px, py, &dx0, &dy0, &dxp, &dxm, &dyp, &dym);
+            count += stencil_row(i, j, nx, ny, k, mass, w, dx0, dy0, dxp, dxm, dyp, dym,
+                                 a, b, c, d, e, idh2, damp,
+                                 irow + count, icol + count, ar + count, ai + count);
+            k++;
+        }
+#undef MU
+    free(mu);
+    return count;
+}
+
+/* ---- TE: src/TE/init_A_TE_9p_pml.c:21-243.  eps is ABSOLUTE permittivity
+ * (src/TE/readmod_TE.c:39,50), sigma in S/m.  Laplacian weights are 1.        */
+int orc_assemble_te(int nx, int ny, float dh, float freq, float s,
+                    const float *eps, const float *sigma, const float *px, const float *py,
+                    int *irow, int *icol, double *ar, double *ai)
+{
+    float a, b, c, d, e, idh2, damp = 0.0, mu0;
+    fd_weights(dh, &a, &b, &c, &d, &e, &idh2);
+    cf omega = 2.0 * M_PI * (freq + s * I);
+    cf omega2 = cpowf(2.0 * M_PI * (freq + s * I), 2.0);
+    mu0 = 4.0 * M_PI * 1e-7;
+    int count = 0, k = 0;
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            lapw w = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+            cf mass[9];
+            for (int t = 0; t < 9; t++) {
+                int ii = i + NB_DI[t], jj = j + NB_DJ[t];
+                if (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny) {
+                    cf epse = eps[IDX(ii, jj)] + (sigma[IDX(ii, jj)] / omega) * I;
+                    mass[t] = omega2 * mu0 * epse;
+                } else {
+                    mass[t] = 0;
+                }
+            }
+            cf dx0, dy0, dxp, dxm, dyp, dym;
+            pml_at(i, j, nx, ny, px, py, &dx0, &dy0, &dxp, &dxm, &dyp, &dym);
+            count += stencil_row(i, j, nx, ny, k, mass, w, dx0, dy0, dxp, dxm, dyp, dym,
+                                 a, b, c, d, e, idh2, damp,
+                                 irow + count, icol + count, ar + count, ai + count);
+            k++;
+        }
+    return count;
+}
+
+/* Smith complex division as libgcc's __divsc3 performs it for finite operands
+ * (exposed so a test can compare it with the C '/' operator). */
+void orc_smith_divf(float a, float b, float c, float d, float *x, float *y)
+{
+    float ratio, denom;
+    if (fabsf(c) < fabsf(d)) {
+        ratio = c / d; denom = (c * ratio) + d;
+        *x = ((a * ratio) + b) / denom; *y = ((b * ratio) - a) / denom;
+    } else {
+        ratio = d / c; denom = (d * ratio) + c;
+        *x = ((b * ratio) + a) / denom; *y = (b - (a * ratio)) / denom;
+    }
+}
+void orc_c_divf(float a, float b, float c, float d, float *x, float *y)
+{
+    volatile cf num = a + b * I, den = c + d * I;
+    cf q = num / den;
+    *x = crealf(q); *y = cimagf(q);
+}
+
+/* ---- shot kernels ----------------------------------------------------------*/
+
+/* discrete source row: src/AC/RHS_source_AC.c:23-32 (and TE :27-36).
+ * Returns k_src = nxsrc + (nysrc-1)*nx, the grid point (nxsrc+1, nysrc).     */
+int orc_source_index(float x, float y, float dh, int npml, int fsshift, int nx)
+{
+    int nxsrc = (int)floor(x / dh + 0.5);
+    int nysrc = (int)floor(y / dh + 0.5);
+    nxsrc += npml;
+    nysrc += fsshift;
+    return nxsrc + (nysrc - 1) * nx;
+}
+
+/* spike amplitude written at k_src: AC src/AC/RHS_source_AC.c:37-38,52-53;
+ * TE src/TE/RHS_source_TE.c:22-23,40-41,55-56 (-(i omega mu0)/DH^2).       */
+void orc_source_amp(int physics, float freq, float s, float dh, double *re, double *im)
+{
+    float amp1, amp2;
+    if (physics == 3) {
+        cf omega = 2.0 * M_PI * (freq + s * I);
+        float mu0 = 4.0 * M_PI * 1e-7;
+        amp1 = -creal((omega * mu0 * I) * (1.0 + 0.0 * I));
+        amp2 = -cimag((omega * mu0 * I) * (1.0 + 0.0 * I));
+    } else {
+        amp1 = 1.0;
+        amp2 = 0.0;
+    }
+    *re = amp1 / (dh * dh);
+    *im = amp2 / (dh * dh);
+}
+
+/* src/AC/calc_res_AC.c:49-95 with STF_INV=0, INVMAT=1, MISFIT=1:
+ * res = pobs - pcal (float complex), muted where pobsr<1e-20 && pobsi<1e-20.
+ * Returns sum |res|^2 over ALL receivers (muted ones included, :88), with the
+ * accumulator starting at zero (the reference leaves it uninitialised).      */
+float orc_residual(int ntr, const float *pcalr, const float *pcali, const float *pobsr,
+                   const float *pobsi, float *presr, float *presi)
+{
+    cf wien = 1.0 + 0.0 * I;
+    float l2 = 0.0;
+    for (int i = 0; i < ntr; i++) {
+        cf res = ((pobsr[i] + pobsi[i] * I) - (wien * (pcalr[i] + pcali[i] * I))) / cabsf(wien);
+        presr[i] = creal(res);
+        presi[i] = cimag(res);
+        if ((pobsr[i] < 1e-20) && (pobsi[i] < 1e-20)) { presr[i] = 0.0; presi[i] = 0.0; }
+        l2 += creal(res * conj(res));
+    }
+    return l2;
+}
+
+/* src/AC/RHS_source_AC_adj.c:21-36: conj(res)/DH^2 at k_rec (last write wins) */
+void orc_adjoint_rhs(int n, int ntr, const int *krec, const float *presr, const float *presi,
+                     float dh, double *rhsr, double *rhsi)
+{
+    for (int i = 0; i < n; i++) { rhsr[i] = 0.0; rhsi[i] = 0.0; }
+    for (int i = 0; i < ntr; i++) {
+        rhsr[krec[i]] = presr[i] / (dh * dh);
+        rhsi[krec[i]] = -presi[i] / (dh * dh);
+    }
+}
+
+/* src/AC/ass_grad_AC.c:19-33 (STF_INV=0 so wien=1): one shot's gradient at
+ * every cell, accumulated in float like :46-52.  u, lam are the float fields
+ * the reference derives with vec2mat (src/vec2mat.c:17-25): cell q reads the
+ * FP64 solution at q+1; the last cell reads x[n], which no solve writes, so it
+ * is passed in explicitly as x_last.                                           */
+void orc_grad_ac_accumulate(int nx, int ny, float freq, float s,
+                            const double *ur, const double *ui, double u_last_r, double u_last_i,
+                            const double *lr, const double *li, double l_last_r, double l_last_i,
+                            const float *vp, const float *rho, float *grad)
+{
+    cf wien = 1.0f + 0.0f * I;
+    cf omega2 = cpowf(((2.0 * M_PI * freq) - (I * s)), 2.0);
+    size_t n = (size_t)nx * ny;
+    for (size_t q = 0; q < n; q++) {
+        float fr = (q + 1 < n) ? (float)ur[q + 1] : (float)u_last_r;
+        float fi = (q + 1 < n) ? (float)ui[q + 1] : (float)u_last_i;
+        float pr = (q + 1 < n) ? (float)lr[q + 1] : (float)l_last_r;
+        float pi = (q + 1 < n) ? (float)li[q + 1] : (float)l_last_i;
+        float g = -2.0 * creal(omega2 * (wien / cabsf(wien)) * (fr + fi * I)
+                               * (1.0 / (rho[q] * vp[q] * vp[q] * vp[q])) * (pr + pi * I));
+        grad[q] += g;
+    }
+}
+
+/* src/TE/ass_grad_TE.c:19-56 (STF_INV=0): sigma and epsilon gradients */
+void orc_grad_te_accumulate(int nx, int ny, float freq, float s,
+                            const double *ur, const double *ui, double u_last_r, double u_last_i,
+                            const double *lr, const double *li, double l_last_r, double l_last_i,
+                            float *grad_sigma, float *grad_eps)
+{
+    cf wien = 1.0f + 0.0f * I;
+    cf omega = 2.0 * M_PI * (freq + (I * s));
+    cf omega2 = cpowf(2.0 * M_PI * (freq + (I * s)), 2.0);
+    size_t n = (size_t)nx * ny;
+    for (size_t q = 0; q < n; q++) {
+        float fr = (q + 1 < n) ? (float)ur[q + 1] : (float)u_last_r;
+        float fi = (q + 1 < n) ? (float)ui[q + 1] : (float)u_last_i;
+        float pr = (q + 1 < n) ? (float)lr[q + 1] : (float)l_last_r;
+        float pi = (q + 1 < n) ? (float)li[q + 1] : (float)l_last_i;
+        float gs = creal((wien / cabsf(wien)) * (fr + fi * I) * (omega * I) * (pr + pi * I));
+        float ge = creal(omega2 * (wien / cabsf(wien)) * (fr + fi * I) * (pr + pi * I));
+        grad_sigma[q] += gs;
+        grad_eps[q] += ge;
+    }
+}

@@ -1,0 +1,213 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front-end of ``liboracle.so`` (oracle/assemble.c, oracle/bandlu.c): the
+CPU restatement of GERMAINE's assembly, shot kernels and an FP64 banded LU
+checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg import this module; the product package never does.
+
+Conventions: grids are nx*ny including the PML; material arrays are float32
+``[ny, nx]``; matrix values are returned in the reference's triplet order
+(row k = (i-1)+(j-1)*nx, columns ascending), which is also the CSC order of
+the matrix M = A^T that UMFPACK factors (src/AC/forward_shot_AC.c:41-48).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+AC, SH, TE = 1, 2, 3
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", _HERE])
+        _LIB = C.CDLL(path)
+        f = np.ctypeslib.ndpointer
+        L = _LIB
+        L.orc_nonzero.restype = C.c_int
+        L.orc_nonzero.argtypes = [C.c_int, C.c_int]
+        L.orc_pml_profiles.restype = None
+        L.orc_pml_profiles.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                       C.c_float, C.c_int, f(np.float32), f(np.float32)]
+        L.orc_omega2.restype = None
+        L.orc_omega2.argtypes = [C.c_int, C.c_float, C.c_float, f(np.float32), f(np.float32)]
+        for name in ("orc_assemble_ac", "orc_assemble_sh", "orc_assemble_te"):
+            fn = getattr(L, name)
+            fn.restype = C.c_int
+            fn.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                           f(np.float32), f(np.float32), f(np.float32), f(np.float32),
+                           f(np.int32), f(np.int32), f(np.float64), f(np.float64)]
+        L.orc_smith_divf.restype = None
+        L.orc_smith_divf.argtypes = [C.c_float] * 4 + [C.POINTER(C.c_float)] * 2
+        L.orc_c_divf.restype = None
+        L.orc_c_divf.argtypes = [C.c_float] * 4 + [C.POINTER(C.c_float)] * 2
+        L.orc_source_index.restype = C.c_int
+        L.orc_source_index.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]
+        L.orc_source_amp.restype = None
+        L.orc_source_amp.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_residual.restype = C.c_float
+        L.orc_residual.argtypes = [C.c_int] + [f(np.float32)] * 6
+        L.orc_adjoint_rhs.restype = None
+        L.orc_adjoint_rhs.argtypes = [C.c_int, C.c_int, f(np.int32), f(np.float32), f(np.float32),
+                                      C.c_float, f(np.float64), f(np.float64)]
+        L.orc_grad_ac_accumulate.restype = None
+        L.orc_grad_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
+                                             f(np.float64), f(np.float64), C.c_double, C.c_double,
+                                             f(np.float64), f(np.float64), C.c_double, C.c_double,
+                                             f(np.float32), f(np.float32), f(np.float32)]
+        L.orc_grad_te_accumulate.restype = None
+        L.orc_grad_te_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
+                                             f(np.float64), f(np.float64), C.c_double, C.c_double,
+                                             f(np.float64), f(np.float64), C.c_double, C.c_double,
+                                             f(np.float32), f(np.float32)]
+        L.orc_band_solve.restype = C.c_int
+        L.orc_band_solve.argtypes = [C.c_int, f(np.int32), f(np.int32), f(np.float64), f(np.float64),
+                                     C.c_void_p, C.c_int, C.c_int, f(np.float64), f(np.float64),
+                                     f(np.float64), f(np.float64)]
+        L.orc_csc_matvec.restype = None
+        L.orc_csc_matvec.argtypes = [C.c_int, f(np.int32), f(np.int32), f(np.float64), f(np.float64),
+                                     C.c_int, f(np.float64), f(np.float64), f(np.float64), f(np.float64)]
+    return _LIB
+
+
+def nonzero(nx, ny):
+    return lib().orc_nonzero(nx, ny)
+
+
+def pml_profiles(nx, ny, npml, dh, a0_pml, freq, s=0.0, free_surf=0):
+    """(px, py): float32 [4, nx+2] / [4, ny+2] rows = damp r, i, half r, half i."""
+    px = np.zeros(4 * (nx + 2), np.float32)
+    py = np.zeros(4 * (ny + 2), np.float32)
+    lib().orc_pml_profiles(nx, ny, npml, dh, a0_pml, freq, s, free_surf, px, py)
+    return px.reshape(4, nx + 2), py.reshape(4, ny + 2)
+
+
+def omega2(physics, freq, s=0.0):
+    o2 = np.zeros(2, np.float32)
+    om = np.zeros(2, np.float32)
+    lib().orc_omega2(physics, freq, s, o2, om)
+    return o2, om
+
+
+def assemble(physics, nx, ny, dh, freq, s, m1, m2, px, py):
+    """Triplets (irow, icol, Ar, Ai) of A in the reference order.
+
+    physics AC: m1=vp, m2=rho; SH: m1=vs, m2=rho; TE: m1=eps (absolute), m2=sigma.
+    """
+    nnz = nonzero(nx, ny)
+    irow = np.zeros(nnz, np.int32)
+    icol = np.zeros(nnz, np.int32)
+    ar = np.zeros(nnz, np.float64)
+    ai = np.zeros(nnz, np.float64)
+    fn = {AC: lib().orc_assemble_ac, SH: lib().orc_assemble_sh, TE: lib().orc_assemble_te}[physics]
+    cnt = fn(nx, ny, dh, freq, s,
+             np.ascontiguousarray(m1, np.float32).ravel(), np.ascontiguousarray(m2, np.float32).ravel(),
+             np.ascontiguousarray(px, np.float32).ravel(), np.ascontiguousarray(py, np.float32).ravel(),
+             irow, icol, ar, ai)
+    assert cnt == nnz
+    return irow, icol, ar, ai
+
+
+def csc_of_M(nx, ny, irow, icol, ar, ai):
+    """CSC arrays (Ap, Ai, Ax, Az) of the matrix UMFPACK sees after the host's
+    value overwrite (src/AC/forward_shot_AC.c:41-48): column l of M holds the
+    triplets of row l of A, so M = A^T and Ax is the triplet value array."""
+    n = nx * ny
+    ap = np.zeros(n + 1, np.int32)
+    np.add.at(ap, irow + 1, 1)
+    ap = np.cumsum(ap).astype(np.int32)
+    return ap, icol.astype(np.int32), ar.copy(), ai.copy()
+
+
+def band_solve(ap, ai, ax, az, b, sys=0, perm=None):
+    """Solve op(M) x = b with the banded-LU oracle; b: complex [n] or [n, nrhs]."""
+    b = np.asarray(b)
+    one = b.ndim == 1
+    B = b.reshape(b.shape[0], -1)
+    n, nrhs = B.shape
+    bx = np.ascontiguousarray(B.real.T, np.float64).ravel()
+    bz = np.ascontiguousarray(B.imag.T, np.float64).ravel()
+    xx = np.zeros(n * nrhs)
+    xz = np.zeros(n * nrhs)
+    p = None
+    if perm is not None:
+        perm = np.ascontiguousarray(perm, np.int32)
+        p = perm.ctypes.data
+    info = lib().orc_band_solve(n, ap, ai, ax, az, p, sys, nrhs, bx, bz, xx, xz)
+    if info < 0:
+        raise MemoryError("band LU allocation failed")
+    X = (xx + 1j * xz).reshape(nrhs, n).T
+    return (X[:, 0] if one else X), info
+
+
+def grid_transpose_perm(nx, ny):
+    """Symmetric permutation that orders the grid column-wise (bandwidth ny+1)."""
+    return (np.arange(nx)[:, None] + nx * np.arange(ny)[None, :]).ravel().astype(np.int32)
+
+
+def matvec(ap, ai, ax, az, x, sys=0):
+    n = len(ap) - 1
+    x = np.asarray(x, np.complex128)
+    yr = np.zeros(n)
+    yi = np.zeros(n)
+    lib().orc_csc_matvec(n, ap, ai, ax, az, sys, np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag), yr, yi)
+    return yr + 1j * yi
+
+
+def source_index(x, y, dh, npml, fsshift, nx):
+    return lib().orc_source_index(x, y, dh, npml, fsshift, nx)
+
+
+def source_amp(physics, freq, s, dh):
+    re = C.c_double()
+    im = C.c_double()
+    lib().orc_source_amp(physics, freq, s, dh, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def residual(pcal, pobs):
+    pcal = np.asarray(pcal, np.complex64)
+    pobs = np.asarray(pobs, np.complex64)
+    n = len(pcal)
+    rr = np.zeros(n, np.float32)
+    ri = np.zeros(n, np.float32)
+    l2 = lib().orc_residual(n, np.ascontiguousarray(pcal.real), np.ascontiguousarray(pcal.imag),
+                            np.ascontiguousarray(pobs.real), np.ascontiguousarray(pobs.imag), rr, ri)
+    return rr + 1j * ri, l2
+
+
+def adjoint_rhs(n, krec, res, dh):
+    res = np.asarray(res, np.complex64)
+    rr = np.zeros(n)
+    ri = np.zeros(n)
+    lib().orc_adjoint_rhs(n, len(krec), np.ascontiguousarray(krec, np.int32),
+                          np.ascontiguousarray(res.real), np.ascontiguousarray(res.imag), dh, rr, ri)
+    return rr + 1j * ri
+
+
+def grad_ac_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, vp, rho, grad):
+    lib().orc_grad_ac_accumulate(nx, ny, freq, s,
+                                 np.ascontiguousarray(u.real), np.ascontiguousarray(u.imag),
+                                 u_last.real, u_last.imag,
+                                 np.ascontiguousarray(lam.real), np.ascontiguousarray(lam.imag),
+                                 lam_last.real, lam_last.imag,
+                                 np.ascontiguousarray(vp, np.float32).ravel(),
+                                 np.ascontiguousarray(rho, np.float32).ravel(), grad)
+    return grad
+
+
+def grad_te_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, gsig, geps):
+    lib().orc_grad_te_accumulate(nx, ny, freq, s,
+                                 np.ascontiguousarray(u.real), np.ascontiguousarray(u.imag),
+                                 u_last.real, u_last.imag,
+                                 np.ascontiguousarray(lam.real), np.ascontiguousarray(lam.imag),
+                                 lam_last.real, lam_last.imag, gsig, geps)
+    return gsig, geps
