@@ -1,0 +1,378 @@
+// gfdfd extension C-ABI (include/gfdfd.h): thin wrappers that validate
+// arguments, catch exceptions, set the status and print a diagnostic.
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "../../include/gfdfd.h"
+#include "assemble.hpp"
+#include "engine.hpp"
+#include "runtime.hpp"
+#include "shot.hpp"
+
+using namespace gfdfd;
+
+struct gfdfd_plan_s {
+    std::shared_ptr<const Plan> host;
+    std::shared_ptr<DevPlan> dev;   // uploaded on first device use
+    DevPlan &device() {
+        if (!dev) dev = device_plan(host);
+        return *dev;
+    }
+};
+
+struct gfdfd_factors_s {
+    int precision = 64;
+    std::unique_ptr<Factors<double>> f64;
+    std::unique_ptr<Factors<float>> f32;
+};
+
+namespace gfdfd {
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_cache_mu;
+std::map<std::tuple<int, int, int, int>, std::shared_ptr<const Plan>> g_host_cache;
+std::map<const Plan *, std::shared_ptr<DevPlan>> g_dev_cache;
+}  // namespace
+
+int report(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    fprintf(stderr, "gfdfd: %s\n", buf);
+    fflush(stderr);
+    return code;
+}
+
+std::shared_ptr<const Plan> cached_host_plan(int nx, int ny, const PlanOptions &opt) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto key = std::make_tuple(nx, ny, opt.leaf_area, opt.small_max);
+    auto it = g_host_cache.find(key);
+    if (it != g_host_cache.end()) return it->second;
+    auto pl = std::make_shared<const Plan>(build_grid_plan(nx, ny, opt));
+    g_host_cache[key] = pl;
+    return pl;
+}
+
+std::shared_ptr<DevPlan> device_plan(std::shared_ptr<const Plan> pl) {
+    ensure_device();
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_dev_cache.find(pl.get());
+    if (it != g_dev_cache.end()) return it->second;
+    auto dp = std::make_shared<DevPlan>(pl);
+    g_dev_cache[pl.get()] = dp;
+    return dp;
+}
+
+std::shared_ptr<DevPlan> cached_grid_plan(int nx, int ny, const PlanOptions &opt) {
+    return device_plan(cached_host_plan(nx, ny, opt));
+}
+
+template <typename T>
+std::unique_ptr<Factors<T>> make_factors(std::shared_ptr<DevPlan> dp, int nfreq, const cx<double> *vals, bool copy_vals,
+                                         hipStream_t st) {
+    const Plan &pl = *dp->host;
+    auto F = std::make_unique<Factors<T>>();
+    F->plan = dp;
+    F->nfreq = nfreq;
+    F->fstride = pl.arena;
+    F->arena = dev_alloc<cx<T>>((size_t)pl.arena * nfreq);
+    F->piv = dev_alloc<int>((size_t)pl.n * nfreq);
+    F->sing = dev_alloc<int>(nfreq);
+    if (copy_vals) {
+        F->vals = dev_alloc<cx<double>>((size_t)pl.nnz * nfreq);
+        F->own_vals = true;
+        GFDFD_HIP(hipMemcpyAsync(F->vals, vals, sizeof(cx<double>) * pl.nnz * nfreq, hipMemcpyDeviceToDevice, st));
+    } else {
+        F->vals = const_cast<cx<double> *>(vals);
+    }
+    factor_numeric<T>(*dp, *F, F->vals, pl.nnz, st);
+    return F;
+}
+template std::unique_ptr<Factors<double>> make_factors<double>(std::shared_ptr<DevPlan>, int, const cx<double> *, bool,
+                                                               hipStream_t);
+template std::unique_ptr<Factors<float>> make_factors<float>(std::shared_ptr<DevPlan>, int, const cx<double> *, bool,
+                                                             hipStream_t);
+
+}  // namespace gfdfd
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                                                          \
+    }                                                                                      \
+    catch (const hip_error &e) {                                                           \
+        return report(GFDFD_ERR_DEVICE, "%s: %s", __func__, e.what());                     \
+    }                                                                                      \
+    catch (const std::bad_alloc &e) {                                                      \
+        return report(GFDFD_ERR_MEMORY, "%s: out of host memory", __func__);               \
+    }                                                                                      \
+    catch (const std::invalid_argument &e) {                                               \
+        return report(GFDFD_ERR_ARG, "%s: %s", __func__, e.what());                        \
+    }                                                                                      \
+    catch (const std::exception &e) {                                                      \
+        return report(GFDFD_ERR_INTERNAL, "%s: %s", __func__, e.what());                   \
+    }
+
+static hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+extern "C" {
+
+const char *gfdfd_last_error(void) { return g_last_error.c_str(); }
+
+int gfdfd_plan_grid_opt(int nx, int ny, int leaf_area, int small_max, gfdfd_plan *out) {
+    if (!out) return report(GFDFD_ERR_ARG, "gfdfd_plan_grid: out is NULL");
+    *out = nullptr;
+    if (nx < 2 || ny < 2) return report(GFDFD_ERR_ARG, "gfdfd_plan_grid: grid %dx%d too small", nx, ny);
+    if ((long long)nx * ny > 0x7fffffffLL / 9)
+        return report(GFDFD_ERR_ARG, "gfdfd_plan_grid: grid %dx%d exceeds 32-bit indexing", nx, ny);
+    if (leaf_area < 1 || small_max < 8 || small_max > 90)
+        return report(GFDFD_ERR_ARG, "gfdfd_plan_grid: leaf_area %d / small_max %d out of range", leaf_area, small_max);
+    GUARD_BEGIN
+    PlanOptions o;
+    o.leaf_area = leaf_area;
+    o.small_max = small_max;
+    *out = new gfdfd_plan_s{cached_host_plan(nx, ny, o), nullptr};
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_plan_grid(int nx, int ny, gfdfd_plan *out) {
+    PlanOptions o;
+    return gfdfd_plan_grid_opt(nx, ny, o.leaf_area, o.small_max, out);
+}
+
+void gfdfd_plan_free(gfdfd_plan *plan) {
+    if (plan && *plan) {
+        delete *plan;
+        *plan = nullptr;
+    }
+}
+
+int gfdfd_plan_query(gfdfd_plan plan, gfdfd_plan_info *info) {
+    if (!plan || !info) return report(GFDFD_ERR_ARG, "gfdfd_plan_query: NULL argument");
+    const Plan &pl = *plan->host;
+    info->n = pl.n;
+    info->nnz = pl.nnz;
+    info->fronts = (int)pl.fronts.size();
+    info->levels = pl.nlevels;
+    info->max_front = pl.max_m;
+    info->max_pivots = pl.max_p;
+    info->arena_elems = pl.arena;
+    info->work_rows = pl.fv_rows;
+    info->factor_flops = pl.factor_flops;
+    info->nnz_lu = pl.nnz_lu;
+    info->plan_device_bytes = plan->dev ? (double)plan->dev->bytes : 0.0;
+    return GFDFD_OK;
+}
+
+int gfdfd_plan_pattern(gfdfd_plan plan, int *colptr, int *rowind) {
+    if (!plan || !colptr || !rowind) return report(GFDFD_ERR_ARG, "gfdfd_plan_pattern: NULL argument");
+    const Plan &pl = *plan->host;
+    std::memcpy(colptr, pl.colptr.data(), sizeof(int) * pl.colptr.size());
+    std::memcpy(rowind, pl.rowind.data(), sizeof(int) * pl.rowind.size());
+    return GFDFD_OK;
+}
+
+int gfdfd_plan_perm(gfdfd_plan plan, int *perm) {
+    if (!plan || !perm) return report(GFDFD_ERR_ARG, "gfdfd_plan_perm: NULL argument");
+    const Plan &pl = *plan->host;
+    std::memcpy(perm, pl.perm.data(), sizeof(int) * pl.perm.size());
+    return GFDFD_OK;
+}
+
+int gfdfd_plan_fronts(gfdfd_plan plan, int *p, int *m, int *pos0, int *parent, int *level, long long *idx_off) {
+    if (!plan || !p || !m || !pos0 || !parent || !level || !idx_off)
+        return report(GFDFD_ERR_ARG, "gfdfd_plan_fronts: NULL argument");
+    const Plan &pl = *plan->host;
+    for (size_t f = 0; f < pl.fronts.size(); f++) {
+        const Front &F = pl.fronts[f];
+        p[f] = F.p; m[f] = F.m; pos0[f] = F.pos0; parent[f] = F.parent; level[f] = F.level;
+        idx_off[f] = F.idx_off;
+    }
+    return GFDFD_OK;
+}
+
+int gfdfd_plan_index(gfdfd_plan plan, int *idx) {
+    if (!plan || !idx) return report(GFDFD_ERR_ARG, "gfdfd_plan_index: NULL argument");
+    const Plan &pl = *plan->host;
+    std::memcpy(idx, pl.idx.data(), sizeof(int) * pl.idx.size());
+    return GFDFD_OK;
+}
+
+void gfdfd_fd_weights(float dh, float *out6);
+float gfdfd_mu0(void);
+
+int gfdfd_assemble(gfdfd_plan plan, int physics, float dh, int nfreq, const gfdfd_freq *freqs, const float *d_m1,
+                   const float *d_m2, double *d_vals, void *stream) {
+    if (!plan || !freqs || !d_m1 || !d_m2 || !d_vals || nfreq < 1)
+        return report(GFDFD_ERR_ARG, "gfdfd_assemble: missing argument or nfreq < 1");
+    if (physics < 1 || physics > 3) return report(GFDFD_ERR_ARG, "gfdfd_assemble: physics %d", physics);
+    if (!(dh > 0)) return report(GFDFD_ERR_ARG, "gfdfd_assemble: DH must be positive");
+    GUARD_BEGIN
+    const Plan &pl = *plan->host;
+    if (!pl.grid) return report(GFDFD_ERR_ARG, "gfdfd_assemble: plan is not a grid plan");
+    DevPlan &dp = plan->device();
+    const int nx = pl.nx, ny = pl.ny;
+    const size_t plen = 4 * (size_t)(nx + 2) + 4 * (size_t)(ny + 2);
+    std::vector<float> prof(plen * nfreq);
+    std::vector<AsmFreq> fr(nfreq);
+    for (int f = 0; f < nfreq; f++) {
+        if (!freqs[f].prof) return report(GFDFD_ERR_ARG, "gfdfd_assemble: frequency %d has no PML profile", f);
+        std::memcpy(&prof[plen * f], freqs[f].prof, plen * sizeof(float));
+        fr[f] = {freqs[f].omega2[0], freqs[f].omega2[1], freqs[f].omega[0], freqs[f].omega[1]};
+    }
+    hipStream_t st = S(stream);
+    float *dprof = dev_alloc<float>(prof.size());
+    AsmFreq *dfr = dev_alloc<AsmFreq>(nfreq);
+    GFDFD_HIP(hipMemcpyAsync(dprof, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    GFDFD_HIP(hipMemcpyAsync(dfr, fr.data(), fr.size() * sizeof(AsmFreq), hipMemcpyHostToDevice, st));
+    float w[6];
+    gfdfd_fd_weights(dh, w);
+    AsmScalars sc{w[0], w[1], w[2], w[3], w[4], w[5], gfdfd_mu0(), dfr};
+    assemble_device(physics, nx, ny, sc, d_m1, d_m2, dprof, dp.colptr, reinterpret_cast<cx<double> *>(d_vals),
+                    pl.nnz, nfreq, st);
+    GFDFD_HIP(hipStreamSynchronize(st));
+    dev_free(dprof);
+    dev_free(dfr);
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_factor(gfdfd_plan plan, int nfreq, const double *d_vals, int precision, void *stream, gfdfd_factors *out) {
+    if (!out) return report(GFDFD_ERR_ARG, "gfdfd_factor: out is NULL");
+    *out = nullptr;
+    if (!plan || !d_vals || nfreq < 1) return report(GFDFD_ERR_ARG, "gfdfd_factor: missing argument or nfreq < 1");
+    if (precision != GFDFD_FP64 && precision != GFDFD_FP32)
+        return report(GFDFD_ERR_ARG, "gfdfd_factor: precision must be 64 or 32");
+    GUARD_BEGIN
+    auto h = std::make_unique<gfdfd_factors_s>();
+    h->precision = precision;
+    const cx<double> *v = reinterpret_cast<const cx<double> *>(d_vals);
+    plan->device();
+    if (precision == GFDFD_FP64)
+        h->f64 = make_factors<double>(plan->dev, nfreq, v, true, S(stream));
+    else
+        h->f32 = make_factors<float>(plan->dev, nfreq, v, true, S(stream));
+    *out = h.release();
+    return GFDFD_OK;
+    GUARD_END
+}
+
+void gfdfd_factors_free(gfdfd_factors *f) {
+    if (f && *f) {
+        delete *f;
+        *f = nullptr;
+    }
+}
+
+int gfdfd_factors_singular(gfdfd_factors f, int *flags) {
+    if (!f || !flags) return report(GFDFD_ERR_ARG, "gfdfd_factors_singular: NULL argument");
+    GUARD_BEGIN
+    int nf = f->f64 ? f->f64->nfreq : f->f32->nfreq;
+    const int *d = f->f64 ? f->f64->sing : f->f32->sing;
+    GFDFD_HIP(hipMemcpy(flags, d, sizeof(int) * nf, hipMemcpyDeviceToHost));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_solve(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x, void *stream) {
+    if (!f || !d_b || !d_x || nrhs < 1) return report(GFDFD_ERR_ARG, "gfdfd_solve: missing argument or nrhs < 1");
+    if (sys < 0 || sys > 2) return report(GFDFD_ERR_ARG, "gfdfd_solve: invalid system %d", sys);
+    GUARD_BEGIN
+    auto B = reinterpret_cast<const cx<double> *>(d_b);
+    auto X = reinterpret_cast<cx<double> *>(d_x);
+    if (f->f64)
+        solve_panel<double, double>(*f->f64, ifreq, sys, nrhs, B, X, S(stream));
+    else
+        solve_panel<float, double>(*f->f32, ifreq, sys, nrhs, B, X, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_solve_refine(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x, double tol,
+                       int max_iter, int *iters, double *relres, void *stream) {
+    if (!f || !d_b || !d_x || nrhs < 1) return report(GFDFD_ERR_ARG, "gfdfd_solve_refine: missing argument");
+    if (d_b == d_x) return report(GFDFD_ERR_ARG, "gfdfd_solve_refine: d_x must not alias d_b");
+    if (sys < 0 || sys > 2) return report(GFDFD_ERR_ARG, "gfdfd_solve_refine: invalid system %d", sys);
+    GUARD_BEGIN
+    auto B = reinterpret_cast<const cx<double> *>(d_b);
+    auto X = reinterpret_cast<cx<double> *>(d_x);
+    RefineStats rs = f->f64 ? solve_refined<double>(*f->f64, ifreq, sys, nrhs, B, X, tol, max_iter, S(stream))
+                            : solve_refined<float>(*f->f32, ifreq, sys, nrhs, B, X, tol, max_iter, S(stream));
+    if (iters) *iters = rs.iterations;
+    if (relres) *relres = rs.max_rel_residual;
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_residual(gfdfd_plan plan, const double *d_vals, int sys, int nrhs, const double *d_b, const double *d_x,
+                   double *d_r, void *stream) {
+    if (!plan || !d_vals || !d_b || !d_x || !d_r || nrhs < 1)
+        return report(GFDFD_ERR_ARG, "gfdfd_residual: missing argument");
+    if (sys < 0 || sys > 2) return report(GFDFD_ERR_ARG, "gfdfd_residual: invalid system %d", sys);
+    GUARD_BEGIN
+    residual_fp64(plan->device(), reinterpret_cast<const cx<double> *>(d_vals), sys, nrhs,
+                  reinterpret_cast<const cx<double> *>(d_b), reinterpret_cast<const cx<double> *>(d_x),
+                  reinterpret_cast<cx<double> *>(d_r), S(stream));
+    GFDFD_HIP(hipGetLastError());
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_spike_panel(int n, int nshots, const int *d_k, double amp_re, double amp_im, double *d_b, void *stream) {
+    if (n < 1 || nshots < 1 || !d_k || !d_b) return report(GFDFD_ERR_ARG, "gfdfd_spike_panel: bad argument");
+    GUARD_BEGIN
+    spike_panel(n, nshots, d_k, cx<double>{amp_re, amp_im}, reinterpret_cast<cx<double> *>(d_b), S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_sample(int n, int nshots, int ntr, const int *d_krec, const double *d_x, float *d_pcal, void *stream) {
+    if (n < 1 || nshots < 1 || ntr < 1 || !d_krec || !d_x || !d_pcal)
+        return report(GFDFD_ERR_ARG, "gfdfd_sample: bad argument");
+    GUARD_BEGIN
+    sample_receivers(n, nshots, ntr, d_krec, reinterpret_cast<const cx<double> *>(d_x),
+                     reinterpret_cast<cx<float> *>(d_pcal), S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_adjoint_sources(int n, int nshots, int ntr, const int *d_krec, const float *d_pcal, const float *d_pobs,
+                          float dh, double *d_b, double *d_l2, void *stream) {
+    if (n < 1 || nshots < 1 || ntr < 1 || !d_krec || !d_pcal || !d_pobs || !d_b || !d_l2)
+        return report(GFDFD_ERR_ARG, "gfdfd_adjoint_sources: bad argument");
+    GUARD_BEGIN
+    adjoint_sources(n, nshots, ntr, d_krec, reinterpret_cast<const cx<float> *>(d_pcal),
+                    reinterpret_cast<const cx<float> *>(d_pobs), dh, reinterpret_cast<cx<double> *>(d_b), d_l2,
+                    S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam, const float *d_vp,
+                  const float *d_rho, double *d_grad, void *stream) {
+    if (n < 1 || nshots < 1 || !omega2 || !d_u || !d_lam || !d_vp || !d_rho || !d_grad)
+        return report(GFDFD_ERR_ARG, "gfdfd_grad_ac: bad argument");
+    GUARD_BEGIN
+    grad_ac(n, nshots, omega2, reinterpret_cast<const cx<double> *>(d_u), reinterpret_cast<const cx<double> *>(d_lam),
+            d_vp, d_rho, d_grad, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u, const double *d_lam,
+                  double *d_grad_sigma, double *d_grad_eps, void *stream) {
+    if (n < 1 || nshots < 1 || !omega || !omega2 || !d_u || !d_lam || !d_grad_sigma || !d_grad_eps)
+        return report(GFDFD_ERR_ARG, "gfdfd_grad_te: bad argument");
+    GUARD_BEGIN
+    grad_te(n, nshots, omega, omega2, reinterpret_cast<const cx<double> *>(d_u),
+            reinterpret_cast<const cx<double> *>(d_lam), d_grad_sigma, d_grad_eps, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+}  // extern "C"

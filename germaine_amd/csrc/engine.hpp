@@ -1,0 +1,78 @@
+// Device-resident plan, factors and the host-side drivers of the numeric
+// factorisation and the panel solves.
+#pragma once
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+#include "symbolic.hpp"
+
+namespace gfdfd {
+
+constexpr int kTile = 64;   // output tile of the trailing-update GEMM
+
+// Plan arrays uploaded once per grid size and shared by every factorisation.
+struct DevPlan {
+    std::shared_ptr<const Plan> host;
+    FrontDev *fronts = nullptr;
+    int *children = nullptr, *idx = nullptr, *inv = nullptr, *perm = nullptr, *iperm = nullptr;
+    int64_t *orig_q = nullptr, *orig_dst = nullptr;
+    int *colptr = nullptr, *rowind = nullptr;
+    int64_t *tq = nullptr;                 // CSR-of-M position -> CSC index (symmetric pattern)
+    std::vector<int *> lv_small, lv_large, lv_all;
+    std::vector<int> n_small, n_large, n_all, mmax_all;
+    TileDev *tiles = nullptr;
+    std::vector<std::vector<std::pair<int64_t, int>>> steps;   // [level][step] -> (offset, count)
+    size_t bytes = 0;
+
+    explicit DevPlan(std::shared_ptr<const Plan> p);
+    ~DevPlan();
+    DevPlan(const DevPlan &) = delete;
+    DevPlan &operator=(const DevPlan &) = delete;
+};
+
+// Numeric factors of nfreq matrices with the same pattern.
+template <typename T>
+struct Factors {
+    std::shared_ptr<DevPlan> plan;
+    int nfreq = 0;
+    int64_t fstride = 0;            // arena elements per frequency
+    cx<T> *arena = nullptr;         // fronts, column-major, factors in place
+    int *piv = nullptr;             // nfreq * n local row swaps
+    int *sing = nullptr;            // nfreq flags: zero pivot met
+    cx<double> *vals = nullptr;     // nfreq * nnz FP64 values of M (CSC order) kept for refinement
+    bool own_vals = false;
+    ~Factors();
+};
+
+// numeric factorisation of nfreq matrices whose CSC values (FP64, interleaved)
+// sit at vals + f*vstride
+template <typename T>
+void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, int64_t vstride, hipStream_t st);
+
+// Panel solve of op(M_f) X = B for one frequency; B and X are column-major
+// n x nrhs interleaved complex of type TIO (may alias).  sys: 0 M, 1 M^H, 2 M^T.
+template <typename T, typename TIO>
+void solve_panel(const Factors<T> &F, int ifreq, int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st);
+
+// r = b - op(M) x in FP64 (CSC values vals of M); optional per-column norms ||r||_2^2 and ||b||_2^2.
+void residual_fp64(const DevPlan &dp, const cx<double> *vals, int sys, int nrhs, const cx<double> *B,
+                   const cx<double> *X, cx<double> *R, hipStream_t st);
+void column_norms2(int n, int nrhs, const cx<double> *V, double *out, hipStream_t st);
+void axpy_fp64(int64_t count, const cx<double> *D, cx<double> *X, hipStream_t st);          // X += D
+void cast_c128_to_c64(int64_t count, const cx<double> *a, cx<float> *b, hipStream_t st);
+void cast_c64_to_c128(int64_t count, const cx<float> *a, cx<double> *b, hipStream_t st);
+
+struct RefineStats {
+    int iterations = 0;
+    double max_rel_residual = 0;   // max over columns of ||b - M x|| / ||b||
+};
+
+// FP64 iterative refinement around the factors F (any precision):
+// x_0 = solve(b); repeat r = b - M x, x += solve(r) until max rel residual <= tol.
+template <typename T>
+RefineStats solve_refined(const Factors<T> &F, int ifreq, int sys, int nrhs, const cx<double> *B, cx<double> *X,
+                          double tol, int max_iter, hipStream_t st);
+
+}  // namespace gfdfd

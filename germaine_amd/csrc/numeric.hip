@@ -1,0 +1,448 @@
+// Numeric multifrontal factorisation on gfx950.
+//
+// Fronts are processed level by level (height above the leaves), all fronts
+// of a level and all frequencies of a batch in the same launches:
+//   1. assemble: every entry of a front is written once, as the sum of the
+//      children's Schur complements (pulled through child->parent index maps);
+//   2. the original matrix entries owned by the level are added (each
+//      destination is unique, so no atomics);
+//   3. partial LU of the first p columns with row pivoting restricted to the
+//      p fully summed rows (the front's structure never changes):
+//      - fronts with m <= small_max: one workgroup, front held in LDS;
+//      - larger fronts: blocked right-looking LU, per panel step a panel
+//        kernel (pivot search, row swaps over the whole front, scaling),
+//        a TRSM of the U row block, and an MFMA trailing update of the
+//        remaining front (which leaves the Schur complement in F22).
+// Factors stay in place in the front arena (L11\U11 and L21 in the first p
+// columns, U12 in the first p rows); the pivot swaps go to piv[pos0 + j].
+#include <algorithm>
+
+#include "engine.hpp"
+
+namespace gfdfd {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// DevPlan
+// ---------------------------------------------------------------------------
+DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
+    const Plan &pl = *host;
+    std::vector<FrontDev> fd(pl.fronts.size());
+    for (size_t f = 0; f < pl.fronts.size(); f++) {
+        const Front &F = pl.fronts[f];
+        fd[f] = {F.p, F.m, F.pos0, F.nchild, F.child_off, F.level, F.buf_off, F.fv_off, F.idx_off, F.inv_off};
+    }
+    fronts = dev_upload(fd);
+    children = dev_upload(pl.children);
+    idx = dev_upload(pl.idx);
+    inv = dev_upload(pl.inv);
+    perm = dev_upload(pl.perm);
+    iperm = dev_upload(pl.iperm);
+    orig_q = dev_upload(pl.orig_q);
+    orig_dst = dev_upload(pl.orig_dst);
+    colptr = dev_upload(pl.colptr);
+    rowind = dev_upload(pl.rowind);
+    bytes += fd.size() * sizeof(FrontDev) + (pl.children.size() + pl.idx.size() + pl.inv.size() +
+                                            2 * pl.perm.size() + pl.colptr.size() + pl.rowind.size()) * 4 +
+             (pl.orig_q.size() + pl.orig_dst.size()) * 8;
+    // CSR-of-M -> CSC map, valid when the pattern is symmetric (grid plans)
+    if (pl.grid) {
+        std::vector<int64_t> t(pl.nnz);
+        std::vector<int> fill(pl.n, 0);
+        for (int c = 0; c < pl.n; c++)
+            for (int q = pl.colptr[c]; q < pl.colptr[c + 1]; q++) {
+                int r = pl.rowind[q];
+                // row r of M: its entries in column order are the rows of CSC column r (symmetric pattern)
+                t[pl.colptr[r] + fill[r]++] = q;
+            }
+        tq = dev_upload(t);
+        bytes += t.size() * 8;
+    }
+    for (const LevelPlan &L : pl.levels) {
+        std::vector<int> all(L.small);
+        all.insert(all.end(), L.large.begin(), L.large.end());
+        lv_small.push_back(dev_upload(L.small));
+        lv_large.push_back(dev_upload(L.large));
+        lv_all.push_back(dev_upload(all));
+        n_small.push_back((int)L.small.size());
+        n_large.push_back((int)L.large.size());
+        n_all.push_back((int)all.size());
+        mmax_all.push_back(std::max(L.small_mmax, L.large_mmax));
+    }
+    // tile lists of every blocked step
+    std::vector<TileDev> tl;
+    std::vector<Tile> tmp;
+    steps.resize(pl.levels.size());
+    for (size_t l = 0; l < pl.levels.size(); l++) {
+        const LevelPlan &L = pl.levels[l];
+        for (int k0 = 0; k0 < L.large_pmax; k0 += pl.nb) {
+            step_tiles(pl, L, k0, kTile, tmp);
+            steps[l].push_back({(int64_t)tl.size(), (int)tmp.size()});
+            for (const Tile &t : tmp) tl.push_back({t.front, t.ti, t.tj});
+        }
+    }
+    tiles = dev_upload(tl);
+    bytes += tl.size() * sizeof(TileDev);
+}
+
+DevPlan::~DevPlan() {
+    dev_free(fronts); dev_free(children); dev_free(idx); dev_free(inv); dev_free(perm); dev_free(iperm);
+    dev_free(orig_q); dev_free(orig_dst); dev_free(colptr); dev_free(rowind); dev_free(tq); dev_free(tiles);
+    for (auto p : lv_small) dev_free(p);
+    for (auto p : lv_large) dev_free(p);
+    for (auto p : lv_all) dev_free(p);
+}
+
+template <typename T>
+Factors<T>::~Factors() {
+    dev_free(arena);
+    dev_free(piv);
+    dev_free(sing);
+    if (own_vals) dev_free(vals);
+}
+template struct Factors<double>;
+template struct Factors<float>;
+
+// ---------------------------------------------------------------------------
+// 1+2: front assembly
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_front_assemble(cx<T> *__restrict__ arena, int64_t fstride,
+                                                        const FrontDev *__restrict__ fronts,
+                                                        const int *__restrict__ list, const int *__restrict__ children,
+                                                        const int *__restrict__ inv) {
+    const FrontDev F = fronts[list[blockIdx.x]];
+    cx<T> *A = arena + blockIdx.z * fstride;
+    cx<T> *G = A + F.buf_off;
+    const int64_t total = (int64_t)F.m * F.m;
+    const int64_t step = (int64_t)gridDim.y * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < total; e += step) {
+        int s = (int)(e / F.m), r = (int)(e - (int64_t)s * F.m);
+        cx<T> v = {T(0), T(0)};
+        for (int c = 0; c < F.nchild; c++) {
+            const FrontDev C = fronts[children[F.child_off + c]];
+            int a = inv[C.inv_off + r], b = inv[C.inv_off + s];
+            if (a >= 0 && b >= 0) v = cadd(v, A[C.buf_off + (C.p + a) + (int64_t)(C.p + b) * C.m]);
+        }
+        G[e] = v;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scatter_orig(cx<T> *__restrict__ arena, int64_t fstride,
+                                                      const cx<double> *__restrict__ vals, int64_t vstride,
+                                                      const int64_t *__restrict__ q, const int64_t *__restrict__ dst,
+                                                      int64_t count) {
+    cx<T> *A = arena + blockIdx.y * fstride;
+    const cx<double> *V = vals + blockIdx.y * vstride;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
+        cx<double> v = V[q[e]];
+        cx<T> &d = A[dst[e]];
+        d.x += (T)v.x;
+        d.y += (T)v.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pivot search helper: max cabs1 with the smallest row on ties
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void better(T &best, int &bi, T ob, int oi) {
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+}
+
+template <typename T>
+__device__ int block_argmax(T best, int bi, T *sb, int *si) {
+    // wave reduction then across waves through LDS
+    for (int off = 32; off > 0; off >>= 1) {
+        T ob = __shfl_xor(best, off);
+        int oi = __shfl_xor(bi, off);
+        better(best, bi, ob, oi);
+    }
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sb[w] = best; si[w] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < nw; k++) better(sb[0], si[0], sb[k], si[k]);
+    }
+    __syncthreads();
+    return si[0];
+}
+
+// ---------------------------------------------------------------------------
+// 3a: small fronts, one workgroup, front in LDS
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int64_t fstride,
+                                                  const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                  int *__restrict__ piv, int n, int *__restrict__ sing) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *S = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ T sb[4];
+    __shared__ int si[4];
+    const FrontDev F = fronts[list[blockIdx.x]];
+    const int fq = blockIdx.y;
+    cx<T> *G = arena + fq * fstride + F.buf_off;
+    const int m = F.m, p = F.p, mm = m * m, tid = threadIdx.x, bs = blockDim.x;
+    for (int e = tid; e < mm; e += bs) S[e] = G[e];
+    __syncthreads();
+    for (int j = 0; j < p; j++) {
+        T best = T(-1);
+        int bi = j;
+        for (int r = j + tid; r < p; r += bs) better(best, bi, cabs1(S[r + j * m]), r);
+        const int pr = block_argmax(best, bi, sb, si);
+        if (tid == 0) {
+            piv[(int64_t)fq * n + F.pos0 + j] = pr;
+            if (sb[0] == T(0)) atomicOr(sing + fq, 1);
+        }
+        if (pr != j)
+            for (int c = tid; c < m; c += bs) {
+                cx<T> t = S[j + c * m];
+                S[j + c * m] = S[pr + c * m];
+                S[pr + c * m] = t;
+            }
+        __syncthreads();
+        const cx<T> d = crecip(S[j + j * m]);
+        for (int r = j + 1 + tid; r < m; r += bs) S[r + j * m] = cmul(S[r + j * m], d);
+        __syncthreads();
+        const int t = m - j - 1;
+        for (int e = tid; e < t * t; e += bs) {
+            int c = j + 1 + e / t, r = j + 1 + e % t;
+            S[r + c * m] = cmsub(S[r + c * m], S[r + j * m], S[j + c * m]);
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < mm; e += bs) G[e] = S[e];
+}
+
+// ---------------------------------------------------------------------------
+// 3b: blocked path for large fronts
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_panel(cx<T> *__restrict__ arena, int64_t fstride,
+                                               const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                               int *__restrict__ piv, int n, int k0, int nb, int *__restrict__ sing) {
+    __shared__ T sb[4];
+    __shared__ int si[4];
+    const FrontDev F = fronts[list[blockIdx.x]];
+    if (F.p <= k0) return;
+    const int fq = blockIdx.y;
+    cx<T> *G = arena + fq * fstride + F.buf_off;
+    const int m = F.m, p = F.p, kb = min(nb, p - k0), tid = threadIdx.x, bs = blockDim.x;
+    for (int j = k0; j < k0 + kb; j++) {
+        T best = T(-1);
+        int bi = j;
+        for (int r = j + tid; r < p; r += bs) better(best, bi, cabs1(G[r + (int64_t)j * m]), r);
+        const int pr = block_argmax(best, bi, sb, si);
+        if (tid == 0) {
+            piv[(int64_t)fq * n + F.pos0 + j] = pr;
+            if (sb[0] == T(0)) atomicOr(sing + fq, 1);
+        }
+        if (pr != j)
+            for (int c = tid; c < m; c += bs) {
+                cx<T> t = G[j + (int64_t)c * m];
+                G[j + (int64_t)c * m] = G[pr + (int64_t)c * m];
+                G[pr + (int64_t)c * m] = t;
+            }
+        __syncthreads();
+        const cx<T> d = crecip(G[j + (int64_t)j * m]);
+        for (int r = j + 1 + tid; r < m; r += bs) G[r + (int64_t)j * m] = cmul(G[r + (int64_t)j * m], d);
+        __syncthreads();
+        const int rows = m - j - 1, ncol = k0 + kb - j - 1;
+        for (int e = tid; e < rows * ncol; e += bs) {
+            int c = j + 1 + e / rows, r = j + 1 + e % rows;
+            G[r + (int64_t)c * m] = cmsub(G[r + (int64_t)c * m], G[r + (int64_t)j * m], G[j + (int64_t)c * m]);
+        }
+        __syncthreads();
+    }
+}
+
+// U12 block row of the panel: rows [k0,k0+kb), columns [k0+kb, m): L_kb^-1 * F
+template <typename T>
+__global__ void __launch_bounds__(64) k_trsm_u(cx<T> *__restrict__ arena, int64_t fstride,
+                                               const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                               int k0, int nb) {
+    __shared__ cx<T> Ls[32][33];
+    __shared__ cx<T> Xs[32][65];
+    const FrontDev F = fronts[list[blockIdx.y]];
+    if (F.p <= k0) return;
+    const int m = F.m, kb = min(nb, F.p - k0);
+    const int c0 = k0 + kb + blockIdx.x * 64;
+    if (c0 >= m) return;
+    cx<T> *G = arena + blockIdx.z * fstride + F.buf_off;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kb * kb; e += 64) {
+        int i = e % kb, t = e / kb;
+        Ls[i][t] = G[(k0 + i) + (int64_t)(k0 + t) * m];
+    }
+    for (int e = tid; e < kb * 64; e += 64) {
+        int i = e % kb, c = e / kb, col = c0 + c;
+        Xs[i][c] = col < m ? G[(k0 + i) + (int64_t)col * m] : cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    for (int i = 1; i < kb; i++) {
+        cx<T> acc = Xs[i][tid];
+        for (int t = 0; t < i; t++) acc = cmsub(acc, Ls[i][t], Xs[t][tid]);
+        Xs[i][tid] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < kb * 64; e += 64) {
+        int i = e % kb, c = e / kb, col = c0 + c;
+        if (col < m) G[(k0 + i) + (int64_t)col * m] = Xs[i][c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// trailing update C -= A*B on MFMA (complex = 4 real products), 64x64 tiles,
+// K <= 32.  4 waves in 2x2, each wave 32x32 = 2x2 blocks of 16x16.
+// MFMA 16x16x4 operand layout: A lane l -> (row l%16, k l/16), B lane l ->
+// (k l/16, col l%16), D lane l, v -> (row 4*(l/16)+v, col l%16).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct mfma_t;
+template <>
+struct mfma_t<double> {
+    typedef d4v acc;
+    static __device__ __forceinline__ acc op(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct mfma_t<float> {
+    typedef f4v acc;
+    static __device__ __forceinline__ acc op(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, int64_t fstride,
+                                                     const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                     const TileDev *__restrict__ tiles, int k0, int nb) {
+    constexpr int KB = 32;
+    __shared__ T Ar[KB][64], Ai[KB][64], Br[KB][64], Bi[KB][64];
+    const TileDev tl = tiles[blockIdx.x];
+    const FrontDev F = fronts[list[tl.front]];
+    const int m = F.m, kb = min(nb, F.p - k0);
+    const int base = k0 + kb;
+    const int r0 = base + tl.ti * 64, c0 = base + tl.tj * 64;
+    cx<T> *G = arena + blockIdx.y * fstride + F.buf_off;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < KB * 64; e += 256) {
+        int i = e & 63, k = e >> 6;
+        cx<T> v = {T(0), T(0)};
+        if (k < kb && r0 + i < m) v = G[(r0 + i) + (int64_t)(k0 + k) * m];
+        Ar[k][i] = v.x;
+        Ai[k][i] = v.y;
+    }
+    for (int e = tid; e < KB * 64; e += 256) {
+        int k = e % KB, j = e / KB;
+        cx<T> v = {T(0), T(0)};
+        if (k < kb && c0 + j < m) v = G[(k0 + k) + (int64_t)(c0 + j) * m];
+        Br[k][j] = v.x;
+        Bi[k][j] = v.y;
+    }
+    __syncthreads();
+    typedef typename mfma_t<T>::acc acc_t;
+    acc_t cr[2][2], ci[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            cr[a][b] = acc_t{0, 0, 0, 0};
+            ci[a][b] = acc_t{0, 0, 0, 0};
+        }
+    const int lane = tid & 63, w = tid >> 6, wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int kend = (kb + 3) & ~3;
+    for (int kk = 0; kk < kend; kk += 4) {
+        const int k = kk + lk;
+        T ar[2], ai[2], br[2], bi[2];
+        for (int a = 0; a < 2; a++) {
+            ar[a] = Ar[k][wr + a * 16 + lr];
+            ai[a] = Ai[k][wr + a * 16 + lr];
+        }
+        for (int b = 0; b < 2; b++) {
+            br[b] = Br[k][wc + b * 16 + lr];
+            bi[b] = Bi[k][wc + b * 16 + lr];
+        }
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                cr[a][b] = mfma_t<T>::op(ar[a], br[b], cr[a][b]);
+                cr[a][b] = mfma_t<T>::op(-ai[a], bi[b], cr[a][b]);
+                ci[a][b] = mfma_t<T>::op(ar[a], bi[b], ci[a][b]);
+                ci[a][b] = mfma_t<T>::op(ai[a], br[b], ci[a][b]);
+            }
+    }
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int v = 0; v < 4; v++) {
+                const int row = r0 + wr + a * 16 + 4 * lk + v, col = c0 + wc + b * 16 + lr;
+                if (row < m && col < m) {
+                    cx<T> &g = G[row + (int64_t)col * m];
+                    g.x -= cr[a][b][v];
+                    g.y -= ci[a][b][v];
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// driver
+// ---------------------------------------------------------------------------
+template <typename T>
+void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, int64_t vstride, hipStream_t st) {
+    const Plan &pl = *dp.host;
+    const int nf = F.nfreq;
+    GFDFD_HIP(hipMemsetAsync(F.sing, 0, sizeof(int) * nf, st));
+    const size_t lds_small_max = 160 * 1024 - 256;
+    static bool attr_set = false;
+    if (!attr_set) {
+        // fronts of up to 90x90 complex doubles live in LDS (gfx950: 160 KB per CU)
+        GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_small_lu<double>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_small_max));
+        GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_small_lu<float>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_small_max));
+        attr_set = true;
+    }
+    for (int l = 0; l < pl.nlevels; l++) {
+        const LevelPlan &L = pl.levels[l];
+        if (dp.n_all[l] > 0) {
+            int64_t mm = (int64_t)dp.mmax_all[l] * dp.mmax_all[l];
+            int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, mm / (256 * 8)), 512);
+            hipLaunchKernelGGL(k_front_assemble<T>, dim3(dp.n_all[l], chunks, nf), dim3(256), 0, st, F.arena,
+                               F.fstride, dp.fronts, dp.lv_all[l], dp.children, dp.inv);
+        }
+        const int64_t cnt = L.orig_end - L.orig_begin;
+        if (cnt > 0) {
+            int blocks = (int)std::min<int64_t>((cnt + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_scatter_orig<T>, dim3(blocks, nf), dim3(256), 0, st, F.arena, F.fstride, vals,
+                               vstride, dp.orig_q + L.orig_begin, dp.orig_dst + L.orig_begin, cnt);
+        }
+        if (dp.n_small[l] > 0) {
+            size_t lds = (size_t)L.small_mmax * L.small_mmax * sizeof(cx<T>);
+            if (lds > lds_small_max) throw std::logic_error("small front exceeds LDS");
+            hipLaunchKernelGGL(k_small_lu<T>, dim3(dp.n_small[l], nf), dim3(256), lds, st, F.arena, F.fstride,
+                               dp.fronts, dp.lv_small[l], F.piv, pl.n, F.sing);
+        }
+        if (dp.n_large[l] > 0) {
+            int si = 0;
+            for (int k0 = 0; k0 < L.large_pmax; k0 += pl.nb, si++) {
+                hipLaunchKernelGGL(k_panel<T>, dim3(dp.n_large[l], nf), dim3(256), 0, st, F.arena, F.fstride,
+                                   dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, F.sing);
+                int colchunks = (L.large_mmax - k0 + 63) / 64;
+                hipLaunchKernelGGL(k_trsm_u<T>, dim3(colchunks, dp.n_large[l], nf), dim3(64), 0, st, F.arena,
+                                   F.fstride, dp.fronts, dp.lv_large[l], k0, pl.nb);
+                const auto &sp = dp.steps[l][si];
+                if (sp.second > 0)
+                    hipLaunchKernelGGL(k_gemm_update<T>, dim3(sp.second, nf), dim3(256), 0, st, F.arena, F.fstride,
+                                       dp.fronts, dp.lv_large[l], dp.tiles + sp.first, k0, pl.nb);
+            }
+        }
+    }
+    GFDFD_HIP(hipGetLastError());
+}
+
+template void factor_numeric<double>(const DevPlan &, Factors<double> &, const cx<double> *, int64_t, hipStream_t);
+template void factor_numeric<float>(const DevPlan &, Factors<float> &, const cx<double> *, int64_t, hipStream_t);
+
+}  // namespace gfdfd

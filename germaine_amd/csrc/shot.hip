@@ -1,0 +1,141 @@
+// Shot kernels on the device: point-source panels, receiver sampling, data
+// residuals with the adjoint source panel, and the FWI gradients reduced over
+// all shots in FP64.  Panels are column-major n x nshots (one shot per column).
+#include <algorithm>
+
+#include "common.hpp"
+#include "shot.hpp"
+
+namespace gfdfd {
+
+namespace {
+
+// src/AC/RHS_source_AC.c:46-53 (TE :50-56) for every shot: zero, then the spike
+__global__ void k_spikes(int n, int nshots, const int *__restrict__ k, cx<double> amp, cx<double> *__restrict__ B) {
+    const int64_t total = (int64_t)n * nshots;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int s = (int)(e / n), row = (int)(e - (int64_t)s * n);
+        B[e] = row == k[s] ? amp : cx<double>{0.0, 0.0};
+    }
+}
+
+// src/AC/calc_seis_AC.c:19-20 after src/vec2mat.c:19-24: the float wavefield
+// sample of receiver t is the solution entry k_rec(t)
+__global__ void k_sample(int n, int nshots, int ntr, const int *__restrict__ krec, const cx<double> *__restrict__ X,
+                         cx<float> *__restrict__ pcal) {
+    const int64_t total = (int64_t)ntr * nshots;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int s = (int)(e / ntr), t = (int)(e - (int64_t)s * ntr);
+        cx<double> v = X[krec[t] + (int64_t)s * n];
+        pcal[e] = {(float)v.x, (float)v.y};
+    }
+}
+
+// src/AC/calc_res_AC.c:49-95 (STF_INV=0, MISFIT=1) and src/AC/RHS_source_AC_adj.c:21-36:
+// res = pobs - pcal in float, muted where pobsr<1e-20 && pobsi<1e-20; the
+// adjoint panel holds conj(res)/DH^2 at k_rec (the last of duplicate receivers
+// wins); per-shot misfit sum |res|^2 (muted included).
+__global__ void k_adjoint(int n, int nshots, int ntr, const int *__restrict__ krec, const cx<float> *__restrict__ pcal, const cx<float> *__restrict__ pobs, float dh,
+                          cx<double> *__restrict__ B, double *__restrict__ l2) {
+    const int s = blockIdx.x;
+    cx<double> *b = B + (int64_t)s * n;
+    for (int row = threadIdx.x; row < n; row += blockDim.x) b[row] = {0.0, 0.0};
+    __syncthreads();
+    __shared__ double part[256];
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < ntr; t += blockDim.x) {
+        cx<float> o = pobs[(int64_t)s * ntr + t], c = pcal[(int64_t)s * ntr + t];
+        float rr = o.x - c.x, ri = o.y - c.y;
+        acc += (double)rr * rr + (double)ri * ri;
+        if ((double)o.x < 1e-20 && (double)o.y < 1e-20) { rr = 0.0f; ri = 0.0f; }
+        bool last = true;
+        for (int t2 = t + 1; t2 < ntr; t2++)
+            if (krec[t2] == krec[t]) { last = false; break; }
+        if (last) b[krec[t]] = {(double)(rr / (dh * dh)), (double)(-ri / (dh * dh))};
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) l2[s] = part[0];
+}
+
+// src/AC/ass_grad_AC.c:30-31 summed over shots in FP64: cell q uses the
+// solutions at q+1 (src/vec2mat.c:19-24); the last cell reads past the
+// solution in the reference and gets 0 here.
+__global__ void k_grad_ac(int n, int nshots, double o2r, double o2i, const cx<double> *__restrict__ U,
+                          const cx<double> *__restrict__ L, const float *__restrict__ vp, const float *__restrict__ rho,
+                          double *__restrict__ grad) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        if (q + 1 >= n) continue;
+        double acc = 0.0;
+        for (int s = 0; s < nshots; s++) {
+            cx<double> u = U[q + 1 + (int64_t)s * n], l = L[q + 1 + (int64_t)s * n];
+            double pr = u.x * l.x - u.y * l.y, pi = u.x * l.y + u.y * l.x;
+            acc += o2r * pr - o2i * pi;
+        }
+        double v = vp[q];
+        grad[q] += -2.0 * acc / ((double)rho[q] * v * v * v);
+    }
+}
+
+// src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
+// grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
+__global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
+                          const cx<double> *__restrict__ U, const cx<double> *__restrict__ L,
+                          double *__restrict__ gsig, double *__restrict__ geps) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        if (q + 1 >= n) continue;
+        double ar = 0.0, ai = 0.0;
+        for (int s = 0; s < nshots; s++) {
+            cx<double> u = U[q + 1 + (int64_t)s * n], l = L[q + 1 + (int64_t)s * n];
+            ar += u.x * l.x - u.y * l.y;
+            ai += u.x * l.y + u.y * l.x;
+        }
+        // Re((i Omega) z) = -(omr*ai + omi*ar);  Re(Omega^2 z) = o2r*ar - o2i*ai
+        gsig[q] += -(omr * ai + omi * ar);
+        geps[q] += o2r * ar - o2i * ai;
+    }
+}
+
+inline int grid_for(int64_t total) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
+}
+
+}  // namespace
+
+void spike_panel(int n, int nshots, const int *k, cx<double> amp, cx<double> *B, hipStream_t st) {
+    hipLaunchKernelGGL(k_spikes, dim3(grid_for((int64_t)n * nshots)), dim3(256), 0, st, n, nshots, k, amp, B);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void sample_receivers(int n, int nshots, int ntr, const int *krec, const cx<double> *X, cx<float> *pcal,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_sample, dim3(grid_for((int64_t)ntr * nshots)), dim3(256), 0, st, n, nshots, ntr, krec, X,
+                       pcal);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void adjoint_sources(int n, int nshots, int ntr, const int *krec, const cx<float> *pcal, const cx<float> *pobs,
+                     float dh, cx<double> *B, double *l2, hipStream_t st) {
+    hipLaunchKernelGGL(k_adjoint, dim3(nshots), dim3(256), 0, st, n, nshots, ntr, krec, pcal, pobs, dh, B, l2);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void grad_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const cx<double> *L, const float *vp,
+             const float *rho, double *grad, hipStream_t st) {
+    hipLaunchKernelGGL(k_grad_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, (double)omega2[0],
+                       (double)omega2[1], U, L, vp, rho, grad);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void grad_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U,
+             const cx<double> *L, double *gsig, double *geps, hipStream_t st) {
+    hipLaunchKernelGGL(k_grad_te, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, (double)omega[0], (double)omega[1],
+                       (double)omega2[0], (double)omega2[1], U, L, gsig, geps);
+    GFDFD_HIP(hipGetLastError());
+}
+
+}  // namespace gfdfd

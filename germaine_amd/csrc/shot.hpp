@@ -1,0 +1,17 @@
+// Shot kernels (see shot.hip).
+#pragma once
+#include "common.hpp"
+
+namespace gfdfd {
+
+void spike_panel(int n, int nshots, const int *k, cx<double> amp, cx<double> *B, hipStream_t st);
+void sample_receivers(int n, int nshots, int ntr, const int *krec, const cx<double> *X, cx<float> *pcal,
+                      hipStream_t st);
+void adjoint_sources(int n, int nshots, int ntr, const int *krec, const cx<float> *pcal, const cx<float> *pobs,
+                     float dh, cx<double> *B, double *l2, hipStream_t st);
+void grad_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const cx<double> *L, const float *vp,
+             const float *rho, double *grad, hipStream_t st);
+void grad_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U,
+             const cx<double> *L, double *gsig, double *geps, hipStream_t st);
+
+}  // namespace gfdfd

@@ -1,0 +1,282 @@
+"""Host-side workflows over libgfdfd.so, mirroring the reference's shot loops.
+
+Device arrays are torch tensors on ``cuda:0`` (torch is used only to own
+device memory and streams); every computation runs in the library's HIP
+kernels.  Complex panels are column-major ``n x nshots``, i.e. torch tensors
+of shape ``[nshots, n]``, dtype complex128.
+
+* :func:`forward_shots` — ``forward_shot_AC`` (src/AC/forward_shot_AC.c:10-122)
+  for a batch of frequencies and all shots at once: device assembly, batched
+  factorisation, one multi-RHS panel solve per frequency, receiver sampling.
+* :func:`grad_obj` — ``grad_obj_AC`` (src/AC/grad_obj_AC.c:10-194) /
+  ``grad_obj_TE``: forward panel, residuals, adjoint panel on the same
+  factors (M for AC, M^T for TE), gradient reduced over shots in FP64.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import AC, FP32, FP64, SH, TE, FreqParams, PlanInfo, check, lib
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+# ---- host parameters (gcc arithmetic of the reference) -------------------------
+
+def fd_weights(dh):
+    out = np.zeros(6, np.float32)
+    lib().gfdfd_fd_weights(dh, out.ctypes.data)
+    return out
+
+
+def pml_profiles(nx, ny, npml, dh, a0_pml, freq, s=0.0, free_surf=0):
+    """src/pml_pro.c: float32 [4(nx+2) + 4(ny+2)] = x r,i,hr,hi then y r,i,hr,hi."""
+    prof = np.zeros(4 * (nx + 2) + 4 * (ny + 2), np.float32)
+    lib().gfdfd_pml_profiles(nx, ny, npml, dh, a0_pml, freq, s, free_surf, prof.ctypes.data)
+    return prof
+
+
+def omega(physics, freq, s=0.0):
+    o2 = np.zeros(2, np.float32)
+    om = np.zeros(2, np.float32)
+    lib().gfdfd_omega(physics, freq, s, o2.ctypes.data, om.ctypes.data)
+    return o2, om
+
+
+def omega2_grad_ac(freq, s=0.0):
+    o2 = np.zeros(2, np.float32)
+    lib().gfdfd_omega2_grad_ac(freq, s, o2.ctypes.data)
+    return o2
+
+
+def source_amp(physics, freq, s, dh):
+    a = np.zeros(2, np.float64)
+    lib().gfdfd_source_amp(physics, freq, s, dh, a.ctypes.data)
+    return complex(a[0], a[1])
+
+
+def grid_index(x, y, dh, npml, fsshift, nx):
+    """Row of a source/receiver at (x, y) metres: nxsrc + (nysrc-1)*NX with
+    nxsrc = iround(x/DH)+NPML, nysrc = iround(y/DH)+FSSHIFT (src/AC/RHS_source_AC.c:23-32)."""
+    xs = int(np.floor(np.float32(np.float32(x) / np.float32(dh)) + 0.5)) + npml
+    ys = int(np.floor(np.float32(np.float32(y) / np.float32(dh)) + 0.5)) + fsshift
+    return xs + (ys - 1) * nx
+
+
+# ---- plan / factors ------------------------------------------------------------
+
+class Plan:
+    """Nested-dissection plan of an nx x ny 9-point grid (cached by size)."""
+
+    def __init__(self, nx, ny, leaf_area=32, small_max=64):
+        self.nx, self.ny = nx, ny
+        h = C.c_void_p()
+        check(lib().gfdfd_plan_grid_opt(nx, ny, leaf_area, small_max, C.byref(h)), "gfdfd_plan_grid")
+        self.h = h
+        self.info = PlanInfo()
+        check(lib().gfdfd_plan_query(self.h, C.byref(self.info)), "gfdfd_plan_query")
+        self.n = int(self.info.n)
+        self.nnz = int(self.info.nnz)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and lib is not None:
+            lib().gfdfd_plan_free(C.byref(self.h))
+            self.h = None
+
+    def pattern(self):
+        cp = np.zeros(self.n + 1, np.int32)
+        ri = np.zeros(self.nnz, np.int32)
+        check(lib().gfdfd_plan_pattern(self.h, cp.ctypes.data, ri.ctypes.data), "gfdfd_plan_pattern")
+        return cp, ri
+
+    def perm(self):
+        p = np.zeros(self.n, np.int32)
+        check(lib().gfdfd_plan_perm(self.h, p.ctypes.data), "gfdfd_plan_perm")
+        return p
+
+    def fronts(self):
+        nf = self.info.fronts
+        arrs = [np.zeros(nf, np.int32) for _ in range(5)] + [np.zeros(nf, np.int64)]
+        check(lib().gfdfd_plan_fronts(self.h, *[a.ctypes.data for a in arrs]), "gfdfd_plan_fronts")
+        p, m, pos0, parent, level, idx_off = arrs
+        idx = np.zeros(int(m.sum()), np.int32)
+        check(lib().gfdfd_plan_index(self.h, idx.ctypes.data), "gfdfd_plan_index")
+        return dict(p=p, m=m, pos0=pos0, parent=parent, level=level, idx_off=idx_off, idx=idx)
+
+    def assemble(self, physics, dh, freqs, m1, m2, stream=None, s=0.0, npml=0, a0_pml=90.0, free_surf=0):
+        """Device assembly for a list of frequencies; m1/m2 float32 cuda [ny, nx].
+        Returns complex128 cuda [nfreq, nnz] (values of M in CSC order)."""
+        import torch
+        nf = len(freqs)
+        profs, fp = [], (FreqParams * nf)()
+        for k, f in enumerate(freqs):
+            prof = pml_profiles(self.nx, self.ny, npml, dh, a0_pml, f, s, free_surf)
+            profs.append(prof)
+            o2, om = omega(physics, f, s)
+            fp[k].omega2[0], fp[k].omega2[1] = float(o2[0]), float(o2[1])
+            fp[k].omega[0], fp[k].omega[1] = float(om[0]), float(om[1])
+            fp[k].prof = prof.ctypes.data_as(C.POINTER(C.c_float))
+        vals = torch.empty((nf, self.nnz), dtype=torch.complex128, device="cuda")
+        m1 = m1.contiguous()
+        m2 = m2.contiguous()
+        check(lib().gfdfd_assemble(self.h, physics, dh, nf, fp, _ptr(m1), _ptr(m2), _ptr(vals), _stream(stream)),
+              "gfdfd_assemble")
+        return vals
+
+    def factor(self, vals, precision=FP64, stream=None):
+        return Factors(self, vals, precision, stream)
+
+    def residual(self, vals_f, B, X, sys=0, stream=None):
+        import torch
+        R = torch.empty_like(B)
+        nrhs = B.shape[0] if B.dim() == 2 else 1
+        check(lib().gfdfd_residual(self.h, _ptr(vals_f), sys, nrhs, _ptr(B), _ptr(X), _ptr(R), _stream(stream)),
+              "gfdfd_residual")
+        return R
+
+
+class Factors:
+    """Numeric factors of nfreq matrices (complex128 cuda [nfreq, nnz] values)."""
+
+    def __init__(self, plan, vals, precision=FP64, stream=None):
+        self.plan = plan
+        self.nfreq = vals.shape[0]
+        self.precision = precision
+        vals = vals.contiguous()
+        h = C.c_void_p()
+        check(lib().gfdfd_factor(plan.h, self.nfreq, _ptr(vals), precision, _stream(stream), C.byref(h)),
+              "gfdfd_factor")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and lib is not None:
+            lib().gfdfd_factors_free(C.byref(self.h))
+            self.h = None
+
+    def singular(self):
+        f = np.zeros(self.nfreq, np.int32)
+        check(lib().gfdfd_factors_singular(self.h, f.ctypes.data), "gfdfd_factors_singular")
+        return f
+
+    def solve(self, ifreq, B, sys=0, out=None, stream=None):
+        import torch
+        X = torch.empty_like(B) if out is None else out
+        nrhs = B.shape[0] if B.dim() == 2 else 1
+        check(lib().gfdfd_solve(self.h, ifreq, sys, nrhs, _ptr(B), _ptr(X), _stream(stream)), "gfdfd_solve")
+        return X
+
+    def solve_refine(self, ifreq, B, sys=0, tol=1e-10, max_iter=10, out=None, stream=None):
+        import torch
+        X = torch.empty_like(B) if out is None else out
+        nrhs = B.shape[0] if B.dim() == 2 else 1
+        it = C.c_int(0)
+        rr = C.c_double(0)
+        check(lib().gfdfd_solve_refine(self.h, ifreq, sys, nrhs, _ptr(B), _ptr(X), tol, max_iter, C.byref(it),
+                                       C.byref(rr), _stream(stream)), "gfdfd_solve_refine")
+        return X, it.value, rr.value
+
+
+# ---- shot panels and workflows ---------------------------------------------------
+
+def spike_panel(n, ksrc, amp, stream=None):
+    import torch
+    k = torch.as_tensor(np.asarray(ksrc, np.int32), device="cuda")
+    B = torch.empty((len(ksrc), n), dtype=torch.complex128, device="cuda")
+    check(lib().gfdfd_spike_panel(n, len(ksrc), _ptr(k), amp.real, amp.imag, _ptr(B), _stream(stream)),
+          "gfdfd_spike_panel")
+    return B
+
+
+def sample(X, krec, stream=None):
+    import torch
+    nshots, n = X.shape
+    k = torch.as_tensor(np.asarray(krec, np.int32), device="cuda")
+    P = torch.empty((nshots, len(krec)), dtype=torch.complex64, device="cuda")
+    check(lib().gfdfd_sample(n, nshots, len(krec), _ptr(k), _ptr(X), _ptr(P), _stream(stream)), "gfdfd_sample")
+    return P
+
+
+def adjoint_sources(n, krec, pcal, pobs, dh, stream=None):
+    import torch
+    nshots, ntr = pcal.shape
+    k = torch.as_tensor(np.asarray(krec, np.int32), device="cuda")
+    B = torch.empty((nshots, n), dtype=torch.complex128, device="cuda")
+    l2 = torch.empty(nshots, dtype=torch.float64, device="cuda")
+    check(lib().gfdfd_adjoint_sources(n, nshots, ntr, _ptr(k), _ptr(pcal.contiguous()), _ptr(pobs.contiguous()), dh,
+                                      _ptr(B), _ptr(l2), _stream(stream)), "gfdfd_adjoint_sources")
+    return B, l2
+
+
+def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
+                  precision=FP64, tol=1e-10, max_iter=10, keep_fields=False):
+    """All shots x all frequencies (src/AC/forward_shot_AC.c:38-101, batched).
+
+    Returns dict(data=[nfreq, nshots, ntr] complex64 receiver samples or None,
+    fields=list of [nshots, n] complex128 panels when keep_fields, iters, relres)."""
+    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
+    F = plan.factor(vals, precision)
+    data, fields, its, res = [], [], [], []
+    for f, fr in enumerate(freqs):
+        B = spike_panel(plan.n, ksrc, source_amp(physics, fr, s, dh))
+        if precision == FP64 and tol is None:
+            X = F.solve(f, B)
+            it, rr = 0, float("nan")
+        else:
+            X, it, rr = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
+        its.append(it)
+        res.append(rr)
+        if krec is not None:
+            data.append(sample(X, krec))
+        if keep_fields:
+            fields.append(X)
+    return dict(data=None if krec is None else data, fields=fields, iters=its, relres=res, factors=F, vals=vals)
+
+
+def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
+             precision=FP64, tol=1e-10, max_iter=10):
+    """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,
+    src/TE/grad_obj_TE.c:53-172).  pobs: list per frequency of complex64 cuda [nshots, ntr].
+    Returns (L2 = 0.5 * sum |res|^2, grad (AC: [n] float64; TE: (grad_sigma, grad_eps)))."""
+    import torch
+    n = plan.n
+    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
+    F = plan.factor(vals, precision)
+    g1 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    g2 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    l2 = 0.0
+    adj_sys = 0 if physics == AC else 2      # AC solves with M (grad_obj_AC.c:147), TE with M^T (grad_obj_TE.c:149)
+    k = torch.as_tensor(np.asarray(krec, np.int32), device="cuda")
+    for f, fr in enumerate(freqs):
+        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
+        if precision == FP64 and tol is None:
+            U = F.solve(f, B)
+        else:
+            U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
+        pcal = sample(U, krec)
+        BA, l2s = adjoint_sources(n, krec, pcal, pobs[f], dh)
+        l2 += float(l2s.sum())
+        if precision == FP64 and tol is None:
+            LAM = F.solve(f, BA, sys=adj_sys)
+        else:
+            LAM, _, _ = F.solve_refine(f, BA, sys=adj_sys, tol=tol, max_iter=max_iter)
+        nshots = len(ksrc)
+        if physics == AC:
+            o2 = omega2_grad_ac(fr, s)
+            check(lib().gfdfd_grad_ac(n, nshots, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(m1.contiguous()),
+                                      _ptr(m2.contiguous()), _ptr(g1), _stream(None)), "gfdfd_grad_ac")
+        elif physics == TE:
+            o2, om = omega(TE, fr, s)
+            check(lib().gfdfd_grad_te(n, nshots, om.ctypes.data, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(g1),
+                                      _ptr(g2), _stream(None)), "gfdfd_grad_te")
+        else:
+            raise ValueError("SH has no gradient path in the reference (src/germaine.c:112-120)")
+    grad = g1 if physics == AC else (g1, g2)
+    return 0.5 * l2, grad

@@ -1,0 +1,157 @@
+/*
+ * gfdfd — extension C-ABI of the MI355X solve path.
+ *
+ * What GERMAINE's one-shot-at-a-time host loop cannot express through the six
+ * UMFPACK entry points (include/umfpack.h): device assembly, several
+ * frequencies factored in one batch, multi-right-hand-side panel solves,
+ * FP32 factors with FP64 refinement, and the shot kernels of the FWI
+ * gradient.  All device arrays are plain device pointers (hipMalloc'd or
+ * from any allocator); complex arrays are interleaved (re, im) doubles or
+ * floats; right-hand-side panels are column-major n x nrhs (one shot per
+ * column).  `stream` is a hipStream_t (NULL = default stream).  Functions
+ * return GFDFD_OK, a positive warning or a negative error, and print a
+ * diagnostic on stderr for every non-zero status; gfdfd_last_error() returns
+ * the text.
+ *
+ * Unknown numbering: k = (i-1) + (j-1)*nx for the reference's 1-based grid
+ * point (i,j) (src/AC/init_A_AC_9p_pml.c:52-53); grids include the PML.
+ */
+#ifndef GFDFD_H
+#define GFDFD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFDFD_OK 0
+#define GFDFD_WARN_SINGULAR 1
+#define GFDFD_ERR_ARG (-1)
+#define GFDFD_ERR_DEVICE (-2)
+#define GFDFD_ERR_MEMORY (-3)
+#define GFDFD_ERR_INTERNAL (-4)
+
+#define GFDFD_AC 1
+#define GFDFD_SH 2
+#define GFDFD_TE 3
+
+#define GFDFD_FP64 64   /* complex double fronts */
+#define GFDFD_FP32 32   /* complex float fronts (FP32 MFMA), refine in FP64 */
+
+typedef struct gfdfd_plan_s *gfdfd_plan;
+typedef struct gfdfd_factors_s *gfdfd_factors;
+
+typedef struct {
+    long long n, nnz;           /* unknowns, entries of M */
+    int fronts, levels, max_front, max_pivots;
+    long long arena_elems;      /* front entries per frequency (sum m^2) */
+    long long work_rows;        /* solve work-panel rows (sum m) */
+    double factor_flops;        /* real flops of one factorisation (complex ops x4) */
+    double nnz_lu;              /* entries of L and U */
+    double plan_device_bytes;
+} gfdfd_plan_info;
+
+typedef struct {
+    float omega2[2];            /* Omega^2 as gfdfd_omega returns it */
+    float omega[2];             /* Omega (TE conductivity term) */
+    const float *prof;          /* host: PML profiles, 4*(nx+2) + 4*(ny+2) floats (gfdfd_pml_profiles) */
+} gfdfd_freq;
+
+const char *gfdfd_last_error(void);
+
+/* ---- symbolic: geometric nested dissection of the nx*ny 9-point grid -------
+ * replaces umfpack_zi_symbolic (src/AC/forward_shot_AC.c:58) for the batched
+ * path; plans are cached by grid size, so repeated calls are cheap.  Plans
+ * are host objects (no device needed) until a device call first uses them.  */
+int gfdfd_plan_grid(int nx, int ny, gfdfd_plan *out);
+int gfdfd_plan_grid_opt(int nx, int ny, int leaf_area, int small_max, gfdfd_plan *out);
+void gfdfd_plan_free(gfdfd_plan *plan);
+int gfdfd_plan_query(gfdfd_plan plan, gfdfd_plan_info *info);
+/* host copies of the canonical CSC pattern of M (colptr[n+1], rowind[nnz]) */
+int gfdfd_plan_pattern(gfdfd_plan plan, int *colptr, int *rowind);
+/* elimination order: perm[pos] = unknown (n ints) */
+int gfdfd_plan_perm(gfdfd_plan plan, int *perm);
+/* front table (nfronts entries each, postorder): pivots, order, first pivot
+ * position, parent (-1 root), height, offset of the front's index list */
+int gfdfd_plan_fronts(gfdfd_plan plan, int *p, int *m, int *pos0, int *parent, int *level, long long *idx_off);
+/* concatenated front index lists (elimination positions, ascending per front) */
+int gfdfd_plan_index(gfdfd_plan plan, int *idx);
+
+/* ---- host-side per-frequency parameters (gcc arithmetic of the reference) -- */
+/* src/AC/init_A_AC_9p_pml.c:34-42: a, b/4, c/4, d, e, 1/DH^2 */
+void gfdfd_fd_weights(float dh, float *out6);
+float gfdfd_mu0(void);
+/* src/pml_pro.c:20-160 */
+void gfdfd_pml_profiles(int nx, int ny, int npml, float dh, float a0_pml, float freq, float s, int free_surf,
+                        float *prof);
+/* src/AC/init_A_AC_9p_pml.c:50 (AC), src/SH/init_A_SH_9p_pml.c:50, src/TE/init_A_TE_9p_pml.c:54-55 */
+void gfdfd_omega(int physics, float freq, float s, float *omega2, float *omega);
+/* Omega^2 of the AC gradient, cpowf(2 pi f - i S, 2) (src/AC/ass_grad_AC.c:22; equals the
+ * assembly's Omega^2 only for S = 0) */
+void gfdfd_omega2_grad_ac(float freq, float s, float *omega2);
+/* src/AC/RHS_source_AC.c:37-53, src/TE/RHS_source_TE.c:22-56 */
+void gfdfd_source_amp(int physics, float freq, float s, float dh, double *amp);
+
+/* ---- device assembly --------------------------------------------------------
+ * replaces init_mat_AC + init_A_AC_9p_pml (src/AC/init_A_AC_9p_pml.c:17-220),
+ * init_A_SH_9p_pml (src/SH/init_A_SH_9p_pml.c:17-222) and init_A_TE_9p_pml
+ * (src/TE/init_A_TE_9p_pml.c:21-243) followed by the CSC value overwrite
+ * (src/AC/forward_shot_AC.c:41-48).  d_m1/d_m2: AC vp/rho, SH vs/rho, TE
+ * absolute permittivity/conductivity, device float [ny][nx].  d_vals receives
+ * nfreq x nnz complex doubles: the values of M in canonical CSC order,
+ * bit-identical to the reference's (double) triplet values.                 */
+int gfdfd_assemble(gfdfd_plan plan, int physics, float dh, int nfreq, const gfdfd_freq *freqs, const float *d_m1,
+                   const float *d_m2, double *d_vals, void *stream);
+
+/* ---- numeric: nfreq matrices of one pattern in one batch -------------------
+ * replaces umfpack_zi_numeric (src/AC/forward_shot_AC.c:61).  d_vals: nfreq x
+ * nnz complex doubles in canonical CSC order (copied; the factors keep an FP64
+ * copy for refinement).  precision GFDFD_FP64 or GFDFD_FP32.               */
+int gfdfd_factor(gfdfd_plan plan, int nfreq, const double *d_vals, int precision, void *stream, gfdfd_factors *out);
+void gfdfd_factors_free(gfdfd_factors *f);
+/* per-frequency zero-pivot flags (nfreq ints) */
+int gfdfd_factors_singular(gfdfd_factors f, int *flags);
+
+/* ---- solves ------------------------------------------------------------------
+ * replaces umfpack_zi_solve (src/AC/forward_shot_AC.c:88, src/AC/grad_obj_AC.c:
+ * 121,147, src/TE/grad_obj_TE.c:149) for a whole panel of shots.
+ * sys: 0 M x = b, 1 M^H x = b, 2 M^T x = b.  d_b and d_x: n x nrhs complex
+ * doubles; gfdfd_solve allows d_x == d_b.                                   */
+int gfdfd_solve(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x, void *stream);
+/* solve with FP64 iterative refinement (residual b - op(M) x from the FP64
+ * values) until max over columns of ||r||/||b|| <= tol or max_iter
+ * corrections; d_x must not alias d_b.  iters/relres may be NULL.           */
+int gfdfd_solve_refine(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x, double tol,
+                       int max_iter, int *iters, double *relres, void *stream);
+/* r = b - op(M) x in FP64 */
+int gfdfd_residual(gfdfd_plan plan, const double *d_vals, int sys, int nrhs, const double *d_b, const double *d_x,
+                   double *d_r, void *stream);
+
+/* ---- shot kernels --------------------------------------------------------------
+ * RHS_source_AC / RHS_source_TE for all shots (src/AC/RHS_source_AC.c:46-53):
+ * column s of d_b is zero except d_b[k[s]] = amp.                           */
+int gfdfd_spike_panel(int n, int nshots, const int *d_k, double amp_re, double amp_im, double *d_b, void *stream);
+/* calc_seis_AC after vec2mat (src/AC/calc_seis_AC.c:19-20): d_pcal[s*ntr+t] =
+ * (complex float) d_x[d_krec[t] + s*n]                                       */
+int gfdfd_sample(int n, int nshots, int ntr, const int *d_krec, const double *d_x, float *d_pcal, void *stream);
+/* calc_res_AC (MISFIT=1, STF_INV=0, src/AC/calc_res_AC.c:49-95) and
+ * RHS_source_AC_adj (src/AC/RHS_source_AC_adj.c:21-36) for all shots:
+ * d_b column s = conj(pobs - pcal)/DH^2 at the receivers (muted where
+ * pobsr<1e-20 && pobsi<1e-20; for duplicate receivers the last one wins),
+ * d_l2[s] = sum |pobs - pcal|^2 (double).                                   */
+int gfdfd_adjoint_sources(int n, int nshots, int ntr, const int *d_krec, const float *d_pcal, const float *d_pobs,
+                          float dh, double *d_b, double *d_l2, void *stream);
+/* ass_grad_AC (src/AC/ass_grad_AC.c:30-31), STF_INV=0, summed over the nshots
+ * columns of d_u (forward) and d_lam (adjoint) in FP64 and added to d_grad
+ * (n doubles).  omega2 from gfdfd_omega2_grad_ac.  Cell q combines the
+ * solutions at q+1 (vec2mat's shift, src/vec2mat.c:19-24); the last cell,
+ * which reads past the solution in the reference, is left unchanged.        */
+int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam, const float *d_vp,
+                  const float *d_rho, double *d_grad, void *stream);
+/* ass_grad_TE (src/TE/ass_grad_TE.c:31-35): sigma and epsilon gradients */
+int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u, const double *d_lam,
+                  double *d_grad_sigma, double *d_grad_eps, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,187 @@
+"""GPU parity of the HIP path against the oracle, through the C-ABI.
+
+* device assembly: bit-identical to the oracle's triplet values (AC, SH, TE;
+  S = 0 and S > 0; free surface) -- integer-exact comparison of the doubles;
+* FP64 factor + solve: M, M^H, M^T against the FP64 banded-LU oracle at
+  relative error <= 1e-10, through both the small-front (LDS) and the blocked
+  (panel/TRSM/MFMA) paths, 1 to 70 right-hand sides;
+* FP32 factors + FP64 refinement: <= 1e-8 against the oracle;
+* batched frequencies equal per-frequency solves;
+* the 201x101 acoustic workload through the six umfpack_zi_* entry points.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import germaine_amd as G
+from germaine_amd import fdfd, models
+from oracle import oracle as O
+from tests._grids import smooth_model, te_model
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def rel(x, ref):
+    return float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
+
+
+def oracle_system(phys, nx, ny, npml, dh, f, m1, m2, s=0.0, fs=0):
+    px, py = O.pml_profiles(nx, ny, npml, dh, 90.0, f, s, fs)
+    tri = O.assemble(phys, nx, ny, dh, f, s, m1, m2, px, py)
+    return tri, O.csc_of_M(nx, ny, *tri)
+
+
+@pytest.mark.parametrize("phys,s,fs", [(G.AC, 0.0, 0), (G.AC, 0.5, 1), (G.SH, 0.3, 0), (G.TE, 0.0, 0)])
+def test_assembly_bit_exact(phys, s, fs):
+    nx, ny, npml = 37, 29, 6
+    if phys == G.TE:
+        m1, m2 = te_model(nx, ny, seed=2)
+        dh, freqs = 0.04, [1.0e8, 3.5e8]
+    else:
+        m1, m2 = smooth_model(nx, ny, seed=5)
+        dh, freqs = 10.0, [4.0, 11.5]
+    say("assembly", phys, s, fs)
+    pl = fdfd.Plan(nx, ny)
+    vals = pl.assemble(phys, dh, freqs, dev(m1), dev(m2), s=s, npml=npml, free_surf=fs).cpu().numpy()
+    for k, f in enumerate(freqs):
+        (irow, icol, ar, ai), _ = oracle_system(phys, nx, ny, npml, dh, f, m1, m2, s, fs)
+        assert np.array_equal(vals[k].real, ar), f"real parts differ at {np.nonzero(vals[k].real != ar)[0][:5]}"
+        assert np.array_equal(vals[k].imag, ai), f"imag parts differ at {np.nonzero(vals[k].imag != ai)[0][:5]}"
+
+
+def system(nx=45, ny=33, npml=6, f=9.0, seed=1):
+    m1, m2 = smooth_model(nx, ny, seed=seed)
+    tri, csc = oracle_system(G.AC, nx, ny, npml, 10.0, f, m1, m2)
+    return m1, m2, csc
+
+
+@pytest.mark.parametrize("small_max", [64, 16])
+@pytest.mark.parametrize("sysno", [0, 1, 2])
+@pytest.mark.parametrize("nrhs", [1, 3, 70])
+def test_fp64_solve_matches_oracle(small_max, sysno, nrhs):
+    nx, ny = 45, 33
+    m1, m2, csc = system(nx, ny)
+    say("fp64 solve", small_max, sysno, nrhs)
+    pl = fdfd.Plan(nx, ny, leaf_area=32, small_max=small_max)
+    vals = dev((csc[2] + 1j * csc[3])[None, :])
+    F = pl.factor(vals, G.FP64)
+    assert F.singular()[0] == 0
+    rng = np.random.default_rng(nrhs)
+    B = rng.standard_normal((nrhs, nx * ny)) + 1j * rng.standard_normal((nrhs, nx * ny))
+    X = F.solve(0, dev(B), sys=sysno).cpu().numpy()
+    Xr, info = O.band_solve(*csc, B.T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+    assert rel(X, Xr.T) <= 1e-10
+
+
+@pytest.mark.parametrize("sysno", [0, 2])
+def test_fp32_factors_with_refinement(sysno):
+    nx, ny = 45, 33
+    m1, m2, csc = system(nx, ny)
+    say("fp32+IR", sysno)
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    F = pl.factor(dev((csc[2] + 1j * csc[3])[None, :]), G.FP32)
+    rng = np.random.default_rng(7)
+    B = rng.standard_normal((5, nx * ny)) + 1j * rng.standard_normal((5, nx * ny))
+    X, it, rr = F.solve_refine(0, dev(B), sys=sysno, tol=1e-12, max_iter=20)
+    X = X.cpu().numpy()
+    Xr, _ = O.band_solve(*csc, B.T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+    say("  iterations", it, "relres", rr)
+    assert rr <= 1e-12 and it >= 1
+    assert rel(X, Xr.T) <= 1e-8
+
+
+def test_batched_frequencies_equal_single():
+    nx, ny, npml, dh = 41, 30, 6, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=9)
+    freqs = [3.0, 6.5, 12.0]
+    say("batched frequencies")
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    vals = pl.assemble(G.AC, dh, freqs, dev(m1), dev(m2), npml=npml)
+    F = pl.factor(vals, G.FP64)
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((4, nx * ny)) + 1j * rng.standard_normal((4, nx * ny))
+    for k, f in enumerate(freqs):
+        _, csc = oracle_system(G.AC, nx, ny, npml, dh, f, m1, m2)
+        X = F.solve(k, dev(B)).cpu().numpy()
+        Xr, _ = O.band_solve(*csc, B.T, perm=O.grid_transpose_perm(nx, ny))
+        assert rel(X, Xr.T) <= 1e-10
+
+
+def test_umfpack_abi_config_201x101():
+    """201x101 grid, h=10 m, 20-cell PML, inclusion model, 5 Hz, 10 sources / 100
+    receivers on row 5: the reference's per-shot loop through the drop-in ABI."""
+    w = models.config("cfg1")
+    nx, ny, n = w.nx, w.ny, w.nx * w.ny
+    say("umfpack ABI cfg1")
+    tri, csc = oracle_system(G.AC, nx, ny, w.npml, w.dh, w.freqs[0], w.m1, w.m2)
+    irow, icol, ar, ai = tri
+    nnz = len(ar)
+    L = G.lib()
+    Ap = np.zeros(n + 1, np.int32)
+    Ai = np.zeros(nnz, np.int32)
+    Ax = np.zeros(nnz)
+    Az = np.zeros(nnz)
+    assert L.umfpack_zi_triplet_to_col(n, n, nnz, irow.ctypes.data, icol.ctypes.data, ar.ctypes.data,
+                                       ai.ctypes.data, Ap.ctypes.data, Ai.ctypes.data, Ax.ctypes.data,
+                                       Az.ctypes.data, None) == 0
+    Ax[:] = ar                                 # the host's overwrite (src/AC/forward_shot_AC.c:43-48)
+    Az[:] = ai
+    assert np.array_equal(Ap, csc[0]) and np.array_equal(Ai, csc[1])
+    sym, num = C.c_void_p(), C.c_void_p()
+    info = np.zeros(90)
+    assert L.umfpack_zi_symbolic(n, n, Ap.ctypes.data, Ai.ctypes.data, Ax.ctypes.data, Az.ctypes.data,
+                                 C.byref(sym), None, None) == 0
+    assert L.umfpack_zi_numeric(Ap.ctypes.data, Ai.ctypes.data, Ax.ctypes.data, Az.ctypes.data, sym,
+                                C.byref(num), None, None) == 0
+    L.umfpack_zi_free_symbolic(C.byref(sym))
+    amp = O.source_amp(O.AC, w.freqs[0], 0.0, w.dh)
+    B = np.zeros((n, len(w.src)), complex)
+    for s, k in enumerate(w.src):
+        B[k, s] = amp
+    Xr, _ = O.band_solve(*csc, B, perm=O.grid_transpose_perm(nx, ny))
+    xr, xi = np.zeros(n + 1), np.zeros(n + 1)      # the host allocates more than n (src/AC/forward_shot_AC.c:34-35)
+    for s in range(len(w.src)):
+        br, bi = np.ascontiguousarray(B[:, s].real), np.ascontiguousarray(B[:, s].imag)
+        st = L.umfpack_zi_solve(0, Ap.ctypes.data, Ai.ctypes.data, Ax.ctypes.data, Az.ctypes.data, xr.ctypes.data,
+                                xi.ctypes.data, br.ctypes.data, bi.ctypes.data, num, None, info.ctypes.data)
+        assert st == 0
+        x = xr[:n] + 1j * xi[:n]
+        assert rel(x, Xr[:, s]) <= 1e-10
+        assert rel(x[w.rec], Xr[w.rec, s]) <= 1e-10
+        assert info[80] <= 2                   # refinement steps taken
+    L.umfpack_zi_free_numeric(C.byref(num))
+    assert num.value is None
+
+
+def test_forward_shots_fp32_refined_config_201x101():
+    w = models.config("cfg1")
+    say("forward_shots cfg1 fp32+IR")
+    pl = fdfd.Plan(w.nx, w.ny)
+    out = fdfd.forward_shots(pl, w.physics, w.dh, w.freqs, dev(w.m1), dev(w.m2), w.src, w.rec, npml=w.npml,
+                             precision=G.FP32, tol=1e-10, keep_fields=True)
+    _, csc = oracle_system(G.AC, w.nx, w.ny, w.npml, w.dh, w.freqs[0], w.m1, w.m2)
+    amp = O.source_amp(O.AC, w.freqs[0], 0.0, w.dh)
+    B = np.zeros((w.nx * w.ny, len(w.src)), complex)
+    for s, k in enumerate(w.src):
+        B[k, s] = amp
+    Xr, _ = O.band_solve(*csc, B, perm=O.grid_transpose_perm(w.nx, w.ny))
+    X = out["fields"][0].cpu().numpy().T
+    say("  iterations", out["iters"], "relres", out["relres"])
+    assert out["relres"][0] <= 1e-10
+    err = np.linalg.norm(X - Xr, axis=0) / np.linalg.norm(Xr, axis=0)
+    assert np.all(err <= 1e-8), err
+    d = out["data"][0].cpu().numpy()
+    np.testing.assert_allclose(d, Xr[w.rec, :].T.astype(np.complex64), rtol=1e-5, atol=1e-6 * np.abs(d).max())

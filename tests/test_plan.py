@@ -1,0 +1,169 @@
+"""Host-side checks of the nested-dissection plan (CPU only, no GPU).
+
+The plan is validated two ways:
+* structure: contiguous pivots covering every unknown once, sorted index
+  lists, contribution rows owned by ancestors and contained in the parent,
+  every matrix entry inside its owning front, heights consistent;
+* numerics: a plain numpy executor of the same multifrontal plan (front
+  assembly by extend-add, partial LU with pivoting restricted to the fully
+  summed rows, forward/backward substitution in the plan's order) must
+  reproduce a dense LAPACK solve of M, M^T and M^H.  This executor is test
+  code; the product runs the plan only in HIP kernels.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from germaine_amd import fdfd
+from oracle import oracle as O
+from tests._grids import smooth_model
+
+
+def plan_arrays(nx, ny, leaf=32, small=64):
+    pl = fdfd.Plan(nx, ny, leaf, small)
+    return pl, pl.fronts(), pl.perm()
+
+
+def check_structure(nx, ny, leaf):
+    pl, fr, perm = plan_arrays(nx, ny, leaf)
+    n = nx * ny
+    assert sorted(perm.tolist()) == list(range(n))
+    iperm = np.empty(n, np.int64)
+    iperm[perm] = np.arange(n)
+    p, m, pos0, parent, level, off, idx = (fr[k] for k in ("p", "m", "pos0", "parent", "level", "idx_off", "idx"))
+    nf = len(p)
+    # pivots: contiguous ranges partitioning 0..n-1, in postorder
+    assert pos0[0] == 0 and np.all(pos0[1:] == pos0[:-1] + p[:-1]) and pos0[-1] + p[-1] == n
+    owner = np.repeat(np.arange(nf), p)
+    for f in range(nf):
+        rows = idx[off[f]:off[f] + m[f]]
+        assert np.all(rows[:p[f]] == np.arange(pos0[f], pos0[f] + p[f]))
+        assert np.all(np.diff(rows) > 0)
+        assert np.all(rows[p[f]:] >= pos0[f] + p[f])
+        if parent[f] >= 0:
+            q = parent[f]
+            assert q > f and level[q] > level[f]
+            prow = set(idx[off[q]:off[q] + m[q]].tolist())
+            assert set(rows[p[f]:].tolist()) <= prow
+        else:
+            assert m[f] == p[f]
+    # every 9-point entry lies in the front that eliminates its earlier endpoint
+    cp, ri = pl.pattern()
+    for c in range(n):
+        for r in ri[cp[c]:cp[c + 1]]:
+            a, b = iperm[r], iperm[c]
+            f = owner[min(a, b)]
+            rows = idx[off[f]:off[f] + m[f]]
+            assert a in rows and b in rows
+    return pl, fr
+
+
+@pytest.mark.parametrize("nx,ny,leaf", [(7, 5, 4), (23, 17, 8), (40, 31, 32), (64, 9, 16), (3, 50, 6)])
+def test_plan_structure(nx, ny, leaf):
+    check_structure(nx, ny, leaf)
+
+
+def mf_solve(fr, perm, M, B, sys):
+    """numpy multifrontal factor + solve following the plan."""
+    n = M.shape[0]
+    Mp = M[perm][:, perm]
+    p, m, pos0, parent, off, idx = (fr[k] for k in ("p", "m", "pos0", "parent", "idx_off", "idx"))
+    nf = len(p)
+    upd, facts, pivs = {}, [], []
+    children = [[] for _ in range(nf)]
+    for f in range(nf):
+        if parent[f] >= 0:
+            children[parent[f]].append(f)
+    for f in range(nf):
+        rows = idx[off[f]:off[f] + m[f]]
+        F = np.zeros((m[f], m[f]), complex)
+        lo, hi = pos0[f], pos0[f] + p[f]
+        sub = Mp[np.ix_(rows, rows)]
+        own = (np.minimum.outer(rows, rows) >= lo) & (np.minimum.outer(rows, rows) < hi)
+        F[own] = sub[own]
+        for c in children[f]:
+            crow = idx[off[c]:off[c] + m[c]][p[c]:]
+            loc = np.searchsorted(rows, crow)
+            F[np.ix_(loc, loc)] += upd.pop(c)
+        piv = np.arange(p[f])
+        for j in range(p[f]):
+            r = j + int(np.argmax(np.abs(F[j:p[f], j].real) + np.abs(F[j:p[f], j].imag)))
+            piv[j] = r
+            F[[j, r]] = F[[r, j]]
+            F[j + 1:, j] /= F[j, j]
+            F[j + 1:, j + 1:] -= np.outer(F[j + 1:, j], F[j, j + 1:])
+        upd[f] = F[p[f]:, p[f]:].copy()
+        facts.append(F)
+        pivs.append(piv)
+    X = np.asarray(B, complex)[perm].copy()
+    trans, cj = sys != 0, sys == 1
+    op = (lambda a: a.conj()) if cj else (lambda a: a)
+    W = {}
+    for f in range(nf):                         # forward
+        rows = idx[off[f]:off[f] + m[f]]
+        w = np.zeros((m[f],) + X.shape[1:], complex)
+        w[:p[f]] = X[pos0[f]:pos0[f] + p[f]]
+        for c in children[f]:
+            crow = idx[off[c]:off[c] + m[c]][p[c]:]
+            w[np.searchsorted(rows, crow)] += W.pop(c)
+        F = op(facts[f])
+        if not trans:
+            for j in range(p[f]):
+                w[[j, pivs[f][j]]] = w[[pivs[f][j], j]]
+            for j in range(p[f]):
+                w[j + 1:] -= np.multiply.outer(F[j + 1:, j], w[j])
+        else:
+            for j in range(p[f]):
+                w[j] /= F[j, j]
+                w[j + 1:] -= np.multiply.outer(F[j, j + 1:], w[j])
+        X[pos0[f]:pos0[f] + p[f]] = w[:p[f]]
+        W[f] = w[p[f]:]
+    for f in reversed(range(nf)):               # backward
+        rows = idx[off[f]:off[f] + m[f]]
+        F = op(facts[f])
+        z = X[pos0[f]:pos0[f] + p[f]].copy()
+        x2 = X[rows[p[f]:]]
+        U12 = F[:p[f], p[f]:] if not trans else F[p[f]:, :p[f]].T
+        z -= U12 @ x2
+        for j in reversed(range(p[f])):
+            if not trans:
+                z[j] /= F[j, j]
+                z[:j] -= np.multiply.outer(F[:j, j], z[j])
+            else:
+                z[:j] -= np.multiply.outer(F[j, :j], z[j])
+        X[pos0[f]:pos0[f] + p[f]] = z
+    if trans:
+        for f in range(nf):
+            seg = X[pos0[f]:pos0[f] + p[f]]
+            for j in reversed(range(p[f])):
+                seg[[j, pivs[f][j]]] = seg[[pivs[f][j], j]]
+    out = np.empty_like(X)
+    out[perm] = X
+    return out
+
+
+@pytest.mark.parametrize("sysno", [0, 1, 2])
+@pytest.mark.parametrize("nx,ny,leaf", [(13, 11, 6), (30, 22, 32)])
+def test_plan_numpy_executor_matches_dense(nx, ny, leaf, sysno):
+    vp, rho = smooth_model(nx, ny, seed=nx)
+    px, py = O.pml_profiles(nx, ny, 4, 10.0, 90.0, 9.0, 0.3, 0)
+    ap, ai, ax, az = O.csc_of_M(nx, ny, *O.assemble(O.AC, nx, ny, 10.0, 9.0, 0.3, vp, rho, px, py))
+    M = sp.csc_matrix((ax + 1j * az, ai, ap), shape=(nx * ny, nx * ny)).toarray()
+    pl, fr, perm = plan_arrays(nx, ny, leaf)
+    rng = np.random.default_rng(sysno)
+    B = rng.standard_normal((nx * ny, 2)) + 1j * rng.standard_normal((nx * ny, 2))
+    X = mf_solve(fr, perm, M, B, sysno)
+    op = {0: M, 1: M.conj().T, 2: M.T}[sysno]
+    Xd = np.linalg.solve(op, B)
+    assert np.max(np.abs(X - Xd)) <= 1e-11 * np.max(np.abs(Xd))
+
+
+def test_plan_stats_north_star_grid():
+    """1601 x 401 grid: sizes that set the device buffers and the roofline counts."""
+    pl = fdfd.Plan(1601, 401)
+    info = pl.info
+    assert info.n == 642001 and info.nnz == 5766001
+    assert 1000 <= info.max_front <= 1400
+    assert 50e9 <= info.factor_flops <= 90e9
+    assert 60e6 <= info.nnz_lu <= 110e6
+    assert info.arena_elems < 400e6          # < 6.4 GB of complex double fronts per frequency
