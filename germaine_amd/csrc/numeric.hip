@@ -298,13 +298,15 @@ __global__ void __launch_bounds__(64) k_trsm_u(cx<T> *__restrict__ arena, int64_
 // trailing update C -= A*B on MFMA (complex = 4 real products), 64x64 tiles,
 // K <= 32.  4 waves in 2x2, each wave 32x32 = 2x2 blocks of 16x16.
 // MFMA 16x16x4 operand layout: A lane l -> (row l%16, k l/16), B lane l ->
-// (k l/16, col l%16), D lane l, v -> (row 4*(l/16)+v, col l%16).
+// (k l/16, col l%16).  Accumulator register v of lane l holds column l%16 and
+// row 4*(l/16)+v for f32, but row 4*v+l/16 for f64.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct mfma_t;
 template <>
 struct mfma_t<double> {
     typedef d4v acc;
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * v + lk; }
     static __device__ __forceinline__ acc op(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
@@ -312,6 +314,7 @@ struct mfma_t<double> {
 template <>
 struct mfma_t<float> {
     typedef f4v acc;
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
     static __device__ __forceinline__ acc op(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -377,7 +380,7 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
             for (int v = 0; v < 4; v++) {
-                const int row = r0 + wr + a * 16 + 4 * lk + v, col = c0 + wc + b * 16 + lr;
+                const int row = r0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = c0 + wc + b * 16 + lr;
                 if (row < m && col < m) {
                     cx<T> &g = G[row + (int64_t)col * m];
                     g.x -= cr[a][b][v];
