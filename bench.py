@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: shot-solves/s of the FDFD solve path on one MI355X.
+
+Workload (default): 1601 x 401 acoustic grid incl. a 30-cell PML, h = 12.5 m,
+seeded layered-fault model, 8 frequencies 2..16 Hz, 200 point sources.
+A step is one pass of the hot path over the batch, timed from "model in HBM"
+to "all solution panels in HBM":
+  device assembly of the 8 matrices -> one batched factorisation of the 8
+  frequencies -> per frequency one 200-column point-source panel and its
+  multi-RHS solve with FP64 iterative refinement to ||r||/||b|| <= 1e-10.
+value = 8 * 200 / (seconds per step), FP32 fronts (FP32 MFMA) + FP64 refinement.
+
+Also reported: the same step with FP64 fronts (no refinement), the plain FP64
+path the drop-in ABI executes (per frequency a factorisation and one
+single-RHS solve per shot, device-resident), the FP32+IR wavefield error
+against the FP64 solution, per-kernel event timings with algorithmic
+flops/bytes (the dominant kernel's roofline), and the CPU baseline (the
+oracle on the host cores, on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64_mfma": 78.6e12, "fp32_mfma": 157.3e12, "hbm": 8.0e12}
+KERNELS = ["assemble", "front_assemble", "scatter_orig", "small_lu", "panel", "trsm_u", "gemm_update",
+           "fwd_gather", "fwd_front", "bwd_front", "fwd_vec", "bwd_vec", "permute", "residual"]
+COMPUTE_KERNELS = {"small_lu", "panel", "trsm_u", "gemm_update", "fwd_front", "bwd_front"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_stats(lib, ctypes):
+    out = {}
+    for k in KERNELS:
+        ms, n, fl, by = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+        lib.gfdfd_profile_read(k.encode(), ctypes.byref(ms), ctypes.byref(n), ctypes.byref(fl), ctypes.byref(by))
+        if n.value:
+            out[k] = dict(ms=ms.value, launches=n.value, flops=fl.value, bytes=by.value)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg3")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-shot plain FP64 path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (frequencies and shot panels would be "
+              "split across GPUs with one gradient all-reduce; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import germaine_amd as G
+    from germaine_amd import fdfd, models
+
+    L = G.lib()
+    w = models.config(args.config)
+    nf, ns = len(w.freqs), len(w.src)
+    log(f"bench: {w.name} {w.nx}x{w.ny} freqs={w.freqs} shots={ns}")
+    m1 = torch.as_tensor(w.m1, device="cuda")
+    m2 = torch.as_tensor(w.m2, device="cuda")
+    ksrc = torch.as_tensor(np.asarray(w.src, np.int32), device="cuda")
+    pl = fdfd.Plan(w.nx, w.ny)
+    n = pl.n
+    info = pl.info
+    st = torch.cuda.current_stream()
+    amps = [fdfd.source_amp(w.physics, f, w.s, w.dh) for f in w.freqs]
+    Bs = torch.empty((nf, ns, n), dtype=torch.complex128, device="cuda")
+    Xs = torch.empty((nf, ns, n), dtype=torch.complex128, device="cuda")
+    freqs = list(range(nf))
+
+    def step(precision, refine):
+        vals = pl.assemble(w.physics, w.dh, w.freqs, m1, m2, s=w.s, npml=w.npml, a0_pml=w.a0_pml)
+        F = pl.factor(vals, precision)
+        for f in range(nf):
+            G.check(L.gfdfd_spike_panel(n, ns, ctypes.c_void_p(ksrc.data_ptr()), amps[f].real, amps[f].imag,
+                                        ctypes.c_void_p(Bs[f].data_ptr()), ctypes.c_void_p(st.cuda_stream)))
+        if refine:
+            _, it, rr = F.solve_refine_batch(freqs, Bs, tol=args.tol, max_iter=20, out=Xs)
+            its = list(zip(it, rr))
+        else:
+            F.solve_batch(freqs, Bs, out=Xs)
+            its = []
+        return F, its
+
+    def timed(precision, refine, steps, warmup, profile=False):
+        for _ in range(warmup):
+            F, _ = step(precision, refine)
+            del F
+        torch.cuda.synchronize()
+        if profile:
+            L.gfdfd_profile_reset()
+            L.gfdfd_profile(1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            F, its = step(precision, refine)
+            del F
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        if profile:
+            L.gfdfd_profile(0)
+        return t, its
+
+    # --- headline: FP32 fronts + FP64 refinement, batched -----------------------------
+    t32, its32 = timed(G.FP32, True, args.steps, args.warmup, profile=True)
+    kst = kernel_stats(L, ctypes)
+    log(f"fp32+IR batched: {t32 * 1e3:.1f} ms/step, refinement (iters, relres) per freq {its32}")
+    X32 = Xs.clone() if not args.no_check else None
+
+    # --- FP64 fronts, batched, no refinement ---------------------------------------------
+    t64, _ = timed(G.FP64, False, max(1, args.steps // 2), 1)
+    log(f"fp64 batched: {t64 * 1e3:.1f} ms/step")
+    err = None
+    if X32 is not None:
+        err = []
+        for f in range(nf):
+            d = torch.linalg.vector_norm(X32[f] - Xs[f], dim=1) / torch.linalg.vector_norm(Xs[f], dim=1)
+            err.append(float(d.max()))
+        log(f"fp32+IR vs fp64 rel wavefield error per freq: {err}")
+        del X32
+
+    # --- plain FP64: what the drop-in executes (per-frequency factor, one solve per shot) ---
+    tplain = None
+    if not args.no_plain:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(nf):
+            vals = pl.assemble(w.physics, w.dh, [w.freqs[f]], m1, m2, s=w.s, npml=w.npml, a0_pml=w.a0_pml)
+            F = pl.factor(vals, G.FP64)
+            G.check(L.gfdfd_spike_panel(n, ns, ctypes.c_void_p(ksrc.data_ptr()), amps[f].real, amps[f].imag,
+                                        ctypes.c_void_p(Bs[f].data_ptr()), ctypes.c_void_p(st.cuda_stream)))
+            for s in range(ns):
+                F.solve(0, Bs[f][s], out=Xs[f][s])
+            del F
+        torch.cuda.synchronize()
+        tplain = time.perf_counter() - t0
+        log(f"plain fp64 per-shot: {tplain * 1e3:.1f} ms")
+
+    # --- roofline of the dominant kernel -----------------------------------------------------
+    dom = max(kst, key=lambda k: kst[k]["ms"]) if kst else None
+    roof = None
+    if dom:
+        d = kst[dom]
+        avg_s = d["ms"] / d["launches"] * 1e-3
+        if dom in COMPUTE_KERNELS:
+            ach = d["flops"] / d["launches"] / avg_s
+            roof = dict(bound="mfma", achieved=ach / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s",
+                        frac=ach / PEAK["fp32_mfma"], traffic=None)
+        else:
+            ach = d["bytes"] / d["launches"] / avg_s
+            roof = dict(bound="hbm", achieved=ach / 1e9, peak=PEAK["hbm"] / 1e9, unit="GB/s",
+                        frac=ach / PEAK["hbm"], traffic=None)
+        roof["kernel"] = dom
+        roof["launches_per_step"] = d["launches"] / args.steps
+        roof["ms_per_step"] = d["ms"] / args.steps
+
+    # --- CPU baseline ---------------------------------------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import cpu_baseline
+            cpu = cpu_baseline.measure(w, threads=args.cpu_threads or None)
+        except Exception as e:                       # reported, never fatal
+            log(f"cpu baseline unavailable: {e}")
+
+    value = nf * ns / t32
+    res = {
+        "metric": "shot-solves/sec (factor+solve) FP32+IR vs FP64 LU at rel err <=1e-8; MFMA %",
+        "value": round(value, 2),
+        "unit": "shot-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 fronts + f64 refinement",
+        "data": "synthetic seeded layered-fault model, point sources",
+        "config": {"workload": f"{w.name}: AC {w.nx}x{w.ny} incl. {w.npml}-cell PML, h={w.dh} m, "
+                               f"{nf} freqs {w.freqs[0]:g}-{w.freqs[-1]:g} Hz batched, {ns}-shot panel",
+                   "n": n, "nnz": pl.nnz, "fronts": info.fronts, "factor_gflop_per_freq": info.factor_flops / 1e9,
+                   "nnz_lu": info.nnz_lu, "refine_tol": args.tol},
+        "fp64_batched": {"ms_per_step": round(t64 * 1e3, 3), "value": round(nf * ns / t64, 2)},
+        "plain_fp64_per_shot": None if tplain is None else {"ms": round(tplain * 1e3, 3),
+                                                             "value": round(nf * ns / tplain, 2)},
+        "speedup_vs_plain_fp64": None if tplain is None else round(tplain / t32, 3),
+        "refinement": [{"iters": a, "relres": b} for a, b in its32],
+        "max_rel_err_vs_fp64": None if err is None else max(err),
+        "roofline": roof,
+        "kernels": {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] / args.steps,
+                        "tflops": round(v["flops"] / max(v["ms"], 1e-9) * 1e-9, 3),
+                        "gbps": round(v["bytes"] / max(v["ms"], 1e-9) * 1e-6, 1)} for k, v in kst.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -64,12 +64,17 @@ SIGNATURES = {
     "gfdfd_factors_singular": (_I, [_P, _P]),
     "gfdfd_solve": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     "gfdfd_solve_refine": (_I, [_P, _I, _I, _I, _P, _P, _D, _I, _IP, _DP, _P]),
+    "gfdfd_solve_batch": (_I, [_P, _I, _P, _I, _I, _P, _P, _P]),
+    "gfdfd_solve_refine_batch": (_I, [_P, _I, _P, _I, _I, _P, _P, _D, _I, _P, _P, _P]),
     "gfdfd_residual": (_I, [_P, _P, _I, _I, _P, _P, _P, _P]),
     "gfdfd_spike_panel": (_I, [_I, _I, _P, _D, _D, _P, _P]),
     "gfdfd_sample": (_I, [_I, _I, _I, _P, _P, _P, _P]),
     "gfdfd_adjoint_sources": (_I, [_I, _I, _I, _P, _P, _P, _F, _P, _P, _P]),
     "gfdfd_grad_ac": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_grad_te": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "gfdfd_profile": (_I, [_I]),
+    "gfdfd_profile_reset": (_V, []),
+    "gfdfd_profile_read": (_I, [C.c_char_p, _DP, _LLP, _DP, _DP]),
     "umfpack_zi_triplet_to_col": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "umfpack_zi_symbolic": (_I, [_I, _I, _P, _P, _P, _P, C.POINTER(_P), _P, _P]),
     "umfpack_zi_numeric": (_I, [_P, _P, _P, _P, _P, C.POINTER(_P), _P, _P]),

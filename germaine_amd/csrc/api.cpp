@@ -83,6 +83,7 @@ std::unique_ptr<Factors<T>> make_factors(std::shared_ptr<DevPlan> dp, int nfreq,
     F->fstride = pl.arena;
     F->arena = dev_alloc<cx<T>>((size_t)pl.arena * nfreq);
     F->piv = dev_alloc<int>((size_t)pl.n * nfreq);
+    F->rowperm = dev_alloc<int>((size_t)pl.n * nfreq);
     F->sing = dev_alloc<int>(nfreq);
     if (copy_vals) {
         F->vals = dev_alloc<cx<double>>((size_t)pl.nnz * nfreq);
@@ -305,6 +306,53 @@ int gfdfd_solve_refine(gfdfd_factors f, int ifreq, int sys, int nrhs, const doub
                             : solve_refined<float>(*f->f32, ifreq, sys, nrhs, B, X, tol, max_iter, S(stream));
     if (iters) *iters = rs.iterations;
     if (relres) *relres = rs.max_rel_residual;
+    return GFDFD_OK;
+    GUARD_END
+}
+
+static int check_sel(gfdfd_factors f, int nsel, const int *freqs, const char *fn) {
+    if (!f || !freqs || nsel < 1 || nsel > kMaxBatch)
+        return report(GFDFD_ERR_ARG, "%s: need 1..%d frequencies", fn, kMaxBatch);
+    const int nf = f->f64 ? f->f64->nfreq : f->f32->nfreq;
+    for (int z = 0; z < nsel; z++)
+        if (freqs[z] < 0 || freqs[z] >= nf) return report(GFDFD_ERR_ARG, "%s: frequency %d out of range", fn, freqs[z]);
+    return GFDFD_OK;
+}
+
+int gfdfd_solve_batch(gfdfd_factors f, int nsel, const int *freqs, int sys, int nrhs, const double *d_b, double *d_x,
+                      void *stream) {
+    if (int e = check_sel(f, nsel, freqs, "gfdfd_solve_batch")) return e;
+    if (!d_b || !d_x || nrhs < 1) return report(GFDFD_ERR_ARG, "gfdfd_solve_batch: missing argument or nrhs < 1");
+    if (sys < 0 || sys > 2) return report(GFDFD_ERR_ARG, "gfdfd_solve_batch: invalid system %d", sys);
+    GUARD_BEGIN
+    std::vector<int> fr(freqs, freqs + nsel), sl(nsel);
+    for (int z = 0; z < nsel; z++) sl[z] = z;
+    auto B = reinterpret_cast<const cx<double> *>(d_b);
+    auto X = reinterpret_cast<cx<double> *>(d_x);
+    if (f->f64)
+        solve_batch<double, double>(*f->f64, fr, sl, nsel, sys, nrhs, B, X, S(stream));
+    else
+        solve_batch<float, double>(*f->f32, fr, sl, nsel, sys, nrhs, B, X, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_solve_refine_batch(gfdfd_factors f, int nsel, const int *freqs, int sys, int nrhs, const double *d_b,
+                             double *d_x, double tol, int max_iter, int *iters, double *relres, void *stream) {
+    if (int e = check_sel(f, nsel, freqs, "gfdfd_solve_refine_batch")) return e;
+    if (!d_b || !d_x || nrhs < 1) return report(GFDFD_ERR_ARG, "gfdfd_solve_refine_batch: missing argument");
+    if (d_b == d_x) return report(GFDFD_ERR_ARG, "gfdfd_solve_refine_batch: d_x must not alias d_b");
+    if (sys < 0 || sys > 2) return report(GFDFD_ERR_ARG, "gfdfd_solve_refine_batch: invalid system %d", sys);
+    GUARD_BEGIN
+    std::vector<int> fr(freqs, freqs + nsel);
+    auto B = reinterpret_cast<const cx<double> *>(d_b);
+    auto X = reinterpret_cast<cx<double> *>(d_x);
+    std::vector<RefineStats> rs = f->f64 ? solve_refined_batch<double>(*f->f64, fr, sys, nrhs, B, X, tol, max_iter, S(stream))
+                                         : solve_refined_batch<float>(*f->f32, fr, sys, nrhs, B, X, tol, max_iter, S(stream));
+    for (int z = 0; z < nsel; z++) {
+        if (iters) iters[z] = rs[z].iterations;
+        if (relres) relres[z] = rs[z].max_rel_residual;
+    }
     return GFDFD_OK;
     GUARD_END
 }

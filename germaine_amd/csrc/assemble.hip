@@ -17,6 +17,7 @@
 #pragma clang fp contract(off)
 #include "assemble.hpp"
 #include "common.hpp"
+#include "prof.hpp"
 
 namespace gfdfd {
 
@@ -173,6 +174,8 @@ __global__ void __launch_bounds__(256) k_assemble(int nx, int ny, AsmScalars sc,
 void assemble_device(int physics, int nx, int ny, const AsmScalars &sc, const float *m1, const float *m2,
                      const float *prof, const int *colptr, cx<double> *out, int64_t nnz, int nfreq, hipStream_t st) {
     dim3 grid((nx * ny + 255) / 256, nfreq);
+    // algorithmic traffic: the values written plus the two material grids read
+    ProfScope ps(K_ASSEMBLE, st, 0.0, (double)nfreq * (16.0 * nnz + 8.0 * nx * ny));
     switch (physics) {
     case 1:
         hipLaunchKernelGGL(k_assemble<1>, grid, dim3(256), 0, st, nx, ny, sc, m1, m2, prof, colptr, out, nnz);

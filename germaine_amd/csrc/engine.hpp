@@ -10,7 +10,8 @@
 
 namespace gfdfd {
 
-constexpr int kTile = 64;   // output tile of the trailing-update GEMM
+constexpr int kTile = 64;       // output tile of the trailing-update GEMM
+constexpr int kMaxBatch = 64;   // frequencies per batched solve
 
 // Plan arrays uploaded once per grid size and shared by every factorisation.
 struct DevPlan {
@@ -21,10 +22,19 @@ struct DevPlan {
     int *colptr = nullptr, *rowind = nullptr;
     int64_t *tq = nullptr;                 // CSR-of-M position -> CSC index (symmetric pattern)
     std::vector<int *> lv_small, lv_large, lv_all;
-    std::vector<int> n_small, n_large, n_all, mmax_all;
+    std::vector<int> n_small, n_large, n_all, mmax_all, pmax_all;
     TileDev *tiles = nullptr;
     std::vector<std::vector<std::pair<int64_t, int>>> steps;   // [level][step] -> (offset, count)
     size_t bytes = 0;
+    // algorithmic counts per level (complex entries or real flops), for measurement
+    struct LevelCounts {
+        double small_flops = 0, small_elems = 0;     // partial LUs in LDS; front entries read+written
+        double asm_elems = 0;                        // front entries written by the assembly
+        double tri_elems = 0;                        // sum p^2/2 + p(m-p): factor entries one sweep reads
+        double sum_m = 0;
+        std::vector<double> gemm_flops, gemm_elems, panel_flops, trsm_flops;   // per blocked step
+    };
+    std::vector<LevelCounts> counts;
 
     explicit DevPlan(std::shared_ptr<const Plan> p);
     ~DevPlan();
@@ -40,6 +50,7 @@ struct Factors {
     int64_t fstride = 0;            // arena elements per frequency
     cx<T> *arena = nullptr;         // fronts, column-major, factors in place
     int *piv = nullptr;             // nfreq * n local row swaps
+    int *rowperm = nullptr;         // nfreq * n: local source row of each pivot position after all swaps
     int *sing = nullptr;            // nfreq flags: zero pivot met
     cx<double> *vals = nullptr;     // nfreq * nnz FP64 values of M (CSC order) kept for refinement
     bool own_vals = false;
@@ -69,10 +80,21 @@ struct RefineStats {
     double max_rel_residual = 0;   // max over columns of ||b - M x|| / ||b||
 };
 
+// Batched solve: launch slot z solves frequency freqs[z] on panel slot slots[z]
+// of B and X (consecutive column-major n x nrhs panels).
+template <typename T, typename TIO>
+void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::vector<int> &slots, int nslot_buf,
+                 int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st);
+
 // FP64 iterative refinement around the factors F (any precision):
 // x_0 = solve(b); repeat r = b - M x, x += solve(r) until max rel residual <= tol.
 template <typename T>
 RefineStats solve_refined(const Factors<T> &F, int ifreq, int sys, int nrhs, const cx<double> *B, cx<double> *X,
                           double tol, int max_iter, hipStream_t st);
+// the same for the frequencies freqs[z] on panel slots z of B and X, solved together
+template <typename T>
+std::vector<RefineStats> solve_refined_batch(const Factors<T> &F, const std::vector<int> &freqs, int sys, int nrhs,
+                                             const cx<double> *B, cx<double> *X, double tol, int max_iter,
+                                             hipStream_t st);
 
 }  // namespace gfdfd

@@ -1,0 +1,44 @@
+// Complex products on gfx950 matrix cores: the real MFMA 16x16x4 instructions
+// for f32 and f64, four real products per complex product.
+//
+// Operand layout (one wave): A lane l -> (row l%16, k l/16), B lane l ->
+// (k l/16, col l%16).  Accumulator register v of lane l holds column l%16 and
+// row 4*(l/16)+v for f32, but row 4*v+l/16 for f64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gfdfd {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct mfma_t;
+template <>
+struct mfma_t<double> {
+    typedef d4v acc;
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * v + lk; }
+    static __device__ __forceinline__ acc op(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct mfma_t<float> {
+    typedef f4v acc;
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+    static __device__ __forceinline__ acc op(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// (cr, ci) += (ar + i ai) * (br + i bi) on one 16x16 block
+template <typename T>
+__device__ __forceinline__ void cmma(T ar, T ai, T br, T bi, typename mfma_t<T>::acc &cr,
+                                     typename mfma_t<T>::acc &ci) {
+    cr = mfma_t<T>::op(ar, br, cr);
+    cr = mfma_t<T>::op(-ai, bi, cr);
+    ci = mfma_t<T>::op(ar, bi, ci);
+    ci = mfma_t<T>::op(ai, br, ci);
+}
+
+}  // namespace gfdfd

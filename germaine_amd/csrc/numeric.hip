@@ -18,11 +18,11 @@
 #include <algorithm>
 
 #include "engine.hpp"
+#include "mfma.hpp"
+#include "prof.hpp"
 
 namespace gfdfd {
 
-typedef double d4v __attribute__((ext_vector_type(4)));
-typedef float f4v __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // DevPlan
@@ -70,6 +70,9 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
         n_large.push_back((int)L.large.size());
         n_all.push_back((int)all.size());
         mmax_all.push_back(std::max(L.small_mmax, L.large_mmax));
+        int pm = 0;
+        for (int f : all) pm = std::max(pm, pl.fronts[f].p);
+        pmax_all.push_back(pm);
     }
     // tile lists of every blocked step
     std::vector<TileDev> tl;
@@ -85,6 +88,42 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
     }
     tiles = dev_upload(tl);
     bytes += tl.size() * sizeof(TileDev);
+    counts.resize(pl.levels.size());
+    for (size_t l = 0; l < pl.levels.size(); l++) {
+        const LevelPlan &L = pl.levels[l];
+        LevelCounts &c = counts[l];
+        auto each = [&](const std::vector<int> &v, auto fn) { for (int f : v) fn(pl.fronts[f]); };
+        auto tri = [&](const Front &F) {
+            double p = F.p, m = F.m;
+            c.tri_elems += 0.5 * p * p + p * (m - p);
+            c.sum_m += m;
+            c.asm_elems += m * m;
+        };
+        each(L.small, tri);
+        each(L.large, tri);
+        each(L.small, [&](const Front &F) {
+            for (int k = 0; k < F.p; k++) {
+                double t = F.m - k - 1;
+                c.small_flops += 4.0 * (2.0 * t * t + t);
+            }
+            c.small_elems += 2.0 * F.m * F.m;
+        });
+        for (int k0 = 0; k0 < L.large_pmax; k0 += pl.nb) {
+            double gf = 0, ge = 0, pf = 0, tf = 0;
+            each(L.large, [&](const Front &F) {
+                if (F.p <= k0) return;
+                double kb = std::min(pl.nb, F.p - k0), t = F.m - k0 - kb;
+                gf += 8.0 * t * t * kb;
+                ge += 2.0 * t * kb + 2.0 * t * t;
+                for (int j = k0; j < k0 + kb; j++) pf += 8.0 * (F.m - j - 1) * (k0 + kb - j - 1) + 6.0 * (F.m - j - 1);
+                tf += 8.0 * 0.5 * kb * kb * t;
+            });
+            c.gemm_flops.push_back(gf);
+            c.gemm_elems.push_back(ge);
+            c.panel_flops.push_back(pf);
+            c.trsm_flops.push_back(tf);
+        }
+    }
 }
 
 DevPlan::~DevPlan() {
@@ -99,6 +138,7 @@ template <typename T>
 Factors<T>::~Factors() {
     dev_free(arena);
     dev_free(piv);
+    dev_free(rowperm);
     dev_free(sing);
     if (own_vals) dev_free(vals);
 }
@@ -302,25 +342,6 @@ __global__ void __launch_bounds__(64) k_trsm_u(cx<T> *__restrict__ arena, int64_
 // row 4*(l/16)+v for f32, but row 4*v+l/16 for f64.
 // ---------------------------------------------------------------------------
 template <typename T>
-struct mfma_t;
-template <>
-struct mfma_t<double> {
-    typedef d4v acc;
-    static __device__ __forceinline__ int row(int lk, int v) { return 4 * v + lk; }
-    static __device__ __forceinline__ acc op(double a, double b, acc c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-template <>
-struct mfma_t<float> {
-    typedef f4v acc;
-    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
-    static __device__ __forceinline__ acc op(float a, float b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
-
-template <typename T>
 __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, int64_t fstride,
                                                      const FrontDev *__restrict__ fronts, const int *__restrict__ list,
                                                      const TileDev *__restrict__ tiles, int k0, int nb) {
@@ -371,10 +392,7 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
         }
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) {
-                cr[a][b] = mfma_t<T>::op(ar[a], br[b], cr[a][b]);
-                cr[a][b] = mfma_t<T>::op(-ai[a], bi[b], cr[a][b]);
-                ci[a][b] = mfma_t<T>::op(ar[a], bi[b], ci[a][b]);
-                ci[a][b] = mfma_t<T>::op(ai[a], br[b], ci[a][b]);
+                cmma<T>(ar[a], ai[a], br[b], bi[b], cr[a][b], ci[a][b]);
             }
     }
     for (int a = 0; a < 2; a++)
@@ -387,6 +405,25 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
                     g.y -= ci[a][b][v];
                 }
             }
+}
+
+// ---------------------------------------------------------------------------
+// row permutation of every front from its swap sequence: rowperm[pos0+i] is
+// the local row that ends at position i
+// ---------------------------------------------------------------------------
+__global__ void k_rowperm(const FrontDev *__restrict__ fronts, int nfronts, const int *__restrict__ piv, int n,
+                          int *__restrict__ rowperm) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfronts) return;
+    const FrontDev F = fronts[f];
+    const int64_t o = (int64_t)blockIdx.y * n;
+    int *rp = rowperm + o + F.pos0;
+    const int *pv = piv + o + F.pos0;
+    for (int i = 0; i < F.p; i++) rp[i] = i;
+    for (int j = 0; j < F.p; j++) {
+        int t = pv[j];
+        if (t != j) { int a = rp[j]; rp[j] = rp[t]; rp[t] = a; }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -409,39 +446,55 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
     }
     for (int l = 0; l < pl.nlevels; l++) {
         const LevelPlan &L = pl.levels[l];
+        const DevPlan::LevelCounts &cnt = dp.counts[l];
+        const double es = sizeof(cx<T>) * (double)nf;
         if (dp.n_all[l] > 0) {
+            ProfScope ps(K_FRONT_ASSEMBLE, st, 0.0, cnt.asm_elems * es);
             int64_t mm = (int64_t)dp.mmax_all[l] * dp.mmax_all[l];
             int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, mm / (256 * 8)), 512);
             hipLaunchKernelGGL(k_front_assemble<T>, dim3(dp.n_all[l], chunks, nf), dim3(256), 0, st, F.arena,
                                F.fstride, dp.fronts, dp.lv_all[l], dp.children, dp.inv);
         }
-        const int64_t cnt = L.orig_end - L.orig_begin;
-        if (cnt > 0) {
-            int blocks = (int)std::min<int64_t>((cnt + 255) / 256, 4096);
+        const int64_t ocnt = L.orig_end - L.orig_begin;
+        if (ocnt > 0) {
+            ProfScope ps(K_SCATTER, st, 0.0, (double)ocnt * (16.0 + 16.0 + 2 * sizeof(cx<T>)) * nf);
+            int blocks = (int)std::min<int64_t>((ocnt + 255) / 256, 4096);
             hipLaunchKernelGGL(k_scatter_orig<T>, dim3(blocks, nf), dim3(256), 0, st, F.arena, F.fstride, vals,
-                               vstride, dp.orig_q + L.orig_begin, dp.orig_dst + L.orig_begin, cnt);
+                               vstride, dp.orig_q + L.orig_begin, dp.orig_dst + L.orig_begin, ocnt);
         }
         if (dp.n_small[l] > 0) {
             size_t lds = (size_t)L.small_mmax * L.small_mmax * sizeof(cx<T>);
             if (lds > lds_small_max) throw std::logic_error("small front exceeds LDS");
+            ProfScope ps(K_SMALL_LU, st, cnt.small_flops * nf, cnt.small_elems * es);
             hipLaunchKernelGGL(k_small_lu<T>, dim3(dp.n_small[l], nf), dim3(256), lds, st, F.arena, F.fstride,
                                dp.fronts, dp.lv_small[l], F.piv, pl.n, F.sing);
         }
         if (dp.n_large[l] > 0) {
             int si = 0;
             for (int k0 = 0; k0 < L.large_pmax; k0 += pl.nb, si++) {
+                {
+                ProfScope ps(K_PANEL, st, cnt.panel_flops[si] * nf, 0.0);
                 hipLaunchKernelGGL(k_panel<T>, dim3(dp.n_large[l], nf), dim3(256), 0, st, F.arena, F.fstride,
                                    dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, F.sing);
+                }
+                {
+                ProfScope ps(K_TRSM, st, cnt.trsm_flops[si] * nf, 0.0);
                 int colchunks = (L.large_mmax - k0 + 63) / 64;
                 hipLaunchKernelGGL(k_trsm_u<T>, dim3(colchunks, dp.n_large[l], nf), dim3(64), 0, st, F.arena,
                                    F.fstride, dp.fronts, dp.lv_large[l], k0, pl.nb);
+                }
                 const auto &sp = dp.steps[l][si];
-                if (sp.second > 0)
+                if (sp.second > 0) {
+                    ProfScope ps(K_GEMM, st, cnt.gemm_flops[si] * nf, cnt.gemm_elems[si] * es);
                     hipLaunchKernelGGL(k_gemm_update<T>, dim3(sp.second, nf), dim3(256), 0, st, F.arena, F.fstride,
                                        dp.fronts, dp.lv_large[l], dp.tiles + sp.first, k0, pl.nb);
+                }
             }
         }
     }
+    const int nfr = (int)pl.fronts.size();
+    hipLaunchKernelGGL(k_rowperm, dim3((nfr + 63) / 64, nf), dim3(64), 0, st, dp.fronts, nfr, F.piv, pl.n,
+                       F.rowperm);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,111 @@
+#include "prof.hpp"
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/gfdfd.h"
+
+namespace gfdfd {
+
+const char *const kClassNames[K_NCLASS] = {"assemble", "front_assemble", "scatter_orig", "small_lu", "panel",
+                                           "trsm_u", "gemm_update", "fwd_gather", "fwd_front", "bwd_front",
+                                           "fwd_vec", "bwd_vec", "permute", "residual"};
+
+namespace {
+struct Rec {
+    KClass c;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+struct Totals {
+    double ms = 0, flops = 0, bytes = 0;
+    long long launches = 0;
+};
+std::mutex mu;
+bool enabled = false;
+std::vector<Rec> pending;
+std::vector<hipEvent_t> pool;
+Totals totals[K_NCLASS];
+
+hipEvent_t get_event() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void drain() {
+    for (Rec &r : pending) {
+        float ms = 0;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            Totals &t = totals[r.c];
+            t.ms += ms;
+            t.flops += r.flops;
+            t.bytes += r.bytes;
+            t.launches++;
+        }
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    pending.clear();
+}
+}  // namespace
+
+bool prof_enabled() { return enabled; }
+
+ProfScope::ProfScope(KClass c, hipStream_t st, double flops, double bytes) : st_(st) {
+    if (!enabled) return;
+    std::lock_guard<std::mutex> lk(mu);
+    Rec r{c, get_event(), get_event(), flops, bytes};
+    if (!r.a || !r.b) return;
+    if (hipEventRecord(r.a, st) != hipSuccess) return;
+    pending.push_back(r);
+    slot_ = (int)pending.size() - 1;
+}
+
+ProfScope::~ProfScope() {
+    if (slot_ < 0) return;
+    std::lock_guard<std::mutex> lk(mu);
+    (void)hipEventRecord(pending[slot_].b, st_);
+    if (pending.size() > 4096) drain();
+}
+
+}  // namespace gfdfd
+
+using namespace gfdfd;
+
+extern "C" {
+
+int gfdfd_profile(int enable) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!enable) drain();
+    enabled = enable != 0;
+    return GFDFD_OK;
+}
+
+void gfdfd_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(mu);
+    drain();
+    for (auto &t : totals) t = Totals();
+}
+
+int gfdfd_profile_read(const char *kernel, double *ms, long long *launches, double *flops, double *bytes) {
+    std::lock_guard<std::mutex> lk(mu);
+    drain();
+    for (int c = 0; c < K_NCLASS; c++)
+        if (kernel && std::strcmp(kernel, kClassNames[c]) == 0) {
+            if (ms) *ms = totals[c].ms;
+            if (launches) *launches = totals[c].launches;
+            if (flops) *flops = totals[c].flops;
+            if (bytes) *bytes = totals[c].bytes;
+            return GFDFD_OK;
+        }
+    return GFDFD_ERR_ARG;
+}
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+// Optional per-kernel-class timing with HIP events recorded on the launch
+// stream (off unless enabled through gfdfd_profile), with the algorithmic
+// flops and bytes of each launch attached by the caller.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gfdfd {
+
+enum KClass {
+    K_ASSEMBLE, K_FRONT_ASSEMBLE, K_SCATTER, K_SMALL_LU, K_PANEL, K_TRSM, K_GEMM,
+    K_FWD_GATHER, K_FWD, K_BWD, K_FWD_VEC, K_BWD_VEC, K_PERMUTE, K_RESIDUAL, K_NCLASS
+};
+
+extern const char *const kClassNames[K_NCLASS];
+
+bool prof_enabled();
+
+// Records a start event on construction and an end event on destruction
+// (both on `st`) when profiling is enabled; otherwise does nothing.
+class ProfScope {
+  public:
+    ProfScope(KClass c, hipStream_t st, double flops, double bytes);
+    ~ProfScope();
+    ProfScope(const ProfScope &) = delete;
+    ProfScope &operator=(const ProfScope &) = delete;
+
+  private:
+    int slot_ = -1;
+    hipStream_t st_;
+};
+
+}  // namespace gfdfd

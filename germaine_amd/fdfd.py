@@ -183,6 +183,26 @@ class Factors:
                                        C.byref(rr), _stream(stream)), "gfdfd_solve_refine")
         return X, it.value, rr.value
 
+    def solve_batch(self, freqs, B, sys=0, out=None, stream=None):
+        """B: complex128 cuda [len(freqs), nrhs, n]; one launch set for all frequencies."""
+        import torch
+        X = torch.empty_like(B) if out is None else out
+        fr = np.ascontiguousarray(freqs, np.int32)
+        check(lib().gfdfd_solve_batch(self.h, len(fr), fr.ctypes.data, sys, B.shape[1], _ptr(B), _ptr(X),
+                                      _stream(stream)), "gfdfd_solve_batch")
+        return X
+
+    def solve_refine_batch(self, freqs, B, sys=0, tol=1e-10, max_iter=10, out=None, stream=None):
+        import torch
+        X = torch.empty_like(B) if out is None else out
+        fr = np.ascontiguousarray(freqs, np.int32)
+        its = np.zeros(len(fr), np.int32)
+        rr = np.zeros(len(fr), np.float64)
+        check(lib().gfdfd_solve_refine_batch(self.h, len(fr), fr.ctypes.data, sys, B.shape[1], _ptr(B), _ptr(X), tol,
+                                             max_iter, its.ctypes.data, rr.ctypes.data, _stream(stream)),
+              "gfdfd_solve_refine_batch")
+        return X, its.tolist(), rr.tolist()
+
 
 # ---- shot panels and workflows ---------------------------------------------------
 

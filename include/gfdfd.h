@@ -122,6 +122,15 @@ int gfdfd_solve(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b
  * corrections; d_x must not alias d_b.  iters/relres may be NULL.           */
 int gfdfd_solve_refine(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x, double tol,
                        int max_iter, int *iters, double *relres, void *stream);
+/* the same for nsel frequencies at once: freqs[z] (host array) is solved on
+ * panel z of d_b / d_x (nsel consecutive n x nrhs panels), all in the same
+ * kernel launches; at most 64 frequencies per call.  The refining variant
+ * drops a frequency from the batch once it has converged; iters/relres
+ * (nsel entries each) may be NULL.                                          */
+int gfdfd_solve_batch(gfdfd_factors f, int nsel, const int *freqs, int sys, int nrhs, const double *d_b, double *d_x,
+                      void *stream);
+int gfdfd_solve_refine_batch(gfdfd_factors f, int nsel, const int *freqs, int sys, int nrhs, const double *d_b,
+                             double *d_x, double tol, int max_iter, int *iters, double *relres, void *stream);
 /* r = b - op(M) x in FP64 */
 int gfdfd_residual(gfdfd_plan plan, const double *d_vals, int sys, int nrhs, const double *d_b, const double *d_x,
                    double *d_r, void *stream);
@@ -150,6 +159,17 @@ int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, con
 /* ass_grad_TE (src/TE/ass_grad_TE.c:31-35): sigma and epsilon gradients */
 int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u, const double *d_lam,
                   double *d_grad_sigma, double *d_grad_eps, void *stream);
+
+/* ---- measurement ------------------------------------------------------------
+ * Per-kernel-class timing with HIP events on the launch stream, plus the
+ * algorithmic flops and HBM bytes of each launch derived from the plan.
+ * Classes: assemble, front_assemble, scatter_orig, small_lu, panel, trsm_u,
+ * gemm_update, fwd_gather, fwd_front, bwd_front, fwd_vec, bwd_vec, permute,
+ * residual.  No counterpart in the reference (it times whole phases with
+ * MPI_Wtime, src/AC/forward_shot_AC.c:54-65).                               */
+int gfdfd_profile(int enable);
+void gfdfd_profile_reset(void);
+int gfdfd_profile_read(const char *kernel, double *ms, long long *launches, double *flops, double *bytes);
 
 #ifdef __cplusplus
 }

@@ -185,3 +185,62 @@ def test_forward_shots_fp32_refined_config_201x101():
     assert np.all(err <= 1e-8), err
     d = out["data"][0].cpu().numpy()
     np.testing.assert_allclose(d, Xr[w.rec, :].T.astype(np.complex64), rtol=1e-5, atol=1e-6 * np.abs(d).max())
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+@pytest.mark.parametrize("sysno", [0, 2])
+def test_batched_solves_match_oracle(precision, sysno):
+    """Several frequencies in one launch set (and refinement that drops converged
+    frequencies) against per-frequency oracle solves."""
+    nx, ny, npml, dh = 41, 30, 6, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=4)
+    freqs = [2.5, 7.0, 13.0]
+    say("batched solves", precision, sysno)
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    F = pl.factor(pl.assemble(G.AC, dh, freqs, dev(m1), dev(m2), npml=npml), precision)
+    rng = np.random.default_rng(11)
+    B = rng.standard_normal((3, 66, nx * ny)) + 1j * rng.standard_normal((3, 66, nx * ny))
+    order = [2, 0, 1]
+    if precision == G.FP64:
+        X = F.solve_batch(order, dev(B), sys=sysno).cpu().numpy()
+        tol = 1e-10
+    else:
+        X, its, rr = F.solve_refine_batch(order, dev(B), sys=sysno, tol=1e-12, max_iter=20)
+        X = X.cpu().numpy()
+        say("  iterations", its, rr)
+        assert max(rr) <= 1e-12
+        tol = 1e-8
+    for z, k in enumerate(order):
+        _, csc = oracle_system(G.AC, nx, ny, npml, dh, freqs[k], m1, m2)
+        Xr, _ = O.band_solve(*csc, B[z].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+        assert rel(X[z], Xr.T) <= tol, (z, k)
+
+
+def random_grid_csc(nx, ny, seed):
+    """Random complex values on the 9-point pattern (no diagonal dominance): the
+    in-front partial pivoting swaps rows, which the assembled Helmholtz matrices
+    of the small tests rarely do."""
+    pl = fdfd.Plan(nx, ny)
+    cp, ri = pl.pattern()
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(len(ri)) + 1j * rng.standard_normal(len(ri))
+    return cp, ri, np.ascontiguousarray(v.real), np.ascontiguousarray(v.imag)
+
+
+@pytest.mark.parametrize("sysno", [0, 1, 2])
+@pytest.mark.parametrize("nrhs", [1, 70])
+def test_pivoting_solves_match_oracle(sysno, nrhs):
+    nx, ny = 37, 26
+    say("pivoting solves", sysno, nrhs)
+    mats = [random_grid_csc(nx, ny, seed) for seed in (21, 22)]
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    vals = dev(np.stack([c[2] + 1j * c[3] for c in mats]))
+    F = pl.factor(vals, G.FP64)
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((2, nrhs, nx * ny)) + 1j * rng.standard_normal((2, nrhs, nx * ny))
+    Xb = F.solve_batch([1, 0], dev(B), sys=sysno).cpu().numpy()
+    for z, k in enumerate([1, 0]):
+        Xr, _ = O.band_solve(*mats[k], B[z].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+        assert rel(Xb[z], Xr.T) <= 1e-8, (z, k)
+        Xs = F.solve(k, dev(B[z]), sys=sysno).cpu().numpy()
+        assert rel(Xs, Xr.T) <= 1e-8, (z, k)
