@@ -27,9 +27,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64_mfma": 78.6e12, "fp32_mfma": 157.3e12, "hbm": 8.0e12}
-KERNELS = ["assemble", "front_assemble", "scatter_orig", "small_lu", "panel", "trsm_u", "gemm_update",
-           "fwd_gather", "fwd_front", "bwd_front", "fwd_vec", "bwd_vec", "permute", "residual"]
-COMPUTE_KERNELS = {"small_lu", "panel", "trsm_u", "gemm_update", "fwd_front", "bwd_front"}
+KERNELS = ["assemble", "front_assemble", "scatter_orig", "small_lu", "panel", "trsm_u", "gemm_update", "diag_inv",
+           "fwd_gather", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk", "bwd_small1", "bwd_front1", "bwd_front",
+           "bwd_contrib", "bwd_blk",
+           "piv_unswap", "fwd_vec", "bwd_vec", "permute", "residual", "axpy"]
+# kernels whose arithmetic runs on the matrix cores (at the factor precision)
+MFMA_KERNELS = {"small_lu", "panel", "trsm_u", "gemm_update", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk",
+                "bwd_small1", "bwd_front1", "bwd_front", "bwd_contrib", "bwd_blk"}
 
 
 def log(*a):
@@ -46,6 +50,22 @@ def kernel_stats(lib, ctypes):
     return out
 
 
+def roofline(name, d, mfma_peak):
+    """Roofline position of one kernel class from its algorithmic flops and
+    bytes per launch and its average event-timed launch duration: the bound is
+    whichever of the MFMA and HBM times is larger at peak."""
+    avg_s = d["ms"] / d["launches"] * 1e-3
+    fl, by = d["flops"] / d["launches"], d["bytes"] / d["launches"]
+    peak_f = mfma_peak if name in MFMA_KERNELS else PEAK["fp64_mfma"]
+    if fl / peak_f > by / PEAK["hbm"]:
+        return dict(bound="mfma", achieved=fl / avg_s / 1e12, peak=peak_f / 1e12, unit="TFLOP/s",
+                    frac=fl / avg_s / peak_f, traffic=None, kernel=name, avg_launch_us=avg_s * 1e6,
+                    flops_per_launch=fl, bytes_per_launch=by)
+    return dict(bound="hbm", achieved=by / avg_s / 1e9, peak=PEAK["hbm"] / 1e9, unit="GB/s",
+                frac=by / avg_s / PEAK["hbm"], traffic=None, kernel=name, avg_launch_us=avg_s * 1e6,
+                flops_per_launch=fl, bytes_per_launch=by)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +77,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the batched FP64 comparison step")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the timed region, one device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -88,9 +111,13 @@ def main():
     Xs = torch.empty((nf, ns, n), dtype=torch.complex128, device="cuda")
     freqs = list(range(nf))
 
+    store = {}
+
     def step(precision, refine):
         vals = pl.assemble(w.physics, w.dh, w.freqs, m1, m2, s=w.s, npml=w.npml, a0_pml=w.a0_pml)
-        F = pl.factor(vals, precision)
+        # factor storage is allocated once per precision (the warm-up step) and refactored in place
+        F = store[precision].refactor(vals) if precision in store else pl.factor(vals, precision)
+        store[precision] = F
         for f in range(nf):
             G.check(L.gfdfd_spike_panel(n, ns, ctypes.c_void_p(ksrc.data_ptr()), amps[f].real, amps[f].imag,
                                         ctypes.c_void_p(Bs[f].data_ptr()), ctypes.c_void_p(st.cuda_stream)))
@@ -104,20 +131,19 @@ def main():
 
     def timed(precision, refine, steps, warmup, profile=False):
         for _ in range(warmup):
-            F, _ = step(precision, refine)
-            del F
+            step(precision, refine)
         torch.cuda.synchronize()
         if profile:
             L.gfdfd_profile_reset()
             L.gfdfd_profile(1)
         t0 = time.perf_counter()
         for _ in range(steps):
-            F, its = step(precision, refine)
-            del F
+            _, its = step(precision, refine)
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / steps
         if profile:
             L.gfdfd_profile(0)
+        store.pop(precision, None)                  # release this precision's factors
         return t, its
 
     # --- headline: FP32 fronts + FP64 refinement, batched -----------------------------
@@ -127,10 +153,14 @@ def main():
     X32 = Xs.clone() if not args.no_check else None
 
     # --- FP64 fronts, batched, no refinement ---------------------------------------------
-    t64, _ = timed(G.FP64, False, max(1, args.steps // 2), 1)
-    log(f"fp64 batched: {t64 * 1e3:.1f} ms/step")
+    n64 = max(1, args.steps // 2)
+    t64, kst64 = None, {}
+    if not args.no_fp64:
+        t64, _ = timed(G.FP64, False, n64, 1, profile=True)
+        kst64 = kernel_stats(L, ctypes)
+        log(f"fp64 batched: {t64 * 1e3:.1f} ms/step")
     err = None
-    if X32 is not None:
+    if X32 is not None and t64 is not None:
         err = []
         for f in range(nf):
             d = torch.linalg.vector_norm(X32[f] - Xs[f], dim=1) / torch.linalg.vector_norm(Xs[f], dim=1)
@@ -155,23 +185,23 @@ def main():
         tplain = time.perf_counter() - t0
         log(f"plain fp64 per-shot: {tplain * 1e3:.1f} ms")
 
+    if args.copy_probe:
+        # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # calibrate the HBM-traffic counters of the solve kernels
+        src = torch.empty(args.copy_probe << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # --- roofline of the dominant kernel -----------------------------------------------------
     dom = max(kst, key=lambda k: kst[k]["ms"]) if kst else None
     roof = None
     if dom:
-        d = kst[dom]
-        avg_s = d["ms"] / d["launches"] * 1e-3
-        if dom in COMPUTE_KERNELS:
-            ach = d["flops"] / d["launches"] / avg_s
-            roof = dict(bound="mfma", achieved=ach / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s",
-                        frac=ach / PEAK["fp32_mfma"], traffic=None)
-        else:
-            ach = d["bytes"] / d["launches"] / avg_s
-            roof = dict(bound="hbm", achieved=ach / 1e9, peak=PEAK["hbm"] / 1e9, unit="GB/s",
-                        frac=ach / PEAK["hbm"], traffic=None)
-        roof["kernel"] = dom
-        roof["launches_per_step"] = d["launches"] / args.steps
-        roof["ms_per_step"] = d["ms"] / args.steps
+        roof = roofline(dom, kst[dom], PEAK["fp32_mfma"])
+        roof["launches_per_step"] = kst[dom]["launches"] / args.steps
+        roof["ms_per_step"] = kst[dom]["ms"] / args.steps
 
     # --- CPU baseline ---------------------------------------------------------------------------
     cpu = None
@@ -200,7 +230,9 @@ def main():
                                f"{nf} freqs {w.freqs[0]:g}-{w.freqs[-1]:g} Hz batched, {ns}-shot panel",
                    "n": n, "nnz": pl.nnz, "fronts": info.fronts, "factor_gflop_per_freq": info.factor_flops / 1e9,
                    "nnz_lu": info.nnz_lu, "refine_tol": args.tol},
-        "fp64_batched": {"ms_per_step": round(t64 * 1e3, 3), "value": round(nf * ns / t64, 2)},
+        "fp64_batched": None if t64 is None else {
+            "ms_per_step": round(t64 * 1e3, 3), "value": round(nf * ns / t64, 2),
+            "kernels_ms_per_step": {k: round(v["ms"] / n64, 3) for k, v in kst64.items()}},
         "plain_fp64_per_shot": None if tplain is None else {"ms": round(tplain * 1e3, 3),
                                                              "value": round(nf * ns / tplain, 2)},
         "speedup_vs_plain_fp64": None if tplain is None else round(tplain / t32, 3),

@@ -85,6 +85,7 @@ std::unique_ptr<Factors<T>> make_factors(std::shared_ptr<DevPlan> dp, int nfreq,
     F->piv = dev_alloc<int>((size_t)pl.n * nfreq);
     F->rowperm = dev_alloc<int>((size_t)pl.n * nfreq);
     F->sing = dev_alloc<int>(nfreq);
+    F->gsrc = dev_alloc<int>((size_t)pl.fv_rows * 3 * nfreq);
     if (copy_vals) {
         F->vals = dev_alloc<cx<double>>((size_t)pl.nnz * nfreq);
         F->own_vals = true;
@@ -95,6 +96,15 @@ std::unique_ptr<Factors<T>> make_factors(std::shared_ptr<DevPlan> dp, int nfreq,
     factor_numeric<T>(*dp, *F, F->vals, pl.nnz, st);
     return F;
 }
+// new values of the same pattern into existing factors (storage reused)
+template <typename T>
+void refactor(Factors<T> &F, const cx<double> *vals, hipStream_t st) {
+    const Plan &pl = *F.plan->host;
+    if (!F.own_vals) throw std::invalid_argument("factors do not own their values");
+    GFDFD_HIP(hipMemcpyAsync(F.vals, vals, sizeof(cx<double>) * pl.nnz * F.nfreq, hipMemcpyDeviceToDevice, st));
+    factor_numeric<T>(*F.plan, F, F.vals, pl.nnz, st);
+}
+
 template std::unique_ptr<Factors<double>> make_factors<double>(std::shared_ptr<DevPlan>, int, const cx<double> *, bool,
                                                                hipStream_t);
 template std::unique_ptr<Factors<float>> make_factors<float>(std::shared_ptr<DevPlan>, int, const cx<double> *, bool,
@@ -259,6 +269,23 @@ int gfdfd_factor(gfdfd_plan plan, int nfreq, const double *d_vals, int precision
     else
         h->f32 = make_factors<float>(plan->dev, nfreq, v, true, S(stream));
     *out = h.release();
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_solve_tuning(int rw_cap, int fused_mmax) {
+    GUARD_BEGIN
+    set_solve_tuning(rw_cap, fused_mmax);
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_refactor(gfdfd_factors f, const double *d_vals, void *stream) {
+    if (!f || !d_vals) return report(GFDFD_ERR_ARG, "gfdfd_refactor: NULL argument");
+    GUARD_BEGIN
+    const cx<double> *v = reinterpret_cast<const cx<double> *>(d_vals);
+    if (f->f64) refactor<double>(*f->f64, v, S(stream));
+    else refactor<float>(*f->f32, v, S(stream));
     return GFDFD_OK;
     GUARD_END
 }

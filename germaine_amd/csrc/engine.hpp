@@ -10,6 +10,7 @@
 
 namespace gfdfd {
 
+constexpr int kSingleMmax = 300;   // largest front order of a single-block level (its panels fit in LDS)
 constexpr int kTile = 64;       // output tile of the trailing-update GEMM
 constexpr int kMaxBatch = 64;   // frequencies per batched solve
 
@@ -31,10 +32,19 @@ struct DevPlan {
         double small_flops = 0, small_elems = 0;     // partial LUs in LDS; front entries read+written
         double asm_elems = 0;                        // front entries written by the assembly
         double tri_elems = 0;                        // sum p^2/2 + p(m-p): factor entries one sweep reads
-        double sum_m = 0;
+        double sum_m = 0, sum_p = 0;
+        double contrib_elems = 0;                    // sum p(m-p): U12 entries of the backward contribution
         std::vector<double> gemm_flops, gemm_elems, panel_flops, trsm_flops;   // per blocked step
+        std::vector<double> fwd_blk_elems, bwd_blk_elems;   // per 32-pivot block: factor entries the block reads
     };
     std::vector<LevelCounts> counts;
+    double diag_elems = 0;                          // entries of all 32x32 diagonal blocks
+    int max_children = 0;                           // largest number of children of one front
+    std::vector<int> single;                        // per level: 1 if stored in the single-block solve format
+    int *lv_single = nullptr;                       // device copy of `single`
+    int *gsrc_id = nullptr;                         // [fv_rows][3] gather sources of front rows before pivoting
+    std::vector<FrontDev *> lv_fr;                  // per level: the level's fronts, contiguous
+    std::vector<int> lv_children;                   // per level: 1 if any front has children
 
     explicit DevPlan(std::shared_ptr<const Plan> p);
     ~DevPlan();
@@ -52,6 +62,7 @@ struct Factors {
     int *piv = nullptr;             // nfreq * n local row swaps
     int *rowperm = nullptr;         // nfreq * n: local source row of each pivot position after all swaps
     int *sing = nullptr;            // nfreq flags: zero pivot met
+    int *gsrc = nullptr;            // nfreq * fv_rows * 3: gather sources after pivoting (forward sweep of M)
     cx<double> *vals = nullptr;     // nfreq * nnz FP64 values of M (CSC order) kept for refinement
     bool own_vals = false;
     ~Factors();
@@ -75,16 +86,22 @@ void axpy_fp64(int64_t count, const cx<double> *D, cx<double> *X, hipStream_t st
 void cast_c128_to_c64(int64_t count, const cx<double> *a, cx<float> *b, hipStream_t st);
 void cast_c64_to_c128(int64_t count, const cx<float> *a, cx<double> *b, hipStream_t st);
 
+// panel-solve kernel selection: widest fused right-hand-side tile (16/32/64) and
+// the largest front order of a multi-block level taking the fused kernels
+// (negative: no limit beyond LDS)
+void set_solve_tuning(int rw_cap, int fused_mmax);
+
 struct RefineStats {
     int iterations = 0;
     double max_rel_residual = 0;   // max over columns of ||b - M x|| / ||b||
 };
 
 // Batched solve: launch slot z solves frequency freqs[z] on panel slot slots[z]
-// of B and X (consecutive column-major n x nrhs panels).
+// of B and X (consecutive column-major n x nrhs panels).  accumulate: X += the
+// solution instead of X = (refinement corrections); B and X must not alias then.
 template <typename T, typename TIO>
 void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::vector<int> &slots, int nslot_buf,
-                 int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st);
+                 int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st, bool accumulate = false);
 
 // FP64 iterative refinement around the factors F (any precision):
 // x_0 = solve(b); repeat r = b - M x, x += solve(r) until max rel residual <= tol.

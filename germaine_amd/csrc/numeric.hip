@@ -47,6 +47,24 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
     bytes += fd.size() * sizeof(FrontDev) + (pl.children.size() + pl.idx.size() + pl.inv.size() +
                                             2 * pl.perm.size() + pl.colptr.size() + pl.rowind.size()) * 4 +
              (pl.orig_q.size() + pl.orig_dst.size()) * 8;
+    // gather sources of every front row before pivoting, at the front's work rows:
+    // [X position or -1, W row of a child's contribution or -1, a second one or -1]
+    {
+        std::vector<int> g((size_t)pl.fv_rows * 3, -1);
+        for (const Front &F : pl.fronts)
+            for (int i = 0; i < F.m; i++) {
+                int *r = &g[(size_t)(F.fv_off + i) * 3];
+                if (i < F.p) r[0] = F.pos0 + i;
+                int k = 1;
+                for (int c = 0; c < F.nchild; c++) {
+                    const Front &C = pl.fronts[pl.children[F.child_off + c]];
+                    const int a = pl.inv[C.inv_off + i];
+                    if (a >= 0 && k < 3) r[k++] = (int)(C.fv_off + C.p + a);
+                }
+            }
+        gsrc_id = dev_upload(g);
+        bytes += g.size() * 4;
+    }
     // CSR-of-M -> CSC map, valid when the pattern is symmetric (grid plans)
     if (pl.grid) {
         std::vector<int64_t> t(pl.nnz);
@@ -63,6 +81,14 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
     for (const LevelPlan &L : pl.levels) {
         std::vector<int> all(L.small);
         all.insert(all.end(), L.large.begin(), L.large.end());
+        std::vector<FrontDev> lf;
+        int haskids = 0;
+        for (int f : all) {
+            lf.push_back(fd[f]);
+            haskids |= pl.fronts[f].nchild > 0;
+        }
+        lv_children.push_back(haskids);
+        lv_fr.push_back(dev_upload(lf));
         lv_small.push_back(dev_upload(L.small));
         lv_large.push_back(dev_upload(L.large));
         lv_all.push_back(dev_upload(all));
@@ -88,6 +114,12 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
     }
     tiles = dev_upload(tl);
     bytes += tl.size() * sizeof(TileDev);
+    for (const Front &F : pl.fronts) max_children = std::max(max_children, F.nchild);
+    // single-block levels: every front of the level has at most one pivot block
+    // and is stored in the one-product solve format (see k_diag_inv)
+    for (size_t l = 0; l < pl.levels.size(); l++)
+        single.push_back(pmax_all[l] <= 32 && mmax_all[l] <= kSingleMmax && max_children <= 2 ? 1 : 0);
+    lv_single = dev_upload(single);
     counts.resize(pl.levels.size());
     for (size_t l = 0; l < pl.levels.size(); l++) {
         const LevelPlan &L = pl.levels[l];
@@ -108,6 +140,25 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
             }
             c.small_elems += 2.0 * F.m * F.m;
         });
+        each(L.small, [&](const Front &F) { c.contrib_elems += (double)F.p * (F.m - F.p); c.sum_p += F.p; });
+        each(L.large, [&](const Front &F) { c.contrib_elems += (double)F.p * (F.m - F.p); c.sum_p += F.p; });
+        int lpmax = 0;
+        for (int f : L.small) lpmax = std::max(lpmax, pl.fronts[f].p);
+        for (int f : L.large) lpmax = std::max(lpmax, pl.fronts[f].p);
+        for (int b0 = 0; b0 < lpmax; b0 += 32) {
+            double fe = 0, be = 0;
+            auto blk = [&](const Front &F) {
+                if (F.p <= b0) return;
+                const double kb = std::min(32, F.p - b0);
+                fe += 0.5 * kb * kb + kb * (F.m - b0 - kb);
+                be += 0.5 * kb * kb + kb * b0;
+                diag_elems += kb * kb;
+            };
+            each(L.small, blk);
+            each(L.large, blk);
+            c.fwd_blk_elems.push_back(fe);
+            c.bwd_blk_elems.push_back(be);
+        }
         for (int k0 = 0; k0 < L.large_pmax; k0 += pl.nb) {
             double gf = 0, ge = 0, pf = 0, tf = 0;
             each(L.large, [&](const Front &F) {
@@ -129,6 +180,9 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
 DevPlan::~DevPlan() {
     dev_free(fronts); dev_free(children); dev_free(idx); dev_free(inv); dev_free(perm); dev_free(iperm);
     dev_free(orig_q); dev_free(orig_dst); dev_free(colptr); dev_free(rowind); dev_free(tq); dev_free(tiles);
+    dev_free(lv_single);
+    dev_free(gsrc_id);
+    for (auto p : lv_fr) dev_free(p);
     for (auto p : lv_small) dev_free(p);
     for (auto p : lv_large) dev_free(p);
     for (auto p : lv_all) dev_free(p);
@@ -140,6 +194,7 @@ Factors<T>::~Factors() {
     dev_free(piv);
     dev_free(rowperm);
     dev_free(sing);
+    dev_free(gsrc);
     if (own_vals) dev_free(vals);
 }
 template struct Factors<double>;
@@ -427,6 +482,121 @@ __global__ void k_rowperm(const FrontDev *__restrict__ fronts, int nfronts, cons
 }
 
 // ---------------------------------------------------------------------------
+// gather sources after pivoting, per frequency: row i of a front takes the
+// sources of its row rowperm(i) (pivot rows) or i (contribution rows), so a
+// forward sweep reads one table entry per row.  grid (row chunks, nfreq)
+// ---------------------------------------------------------------------------
+__global__ void k_gsrc(const FrontDev *__restrict__ fronts, int nfronts, const int *__restrict__ rowperm, int n,
+                       const int *__restrict__ gsrc_id, int64_t fv_rows, int *__restrict__ gsrc) {
+    const int f = blockIdx.x;
+    if (f >= nfronts) return;
+    const FrontDev F = fronts[f];
+    const int *rp = rowperm + (int64_t)blockIdx.y * n + F.pos0;
+    int *out = gsrc + ((int64_t)blockIdx.y * fv_rows + F.fv_off) * 3;
+    const int *in = gsrc_id + F.fv_off * 3;
+    for (int i = threadIdx.x; i < F.m; i += blockDim.x) {
+        const int s = i < F.p ? rp[i] : i;
+        out[3 * i] = in[3 * s];
+        out[3 * i + 1] = in[3 * s + 1];
+        out[3 * i + 2] = in[3 * s + 2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// solve format: every 32x32 diagonal block of a front's pivot block is
+// replaced in place by its inverses -- strictly lower part: inverse of the
+// unit lower block of L11, upper part with diagonal: inverse of the upper
+// block of U11 -- so that the substitutions apply each diagonal block as one
+// small dense product on the matrix cores instead of a 32-step dependent
+// chain.  Fronts of single-block levels (p <= 32) go one step further: L21 is
+// replaced by L21 inv(L11) and U12 by inv(U11) U12, which turns each of their
+// forward and backward substitutions -- for M as for M^T -- into one product
+// with no dependent phase:  [y; w2 - L21 y] = [inv(L11); L21 inv(L11)] w1  and
+// x1 = [inv(U11), -inv(U11) U12] [y1; x2].  Computed in FP64 for both factor
+// precisions.  One workgroup per front (thread j builds column j of both
+// inverses; the products run over all threads, 32 rows or columns at a time),
+// grid (fronts, nfreq).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int64_t fstride,
+                                                  const FrontDev *__restrict__ fronts,
+                                                  const int *__restrict__ lv_single) {
+    __shared__ cx<double> D[32][33];
+    __shared__ cx<double> R[32][33];
+    __shared__ cx<double> V[32][33];
+    const FrontDev F = fronts[blockIdx.x];
+    cx<T> *G = arena + blockIdx.y * fstride + F.buf_off;
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    for (int b0 = 0; b0 < p; b0 += 32) {
+        const int kb = min(32, p - b0);
+        for (int e = tid; e < kb * kb; e += 256) {
+            const int i = e % kb, t = e / kb;
+            const cx<T> v = G[b0 + i + (int64_t)(b0 + t) * m];
+            D[i][t] = {(double)v.x, (double)v.y};
+        }
+        __syncthreads();
+        if (tid < kb) {
+            const int j = tid;
+            // column j of inv(L_bb), L_bb unit lower: x_j = 1, x_i = -sum_{j<=t<i} L[i][t] x_t
+            R[j][j] = {1.0, 0.0};
+            for (int i = j + 1; i < kb; i++) {
+                cx<double> acc = {0.0, 0.0};
+                for (int t = j; t < i; t++) acc = cmsub(acc, D[i][t], R[t][j]);
+                R[i][j] = acc;
+            }
+            // column j of inv(U_bb): x_j = 1/U[j][j], x_i = -(sum_{i<t<=j} U[i][t] x_t) / U[i][i]
+            R[j][j] = crecip(D[j][j]);
+            for (int i = j - 1; i >= 0; i--) {
+                cx<double> acc = {0.0, 0.0};
+                for (int t = i + 1; t <= j; t++) acc = cmsub(acc, D[i][t], R[t][j]);
+                R[i][j] = cdiv(acc, D[i][i]);
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < kb * kb; e += 256) {
+            const int i = e % kb, t = e / kb;
+            G[b0 + i + (int64_t)(b0 + t) * m] = {(T)R[i][t].x, (T)R[i][t].y};
+        }
+        __syncthreads();
+    }
+    if (!lv_single[F.level] || p > 32) return;
+    // L21 <- L21 inv(L11), 32 rows at a time staged in V (all reads before any write)
+    for (int r0 = p; r0 < m; r0 += 32) {
+        const int rows = min(32, m - r0);
+        for (int e = tid; e < rows * p; e += 256) {
+            const int i = e % rows, s = e / rows;
+            const cx<T> v = G[r0 + i + (int64_t)s * m];
+            V[i][s] = {(double)v.x, (double)v.y};
+        }
+        __syncthreads();
+        for (int e = tid; e < rows * p; e += 256) {
+            const int i = e % rows, t = e / rows;
+            cx<double> acc = V[i][t];
+            for (int s = t + 1; s < p; s++) acc = cadd(acc, cmul(V[i][s], R[s][t]));
+            G[r0 + i + (int64_t)t * m] = {(T)acc.x, (T)acc.y};
+        }
+        __syncthreads();
+    }
+    // U12 <- inv(U11) U12, 32 columns at a time
+    for (int c0 = p; c0 < m; c0 += 32) {
+        const int cols = min(32, m - c0);
+        for (int e = tid; e < cols * p; e += 256) {
+            const int s = e % p, c = e / p;
+            const cx<T> v = G[s + (int64_t)(c0 + c) * m];
+            V[c][s] = {(double)v.x, (double)v.y};
+        }
+        __syncthreads();
+        for (int e = tid; e < cols * p; e += 256) {
+            const int i = e % p, c = e / p;
+            cx<double> acc = {0.0, 0.0};
+            for (int s = i; s < p; s++) acc = cadd(acc, cmul(R[i][s], V[c][s]));
+            G[i + (int64_t)(c0 + c) * m] = {(T)acc.x, (T)acc.y};
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // driver
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -495,6 +665,13 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
     const int nfr = (int)pl.fronts.size();
     hipLaunchKernelGGL(k_rowperm, dim3((nfr + 63) / 64, nf), dim3(64), 0, st, dp.fronts, nfr, F.piv, pl.n,
                        F.rowperm);
+    hipLaunchKernelGGL(k_gsrc, dim3(nfr, nf), dim3(64), 0, st, dp.fronts, nfr, F.rowperm, pl.n, dp.gsrc_id, pl.fv_rows,
+                       F.gsrc);
+    {
+        ProfScope ps(K_DIAG_INV, st, 0.0, 2.0 * dp.diag_elems * sizeof(cx<T>) * nf);
+        hipLaunchKernelGGL(k_diag_inv<T>, dim3(nfr, nf), dim3(256), 0, st, F.arena, F.fstride, dp.fronts,
+                           dp.lv_single);
+    }
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -19,6 +19,7 @@
 // workgroup per front with the work vector in LDS.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "engine.hpp"
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(256) k_permute_in_slots(const cx<TIO> *__restr
 template <typename T, typename TIO>
 __global__ void __launch_bounds__(256) k_permute_out_slots(const cx<T> *__restrict__ Xp, cx<TIO> *__restrict__ X,
                                                            const int *__restrict__ iout, int64_t map_stride, int n,
-                                                           int nrhs, Sel sel) {
+                                                           int nrhs, int accumulate, Sel sel) {
     __shared__ cx<T> S[PT_R][PT_K + 1];
     const int ktiles = (n + PT_K - 1) / PT_K;
     const int k0 = (blockIdx.x % ktiles) * PT_K, r0 = (blockIdx.x / ktiles) * PT_R;
@@ -97,8 +98,10 @@ __global__ void __launch_bounds__(256) k_permute_out_slots(const cx<T> *__restri
     for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
         int kk = e % PT_K, rr = e / PT_K;
         if (k0 + kk < n && r0 + rr < nrhs) {
-            cx<T> v = S[rr][kk];
-            X[k0 + kk + (int64_t)(r0 + rr) * n] = {(TIO)v.x, (TIO)v.y};
+            const cx<T> v = S[rr][kk];
+            cx<TIO> &x = X[k0 + kk + (int64_t)(r0 + rr) * n];
+            if (accumulate) x = {x.x + (TIO)v.x, x.y + (TIO)v.y};
+            else x = {(TIO)v.x, (TIO)v.y};
         }
     }
 }
@@ -114,10 +117,11 @@ __global__ void __launch_bounds__(256) k_vec_in(const cx<TIO> *__restrict__ B, c
 }
 template <typename T, typename TIO>
 __global__ void __launch_bounds__(256) k_vec_out(const cx<T> *__restrict__ Xp, cx<TIO> *__restrict__ X,
-                                                 const int *__restrict__ iout, int n) {
+                                                 const int *__restrict__ iout, int n, int accumulate) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        cx<T> v = Xp[iout[k]];
-        X[k] = {(TIO)v.x, (TIO)v.y};
+        const cx<T> v = Xp[iout[k]];
+        if (accumulate) X[k] = {X[k].x + (TIO)v.x, X[k].y + (TIO)v.y};
+        else X[k] = {(TIO)v.x, (TIO)v.y};
     }
 }
 
@@ -177,9 +181,22 @@ constexpr size_t blk_smem_bytes() {
     return sizeof(T) * 4 * BK * 64 + sizeof(cx<T>) * BK * (BK + 1);
 }
 
-// acc = A[64 x kb] * Y[kb x 64] from LDS (4 waves in 2x2, 32x32 each)
+// element (i, t) of the operator that solves 32-row diagonal block b0 in the
+// solve format of the factors (diagonal blocks inverted in place): forward
+// sweeps apply a lower operator -- inv(L_bb) (unit diagonal) or, for
+// transposed solves, inv(U_bb)^T -- backward sweeps an upper one -- inv(U_bb)
+// or inv(L_bb)^T
 template <typename T>
-__device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, typename mfma_t<T>::acc (&cr)[2][2],
+__device__ __forceinline__ cx<T> dop(const cx<T> *G, int m, int b0, int i, int t, bool fwd, bool trans, bool cj) {
+    if (fwd ? t > i : t < i) return {T(0), T(0)};
+    if (i == t && fwd != trans) return {T(1), T(0)};
+    return trans ? ld(G + b0 + t + (int64_t)(b0 + i) * m, cj) : ld(G + b0 + i + (int64_t)(b0 + t) * m, cj);
+}
+
+// acc = A[64 x kb] * Y[kb x 64] from LDS (4 waves in 2x2, 32x32 each); column
+// sub-tiles at or beyond ncol (right-hand sides past the end) are skipped
+template <typename T>
+__device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, int ncol, typename mfma_t<T>::acc (&cr)[2][2],
                                          typename mfma_t<T>::acc (&ci)[2][2]) {
     typedef typename mfma_t<T>::acc acc_t;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = (w >> 1) * 32, wc = (w & 1) * 32;
@@ -201,8 +218,9 @@ __device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, typename m
             br[b] = s.Yr[k * 64 + wc + b * 16 + lr];
             bi[b] = s.Yi[k * 64 + wc + b * 16 + lr];
         }
-        for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) cmma<T>(ar[a], ai[a], br[b], bi[b], cr[a][b], ci[a][b]);
+        for (int b = 0; b < 2; b++)
+            if (wc + b * 16 < ncol)
+                for (int a = 0; a < 2; a++) cmma<T>(ar[a], ai[a], br[b], bi[b], cr[a][b], ci[a][b]);
     }
 }
 
@@ -224,8 +242,40 @@ __device__ __forceinline__ void sub_tile(cx<T> *w, int nrhs, int row0, int rowen
             }
 }
 
-// forward block b0 of every front of a level: y_b = Lop_bb^-1 w_b (written to
-// X by the row-tile-0 workgroup), then rows [b0+nb, m) of W -= Lop[:, b] y_b.
+// Y[0:nb] = D Y[0:nb] in LDS (D: the block operator, zero outside nb x nb):
+// waves own 16 rows x 32 right-hand sides each
+template <typename T>
+__device__ __forceinline__ void diag_apply(const BlkSmem<T> &s, int nb, int ncol) {
+    typedef typename mfma_t<T>::acc acc_t;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rw = (w >> 1) * 16, wc = (w & 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    acc_t cr[2], ci[2];
+    for (int b = 0; b < 2; b++) {
+        cr[b] = acc_t{0, 0, 0, 0};
+        ci[b] = acc_t{0, 0, 0, 0};
+    }
+    const int kend = (nb + 3) & ~3;
+    for (int kk = 0; kk < kend; kk += 4) {
+        const int k = kk + lk;
+        const cx<T> a = s.D[(rw + lr) * (BK + 1) + k];
+        for (int b = 0; b < 2; b++)
+            if (wc + b * 16 < ncol)
+                cmma<T>(a.x, a.y, s.Yr[k * 64 + wc + b * 16 + lr], s.Yi[k * 64 + wc + b * 16 + lr], cr[b], ci[b]);
+    }
+    __syncthreads();
+    for (int b = 0; b < 2; b++)
+        for (int v = 0; v < 4; v++) {
+            const int row = rw + mfma_t<T>::row(lk, v), col = wc + b * 16 + lr;
+            if (row < nb) {
+                s.Yr[row * 64 + col] = cr[b][v];
+                s.Yi[row * 64 + col] = ci[b][v];
+            }
+        }
+    __syncthreads();
+}
+
+// forward block b0 of every front of a level: y_b = Dop_bb w_b (written to X
+// by the row-tile-0 workgroup), then rows [b0+nb, m) of W -= Lop[:, b] y_b.
 // grid: (row tiles, fronts, right-hand-side tiles)
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena, int64_t fstride,
@@ -239,39 +289,28 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
     const int m = F.m, nb = min(BK, F.p - b0), tid = threadIdx.x;
     const int row0 = b0 + nb + blockIdx.x * 64;
     if (blockIdx.x > 0 && row0 >= m) return;
-    const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64;
+    const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64, ncol = min(64, nrhs - r0);
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
     auto Lop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
-    for (int e = tid; e < nb * nb; e += 256) {
-        int i = e % nb, t = e / nb;
-        s.D[i * (BK + 1) + t] = Lop(b0 + i, b0 + t);
+    for (int e = tid; e < BK * BK; e += 256) {
+        const int i = e % BK, t = e / BK;
+        s.D[i * (BK + 1) + t] = (i < nb && t < nb) ? dop(G, m, b0, i, t, true, trans, cj) : cx<T>{T(0), T(0)};
     }
     for (int e = tid; e < BK * 64; e += 256) {
         int k = e >> 6, j = e & 63;
         cx<T> v = {T(0), T(0)};
-        if (k < nb && r0 + j < nrhs) v = w[(int64_t)(b0 + k) * nrhs + r0 + j];
+        if (k < nb && j < ncol) v = w[(int64_t)(b0 + k) * nrhs + r0 + j];
         s.Yr[e] = v.x;
         s.Yi[e] = v.y;
     }
     __syncthreads();
-    if (tid < 64) {   // diagonal triangle, one right-hand side per thread
-        const int j = tid;
-        for (int k = 0; k < nb; k++) {
-            cx<T> acc = {s.Yr[k * 64 + j], s.Yi[k * 64 + j]};
-            for (int t = 0; t < k; t++)
-                acc = cmsub(acc, s.D[k * (BK + 1) + t], cx<T>{s.Yr[t * 64 + j], s.Yi[t * 64 + j]});
-            if (trans) acc = cdiv(acc, s.D[k * (BK + 1) + k]);
-            s.Yr[k * 64 + j] = acc.x;
-            s.Yi[k * 64 + j] = acc.y;
-        }
-    }
-    __syncthreads();
+    diag_apply<T>(s, nb, ncol);
     if (blockIdx.x == 0)
         for (int e = tid; e < nb * 64; e += 256) {
             int k = e >> 6, j = e & 63;
-            if (r0 + j < nrhs) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
+            if (j < ncol) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
         }
     if (row0 >= m) return;
     for (int e = tid; e < BK * 64; e += 256) {
@@ -285,7 +324,7 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
     }
     __syncthreads();
     typename mfma_t<T>::acc cr[2][2], ci[2][2];
-    mma_tile<T>(s, nb, cr, ci);
+    mma_tile<T>(s, nb, ncol, cr, ci);
     sub_tile<T>(w, nrhs, row0, m, r0, cr, ci);
 }
 
@@ -303,7 +342,7 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
     const int m = F.m, p = F.p, tid = threadIdx.x;
     const int row0 = blockIdx.x * 64;
     if (row0 >= p) return;
-    const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64;
+    const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64, ncol = min(64, nrhs - r0);
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
@@ -330,12 +369,12 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
         for (int e = tid; e < BK * 64; e += 256) {
             int k = e >> 6, j = e & 63;
             cx<T> v = {T(0), T(0)};
-            if (k < kb && r0 + j < nrhs) v = X[(int64_t)idx[F.idx_off + s0 + k] * nrhs + r0 + j];
+            if (k < kb && j < ncol) v = X[(int64_t)idx[F.idx_off + s0 + k] * nrhs + r0 + j];
             s.Yr[e] = v.x;
             s.Yi[e] = v.y;
         }
         __syncthreads();
-        mma_tile<T>(s, kb, tr, ti);
+        mma_tile<T>(s, kb, ncol, tr, ti);
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) {
                 cr[a][b] += tr[a][b];
@@ -345,14 +384,14 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
     // w[0:p] = y1 (from X) - accumulated product
     for (int e = tid; e < 64 * 64; e += 256) {
         int i = e >> 6, j = e & 63;
-        if (row0 + i < p && r0 + j < nrhs)
+        if (row0 + i < p && j < ncol)
             w[(int64_t)(row0 + i) * nrhs + r0 + j] = X[(int64_t)(F.pos0 + row0 + i) * nrhs + r0 + j];
     }
     __syncthreads();
     sub_tile<T>(w, nrhs, row0, p, r0, cr, ci);
 }
 
-// backward block b0 (blocks from the bottom): x_b = Uop_bb^-1 z_b (to X by the
+// backward block b0 (blocks from the bottom): x_b = Dop_bb z_b (to X by the
 // row-tile-0 workgroup), then rows [0, b0) of W -= Uop[:, b] x_b
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena, int64_t fstride,
@@ -366,39 +405,28 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
     const int m = F.m, nb = min(BK, F.p - b0), tid = threadIdx.x;
     const int row0 = blockIdx.x * 64;
     if (blockIdx.x > 0 && row0 >= b0) return;
-    const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64;
+    const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64, ncol = min(64, nrhs - r0);
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
     auto Uop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
-    for (int e = tid; e < nb * nb; e += 256) {
-        int i = e % nb, t = e / nb;
-        s.D[i * (BK + 1) + t] = Uop(b0 + i, b0 + t);
+    for (int e = tid; e < BK * BK; e += 256) {
+        const int i = e % BK, t = e / BK;
+        s.D[i * (BK + 1) + t] = (i < nb && t < nb) ? dop(G, m, b0, i, t, false, trans, cj) : cx<T>{T(0), T(0)};
     }
     for (int e = tid; e < BK * 64; e += 256) {
         int k = e >> 6, j = e & 63;
         cx<T> v = {T(0), T(0)};
-        if (k < nb && r0 + j < nrhs) v = w[(int64_t)(b0 + k) * nrhs + r0 + j];
+        if (k < nb && j < ncol) v = w[(int64_t)(b0 + k) * nrhs + r0 + j];
         s.Yr[e] = v.x;
         s.Yi[e] = v.y;
     }
     __syncthreads();
-    if (tid < 64) {
-        const int j = tid;
-        for (int k = nb - 1; k >= 0; k--) {
-            cx<T> acc = {s.Yr[k * 64 + j], s.Yi[k * 64 + j]};
-            for (int t = k + 1; t < nb; t++)
-                acc = cmsub(acc, s.D[k * (BK + 1) + t], cx<T>{s.Yr[t * 64 + j], s.Yi[t * 64 + j]});
-            if (!trans) acc = cdiv(acc, s.D[k * (BK + 1) + k]);
-            s.Yr[k * 64 + j] = acc.x;
-            s.Yi[k * 64 + j] = acc.y;
-        }
-    }
-    __syncthreads();
+    diag_apply<T>(s, nb, ncol);
     if (blockIdx.x == 0)
         for (int e = tid; e < nb * 64; e += 256) {
             int k = e >> 6, j = e & 63;
-            if (r0 + j < nrhs) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
+            if (j < ncol) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
         }
     if (row0 >= b0) return;
     for (int e = tid; e < BK * 64; e += 256) {
@@ -412,8 +440,763 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
     }
     __syncthreads();
     typename mfma_t<T>::acc cr[2][2], ci[2][2];
-    mma_tile<T>(s, nb, cr, ci);
+    mma_tile<T>(s, nb, ncol, cr, ci);
     sub_tile<T>(w, nrhs, row0, b0, r0, cr, ci);
+}
+
+// ---- fused per-front substitutions (fronts whose panel fits in LDS) ---------
+// One workgroup owns one front and RW right-hand sides of one frequency and
+// keeps the front's m x RW panel in LDS (re/im planes, row stride RW + 16 so the
+// four k-rows of one MFMA operand read fall in different banks).  It gathers
+// the panel, runs every 32-pivot block -- the inverted diagonal block, then the
+// block column's update of the rows below, both as MFMA products with the
+// factor streamed from global memory as the A operand (waves own 16-row
+// strips) -- and writes the panel back: one launch per level and sweep.
+template <typename T, int RW>
+constexpr size_t fused_row_bytes() {
+    return 2 * sizeof(T) * (RW + 16) + 4 * sizeof(int);     // panel row + the row's gather sources
+}
+
+// The dispatcher hands consecutive workgroup ids to the 8 XCDs round-robin;
+// with gridDim.x a multiple of 8, blockIdx.x mod 8 picks the XCD.  Keeping
+// front = (group, blockIdx.x mod 8) puts all right-hand-side tiles of a front on
+// one XCD, so its factor entries come from HBM once and from that L2 after.
+__device__ __forceinline__ void xcd_front_tile(int bx, int rt, int &fi, int &tile) {
+    const int grp = bx / (8 * rt), rem = bx - grp * 8 * rt;
+    tile = rem >> 3;
+    fi = grp * 8 + (rem & 7);
+}
+
+// Operand A of a strip product: A(i, k) = A[i * ars + k * acs] (conjugated if
+// cj) for i < arows, k < K, zero elsewhere.  Masks in absolute indices
+// (row ioff + i, column koff + k): tri 1 keeps columns <= row (lower), tri 2
+// columns >= row (upper); unit takes the diagonal as 1; columns >= kneg are
+// negated.
+struct AOp {
+    int64_t ars, acs;
+    int arows, K, ioff, koff, tri, kneg;
+    bool unit, cj;
+};
+
+// the 8 A values of lane (lr, lk) for k = k0 + 4s + lk, s < 8
+template <typename T>
+__device__ __forceinline__ void load_a(const cx<T> *__restrict__ A, const AOp &o, int k0, cx<T> (&av)[8]) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4, di = o.ioff + lr;
+    const bool rok = lr < o.arows;
+    const cx<T> *arow = A + lr * o.ars;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int k = k0 + 4 * s + lk, ka = o.koff + k;
+        cx<T> v = {T(0), T(0)};
+        if (rok && k < o.K) {
+            v = arow[k * o.acs];
+            if (o.cj) v.y = -v.y;
+            if ((o.tri == 1 && ka > di) || (o.tri == 2 && ka < di)) v = {T(0), T(0)};
+            if (o.unit && ka == di) v = {T(1), T(0)};
+            if (ka >= o.kneg) v = {-v.x, -v.y};
+        }
+        av[s] = v;
+    }
+}
+
+// acc[c] += A(16 x [k0, k0+32)) * P[prow0 + k][c*16 .. c*16 + 16) with A from av
+template <typename T, int NC>
+__device__ __forceinline__ void mma_a(const cx<T> (&av)[8], int k0, int K, const T *Pr, const T *Pi, int prow0, int rs,
+                                      typename mfma_t<T>::acc (&cr)[NC], typename mfma_t<T>::acc (&ci)[NC]) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        if (k0 + 4 * s >= K) break;
+        const int k = k0 + 4 * s + lk;
+        const bool kin = k < K;
+        const int off = (prow0 + k) * rs + lr;
+        // every column sub-tile, also past the right-hand sides (zero panel
+        // columns): a per-sub-tile branch costs far more registers than the MFMAs
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const T br = kin ? Pr[off + c * 16] : T(0), bi = kin ? Pi[off + c * 16] : T(0);
+            cmma<T>(av[s].x, av[s].y, br, bi, cr[c], ci[c]);
+        }
+    }
+}
+
+// acc[c] += A(16 x K) * P[prow0 .. prow0 + K)[c*16 .. c*16 + 16), K in chunks of
+// 32; av holds chunk 0 on entry (loaded by the caller), and each next chunk is
+// requested before the current one's MFMAs
+template <typename T, int NC>
+__device__ __forceinline__ void strip_mma_from(cx<T> (&av)[8], const cx<T> *__restrict__ A, const AOp &o,
+                                               const T *Pr, const T *Pi, int prow0, int rs,
+                                               typename mfma_t<T>::acc (&cr)[NC], typename mfma_t<T>::acc (&ci)[NC]) {
+    cx<T> an[8];
+    for (int k0 = 0; k0 < o.K; k0 += 32) {
+        if (k0 + 32 < o.K) load_a<T>(A, o, k0 + 32, an);
+        mma_a<T, NC>(av, k0, o.K, Pr, Pi, prow0, rs, cr, ci);
+#pragma unroll
+        for (int s = 0; s < 8; s++) av[s] = an[s];
+    }
+}
+
+template <typename T, int NC>
+__device__ __forceinline__ void strip_mma(const cx<T> *__restrict__ A, const AOp &o, const T *Pr, const T *Pi,
+                                          int prow0, int rs, typename mfma_t<T>::acc (&cr)[NC],
+                                          typename mfma_t<T>::acc (&ci)[NC]) {
+    cx<T> av[8];
+    load_a<T>(A, o, 0, av);
+    strip_mma_from<T, NC>(av, A, o, Pr, Pi, prow0, rs, cr, ci);
+}
+
+template <typename T, int NC>
+__device__ __forceinline__ void acc_zero(typename mfma_t<T>::acc (&cr)[NC], typename mfma_t<T>::acc (&ci)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        cr[c] = typename mfma_t<T>::acc{0, 0, 0, 0};
+        ci[c] = typename mfma_t<T>::acc{0, 0, 0, 0};
+    }
+}
+
+// P rows [row0 + 0..16) (those below rowend) op= acc; sub: -=, else =.  All
+// columns are stored: those past the right-hand sides hold zeros throughout.
+template <typename T, int NC>
+__device__ __forceinline__ void acc_store(T *Pr, T *Pi, int rs, int row0, int rowend, bool sub,
+                                          typename mfma_t<T>::acc (&cr)[NC], typename mfma_t<T>::acc (&ci)[NC]) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = row0 + mfma_t<T>::row(lk, v);
+            if (row < rowend) {
+                const int o = row * rs + c * 16 + lr;
+                if (sub) {
+                    Pr[o] -= cr[c][v];
+                    Pi[o] -= ci[c][v];
+                } else {
+                    Pr[o] = cr[c][v];
+                    Pi[o] = ci[c][v];
+                }
+            }
+        }
+    }
+}
+
+// Panel gather of a forward front into LDS: row s of the assembled vector --
+// its X row plus the contributions of up to two children (nested-dissection
+// fronts have at most two) -- lands at row i = P s.  The rows' sources are
+// resolved first (one thread per row), then all element loads of a batch are
+// issued before any is stored.  Ends with a barrier.
+template <typename T, int RW>
+__device__ __forceinline__ void gather_fwd(const FrontDev &F, const int *__restrict__ rp,
+                                           const FrontDev *__restrict__ fronts, const int *__restrict__ children,
+                                           const int *__restrict__ inv, const cx<T> *__restrict__ X,
+                                           const cx<T> *__restrict__ W, int nrhs, int r0, int ncol, T *Pr, T *Pi) {
+    constexpr int RS = RW + 16, GB = 8;
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    int *sx = reinterpret_cast<int *>(Pi + m * RS), *sw0 = sx + m, *sw1 = sw0 + m;
+    for (int i = tid; i < m; i += 256) {
+        const int src = (rp && i < p) ? rp[F.pos0 + i] : i;
+        int w0 = -1, w1 = -1;
+        for (int c = 0; c < F.nchild; c++) {
+            const FrontDev C = fronts[children[F.child_off + c]];
+            const int a = inv[C.inv_off + src];
+            if (a >= 0) {
+                const int r = (int)(C.fv_off + C.p + a);
+                if (w0 < 0) w0 = r;
+                else w1 = r;
+            }
+        }
+        sx[i] = src < p ? F.pos0 + src : -1;
+        sw0[i] = w0;
+        sw1[i] = w1;
+    }
+    __syncthreads();
+    const int total = m * RW;
+    for (int e0 = 0; e0 < total; e0 += 256 * GB) {
+        cx<T> v[GB];
+#pragma unroll
+        for (int u = 0; u < GB; u++) {
+            const int e = e0 + u * 256 + tid, i = e / RW, j = e - i * RW;
+            v[u] = {T(0), T(0)};
+            if (e < total && j < ncol) {
+                const int a = sx[i], b = sw0[i], c = sw1[i];
+                if (a >= 0) v[u] = X[(int64_t)a * nrhs + r0 + j];
+                if (b >= 0) v[u] = cadd(v[u], W[(int64_t)b * nrhs + r0 + j]);
+                if (c >= 0) v[u] = cadd(v[u], W[(int64_t)c * nrhs + r0 + j]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GB; u++) {
+            const int e = e0 + u * 256 + tid, i = e / RW, j = e - i * RW;
+            if (e < total) {
+                Pr[i * RS + j] = v[u].x;
+                Pi[i * RS + j] = v[u].y;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Panel gather of a backward front: pivot rows and the ancestors' rows of X.
+// Ends with a barrier.
+template <typename T, int RW>
+__device__ __forceinline__ void gather_bwd(const FrontDev &F, const int *__restrict__ idx,
+                                           const cx<T> *__restrict__ X, int nrhs, int r0, int ncol, T *Pr, T *Pi) {
+    constexpr int RS = RW + 16, GB = 8;
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    int *srow = reinterpret_cast<int *>(Pi + m * RS);
+    for (int i = tid; i < m; i += 256) srow[i] = i < p ? F.pos0 + i : idx[F.idx_off + i];
+    __syncthreads();
+    const int total = m * RW;
+    for (int e0 = 0; e0 < total; e0 += 256 * GB) {
+        cx<T> v[GB];
+#pragma unroll
+        for (int u = 0; u < GB; u++) {
+            const int e = e0 + u * 256 + tid, i = e / RW, j = e - i * RW;
+            v[u] = {T(0), T(0)};
+            if (e < total && j < ncol) v[u] = X[(int64_t)srow[i] * nrhs + r0 + j];
+        }
+#pragma unroll
+        for (int u = 0; u < GB; u++) {
+            const int e = e0 + u * 256 + tid, i = e / RW, j = e - i * RW;
+            if (e < total) {
+                Pr[i * RS + j] = v[u].x;
+                Pi[i * RS + j] = v[u].y;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// forward sweep of a level's fronts.  grid: (ceil(fronts/8)*8 * rt, frequency slots)
+template <typename T, int RW>
+__global__ void __launch_bounds__(256) k_fwd_front(const cx<T> *__restrict__ arena, int64_t fstride,
+                                                   const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                   int nfr, const int *__restrict__ children,
+                                                   const int *__restrict__ inv, const int *__restrict__ rowperm,
+                                                   cx<T> *__restrict__ X, cx<T> *__restrict__ W, int n,
+                                                   int64_t wstride, int nrhs, int rt, int trans, int cj, Sel sel) {
+    constexpr int RS = RW + 16, NC = RW / 16;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int fi, tile;
+    xcd_front_tile(blockIdx.x, rt, fi, tile);
+    if (fi >= nfr) return;
+    const FrontDev F = fronts[list[fi]];
+    const int z = blockIdx.y, r0 = tile * RW, ncol = min(RW, nrhs - r0);
+    const int m = F.m, p = F.p, tid = threadIdx.x, wave = tid >> 6;
+    T *Pr = reinterpret_cast<T *>(smem_raw), *Pi = Pr + m * RS;
+    const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
+    X += (int64_t)sel.s[z] * n * nrhs;
+    W += sel.s[z] * wstride;
+    gather_fwd<T, RW>(F, rowperm ? rowperm + (int64_t)sel.f[z] * n : nullptr, fronts, children, inv, X, W, nrhs, r0,
+                      ncol, Pr, Pi);
+    // op(L) entry (i, t) sits at G[i * ars + t * acs]
+    const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
+    const bool unit = !trans;
+    typename mfma_t<T>::acc cr[NC], ci[NC];
+    for (int b0 = 0; b0 < p; b0 += BK) {
+        const int kb = min(BK, p - b0), sr = wave * 16;
+        const bool act = wave < 2 && sr < kb;
+        if (act) {
+            acc_zero<T, NC>(cr, ci);
+            strip_mma<T, NC>(G + (b0 + sr) * ars + b0 * acs, AOp{ars, acs, kb - sr, kb, sr, 0, 1, 1 << 30, unit, (bool)cj},
+                             Pr, Pi, b0, RS, cr, ci);
+        }
+        __syncthreads();
+        if (act) acc_store<T, NC>(Pr, Pi, RS, b0 + sr, b0 + kb, false, cr, ci);
+        __syncthreads();
+        // trailing strips: each next strip's factor entries are requested
+        // before the current strip's MFMAs
+        AOp ot{ars, acs, 0, kb, 0, 0, 0, 1 << 30, false, (bool)cj};
+        cx<T> at[8], an[8];
+        int s0 = b0 + kb + sr;
+        if (s0 < m) {
+            ot.arows = m - s0;
+            load_a<T>(G + s0 * ars + b0 * acs, ot, 0, at);
+        }
+        for (; s0 < m; s0 += 64) {
+            const int s1 = s0 + 64;
+            if (s1 < m) {
+                ot.arows = m - s1;
+                load_a<T>(G + s1 * ars + b0 * acs, ot, 0, an);
+            }
+            acc_zero<T, NC>(cr, ci);
+            mma_a<T, NC>(at, 0, kb, Pr, Pi, b0, RS, cr, ci);
+            acc_store<T, NC>(Pr, Pi, RS, s0, m, true, cr, ci);
+#pragma unroll
+            for (int s = 0; s < 8; s++) at[s] = an[s];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < m * RW; e += 256) {
+        const int i = e / RW, j = e - i * RW;
+        if (j >= ncol) continue;
+        const cx<T> v = {Pr[i * RS + j], Pi[i * RS + j]};
+        if (i < p) X[(int64_t)(F.pos0 + i) * nrhs + r0 + j] = v;
+        else W[(F.fv_off + i) * nrhs + r0 + j] = v;
+    }
+}
+
+// ---- single-block levels: one product per front and sweep, one wave per task --
+// Fronts with p <= 32 are stored as [inv(L11); L21 inv(L11)] (m x p) and
+// [inv(U11), inv(U11) U12] (p x m).  A task is one front, one frequency and 16
+// right-hand sides, and belongs to one wave: no barrier with the other waves
+// of the workgroup, which take the next 16-column tiles of the same front (so
+// they meet the same factor entries in the caches).  Small LDS and register
+// footprints keep many tasks resident per CU, which is what hides the memory
+// latency of these small fronts.
+constexpr int WT = 16;
+
+// orders the wave's own LDS writes before its later LDS reads by other lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward: out = Lop w1 over all m rows (K = p); rows < p are y (to X), rows
+// >= p are subtracted from the gathered contribution rows (to W).  LDS per
+// wave: the p x 16 pivot rows (B operand) and the m rows' gather sources.
+// grid: (ceil(fronts/8)*8 * tile groups, frequency slots), block 64 * waves
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_front1(const cx<T> *__restrict__ arena, int64_t fstride,
+                                                    const FrontDev *__restrict__ lvF, int nfr,
+                                                    const int *__restrict__ gsrc, int64_t gstride,
+                                                    cx<T> *__restrict__ X, cx<T> *__restrict__ W, int n,
+                                                    int64_t wstride, int nrhs, int rtg, int wave_lds, int trans, int cj,
+                                                    Sel sel) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    int fi, tg;
+    xcd_front_tile(blockIdx.x, rtg, fi, tg);
+    const int r0 = (tg * (int)(blockDim.x >> 6) + wave) * WT;
+    if (fi >= nfr || r0 >= nrhs) return;
+    const int z = blockIdx.y, ncol = min(WT, nrhs - r0);
+    const FrontDev F = lvF[fi];
+    const int m = F.m, p = F.p;
+    T *Br = reinterpret_cast<T *>(smem_raw + wave * wave_lds), *Bi = Br + p * WT;
+    int *src = reinterpret_cast<int *>(Bi + p * WT);
+    const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
+    X += (int64_t)sel.s[z] * n * nrhs;
+    W += sel.s[z] * wstride;
+    const int *gs = gsrc + sel.f[z] * gstride + F.fv_off * 3;
+    const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
+    const AOp op{ars, acs, m, p, 0, 0, 1, 1 << 30, !trans, (bool)cj};
+    // strip 0's factor entries and the rows' gather sources in flight together
+    cx<T> av[8];
+    load_a<T>(G, op, 0, av);
+    for (int e = lane; e < 3 * m; e += 64) src[e] = gs[e];
+    wave_sync();
+    {   // pivot rows (p * 16 <= 512 elements: at most 8 per lane), all loads before the stores
+        cx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = lane + 64 * u, i = e >> 4, j = e & 15;
+            v[u] = {T(0), T(0)};
+            if (i < p && j < ncol) {
+                const int a = src[3 * i], b = src[3 * i + 1], c = src[3 * i + 2];
+                if (a >= 0) v[u] = X[(int64_t)a * nrhs + r0 + j];
+                if (b >= 0) v[u] = cadd(v[u], W[(int64_t)b * nrhs + r0 + j]);
+                if (c >= 0) v[u] = cadd(v[u], W[(int64_t)c * nrhs + r0 + j]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = lane + 64 * u, i = e >> 4, j = e & 15;
+            if (i < p) {
+                Br[i * WT + j] = v[u].x;
+                Bi[i * WT + j] = v[u].y;
+            }
+        }
+    }
+    wave_sync();
+    // the lane's four accumulator rows of a strip: contribution-row values
+    auto load_w2 = [&](int s0, cx<T> (&w)[4]) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = s0 + mfma_t<T>::row(lk, v);
+            w[v] = {T(0), T(0)};
+            if (row >= p && row < m && lr < ncol) {
+                const int b = src[3 * row + 1], c = src[3 * row + 2];
+                if (b >= 0) w[v] = W[(int64_t)b * nrhs + r0 + lr];
+                if (c >= 0) w[v] = cadd(w[v], W[(int64_t)c * nrhs + r0 + lr]);
+            }
+        }
+    };
+    cx<T> w2[4];
+    typename mfma_t<T>::acc cr[1], ci[1];
+    for (int s0 = 0; s0 < m; s0 += 16) {
+        // (strip 0's factor entries were requested before the gather)
+        if (s0 > 0) {
+            AOp o = op;
+            o.arows = m - s0;
+            o.ioff = s0;
+            load_a<T>(G + s0 * ars, o, 0, av);
+        }
+        load_w2(s0, w2);
+        acc_zero<T, 1>(cr, ci);
+        mma_a<T, 1>(av, 0, p, Br, Bi, 0, WT, cr, ci);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = s0 + mfma_t<T>::row(lk, v);
+            if (row < m && lr < ncol) {
+                if (row < p) X[(int64_t)(F.pos0 + row) * nrhs + r0 + lr] = {cr[0][v], ci[0][v]};
+                else W[(F.fv_off + row) * nrhs + r0 + lr] = {w2[v].x - cr[0][v], w2[v].y - ci[0][v]};
+            }
+        }
+    }
+}
+
+// Backward: x1 = [Uop_bb, -Uop12] [y1; x2] for the p <= 32 pivot rows (two
+// 16-row strips, K = m in chunks of 32).  LDS per wave: the m x 16 panel and its rows' sources.
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ arena, int64_t fstride,
+                                                    const FrontDev *__restrict__ lvF, int nfr,
+                                                    const int *__restrict__ idx, const int *__restrict__ rowperm,
+                                                    cx<T> *__restrict__ X, int n, int nrhs, int rtg, int wave_lds,
+                                                    int trans, int cj, Sel sel) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    int fi, tg;
+    xcd_front_tile(blockIdx.x, rtg, fi, tg);
+    const int r0 = (tg * (int)(blockDim.x >> 6) + wave) * WT;
+    if (fi >= nfr || r0 >= nrhs) return;
+    const int z = blockIdx.y, ncol = min(WT, nrhs - r0);
+    const FrontDev F = lvF[fi];
+    const int m = F.m, p = F.p;
+    T *Pr = reinterpret_cast<T *>(smem_raw + wave * wave_lds), *Pi = Pr + m * WT;
+    int *srow = reinterpret_cast<int *>(Pi + m * WT);
+    const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
+    X += (int64_t)sel.s[z] * n * nrhs;
+    const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
+    const bool two = p > 16;
+    const AOp o0{ars, acs, p, m, 0, 0, 2, p, (bool)trans, (bool)cj};
+    const AOp o1{ars, acs, p - 16, m, 16, 0, 2, p, (bool)trans, (bool)cj};
+    cx<T> a0[8], a1[8];
+    load_a<T>(G, o0, 0, a0);
+    if (two) load_a<T>(G + 16 * ars, o1, 0, a1);
+    for (int i = lane; i < m; i += 64) srow[i] = i < p ? F.pos0 + i : idx[F.idx_off + i];
+    int dst[8];   // output positions of the lane's accumulator rows
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+        const int row = (v >> 2) * 16 + mfma_t<T>::row(lk, v & 3);
+        dst[v] = row < p ? F.pos0 + (trans ? rowperm[(int64_t)sel.f[z] * n + F.pos0 + row] : row) : -1;
+    }
+    wave_sync();
+    for (int e0 = 0; e0 < m * WT; e0 += 64 * 8) {
+        cx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + lane + 64 * u, i = e >> 4, j = e & 15;
+            v[u] = {T(0), T(0)};
+            if (i < m && j < ncol) v[u] = X[(int64_t)srow[i] * nrhs + r0 + j];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + lane + 64 * u, i = e >> 4, j = e & 15;
+            if (i < m) {
+                Pr[i * WT + j] = v[u].x;
+                Pi[i * WT + j] = v[u].y;
+            }
+        }
+    }
+    wave_sync();
+    typename mfma_t<T>::acc c0r[1], c0i[1], c1r[1], c1i[1];
+    acc_zero<T, 1>(c0r, c0i);
+    acc_zero<T, 1>(c1r, c1i);
+    for (int k0 = 0; k0 < m; k0 += 32) {
+        if (k0 > 0) {   // (chunk 0 was requested before the gather)
+            load_a<T>(G, o0, k0, a0);
+            if (two) load_a<T>(G + 16 * ars, o1, k0, a1);
+        }
+        mma_a<T, 1>(a0, k0, m, Pr, Pi, 0, WT, c0r, c0i);
+        if (two) mma_a<T, 1>(a1, k0, m, Pr, Pi, 0, WT, c1r, c1i);
+    }
+    if (lr < ncol) {
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            if (dst[v] < 0) continue;
+            const T xr = v < 4 ? c0r[0][v] : c1r[0][v - 4], xi = v < 4 ? c0i[0][v] : c1i[0][v - 4];
+            X[(int64_t)dst[v] * nrhs + r0 + lr] = {xr, xi};
+        }
+    }
+}
+
+// ---- single-block levels with small fronts (m <= 64): staged operator -------
+// The workgroup (up to 8 waves = 8 consecutive 16-column tiles of one front)
+// stages the front's whole operator and its rows' gather sources in LDS, with
+// all loads in flight at once; then every wave loads its right-hand-side
+// operand straight from global memory into registers in MFMA layout (k = 4s +
+// lane/16, column lane%16: 16 lanes read one 128-byte row segment) and runs all
+// its strips with the operator from LDS.  About three memory latencies per
+// task, and the only LDS is the operator, so many tasks stay resident.
+// Operator element (i, k) sits at A[k * as + i].
+constexpr int kSmallM = 64;
+
+// operator row stride: rows rounded to 16, then to 16 or 48 mod 64, so the four
+// k-rows of one operand read start in different banks
+__host__ __device__ __forceinline__ int small_stride(int rows) {
+    int as = (rows + 15) & ~15;
+    if ((as & 63) == 0 || (as & 63) == 32) as += 16;
+    return as;
+}
+
+// Forward.  CH: the level has fronts with children (contribution rows to add);
+// the leaves' level has none and keeps no registers for them.
+// grid: (ceil(fronts/8)*8 * tile groups, frequency slots)
+template <typename T, bool CH>
+__global__ void __launch_bounds__(512) k_fwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
+                                                    const FrontDev *__restrict__ lvF, int nfr,
+                                                    const int *__restrict__ gsrc, int64_t gstride,
+                                                    cx<T> *__restrict__ X, cx<T> *__restrict__ W, int n,
+                                                    int64_t wstride, int nrhs, int rtg, int trans, int cj, Sel sel) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    const int nth = blockDim.x;
+    int fi, tg;
+    xcd_front_tile(blockIdx.x, rtg, fi, tg);
+    if (fi >= nfr) return;
+    const int z = blockIdx.y, r0 = (tg * (nth >> 6) + wave) * WT, ncol = min(WT, nrhs - r0);
+    const FrontDev F = lvF[fi];
+    const int m = F.m, p = F.p, as = small_stride(m);
+    T *Ar = reinterpret_cast<T *>(smem_raw), *Ai = Ar + p * as;
+    int *src = reinterpret_cast<int *>(Ai + p * as);
+    const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
+    X += (int64_t)sel.s[z] * n * nrhs;
+    W += sel.s[z] * wstride;
+    const int *gs = gsrc + sel.f[z] * gstride + F.fv_off * 3;
+    const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
+    // stage the operator [inv(L11); L21 inv(L11)] (or its transposed-solve form) and the sources
+    for (int e0 = 0; e0 < m * p; e0 += 8 * nth) {
+        cx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * nth + threadIdx.x, k = e / m, i = e - k * m;
+            v[u] = {T(0), T(0)};
+            if (e < m * p) {
+                v[u] = G[i * ars + k * acs];
+                if (cj) v[u].y = -v[u].y;
+                if (i < p && k > i) v[u] = {T(0), T(0)};
+                if (i == k && !trans) v[u] = {T(1), T(0)};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * nth + threadIdx.x, k = e / m, i = e - k * m;
+            if (e < m * p) {
+                Ar[k * as + i] = v[u].x;
+                Ai[k * as + i] = v[u].y;
+            }
+        }
+    }
+    for (int e = threadIdx.x; e < 3 * m; e += nth) src[e] = gs[e];
+    __syncthreads();
+    if (r0 >= nrhs) return;
+    const bool cin = lr < ncol;
+    // pivot rows in operand layout (k = 4s + lk), and the contribution rows the
+    // lane's accumulators will meet in the epilogue (rows >= p; m <= 64: 4 strips)
+    cx<T> b[8], w2[CH ? 16 : 1];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int k = 4 * s + lk;
+        b[s] = {T(0), T(0)};
+        if (k < p && cin) {
+            const int a = src[3 * k];
+            if (a >= 0) b[s] = X[(int64_t)a * nrhs + r0 + lr];
+            if (CH) {
+                const int u = src[3 * k + 1], w = src[3 * k + 2];
+                if (u >= 0) b[s] = cadd(b[s], W[(int64_t)u * nrhs + r0 + lr]);
+                if (w >= 0) b[s] = cadd(b[s], W[(int64_t)w * nrhs + r0 + lr]);
+            }
+        }
+    }
+    if (CH)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 16 * q + mfma_t<T>::row(lk, v);
+                cx<T> &t = w2[CH ? 4 * q + v : 0];
+                t = {T(0), T(0)};
+                if (row >= p && row < m && cin) {
+                    const int u = src[3 * row + 1], w = src[3 * row + 2];
+                    if (u >= 0) t = W[(int64_t)u * nrhs + r0 + lr];
+                    if (w >= 0) t = cadd(t, W[(int64_t)w * nrhs + r0 + lr]);
+                }
+            }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (16 * q >= m) break;
+        typename mfma_t<T>::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            if (4 * s >= p) break;
+            const int k = 4 * s + lk;
+            const T ar = k < p ? Ar[k * as + 16 * q + lr] : T(0), ai = k < p ? Ai[k * as + 16 * q + lr] : T(0);
+            cmma<T>(ar, ai, b[s].x, b[s].y, cr, ci);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * q + mfma_t<T>::row(lk, v);
+            if (row < m && cin) {
+                if (row < p) X[(int64_t)(F.pos0 + row) * nrhs + r0 + lr] = {cr[v], ci[v]};
+                else if (CH)
+                    W[(F.fv_off + row) * nrhs + r0 + lr] = {w2[CH ? 4 * q + v : 0].x - cr[v],
+                                                            w2[CH ? 4 * q + v : 0].y - ci[v]};
+                else W[(F.fv_off + row) * nrhs + r0 + lr] = {-cr[v], -ci[v]};
+            }
+        }
+    }
+}
+
+// Backward.  Operator [p][m] staged as A[k * as + i] with as = p rounded to 16;
+// right-hand-side operand: all m rows (K = m <= 64, 16 per lane) in registers.
+template <typename T>
+__global__ void __launch_bounds__(512) k_bwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
+                                                    const FrontDev *__restrict__ lvF, int nfr,
+                                                    const int *__restrict__ idx, const int *__restrict__ rowperm,
+                                                    cx<T> *__restrict__ X, int n, int nrhs, int rtg, int trans, int cj,
+                                                    Sel sel) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    const int nth = blockDim.x;
+    int fi, tg;
+    xcd_front_tile(blockIdx.x, rtg, fi, tg);
+    if (fi >= nfr) return;
+    const int z = blockIdx.y, r0 = (tg * (nth >> 6) + wave) * WT, ncol = min(WT, nrhs - r0);
+    const FrontDev F = lvF[fi];
+    const int m = F.m, p = F.p, as = (p + 15) & ~15;
+    T *Ar = reinterpret_cast<T *>(smem_raw), *Ai = Ar + m * as;
+    int *srow = reinterpret_cast<int *>(Ai + m * as);
+    int *dstl = srow + kSmallM;
+    const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
+    X += (int64_t)sel.s[z] * n * nrhs;
+    const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
+    // stage [inv(U11), -inv(U11) U12] (or its transposed-solve form), rows, output positions
+    for (int e0 = 0; e0 < m * p; e0 += 8 * nth) {
+        cx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * nth + threadIdx.x, k = e / p, i = e - k * p;
+            v[u] = {T(0), T(0)};
+            if (e < m * p) {
+                v[u] = G[i * ars + k * acs];
+                if (cj) v[u].y = -v[u].y;
+                if (k < p && k < i) v[u] = {T(0), T(0)};
+                if (i == k && trans) v[u] = {T(1), T(0)};
+                if (k >= p) v[u] = {-v[u].x, -v[u].y};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * nth + threadIdx.x, k = e / p, i = e - k * p;
+            if (e < m * p) {
+                Ar[k * as + i] = v[u].x;
+                Ai[k * as + i] = v[u].y;
+            }
+        }
+    }
+    for (int i = threadIdx.x; i < m; i += nth) srow[i] = i < p ? F.pos0 + i : idx[F.idx_off + i];
+    for (int i = threadIdx.x; i < p; i += nth)
+        dstl[i] = F.pos0 + (trans ? rowperm[(int64_t)sel.f[z] * n + F.pos0 + i] : i);
+    __syncthreads();
+    if (r0 >= nrhs) return;
+    const bool cin = lr < ncol;
+    cx<T> b[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const int k = 4 * s + lk;
+        b[s] = {T(0), T(0)};
+        if (k < m && cin) b[s] = X[(int64_t)srow[k] * nrhs + r0 + lr];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        if (16 * q >= p) break;
+        typename mfma_t<T>::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            if (4 * s >= m) break;
+            const int k = 4 * s + lk;
+            const T ar = k < m ? Ar[k * as + 16 * q + lr] : T(0), ai = k < m ? Ai[k * as + 16 * q + lr] : T(0);
+            cmma<T>(ar, ai, b[s].x, b[s].y, cr, ci);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * q + mfma_t<T>::row(lk, v);
+            if (row < p && cin) X[(int64_t)dstl[row] * nrhs + r0 + lr] = {cr[v], ci[v]};
+        }
+    }
+}
+
+// backward sweep of a level's fronts: z = y1 - Uop12 x2 (x2 gathered from the
+// ancestors' rows of X), blocks from the bottom, x written to the pivot rows
+// (for transposed solves through P_f^T: the descendants' L21 rows are in this
+// front's original row order).  grid as k_fwd_front.
+template <typename T, int RW>
+__global__ void __launch_bounds__(256) k_bwd_front(const cx<T> *__restrict__ arena, int64_t fstride,
+                                                   const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                   int nfr, const int *__restrict__ idx,
+                                                   const int *__restrict__ rowperm, cx<T> *__restrict__ X, int n,
+                                                   int nrhs, int rt, int trans, int cj, Sel sel) {
+    constexpr int RS = RW + 16, NC = RW / 16;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int fi, tile;
+    xcd_front_tile(blockIdx.x, rt, fi, tile);
+    if (fi >= nfr) return;
+    const FrontDev F = fronts[list[fi]];
+    const int z = blockIdx.y, r0 = tile * RW, ncol = min(RW, nrhs - r0);
+    const int m = F.m, p = F.p, tid = threadIdx.x, wave = tid >> 6;
+    T *Pr = reinterpret_cast<T *>(smem_raw), *Pi = Pr + m * RS;
+    const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
+    X += (int64_t)sel.s[z] * n * nrhs;
+    gather_bwd<T, RW>(F, idx, X, nrhs, r0, ncol, Pr, Pi);
+    // op(U) entry (i, t) sits at G[i * ars + t * acs]
+    const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
+    const bool unit = trans;
+    typename mfma_t<T>::acc cr[NC], ci[NC];
+    for (int s0 = wave * 16; s0 < p && m > p; s0 += 64) {
+        acc_zero<T, NC>(cr, ci);
+        strip_mma<T, NC>(G + s0 * ars + p * acs, AOp{ars, acs, p - s0, m - p, 0, 0, 0, 1 << 30, false, (bool)cj}, Pr,
+                         Pi, p, RS, cr, ci);
+        acc_store<T, NC>(Pr, Pi, RS, s0, p, true, cr, ci);
+    }
+    __syncthreads();
+    for (int b0 = ((p - 1) / BK) * BK; b0 >= 0; b0 -= BK) {
+        const int kb = min(BK, p - b0), sr = wave * 16;
+        const bool act = wave < 2 && sr < kb;
+        if (act) {
+            acc_zero<T, NC>(cr, ci);
+            strip_mma<T, NC>(G + (b0 + sr) * ars + b0 * acs, AOp{ars, acs, kb - sr, kb, sr, 0, 2, 1 << 30, unit, (bool)cj},
+                             Pr, Pi, b0, RS, cr, ci);
+        }
+        __syncthreads();
+        if (act) acc_store<T, NC>(Pr, Pi, RS, b0 + sr, b0 + kb, false, cr, ci);
+        __syncthreads();
+        AOp ot{ars, acs, 0, kb, 0, 0, 0, 1 << 30, false, (bool)cj};
+        cx<T> at[8], an[8];
+        int s0 = sr;
+        if (s0 < b0) {
+            ot.arows = b0 - s0;
+            load_a<T>(G + s0 * ars + b0 * acs, ot, 0, at);
+        }
+        for (; s0 < b0; s0 += 64) {
+            const int s1 = s0 + 64;
+            if (s1 < b0) {
+                ot.arows = b0 - s1;
+                load_a<T>(G + s1 * ars + b0 * acs, ot, 0, an);
+            }
+            acc_zero<T, NC>(cr, ci);
+            mma_a<T, NC>(at, 0, kb, Pr, Pi, b0, RS, cr, ci);
+            acc_store<T, NC>(Pr, Pi, RS, s0, b0, true, cr, ci);
+#pragma unroll
+            for (int s = 0; s < 8; s++) at[s] = an[s];
+        }
+        __syncthreads();
+    }
+    const int *rp = rowperm + (int64_t)sel.f[z] * n + F.pos0;
+    for (int e = tid; e < p * RW; e += 256) {
+        const int i = e / RW, j = e - i * RW;
+        if (j >= ncol) continue;
+        X[(int64_t)(F.pos0 + (trans ? rp[i] : i)) * nrhs + r0 + j] = {Pr[i * RS + j], Pi[i * RS + j]};
+    }
 }
 
 // transposed solves, after a level's backward blocks: x = P_f^T w for every
@@ -446,22 +1229,19 @@ __global__ void __launch_bounds__(256) k_piv_unswap(const FrontDev *__restrict__
 }
 
 // ---- single right-hand side: one workgroup per front, work vector in LDS ----
-template <typename T>
-__device__ __forceinline__ cx<T> shfl_c(cx<T> v, int src) {
-    return {__shfl(v.x, src), __shfl(v.y, src)};
-}
-
 // forward sweep of one front for one right-hand side: gather (row-permuted for
-// M solves), blocked lower solve (32-row diagonal blocks solved by wave 0 with
-// shuffles, block updates coalesced over rows), pivot rows to X and
-// contribution rows to W for the parent
+// M solves), blocked lower solve (each 32-row diagonal block applied as a
+// product with its inverse staged in LDS, block updates coalesced over rows),
+// pivot rows to X and contribution rows to W for the parent
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_vec(const cx<T> *__restrict__ arena, const FrontDev *__restrict__ fronts,
                                                  const int *__restrict__ list, const int *__restrict__ children,
                                                  const int *__restrict__ inv, const int *__restrict__ rowperm,
-                                                 cx<T> *__restrict__ X, cx<T> *__restrict__ W, int trans, int cj) {
+                                                 cx<T> *__restrict__ X, cx<T> *__restrict__ W, int trans, int cj,
+                                                 int single) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *sw = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ cx<T> Dsh[32][33];
     const FrontDev F = fronts[list[blockIdx.x]];
     const int m = F.m, p = F.p, tid = threadIdx.x;
     const cx<T> *G = arena + F.buf_off;
@@ -471,18 +1251,33 @@ __global__ void __launch_bounds__(256) k_fwd_vec(const cx<T> *__restrict__ arena
         sw[i] = front_row(F, src, 0, 1, X, W, fronts, children, inv);
     }
     __syncthreads();
-    for (int tb = 0; tb < p; tb += 32) {
-        const int nb = min(32, p - tb);
-        if (tid < 64) {
-            const int k = tid;
-            cx<T> y = k < nb ? sw[tb + k] : cx<T>{T(0), T(0)};
-            for (int t = 0; t < nb; t++) {
-                if (trans && k == t) y = cdiv(y, Lop(tb + t, tb + t));
-                const cx<T> yt = shfl_c(y, t);
-                if (k > t && k < nb) y = cmsub(y, Lop(tb + k, tb + t), yt);
-            }
-            if (k < nb) sw[tb + k] = y;
+    if (single) {
+        // one product over all rows: out = [Dop; Lop21'] w1 (m <= 512 rows, two per thread)
+        cx<T> acc[2] = {{T(0), T(0)}, {T(0), T(0)}};
+        for (int u = 0; u < 2; u++) {
+            const int i = tid + 256 * u;
+            if (i < m)
+                for (int t = 0; t < p; t++) {
+                    const cx<T> a = i < p ? dop(G, m, 0, i, t, true, trans, cj) : Lop(i, t);
+                    acc[u] = cadd(acc[u], cmul(a, sw[t]));
+                }
         }
+        __syncthreads();
+        for (int u = 0; u < 2; u++) {
+            const int i = tid + 256 * u;
+            if (i < m) sw[i] = i < p ? acc[u] : csub(sw[i], acc[u]);
+        }
+        __syncthreads();
+    }
+    for (int tb = 0; tb < (single ? 0 : p); tb += 32) {
+        const int nb = min(32, p - tb);
+        for (int e = tid; e < nb * nb; e += 256) Dsh[e % nb][e / nb] = dop(G, m, tb, e % nb, e / nb, true, trans, cj);
+        __syncthreads();
+        cx<T> y = {T(0), T(0)};
+        if (tid < nb)
+            for (int t = 0; t <= tid; t++) y = cadd(y, cmul(Dsh[tid][t], sw[tb + t]));
+        __syncthreads();
+        if (tid < nb) sw[tb + tid] = y;
         __syncthreads();
         for (int i = tb + nb + tid; i < m; i += 256) {
             cx<T> acc = sw[i];
@@ -502,15 +1297,32 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena, const FrontDev *__restrict__ fronts,
                                                  const int *__restrict__ list, const int *__restrict__ idx,
                                                  const int *__restrict__ rowperm, cx<T> *__restrict__ X, int trans,
-                                                 int cj) {
+                                                 int cj, int single) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *sw = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ cx<T> Dsh[32][33];
     const FrontDev F = fronts[list[blockIdx.x]];
     const int m = F.m, p = F.p, tid = threadIdx.x;
     const cx<T> *G = arena + F.buf_off;
     auto Uop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
     for (int i = tid; i < m; i += 256) sw[i] = i < p ? X[F.pos0 + i] : X[idx[F.idx_off + i]];
     __syncthreads();
+    if (single) {
+        // x1 = [Dop, -Uop12'] [y1; x2]: 8 threads per pivot row (p <= 32), shuffle sum
+        const int i = tid >> 3, part = tid & 7;
+        cx<T> acc = {T(0), T(0)};
+        if (i < p)
+            for (int t = part; t < m; t += 8) {
+                const cx<T> a = t < p ? dop(G, m, 0, i, t, false, trans, cj) : Uop(i, t);
+                acc = t < p ? cadd(acc, cmul(a, sw[t])) : csub(acc, cmul(a, sw[t]));
+            }
+        for (int o = 4; o > 0; o >>= 1) {
+            acc.x += __shfl_xor(acc.x, o, 8);
+            acc.y += __shfl_xor(acc.y, o, 8);
+        }
+        if (i < p && part == 0) X[F.pos0 + (trans ? rowperm[F.pos0 + i] : i)] = acc;
+        return;
+    }
     for (int i = tid; i < p; i += 256) {
         cx<T> acc = sw[i];
         for (int s = p; s < m; s++) acc = cmsub(acc, Uop(i, s), sw[s]);
@@ -519,16 +1331,13 @@ __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena
     __syncthreads();
     for (int tb = ((p - 1) / 32) * 32; tb >= 0; tb -= 32) {
         const int nb = min(32, p - tb);
-        if (tid < 64) {
-            const int k = tid;
-            cx<T> x = k < nb ? sw[tb + k] : cx<T>{T(0), T(0)};
-            for (int t = nb - 1; t >= 0; t--) {
-                if (!trans && k == t) x = cdiv(x, Uop(tb + t, tb + t));
-                const cx<T> xt = shfl_c(x, t);
-                if (k < t) x = cmsub(x, Uop(tb + k, tb + t), xt);
-            }
-            if (k < nb) sw[tb + k] = x;
-        }
+        for (int e = tid; e < nb * nb; e += 256) Dsh[e % nb][e / nb] = dop(G, m, tb, e % nb, e / nb, false, trans, cj);
+        __syncthreads();
+        cx<T> x = {T(0), T(0)};
+        if (tid < nb)
+            for (int t = tid; t < nb; t++) x = cadd(x, cmul(Dsh[tid][t], sw[tb + t]));
+        __syncthreads();
+        if (tid < nb) sw[tb + tid] = x;
         __syncthreads();
         for (int i = tid; i < tb; i += 256) {
             cx<T> acc = sw[i];
@@ -546,41 +1355,75 @@ __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena
 // r = b - op(M) x, column-major n x nrhs panels; one thread per row keeps the
 // row's (at most nine) entries in registers and walks the right-hand sides.
 // The pattern is symmetric: row `row` of M has the columns listed in CSC
-// column `row` (values through tq for M, directly for M^T / M^H).
+// column `row` (values through tq for M, directly for M^T / M^H).  With
+// `partial`, each workgroup also leaves the sum of |r|^2 over its 256 rows for
+// every column in partial[(z * gridDim.x + block) * nrhs + r] (wave sums through
+// LDS, no atomics); k_norm_reduce adds them up.  grid: (ceil(n/256), slots).
 __global__ void __launch_bounds__(256) k_residual(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
                                                   const int *__restrict__ rowind, const int64_t *__restrict__ tq,
                                                   int n, int nrhs, int sys, int64_t nnz,
                                                   const cx<double> *__restrict__ B, const cx<double> *__restrict__ X,
-                                                  cx<double> *__restrict__ R, Sel sel) {
+                                                  cx<double> *__restrict__ R, double *__restrict__ partial, Sel sel) {
+    extern __shared__ double wsum[];   // [4][nrhs] when partial != nullptr
     vals += sel.f[blockIdx.y] * nnz;
     const int64_t off = (int64_t)sel.s[blockIdx.y] * n * nrhs;
     B += off;
     X += off;
     R += off;
-    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
-        const int q0 = colptr[row], cnt = colptr[row + 1] - q0;
-        cx<double> a[9];
-        int c[9];
+    const int row = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool valid = row < n;
+    cx<double> a[9];
+    int c[9];
+    const int q0 = valid ? colptr[row] : 0, cnt = valid ? colptr[row + 1] - q0 : 0;
 #pragma unroll
-        for (int t = 0; t < 9; t++) {
-            if (t < cnt) {
-                const int q = q0 + t;
-                a[t] = sys == 0 ? vals[tq[q]] : vals[q];
-                if (sys == 1) a[t].y = -a[t].y;
-                c[t] = rowind[q];
-            } else {
-                a[t] = {0.0, 0.0};
-                c[t] = row;
-            }
+    for (int t = 0; t < 9; t++) {
+        if (t < cnt) {
+            const int q = q0 + t;
+            a[t] = sys == 0 ? vals[tq[q]] : vals[q];
+            if (sys == 1) a[t].y = -a[t].y;
+            c[t] = rowind[q];
+        } else {
+            a[t] = {0.0, 0.0};
+            c[t] = valid ? row : 0;
         }
-        for (int r = 0; r < nrhs; r++) {
+    }
+    for (int r = 0; r < nrhs; r++) {
+        cx<double> acc = {0.0, 0.0};
+        if (valid) {
             const cx<double> *x = X + (int64_t)r * n;
-            cx<double> acc = B[(int64_t)r * n + row];
+            acc = B[(int64_t)r * n + row];
 #pragma unroll
             for (int t = 0; t < 9; t++) acc = cmsub(acc, a[t], x[c[t]]);
             R[(int64_t)r * n + row] = acc;
         }
+        if (partial) {
+            double v = acc.x * acc.x + acc.y * acc.y;
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == 0) wsum[wave * nrhs + r] = v;
+        }
     }
+    if (partial) {
+        __syncthreads();
+        for (int r = threadIdx.x; r < nrhs; r += blockDim.x)
+            partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * nrhs + r] =
+                wsum[r] + wsum[nrhs + r] + wsum[2 * nrhs + r] + wsum[3 * nrhs + r];
+    }
+}
+
+// out[sel.s[z] * nrhs + r] = sum over the nblk workgroups' partial sums. grid (nrhs, slots)
+__global__ void __launch_bounds__(256) k_norm_reduce(const double *__restrict__ partial, int nblk, int nrhs,
+                                                     double *__restrict__ out, Sel sel) {
+    __shared__ double s[256];
+    const int r = blockIdx.x, z = blockIdx.y;
+    double acc = 0;
+    for (int b = threadIdx.x; b < nblk; b += 256) acc += partial[((int64_t)z * nblk + b) * nrhs + r];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(int64_t)sel.s[z] * nrhs + r] = s[0];
 }
 
 // backward error of one column: residual and max_i |r_i| / (|op(M)||x| + |b|)_i
@@ -662,14 +1505,82 @@ struct Workspace {
     }
     ~Workspace() { dev_free(p); }
 };
-thread_local Workspace ws_x, ws_w, ws_r, ws_d;
-thread_local Workspace ws_n;
+thread_local Workspace ws_x, ws_w, ws_r;
+thread_local Workspace ws_n, ws_p;
+
+// fused per-front kernels: widest right-hand-side tile whose panel of the
+// level's largest front fits in LDS (0: the level takes the blocked kernels).
+// Tuning (set_solve_tuning, or GFDFD_FUSED_RW at start-up): the widest tile
+// (64, 32 or 16) and the largest front order a multi-block level may have to
+// take the fused kernels.  Single-block levels always take them.
+constexpr size_t kFusedLdsMax = 160 * 1024 - 1024;
+
+template <typename T>
+size_t fused_bytes(int rw) {
+    return rw == 64 ? fused_row_bytes<T, 64>() : rw == 32 ? fused_row_bytes<T, 32>() : fused_row_bytes<T, 16>();
+}
+
+struct SolveTuning {
+    int rw_cap = 64, fused_mmax = 1 << 30;
+    SolveTuning() {
+        if (const char *e = std::getenv("GFDFD_FUSED_RW")) {
+            const int v = std::atoi(e);
+            if (v == 16 || v == 32 || v == 64) rw_cap = v;
+        }
+    }
+};
+SolveTuning &tuning() {
+    static SolveTuning t;
+    return t;
+}
+
+template <typename T>
+int fused_rw(int mmax, int max_children, bool single) {
+    if (max_children > 2 || (!single && mmax > tuning().fused_mmax)) return 0;
+    for (int rw = tuning().rw_cap; rw >= 16; rw /= 2)
+        if ((size_t)mmax * fused_bytes<T>(rw) <= kFusedLdsMax) return rw;
+    return 0;
+}
+
+// wave-task launch of a single-block level: up to four waves (consecutive
+// 16-column tiles of one front) per workgroup as the per-wave LDS allows
+struct WaveLaunch {
+    dim3 grid, block;
+    size_t lds;
+    int rtg, wave_lds;
+};
+template <typename T>
+WaveLaunch wave_launch(int nfr, int nrhs, int nslots, size_t per_wave) {
+    const size_t wl = (per_wave + 15) & ~size_t(15);
+    if (wl > kFusedLdsMax) throw std::logic_error("single-block front too large for a wave task");
+    const int rt = (nrhs + WT - 1) / WT;
+    const int wpg = (int)std::max<size_t>(1, std::min<size_t>({4, (size_t)rt, kFusedLdsMax / wl}));
+    const int rtg = (rt + wpg - 1) / wpg;
+    return {dim3(((nfr + 7) / 8) * 8 * rtg, nslots), dim3(64 * wpg), wpg * wl, rtg, (int)wl};
+}
+
+// staged small-front launch: the workgroup's LDS is the staged operator and
+// sources only; up to 8 waves (16-column tiles), as many workgroups as fit
+inline WaveLaunch small_launch(int nfr, int nrhs, int nslots, size_t lds) {
+    const int rt = (nrhs + WT - 1) / WT;
+    const int wpg = std::min(8, rt);
+    const int rtg = (rt + wpg - 1) / wpg;
+    return {dim3(((nfr + 7) / 8) * 8 * rtg, nslots), dim3(64 * wpg), (lds + 15) & ~size_t(15), rtg, 0};
+}
+
+#define FUSED_LAUNCH(kern, T, rw, grid, lds, st, ...)                                                   \
+    do {                                                                                              \
+        if ((rw) == 64) hipLaunchKernelGGL((kern<T, 64>), grid, dim3(256), lds, st, __VA_ARGS__);     \
+        else if ((rw) == 32) hipLaunchKernelGGL((kern<T, 32>), grid, dim3(256), lds, st, __VA_ARGS__); \
+        else hipLaunchKernelGGL((kern<T, 16>), grid, dim3(256), lds, st, __VA_ARGS__);                \
+    } while (0)
 
 template <typename T>
 void set_smem_attrs() {
     static bool done = false;
     if (done) return;
-    const int big = 160 * 1024;
+    // the vector kernels also hold a static 32 x 33 diagonal block
+    const int big = 160 * 1024 - (int)(32 * 33 * sizeof(cx<T>));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_vec<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, big));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_vec<T>),
@@ -681,10 +1592,39 @@ void set_smem_attrs() {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, blk));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_contrib<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, blk));
+    const int fl = (int)kFusedLdsMax;
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_front<T, 64>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_front<T, 32>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_front<T, 16>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_front<T, 64>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_front<T, 32>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_front<T, 16>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_small1<T, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_small1<T, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_small1<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_front1<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_front1<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
     done = true;
 }
 
 }  // namespace
+
+void set_solve_tuning(int rw_cap, int fused_mmax) {
+    if (rw_cap != 16 && rw_cap != 32 && rw_cap != 64) throw std::invalid_argument("tile width must be 16, 32 or 64");
+    tuning().rw_cap = rw_cap;
+    tuning().fused_mmax = fused_mmax < 0 ? (1 << 30) : fused_mmax;
+}
 
 Sel make_sel(const std::vector<int> &freqs, const std::vector<int> &slots) {
     if (freqs.empty() || freqs.size() > (size_t)kMaxBatch || slots.size() != freqs.size())
@@ -700,7 +1640,7 @@ Sel make_sel(const std::vector<int> &freqs, const std::vector<int> &slots) {
 
 template <typename T, typename TIO>
 void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::vector<int> &slots, int nslot_buf,
-                 int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st) {
+                 int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st, bool accumulate) {
     const DevPlan &dp = *F.plan;
     const Plan &pl = *dp.host;
     const int n = pl.n;
@@ -735,7 +1675,8 @@ void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::
                     ProfScope ps(K_FWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
                     size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
                     hipLaunchKernelGGL(k_fwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
-                                       dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj);
+                                       dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj,
+                                       dp.single[l]);
                 }
                 for (int l = pl.nlevels - 1; l >= 0; l--) {
                     if (dp.n_all[l] == 0) continue;
@@ -743,12 +1684,12 @@ void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::
                     ProfScope ps(K_BWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
                     size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
                     hipLaunchKernelGGL(k_bwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
-                                       dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj);
+                                       dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj, dp.single[l]);
                 }
                 {
                     ProfScope ps(K_PERMUTE, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
                     hipLaunchKernelGGL((k_vec_out<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, Xp, X + off,
-                                       iout + f * mstride, n);
+                                       iout + f * mstride, n, (int)accumulate);
                 }
             }
         }
@@ -772,51 +1713,122 @@ void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::
         hipLaunchKernelGGL((k_permute_in_slots<T, TIO>), dim3(tiles, ns), dim3(256), 0, st, B, Xp, dp.iperm, n, nrhs,
                            sel);
     }
+    const bool prof = prof_enabled();
     for (int l = 0; l < pl.nlevels; l++) {
         if (dp.n_all[l] == 0) continue;
         const DevPlan::LevelCounts &c = dp.counts[l];
+        const int nfr = dp.n_all[l];
+        if (dp.single[l] && dp.mmax_all[l] <= kSmallM) {
+            const WaveLaunch wl =
+                small_launch(nfr, nrhs, ns, 2 * sizeof(T) * dp.pmax_all[l] * small_stride(dp.mmax_all[l]) + 12 * kSmallM);
+            ProfScope ps(K_FWD_SMALL1, st, nz * 8.0 * nr * c.tri_elems, nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
+            if (dp.lv_children[l])
+                hipLaunchKernelGGL((k_fwd_small1<T, true>), wl.grid, wl.block, wl.lds, st, F.arena, F.fstride,
+                                   dp.lv_fr[l], nfr, trans ? dp.gsrc_id : F.gsrc, trans ? (int64_t)0 : pl.fv_rows * 3,
+                                   Xp, Wp, n, wstride, nrhs, wl.rtg, (int)trans, (int)cj, in);
+            else
+                hipLaunchKernelGGL((k_fwd_small1<T, false>), wl.grid, wl.block, wl.lds, st, F.arena, F.fstride,
+                                   dp.lv_fr[l], nfr, trans ? dp.gsrc_id : F.gsrc, trans ? (int64_t)0 : pl.fv_rows * 3,
+                                   Xp, Wp, n, wstride, nrhs, wl.rtg, (int)trans, (int)cj, in);
+            continue;
+        }
+        if (dp.single[l]) {
+            const WaveLaunch wl = wave_launch<T>(nfr, nrhs, ns, dp.pmax_all[l] * WT * 2 * sizeof(T) + 12 * dp.mmax_all[l]);
+            ProfScope ps(K_FWD_FRONT1, st, nz * 8.0 * nr * c.tri_elems, nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
+            hipLaunchKernelGGL(k_fwd_front1<T>, wl.grid, wl.block, wl.lds, st, F.arena, F.fstride, dp.lv_fr[l], nfr,
+                               trans ? dp.gsrc_id : F.gsrc, trans ? (int64_t)0 : pl.fv_rows * 3, Xp, Wp, n, wstride,
+                               nrhs, wl.rtg, wl.wave_lds, (int)trans, (int)cj, in);
+            continue;
+        }
+        const int rw = fused_rw<T>(dp.mmax_all[l], dp.max_children, false);
+        if (rw) {
+            const int rtf = (nrhs + rw - 1) / rw;
+            const dim3 grid(((nfr + 7) / 8) * 8 * rtf, ns);
+            const size_t lds = (size_t)dp.mmax_all[l] * fused_bytes<T>(rw);
+            ProfScope ps(K_FWD_FRONT, st, nz * 8.0 * nr * c.tri_elems, nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
+            FUSED_LAUNCH(k_fwd_front, T, rw, grid, lds, st, F.arena, F.fstride, dp.fronts, dp.lv_all[l], nfr,
+                         dp.children, dp.inv, trans ? nullptr : F.rowperm, Xp, Wp, n, wstride, nrhs, rtf, (int)trans,
+                         (int)cj, in);
+            continue;
+        }
         {
             ProfScope ps(K_FWD_GATHER, st, 0.0, nz * 2.0 * c.sum_m * nr * es);
             int64_t work = (int64_t)dp.mmax_all[l] * nrhs;
             int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, work / 1024), 256);
-            hipLaunchKernelGGL(k_fwd_gather<T>, dim3(dp.n_all[l], chunks, ns), dim3(256), 0, st, Xp, Wp, dp.fronts,
+            hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nfr, chunks, ns), dim3(256), 0, st, Xp, Wp, dp.fronts,
                                dp.lv_all[l], dp.children, dp.inv, trans ? nullptr : F.rowperm, n, wstride, nrhs, in);
         }
         const int pmax = dp.pmax_all[l];
-        ProfScope ps(K_FWD, st, nz * 8.0 * nr * c.tri_elems, nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
         for (int b0 = 0; b0 < pmax; b0 += BK) {
+            // factor entries of the block column, and the panel rows below the block read and written
+            const double fe = prof ? c.fwd_blk_elems[b0 / BK] : 0.0;
+            ProfScope ps(K_FWD_BLK, st, nz * 8.0 * nr * fe, nz * (fe + 2.0 * nr * fe / BK) * es);
             const int tiles = std::max(1, (dp.mmax_all[l] - b0 - 1) / 64 + 1);
-            hipLaunchKernelGGL(k_fwd_blk<T>, dim3(tiles, dp.n_all[l], rt * ns), dim3(256), blk, st, F.arena, F.fstride,
+            hipLaunchKernelGGL(k_fwd_blk<T>, dim3(tiles, nfr, rt * ns), dim3(256), blk, st, F.arena, F.fstride,
                                dp.fronts, dp.lv_all[l], Xp, Wp, n, wstride, nrhs, rt, b0, (int)trans, (int)cj, in);
         }
     }
     for (int l = pl.nlevels - 1; l >= 0; l--) {
         if (dp.n_all[l] == 0) continue;
         const DevPlan::LevelCounts &c = dp.counts[l];
+        const int nfr = dp.n_all[l];
+        if (dp.single[l] && dp.mmax_all[l] <= kSmallM) {
+            const WaveLaunch wl = small_launch(nfr, nrhs, ns,
+                                               2 * sizeof(T) * dp.mmax_all[l] * 32 + 4 * (kSmallM + 32));
+            ProfScope ps(K_BWD_SMALL1, st, nz * 8.0 * nr * c.tri_elems,
+                         nz * (c.tri_elems + (c.sum_m + c.sum_p) * nr) * es);
+            hipLaunchKernelGGL(k_bwd_small1<T>, wl.grid, wl.block, wl.lds, st, F.arena, F.fstride, dp.lv_fr[l], nfr,
+                               dp.idx, F.rowperm, Xp, n, nrhs, wl.rtg, (int)trans, (int)cj, in);
+            continue;
+        }
+        if (dp.single[l]) {
+            const WaveLaunch wl = wave_launch<T>(nfr, nrhs, ns, dp.mmax_all[l] * (WT * 2 * sizeof(T) + 4));
+            ProfScope ps(K_BWD_FRONT1, st, nz * 8.0 * nr * c.tri_elems,
+                         nz * (c.tri_elems + (c.sum_m + c.sum_p) * nr) * es);
+            hipLaunchKernelGGL(k_bwd_front1<T>, wl.grid, wl.block, wl.lds, st, F.arena, F.fstride, dp.lv_fr[l], nfr,
+                               dp.idx, F.rowperm, Xp, n, nrhs, wl.rtg, wl.wave_lds, (int)trans, (int)cj, in);
+            continue;
+        }
+        const int rw = fused_rw<T>(dp.mmax_all[l], dp.max_children, false);
+        if (rw) {
+            const int rtf = (nrhs + rw - 1) / rw;
+            const dim3 grid(((nfr + 7) / 8) * 8 * rtf, ns);
+            const size_t lds = (size_t)dp.mmax_all[l] * fused_bytes<T>(rw);
+            ProfScope ps(K_BWD_FRONT, st, nz * 8.0 * nr * c.tri_elems,
+                         nz * (c.tri_elems + (c.sum_m + c.sum_p) * nr) * es);
+            FUSED_LAUNCH(k_bwd_front, T, rw, grid, lds, st, F.arena, F.fstride, dp.fronts, dp.lv_all[l], nfr, dp.idx,
+                         F.rowperm, Xp, n, nrhs, rtf, (int)trans, (int)cj, in);
+            continue;
+        }
         const int pmax = dp.pmax_all[l];
-        ProfScope ps(K_BWD, st, nz * 8.0 * nr * c.tri_elems, nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
-        hipLaunchKernelGGL(k_bwd_contrib<T>, dim3((pmax + 63) / 64, dp.n_all[l], rt * ns), dim3(256), blk, st, F.arena,
-                           F.fstride, dp.fronts, dp.lv_all[l], dp.idx, Xp, Wp, n, wstride, nrhs, rt, (int)trans,
-                           (int)cj, in);
-        for (int b0 = ((pmax - 1) / BK) * BK; b0 >= 0; b0 -= BK) {
-            const int tiles = std::max(1, (b0 + 63) / 64);
-            hipLaunchKernelGGL(k_bwd_blk<T>, dim3(tiles, dp.n_all[l], rt * ns), dim3(256), blk, st, F.arena,
-                               F.fstride, dp.fronts, dp.lv_all[l], Xp, Wp, n, wstride, nrhs, rt, b0, (int)trans,
+        {
+            ProfScope ps(K_BWD_CONTRIB, st, nz * 8.0 * nr * c.contrib_elems,
+                         nz * (c.contrib_elems + (c.sum_m + c.sum_p) * nr) * es);
+            hipLaunchKernelGGL(k_bwd_contrib<T>, dim3((pmax + 63) / 64, nfr, rt * ns), dim3(256), blk, st, F.arena,
+                               F.fstride, dp.fronts, dp.lv_all[l], dp.idx, Xp, Wp, n, wstride, nrhs, rt, (int)trans,
                                (int)cj, in);
         }
+        for (int b0 = ((pmax - 1) / BK) * BK; b0 >= 0; b0 -= BK) {
+            const double be = prof ? c.bwd_blk_elems[b0 / BK] : 0.0;
+            ProfScope ps(K_BWD_BLK, st, nz * 8.0 * nr * be, nz * (be + 2.0 * nr * be / BK) * es);
+            const int tiles = std::max(1, (b0 + 63) / 64);
+            hipLaunchKernelGGL(k_bwd_blk<T>, dim3(tiles, nfr, rt * ns), dim3(256), blk, st, F.arena, F.fstride,
+                               dp.fronts, dp.lv_all[l], Xp, Wp, n, wstride, nrhs, rt, b0, (int)trans, (int)cj, in);
+        }
         if (trans) {
+            ProfScope ps(K_PIV_UNSWAP, st, 0.0, nz * 4.0 * c.sum_p * nr * es);
             const int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, (int64_t)pmax * nrhs / 1024), 256);
-            hipLaunchKernelGGL(k_piv_copy<T>, dim3(dp.n_all[l], chunks, ns), dim3(256), 0, st, dp.fronts,
-                               dp.lv_all[l], Xp, Wp, n, wstride, nrhs, in);
-            hipLaunchKernelGGL(k_piv_unswap<T>, dim3(dp.n_all[l], chunks, ns), dim3(256), 0, st, dp.fronts,
-                               dp.lv_all[l], F.rowperm, Xp, Wp, n, wstride, nrhs, in);
+            hipLaunchKernelGGL(k_piv_copy<T>, dim3(nfr, chunks, ns), dim3(256), 0, st, dp.fronts, dp.lv_all[l], Xp,
+                               Wp, n, wstride, nrhs, in);
+            hipLaunchKernelGGL(k_piv_unswap<T>, dim3(nfr, chunks, ns), dim3(256), 0, st, dp.fronts, dp.lv_all[l],
+                               F.rowperm, Xp, Wp, n, wstride, nrhs, in);
         }
     }
     {
         ProfScope ps(K_PERMUTE, st, 0.0, nz * ((double)n * nr * (sizeof(cx<TIO>) + es) + 4.0 * n));
         const int tiles = ((n + PT_K - 1) / PT_K) * ((nrhs + PT_R - 1) / PT_R);
         hipLaunchKernelGGL((k_permute_out_slots<T, TIO>), dim3(tiles, ns), dim3(256), 0, st, Xp, X, iout, mstride, n,
-                           nrhs, sel);
+                           nrhs, (int)accumulate, sel);
     }
     GFDFD_HIP(hipGetLastError());
 }
@@ -833,19 +1845,32 @@ template void solve_panel<float, double>(const Factors<float> &, int, int, int, 
 template void solve_panel<float, float>(const Factors<float> &, int, int, int, const cx<float> *, cx<float> *,
                                         hipStream_t);
 template void solve_batch<double, double>(const Factors<double> &, const std::vector<int> &, const std::vector<int> &,
-                                          int, int, int, const cx<double> *, cx<double> *, hipStream_t);
+                                          int, int, int, const cx<double> *, cx<double> *, hipStream_t, bool);
 template void solve_batch<float, double>(const Factors<float> &, const std::vector<int> &, const std::vector<int> &,
-                                         int, int, int, const cx<double> *, cx<double> *, hipStream_t);
+                                         int, int, int, const cx<double> *, cx<double> *, hipStream_t, bool);
 
-// r_s = b_s - op(M_f) x_s for launch slots z (frequency sel.f[z], buffer slot sel.s[z])
+// r_s = b_s - op(M_f) x_s for launch slots z (frequency sel.f[z], buffer slot
+// sel.s[z]); with nrm, also the squared column norms nrm[sel.s[z] * nrhs + r]
 static void residual_sel(const DevPlan &dp, const cx<double> *vals, int sys, int nrhs, const cx<double> *B,
-                         const cx<double> *X, cx<double> *R, const Sel &sel, hipStream_t st) {
+                         const cx<double> *X, cx<double> *R, const Sel &sel, hipStream_t st, double *nrm = nullptr) {
     if (!dp.tq) throw std::logic_error("residual needs a grid plan");
     const int n = dp.host->n;
     const double nnz = (double)dp.host->nnz;
-    ProfScope ps(K_RESIDUAL, st, sel.n * 8.0 * nnz * nrhs, sel.n * (nnz * (16.0 + 12.0) + 3.0 * n * 16.0 * nrhs));
-    hipLaunchKernelGGL(k_residual, dim3(grid_for(n, 256, 4096), sel.n), dim3(256), 0, st, vals, dp.colptr, dp.rowind,
-                       dp.tq, n, nrhs, sys, dp.host->nnz, B, X, R, sel);
+    const int nblk = (n + 255) / 256;
+    // column sums in LDS: 4 waves x nrhs doubles; wider panels take the separate norm kernel
+    const bool fused = nrm && nrhs <= 1024;
+    double *partial = fused ? (double *)ws_p.get(sizeof(double) * (size_t)nblk * nrhs * sel.n) : nullptr;
+    {
+        ProfScope ps(K_RESIDUAL, st, sel.n * 8.0 * nnz * nrhs, sel.n * (nnz * (16.0 + 12.0) + 3.0 * n * 16.0 * nrhs));
+        hipLaunchKernelGGL(k_residual, dim3(nblk, sel.n), dim3(256), fused ? 4 * sizeof(double) * nrhs : 0, st, vals,
+                           dp.colptr, dp.rowind, dp.tq, n, nrhs, sys, dp.host->nnz, B, X, R, partial, sel);
+    }
+    if (fused)
+        hipLaunchKernelGGL(k_norm_reduce, dim3(nrhs, sel.n), dim3(256), 0, st, partial, nblk, nrhs, nrm, sel);
+    else if (nrm)
+        for (int z = 0; z < sel.n; z++)
+            hipLaunchKernelGGL(k_colnorm2, dim3(nrhs, 1), dim3(256), 0, st, R + (int64_t)sel.s[z] * n * nrhs, n, nrhs,
+                               nrm + (int64_t)sel.s[z] * nrhs);
 }
 
 void residual_fp64(const DevPlan &dp, const cx<double> *vals, int sys, int nrhs, const cx<double> *B,
@@ -874,6 +1899,7 @@ void column_norms2(int n, int nrhs, const cx<double> *V, double *out, hipStream_
 }
 
 void axpy_fp64(int64_t count, const cx<double> *D, cx<double> *X, hipStream_t st) {
+    ProfScope ps(K_AXPY, st, 2.0 * count, 48.0 * count);
     hipLaunchKernelGGL(k_axpy, dim3(grid_for(count)), dim3(256), 0, st, count, D, X);
 }
 
@@ -894,7 +1920,6 @@ std::vector<RefineStats> solve_refined_batch(const Factors<T> &F, const std::vec
     const int n = dp.host->n, ns = (int)freqs.size();
     const int64_t panel = (int64_t)n * nrhs;
     cx<double> *R = (cx<double> *)ws_r.get(sizeof(cx<double>) * panel * ns);
-    cx<double> *D = (cx<double> *)ws_d.get(sizeof(cx<double>) * panel * ns);
     double *nrm = (double *)ws_n.get(sizeof(double) * 2 * nrhs * ns);
     std::vector<double> h(2 * (size_t)nrhs * ns);
     std::vector<int> slots(ns);
@@ -905,8 +1930,7 @@ std::vector<RefineStats> solve_refined_batch(const Factors<T> &F, const std::vec
     std::vector<int> act_f = freqs, act_s = slots;
     for (int it = 0;; it++) {
         const Sel sel = make_sel(act_f, act_s);
-        residual_sel(dp, F.vals, sys, nrhs, B, X, R, sel, st);
-        hipLaunchKernelGGL(k_colnorm2, dim3(nrhs, ns), dim3(256), 0, st, R, n, nrhs, nrm);
+        residual_sel(dp, F.vals, sys, nrhs, B, X, R, sel, st, nrm);
         GFDFD_HIP(hipMemcpyAsync(h.data(), nrm, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
         GFDFD_HIP(hipStreamSynchronize(st));
         std::vector<int> nf, nsl;
@@ -927,8 +1951,7 @@ std::vector<RefineStats> solve_refined_batch(const Factors<T> &F, const std::vec
         if (nf.empty()) break;
         act_f = nf;
         act_s = nsl;
-        solve_batch<T, double>(F, act_f, act_s, ns, sys, nrhs, R, D, st);
-        for (int s : act_s) axpy_fp64(panel, D + s * panel, X + s * panel, st);
+        solve_batch<T, double>(F, act_f, act_s, ns, sys, nrhs, R, X, st, true);   // X += op(M)^-1 r
     }
     return rs;
 }

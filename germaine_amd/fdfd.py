@@ -73,6 +73,13 @@ def grid_index(x, y, dh, npml, fsshift, nx):
 
 # ---- plan / factors ------------------------------------------------------------
 
+def solve_tuning(rw_cap=64, fused_mmax=-1):
+    """Panel-solve kernel selection (process-wide): widest fused right-hand-side
+    tile, and the largest front order of a multi-block level that takes the
+    fused per-front kernels (0: all such levels take the per-block kernels)."""
+    check(lib().gfdfd_solve_tuning(rw_cap, fused_mmax), "gfdfd_solve_tuning")
+
+
 class Plan:
     """Nested-dissection plan of an nx x ny 9-point grid (cached by size)."""
 
@@ -155,6 +162,13 @@ class Factors:
         check(lib().gfdfd_factor(plan.h, self.nfreq, _ptr(vals), precision, _stream(stream), C.byref(h)),
               "gfdfd_factor")
         self.h = h
+
+    def refactor(self, vals, stream=None):
+        """New values (same pattern and frequency count) into these factors' storage."""
+        vals = vals.contiguous()
+        assert vals.shape[0] == self.nfreq
+        check(lib().gfdfd_refactor(self.h, _ptr(vals), _stream(stream)), "gfdfd_refactor")
+        return self
 
     def __del__(self):
         if getattr(self, "h", None) is not None and lib is not None:

@@ -107,6 +107,11 @@ int gfdfd_assemble(gfdfd_plan plan, int physics, float dh, int nfreq, const gfdf
  * nnz complex doubles in canonical CSC order (copied; the factors keep an FP64
  * copy for refinement).  precision GFDFD_FP64 or GFDFD_FP32.               */
 int gfdfd_factor(gfdfd_plan plan, int nfreq, const double *d_vals, int precision, void *stream, gfdfd_factors *out);
+/* refactor in place: new values (same pattern, same nfreq, nfreq x nnz) into
+ * existing factors, reusing their device storage -- the next FWI iteration or
+ * line-search trial without a new allocation (src/AC/fwi_FD_AC.c:229-409
+ * refactors every frequency with umfpack_zi_numeric each time)              */
+int gfdfd_refactor(gfdfd_factors f, const double *d_vals, void *stream);
 void gfdfd_factors_free(gfdfd_factors *f);
 /* per-frequency zero-pivot flags (nfreq ints) */
 int gfdfd_factors_singular(gfdfd_factors f, int *flags);
@@ -134,6 +139,13 @@ int gfdfd_solve_refine_batch(gfdfd_factors f, int nsel, const int *freqs, int sy
 /* r = b - op(M) x in FP64 */
 int gfdfd_residual(gfdfd_plan plan, const double *d_vals, int sys, int nrhs, const double *d_b, const double *d_x,
                    double *d_r, void *stream);
+
+/* kernel selection of the panel solves (process-wide; no counterpart in the
+ * reference): the widest right-hand-side tile of the fused per-front kernels
+ * (16, 32 or 64; default 64 or GFDFD_FUSED_RW) and the largest front order of
+ * a level with multi-block fronts that still takes them (negative: as far as
+ * LDS allows; 0: such levels take the per-block kernels).                   */
+int gfdfd_solve_tuning(int rw_cap, int fused_mmax);
 
 /* ---- shot kernels --------------------------------------------------------------
  * RHS_source_AC / RHS_source_TE for all shots (src/AC/RHS_source_AC.c:46-53):
@@ -163,10 +175,12 @@ int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, co
 /* ---- measurement ------------------------------------------------------------
  * Per-kernel-class timing with HIP events on the launch stream, plus the
  * algorithmic flops and HBM bytes of each launch derived from the plan.
- * Classes: assemble, front_assemble, scatter_orig, small_lu, panel, trsm_u,
- * gemm_update, fwd_gather, fwd_front, bwd_front, fwd_vec, bwd_vec, permute,
- * residual.  No counterpart in the reference (it times whole phases with
- * MPI_Wtime, src/AC/forward_shot_AC.c:54-65).                               */
+ * Classes (one kernel each, piv_unswap two): assemble, front_assemble,
+ * scatter_orig, small_lu, panel, trsm_u, gemm_update, diag_inv, fwd_gather,
+ * fwd_small1, fwd_front1, fwd_front, fwd_blk, bwd_small1, bwd_front1,
+ * bwd_front, bwd_contrib, bwd_blk, piv_unswap, fwd_vec, bwd_vec, permute,
+ * residual, axpy.  No counterpart in the reference (it times whole phases
+ * with MPI_Wtime, src/AC/forward_shot_AC.c:54-65).                          */
 int gfdfd_profile(int enable);
 void gfdfd_profile_reset(void);
 int gfdfd_profile_read(const char *kernel, double *ms, long long *launches, double *flops, double *bytes);

@@ -7,6 +7,11 @@
   (panel/TRSM/MFMA) paths, 1 to 70 right-hand sides;
 * FP32 factors + FP64 refinement: <= 1e-8 against the oracle;
 * batched frequencies equal per-frequency solves;
+* every solve-kernel path -- single-block levels (one product per front),
+  multi-block levels on the fused per-front kernels at each tile width, and
+  on the per-block kernels -- for M, M^H, M^T in FP64 and FP32+IR, on
+  assembled and on random-valued (row-swapping) matrices;
+* refactoring in place equals a fresh factorisation;
 * the 201x101 acoustic workload through the six umfpack_zi_* entry points.
 """
 import ctypes as C
@@ -244,3 +249,57 @@ def test_pivoting_solves_match_oracle(sysno, nrhs):
         assert rel(Xb[z], Xr.T) <= 1e-8, (z, k)
         Xs = F.solve(k, dev(B[z]), sys=sysno).cpu().numpy()
         assert rel(Xs, Xr.T) <= 1e-8, (z, k)
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+@pytest.mark.parametrize("path", [(64, -1), (16, -1), (32, 0)])
+@pytest.mark.parametrize("sysno", [0, 1, 2])
+def test_solve_paths_match_oracle(precision, path, sysno):
+    """90x64 grid: separators of up to 64 unknowns make multi-block fronts
+    above the single-block levels.  path = (widest fused tile, largest
+    multi-block front order on the fused kernels; 0 sends those levels to the
+    per-block kernels)."""
+    nx, ny, npml, dh = 90, 64, 8, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=7)
+    freqs = [4.0, 11.0]
+    say("solve paths", precision, path, sysno)
+    mats = [oracle_system(G.AC, nx, ny, npml, dh, f, m1, m2)[1] for f in freqs]
+    mats.append(random_grid_csc(nx, ny, 31 + sysno))
+    vals = dev(np.stack([c[2] + 1j * c[3] for c in mats]))
+    pl = fdfd.Plan(nx, ny)
+    assert pl.info.max_pivots > 32
+    rng = np.random.default_rng(5 + sysno)
+    B = rng.standard_normal((3, 40, nx * ny)) + 1j * rng.standard_normal((3, 40, nx * ny))
+    fdfd.solve_tuning(*path)
+    try:
+        F = pl.factor(vals, precision)
+        if precision == G.FP64:
+            X = F.solve_batch([0, 1, 2], dev(B), sys=sysno).cpu().numpy()
+            tol = 1e-10
+        else:
+            X, its, rr = F.solve_refine_batch([0, 1, 2], dev(B), sys=sysno, tol=1e-12, max_iter=30)
+            X = X.cpu().numpy()
+            say("  iterations", its, rr)
+            tol = 1e-8
+    finally:
+        fdfd.solve_tuning(64, -1)
+    for z, csc in enumerate(mats):
+        Xr, info = O.band_solve(*csc, B[z].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+        assert info == 0
+        assert rel(X[z], Xr.T) <= (tol if z < 2 else 100 * tol), z
+
+
+def test_refactor_equals_fresh_factorisation():
+    nx, ny, npml, dh = 61, 41, 6, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=9)
+    say("refactor")
+    pl = fdfd.Plan(nx, ny)
+    va = pl.assemble(G.AC, dh, [3.0, 8.0], dev(m1), dev(m2), npml=npml)
+    vb = pl.assemble(G.AC, dh, [5.0, 12.0], dev(m1), dev(m2), npml=npml)
+    rng = np.random.default_rng(2)
+    B = dev(rng.standard_normal((2, 8, nx * ny)) + 1j * rng.standard_normal((2, 8, nx * ny)))
+    for precision in (G.FP64, G.FP32):
+        F = pl.factor(va, precision).refactor(vb)
+        Xa = F.solve_batch([0, 1], B).cpu().numpy()
+        Xb = pl.factor(vb, precision).solve_batch([0, 1], B).cpu().numpy()
+        assert rel(Xa, Xb) <= 1e-13
