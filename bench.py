@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64_mfma": 78.6e12, "fp32_mfma": 157.3e12, "hbm": 8.0e12}
-KERNELS = ["assemble", "front_assemble", "scatter_orig", "small_lu", "panel", "trsm_u", "gemm_update", "diag_inv",
+KERNELS = ["assemble", "front_assemble", "scatter_orig", "small_lu", "panel", "panel_apply", "trsm_u", "gemm_update", "diag_inv",
            "fwd_gather", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk", "bwd_small1", "bwd_front1", "bwd_front",
            "bwd_contrib", "bwd_blk",
            "piv_unswap", "fwd_vec", "bwd_vec", "permute", "residual", "axpy"]
@@ -78,6 +78,10 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true", help="skip the batched FP64 comparison step")
+    ap.add_argument("--fused-rw", type=int, default=64, help="widest fused solve tile (16, 32, 64)")
+    ap.add_argument("--fused-mmax", type=int, default=0,
+                    help="largest front order of a multi-block level on the fused solve kernels "
+                         "(0: none, -1: as far as LDS allows)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed region, one device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -96,6 +100,7 @@ def main():
     from germaine_amd import fdfd, models
 
     L = G.lib()
+    fdfd.solve_tuning(args.fused_rw, args.fused_mmax)
     w = models.config(args.config)
     nf, ns = len(w.freqs), len(w.src)
     log(f"bench: {w.name} {w.nx}x{w.ny} freqs={w.freqs} shots={ns}")
@@ -229,7 +234,8 @@ def main():
         "config": {"workload": f"{w.name}: AC {w.nx}x{w.ny} incl. {w.npml}-cell PML, h={w.dh} m, "
                                f"{nf} freqs {w.freqs[0]:g}-{w.freqs[-1]:g} Hz batched, {ns}-shot panel",
                    "n": n, "nnz": pl.nnz, "fronts": info.fronts, "factor_gflop_per_freq": info.factor_flops / 1e9,
-                   "nnz_lu": info.nnz_lu, "refine_tol": args.tol},
+                   "nnz_lu": info.nnz_lu, "refine_tol": args.tol, "fused_rw": args.fused_rw,
+                   "fused_mmax": args.fused_mmax},
         "fp64_batched": None if t64 is None else {
             "ms_per_step": round(t64 * 1e3, 3), "value": round(nf * ns / t64, 2),
             "kernels_ms_per_step": {k: round(v["ms"] / n64, 3) for k, v in kst64.items()}},

@@ -355,6 +355,118 @@ __global__ void __launch_bounds__(256) k_panel(cx<T> *__restrict__ arena, int64_
 }
 
 // U12 block row of the panel: rows [k0,k0+kb), columns [k0+kb, m): L_kb^-1 * F
+// ---------------------------------------------------------------------------
+// 3c: blocked path, panel held in LDS.  The pivot candidates of a 32-column
+// panel are the front's remaining fully-summed rows [k0, p); their panel
+// columns are factored entirely in LDS (column-major, np rows), with the row
+// swaps kept inside the panel.  k_panel_apply then applies the swap sequence to
+// the front's other columns and finishes the panel columns of the
+// contribution rows [p, m), which need no pivoting: row a of them becomes the
+// solution of x U_kk = a.  Same result as k_panel, with the per-column steps in
+// LDS instead of global memory.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_panel_lu(cx<T> *__restrict__ arena, int64_t fstride,
+                                                  const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                  int *__restrict__ piv, int n, int k0, int nb, int *__restrict__ sing) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *P = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ T sb[4];
+    __shared__ int si[4];
+    const FrontDev F = fronts[list[blockIdx.x]];
+    if (F.p <= k0) return;
+    const int fq = blockIdx.y;
+    cx<T> *G = arena + fq * fstride + F.buf_off;
+    const int m = F.m, p = F.p, kb = min(nb, p - k0), np = p - k0, tid = threadIdx.x;
+    for (int e = tid; e < np * kb; e += 256) {
+        const int r = e % np, c = e / np;
+        P[c * np + r] = G[(k0 + r) + (int64_t)(k0 + c) * m];
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; j++) {
+        T best = T(-1);
+        int bi = j;
+        for (int r = j + tid; r < np; r += 256) better(best, bi, cabs1(P[j * np + r]), r);
+        const int pr = block_argmax(best, bi, sb, si);
+        if (tid == 0) {
+            piv[(int64_t)fq * n + F.pos0 + k0 + j] = k0 + pr;
+            if (sb[0] == T(0)) atomicOr(sing + fq, 1);
+        }
+        if (pr != j && tid < kb) {
+            const cx<T> t = P[tid * np + j];
+            P[tid * np + j] = P[tid * np + pr];
+            P[tid * np + pr] = t;
+        }
+        __syncthreads();
+        const cx<T> d = crecip(P[j * np + j]);
+        for (int r = j + 1 + tid; r < np; r += 256) P[j * np + r] = cmul(P[j * np + r], d);
+        __syncthreads();
+        const int rows = np - j - 1, ncol = kb - j - 1;
+        for (int e = tid; e < rows * ncol; e += 256) {
+            const int c = j + 1 + e / rows, r = j + 1 + e % rows;
+            P[c * np + r] = cmsub(P[c * np + r], P[j * np + r], P[c * np + j]);
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < np * kb; e += 256) {
+        const int r = e % np, c = e / np;
+        G[(k0 + r) + (int64_t)(k0 + c) * m] = P[c * np + r];
+    }
+}
+
+// grid (column or row chunks of 64, fronts, nfreq): chunk x < ccols handles 64
+// columns outside the panel (swap sequence), the rest 64 contribution rows
+template <typename T>
+__global__ void __launch_bounds__(64) k_panel_apply(cx<T> *__restrict__ arena, int64_t fstride,
+                                                    const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                    const int *__restrict__ piv, int n, int k0, int nb, int ccols) {
+    __shared__ cx<T> U[32][33];
+    __shared__ int ps[32];
+    const FrontDev F = fronts[list[blockIdx.y]];
+    if (F.p <= k0) return;
+    const int fq = blockIdx.z;
+    cx<T> *G = arena + fq * fstride + F.buf_off;
+    const int m = F.m, p = F.p, kb = min(nb, p - k0), tid = threadIdx.x;
+    if ((int)blockIdx.x < ccols) {
+        // columns c outside [k0, k0+kb): the panel's row swaps, in order
+        if (tid < kb) ps[tid] = piv[(int64_t)fq * n + F.pos0 + k0 + tid];
+        __syncthreads();
+        int c = blockIdx.x * 64 + tid;
+        if (c >= k0) c += kb;
+        if (c >= m) return;
+        cx<T> *col = G + (int64_t)c * m;
+        for (int j = 0; j < kb; j++) {
+            const int r = ps[j];
+            if (r != k0 + j) {
+                const cx<T> t = col[k0 + j];
+                col[k0 + j] = col[r];
+                col[r] = t;
+            }
+        }
+        return;
+    }
+    // contribution rows: x U_kk = a, U_kk from the factored panel
+    const int i = p + (blockIdx.x - ccols) * 64 + tid;
+    for (int e = tid; e < kb * kb; e += 64) U[e % kb][e / kb] = G[(k0 + e % kb) + (int64_t)(k0 + e / kb) * m];
+    __syncthreads();
+    if (i >= m) return;
+    cx<T> x[32];
+#pragma unroll
+    for (int t = 0; t < 32; t++) x[t] = t < kb ? G[i + (int64_t)(k0 + t) * m] : cx<T>{T(0), T(0)};
+#pragma unroll
+    for (int t = 0; t < 32; t++) {
+        if (t < kb) {
+            cx<T> a = x[t];
+#pragma unroll
+            for (int s = 0; s < t; s++) a = cmsub(a, x[s], U[s][t]);
+            x[t] = cmul(a, crecip(U[t][t]));
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 32; t++)
+        if (t < kb) G[i + (int64_t)(k0 + t) * m] = x[t];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64) k_trsm_u(cx<T> *__restrict__ arena, int64_t fstride,
                                                const FrontDev *__restrict__ fronts, const int *__restrict__ list,
@@ -605,6 +717,7 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
     const int nf = F.nfreq;
     GFDFD_HIP(hipMemsetAsync(F.sing, 0, sizeof(int) * nf, st));
     const size_t lds_small_max = 160 * 1024 - 256;
+    const size_t panel_lds_max = 160 * 1024 - 1024;   // panels of up to 640 c64 / 320 c128 candidate rows
     static bool attr_set = false;
     if (!attr_set) {
         // fronts of up to 90x90 complex doubles live in LDS (gfx950: 160 KB per CU)
@@ -612,6 +725,10 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_small_max));
         GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_small_lu<float>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_small_max));
+        GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_panel_lu<double>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)panel_lds_max));
+        GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_panel_lu<float>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)panel_lds_max));
         attr_set = true;
     }
     for (int l = 0; l < pl.nlevels; l++) {
@@ -642,10 +759,22 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
         if (dp.n_large[l] > 0) {
             int si = 0;
             for (int k0 = 0; k0 < L.large_pmax; k0 += pl.nb, si++) {
-                {
-                ProfScope ps(K_PANEL, st, cnt.panel_flops[si] * nf, 0.0);
-                hipLaunchKernelGGL(k_panel<T>, dim3(dp.n_large[l], nf), dim3(256), 0, st, F.arena, F.fstride,
-                                   dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, F.sing);
+                const size_t plds = (size_t)(L.large_pmax - k0) * pl.nb * sizeof(cx<T>);
+                if (plds <= panel_lds_max) {
+                    {
+                        ProfScope ps(K_PANEL, st, cnt.panel_flops[si] * nf, 0.0);
+                        hipLaunchKernelGGL(k_panel_lu<T>, dim3(dp.n_large[l], nf), dim3(256), plds, st, F.arena,
+                                           F.fstride, dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, F.sing);
+                    }
+                    const int ccols = (L.large_mmax + 63) / 64;   // m - kb columns of any front fit
+                    const int crows = (L.large_mmax + 63) / 64;
+                    ProfScope ps(K_PANEL_APPLY, st, 0.0, 0.0);
+                    hipLaunchKernelGGL(k_panel_apply<T>, dim3(ccols + crows, dp.n_large[l], nf), dim3(64), 0, st,
+                                       F.arena, F.fstride, dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, ccols);
+                } else {
+                    ProfScope ps(K_PANEL, st, cnt.panel_flops[si] * nf, 0.0);
+                    hipLaunchKernelGGL(k_panel<T>, dim3(dp.n_large[l], nf), dim3(256), 0, st, F.arena, F.fstride,
+                                       dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, F.sing);
                 }
                 {
                 ProfScope ps(K_TRSM, st, cnt.trsm_flops[si] * nf, 0.0);

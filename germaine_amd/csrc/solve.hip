@@ -1521,7 +1521,9 @@ size_t fused_bytes(int rw) {
 }
 
 struct SolveTuning {
-    int rw_cap = 64, fused_mmax = 1 << 30;
+    // multi-block levels take the per-block kernels by default: measured faster
+    // than the fused per-front kernels on the 1601x401 workload in FP32
+    int rw_cap = 64, fused_mmax = 0;
     SolveTuning() {
         if (const char *e = std::getenv("GFDFD_FUSED_RW")) {
             const int v = std::atoi(e);
@@ -1554,7 +1556,12 @@ WaveLaunch wave_launch(int nfr, int nrhs, int nslots, size_t per_wave) {
     const size_t wl = (per_wave + 15) & ~size_t(15);
     if (wl > kFusedLdsMax) throw std::logic_error("single-block front too large for a wave task");
     const int rt = (nrhs + WT - 1) / WT;
-    const int wpg = (int)std::max<size_t>(1, std::min<size_t>({4, (size_t)rt, kFusedLdsMax / wl}));
+    // waves per workgroup: the most resident waves per CU (LDS-bound), then the widest
+    int wpg = 1, best = 0;
+    for (int w = 1; w <= std::min(4, rt); w++) {
+        const int res = w * (int)(kFusedLdsMax / (w * wl));
+        if (res >= best) best = res, wpg = w;
+    }
     const int rtg = (rt + wpg - 1) / wpg;
     return {dim3(((nfr + 7) / 8) * 8 * rtg, nslots), dim3(64 * wpg), wpg * wl, rtg, (int)wl};
 }
@@ -1638,81 +1645,19 @@ Sel make_sel(const std::vector<int> &freqs, const std::vector<int> &slots) {
     return sel;
 }
 
-template <typename T, typename TIO>
-void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::vector<int> &slots, int nslot_buf,
-                 int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st, bool accumulate) {
+// Forward and backward sweeps of op(M_f) on internal panels (elimination
+// order, row-major, right-hand sides contiguous): launch slot z solves
+// frequency in.f[z] in place on panel in.s[z] of Xp, with work rows Wp.
+template <typename T>
+void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx<T> *Wp, hipStream_t st) {
     const DevPlan &dp = *F.plan;
     const Plan &pl = *dp.host;
-    const int n = pl.n;
-    for (int f : freqs)
-        if (f < 0 || f >= F.nfreq) throw std::invalid_argument("frequency index out of range");
-    if (sys < 0 || sys > 2) throw std::invalid_argument("invalid system");
-    set_smem_attrs<T>();
-    const Sel sel = make_sel(freqs, slots);
-    const int ns = sel.n;
+    const int n = pl.n, ns = in.n;
     const bool trans = sys != 0, cj = sys == 1;
-    const int *iout = dp.iperm;
-    const int64_t mstride = 0;
-    const double es = sizeof(cx<T>);
-    if (nrhs <= kVecMaxRhs) {
-        // few right-hand sides: one pass per frequency and column through the single-vector kernels
-        cx<T> *Xp = (cx<T> *)ws_x.get(sizeof(cx<T>) * (size_t)n);
-        cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)pl.fv_rows);
-        for (int z = 0; z < ns; z++) {
-            const int f = sel.f[z];
-            const cx<T> *arena = F.arena + f * F.fstride;
-            const int *rowperm = F.rowperm + (int64_t)f * n;
-            for (int r = 0; r < nrhs; r++) {
-                const int64_t off = ((int64_t)sel.s[z] * nrhs + r) * n;
-                {
-                    ProfScope ps(K_PERMUTE, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
-                    hipLaunchKernelGGL((k_vec_in<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, B + off, Xp, dp.perm,
-                                       n);
-                }
-                for (int l = 0; l < pl.nlevels; l++) {
-                    if (dp.n_all[l] == 0) continue;
-                    const DevPlan::LevelCounts &c = dp.counts[l];
-                    ProfScope ps(K_FWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
-                    size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-                    hipLaunchKernelGGL(k_fwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
-                                       dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj,
-                                       dp.single[l]);
-                }
-                for (int l = pl.nlevels - 1; l >= 0; l--) {
-                    if (dp.n_all[l] == 0) continue;
-                    const DevPlan::LevelCounts &c = dp.counts[l];
-                    ProfScope ps(K_BWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
-                    size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-                    hipLaunchKernelGGL(k_bwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
-                                       dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj, dp.single[l]);
-                }
-                {
-                    ProfScope ps(K_PERMUTE, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
-                    hipLaunchKernelGGL((k_vec_out<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, Xp, X + off,
-                                       iout + f * mstride, n, (int)accumulate);
-                }
-            }
-        }
-        GFDFD_HIP(hipGetLastError());
-        return;
-    }
-    // internal panels are indexed by launch slot: slot z of Xp/Wp serves frequency sel.f[z]
-    Sel in = sel;
-    for (int z = 0; z < ns; z++) in.s[z] = z;
     const int64_t wstride = pl.fv_rows * (int64_t)nrhs;
-    cx<T> *Xp = (cx<T> *)ws_x.get(sizeof(cx<T>) * (size_t)n * nrhs * ns);
-    cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)wstride * ns);
-    const double nr = nrhs, nz = ns;
+    const double nr = nrhs, nz = ns, es = sizeof(cx<T>);
     const size_t blk = blk_smem_bytes<T>();
     const int rt = (nrhs + 63) / 64;
-    (void)nslot_buf;
-    {
-        // B slot sel.s[z] -> internal slot z
-        ProfScope ps(K_PERMUTE, st, 0.0, nz * ((double)n * nr * (sizeof(cx<TIO>) + es) + 4.0 * n));
-        const int tiles = ((n + PT_K - 1) / PT_K) * ((nrhs + PT_R - 1) / PT_R);
-        hipLaunchKernelGGL((k_permute_in_slots<T, TIO>), dim3(tiles, ns), dim3(256), 0, st, B, Xp, dp.iperm, n, nrhs,
-                           sel);
-    }
     const bool prof = prof_enabled();
     for (int l = 0; l < pl.nlevels; l++) {
         if (dp.n_all[l] == 0) continue;
@@ -1824,6 +1769,82 @@ void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::
                                F.rowperm, Xp, Wp, n, wstride, nrhs, in);
         }
     }
+}
+
+template <typename T, typename TIO>
+void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::vector<int> &slots, int nslot_buf,
+                 int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st, bool accumulate) {
+    const DevPlan &dp = *F.plan;
+    const Plan &pl = *dp.host;
+    const int n = pl.n;
+    for (int f : freqs)
+        if (f < 0 || f >= F.nfreq) throw std::invalid_argument("frequency index out of range");
+    if (sys < 0 || sys > 2) throw std::invalid_argument("invalid system");
+    set_smem_attrs<T>();
+    const Sel sel = make_sel(freqs, slots);
+    const int ns = sel.n;
+    const bool trans = sys != 0, cj = sys == 1;
+    const int *iout = dp.iperm;
+    const int64_t mstride = 0;
+    const double es = sizeof(cx<T>);
+    if (nrhs <= kVecMaxRhs) {
+        // few right-hand sides: one pass per frequency and column through the single-vector kernels
+        cx<T> *Xp = (cx<T> *)ws_x.get(sizeof(cx<T>) * (size_t)n);
+        cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)pl.fv_rows);
+        for (int z = 0; z < ns; z++) {
+            const int f = sel.f[z];
+            const cx<T> *arena = F.arena + f * F.fstride;
+            const int *rowperm = F.rowperm + (int64_t)f * n;
+            for (int r = 0; r < nrhs; r++) {
+                const int64_t off = ((int64_t)sel.s[z] * nrhs + r) * n;
+                {
+                    ProfScope ps(K_PERMUTE, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
+                    hipLaunchKernelGGL((k_vec_in<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, B + off, Xp, dp.perm,
+                                       n);
+                }
+                for (int l = 0; l < pl.nlevels; l++) {
+                    if (dp.n_all[l] == 0) continue;
+                    const DevPlan::LevelCounts &c = dp.counts[l];
+                    ProfScope ps(K_FWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
+                    size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
+                    hipLaunchKernelGGL(k_fwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
+                                       dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj,
+                                       dp.single[l]);
+                }
+                for (int l = pl.nlevels - 1; l >= 0; l--) {
+                    if (dp.n_all[l] == 0) continue;
+                    const DevPlan::LevelCounts &c = dp.counts[l];
+                    ProfScope ps(K_BWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
+                    size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
+                    hipLaunchKernelGGL(k_bwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
+                                       dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj, dp.single[l]);
+                }
+                {
+                    ProfScope ps(K_PERMUTE, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
+                    hipLaunchKernelGGL((k_vec_out<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, Xp, X + off,
+                                       iout + f * mstride, n, (int)accumulate);
+                }
+            }
+        }
+        GFDFD_HIP(hipGetLastError());
+        return;
+    }
+    // internal panels are indexed by launch slot: slot z of Xp/Wp serves frequency sel.f[z]
+    Sel in = sel;
+    for (int z = 0; z < ns; z++) in.s[z] = z;
+    const int64_t wstride = pl.fv_rows * (int64_t)nrhs;
+    cx<T> *Xp = (cx<T> *)ws_x.get(sizeof(cx<T>) * (size_t)n * nrhs * ns);
+    cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)wstride * ns);
+    const double nr = nrhs, nz = ns;
+    (void)nslot_buf;
+    {
+        // B slot sel.s[z] -> internal slot z
+        ProfScope ps(K_PERMUTE, st, 0.0, nz * ((double)n * nr * (sizeof(cx<TIO>) + es) + 4.0 * n));
+        const int tiles = ((n + PT_K - 1) / PT_K) * ((nrhs + PT_R - 1) / PT_R);
+        hipLaunchKernelGGL((k_permute_in_slots<T, TIO>), dim3(tiles, ns), dim3(256), 0, st, B, Xp, dp.iperm, n, nrhs,
+                           sel);
+    }
+    sweeps<T>(F, in, sys, nrhs, Xp, Wp, st);
     {
         ProfScope ps(K_PERMUTE, st, 0.0, nz * ((double)n * nr * (sizeof(cx<TIO>) + es) + 4.0 * n));
         const int tiles = ((n + PT_K - 1) / PT_K) * ((nrhs + PT_R - 1) / PT_R);
@@ -1911,7 +1932,9 @@ void cast_c64_to_c128(int64_t count, const cx<float> *a, cx<double> *b, hipStrea
 }
 
 // FP64 refinement of nsel frequencies at once; frequencies leave the batch as
-// they converge (the correction solves batch the remaining ones).
+// they converge (the correction solves batch the remaining ones).  The
+// residual kernel also leaves the column norms, and each correction is added
+// to X by the solve's final transpose.
 template <typename T>
 std::vector<RefineStats> solve_refined_batch(const Factors<T> &F, const std::vector<int> &freqs, int sys, int nrhs,
                                              const cx<double> *B, cx<double> *X, double tol, int max_iter,

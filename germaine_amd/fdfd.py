@@ -73,10 +73,11 @@ def grid_index(x, y, dh, npml, fsshift, nx):
 
 # ---- plan / factors ------------------------------------------------------------
 
-def solve_tuning(rw_cap=64, fused_mmax=-1):
+def solve_tuning(rw_cap=64, fused_mmax=0):
     """Panel-solve kernel selection (process-wide): widest fused right-hand-side
     tile, and the largest front order of a multi-block level that takes the
-    fused per-front kernels (0: all such levels take the per-block kernels)."""
+    fused per-front kernels (0, the default: all such levels take the
+    per-block kernels; negative: as far as LDS allows)."""
     check(lib().gfdfd_solve_tuning(rw_cap, fused_mmax), "gfdfd_solve_tuning")
 
 

@@ -144,7 +144,7 @@ int gfdfd_residual(gfdfd_plan plan, const double *d_vals, int sys, int nrhs, con
  * reference): the widest right-hand-side tile of the fused per-front kernels
  * (16, 32 or 64; default 64 or GFDFD_FUSED_RW) and the largest front order of
  * a level with multi-block fronts that still takes them (negative: as far as
- * LDS allows; 0: such levels take the per-block kernels).                   */
+ * LDS allows; 0, the default: such levels take the per-block kernels).      */
 int gfdfd_solve_tuning(int rw_cap, int fused_mmax);
 
 /* ---- shot kernels --------------------------------------------------------------
@@ -176,11 +176,11 @@ int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, co
  * Per-kernel-class timing with HIP events on the launch stream, plus the
  * algorithmic flops and HBM bytes of each launch derived from the plan.
  * Classes (one kernel each, piv_unswap two): assemble, front_assemble,
- * scatter_orig, small_lu, panel, trsm_u, gemm_update, diag_inv, fwd_gather,
- * fwd_small1, fwd_front1, fwd_front, fwd_blk, bwd_small1, bwd_front1,
- * bwd_front, bwd_contrib, bwd_blk, piv_unswap, fwd_vec, bwd_vec, permute,
- * residual, axpy.  No counterpart in the reference (it times whole phases
- * with MPI_Wtime, src/AC/forward_shot_AC.c:54-65).                          */
+ * scatter_orig, small_lu, panel, panel_apply, trsm_u, gemm_update, diag_inv,
+ * fwd_gather, fwd_small1, fwd_front1, fwd_front, fwd_blk, bwd_small1,
+ * bwd_front1, bwd_front, bwd_contrib, bwd_blk, piv_unswap, fwd_vec, bwd_vec,
+ * permute, residual, axpy.  No counterpart in the reference (it times whole
+ * phases with MPI_Wtime, src/AC/forward_shot_AC.c:54-65).                   */
 int gfdfd_profile(int enable);
 void gfdfd_profile_reset(void);
 int gfdfd_profile_read(const char *kernel, double *ms, long long *launches, double *flops, double *bytes);

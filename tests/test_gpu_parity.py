@@ -282,7 +282,7 @@ def test_solve_paths_match_oracle(precision, path, sysno):
             say("  iterations", its, rr)
             tol = 1e-8
     finally:
-        fdfd.solve_tuning(64, -1)
+        fdfd.solve_tuning()
     for z, csc in enumerate(mats):
         Xr, info = O.band_solve(*csc, B[z].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
         assert info == 0
