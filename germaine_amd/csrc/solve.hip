@@ -55,26 +55,30 @@ __device__ __forceinline__ cx<T> ld(const cx<T> *p, bool cj) {
 // user index k, internal rows are contiguous in the right-hand sides.
 constexpr int PT_K = 64, PT_R = 32;
 
+// Internal panels have a row stride ld >= nrhs (whole 128-byte lines per row);
+// the padding columns are zero and stay zero through the sweeps.
 template <typename T, typename TIO>
 __global__ void __launch_bounds__(256) k_permute_in_slots(const cx<TIO> *__restrict__ B, cx<T> *__restrict__ Xp,
-                                                          const int *__restrict__ iperm, int n, int nrhs, Sel sel) {
+                                                          const int *__restrict__ iperm, int n, int nrhs, int ld,
+                                                          Sel sel) {
     __shared__ cx<T> S[PT_R][PT_K + 1];
     const int ktiles = (n + PT_K - 1) / PT_K;
     const int k0 = (blockIdx.x % ktiles) * PT_K, r0 = (blockIdx.x / ktiles) * PT_R;
-    const int64_t panel = (int64_t)n * nrhs;
-    B += sel.s[blockIdx.y] * panel;
-    Xp += (int64_t)blockIdx.y * panel;
+    B += sel.s[blockIdx.y] * (int64_t)n * nrhs;
+    Xp += (int64_t)blockIdx.y * n * ld;
     for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
         int kk = e % PT_K, rr = e / PT_K;
+        cx<T> v = {T(0), T(0)};
         if (k0 + kk < n && r0 + rr < nrhs) {
-            cx<TIO> v = B[k0 + kk + (int64_t)(r0 + rr) * n];
-            S[rr][kk] = {(T)v.x, (T)v.y};
+            const cx<TIO> b = B[k0 + kk + (int64_t)(r0 + rr) * n];
+            v = {(T)b.x, (T)b.y};
         }
+        S[rr][kk] = v;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
-        int rr = e % PT_R, kk = e / PT_R;
-        if (k0 + kk < n && r0 + rr < nrhs) Xp[(int64_t)iperm[k0 + kk] * nrhs + r0 + rr] = S[rr][kk];
+        const int rr = e % PT_R, kk = e / PT_R;
+        if (k0 + kk < n && r0 + rr < ld) Xp[(int64_t)iperm[k0 + kk] * ld + r0 + rr] = S[rr][kk];
     }
 }
 
@@ -82,17 +86,16 @@ __global__ void __launch_bounds__(256) k_permute_in_slots(const cx<TIO> *__restr
 template <typename T, typename TIO>
 __global__ void __launch_bounds__(256) k_permute_out_slots(const cx<T> *__restrict__ Xp, cx<TIO> *__restrict__ X,
                                                            const int *__restrict__ iout, int64_t map_stride, int n,
-                                                           int nrhs, int accumulate, Sel sel) {
+                                                           int nrhs, int ld, int accumulate, Sel sel) {
     __shared__ cx<T> S[PT_R][PT_K + 1];
     const int ktiles = (n + PT_K - 1) / PT_K;
     const int k0 = (blockIdx.x % ktiles) * PT_K, r0 = (blockIdx.x / ktiles) * PT_R;
-    const int64_t panel = (int64_t)n * nrhs;
-    X += sel.s[blockIdx.y] * panel;
-    Xp += (int64_t)blockIdx.y * panel;
+    X += sel.s[blockIdx.y] * (int64_t)n * nrhs;
+    Xp += (int64_t)blockIdx.y * n * ld;
     iout += sel.f[blockIdx.y] * map_stride;
     for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
         int rr = e % PT_R, kk = e / PT_R;
-        if (k0 + kk < n && r0 + rr < nrhs) S[rr][kk] = Xp[(int64_t)iout[k0 + kk] * nrhs + r0 + rr];
+        if (k0 + kk < n && r0 + rr < nrhs) S[rr][kk] = Xp[(int64_t)iout[k0 + kk] * ld + r0 + rr];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
@@ -1648,14 +1651,17 @@ Sel make_sel(const std::vector<int> &freqs, const std::vector<int> &slots) {
 // Forward and backward sweeps of op(M_f) on internal panels (elimination
 // order, row-major, right-hand sides contiguous): launch slot z solves
 // frequency in.f[z] in place on panel in.s[z] of Xp, with work rows Wp.
+// nrhs is the panels' row stride (right-hand sides plus zero padding); the
+// algorithmic counts attached to the launches use nreal right-hand sides.
 template <typename T>
-void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx<T> *Wp, hipStream_t st) {
+void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx<T> *Wp, hipStream_t st,
+            int nreal) {
     const DevPlan &dp = *F.plan;
     const Plan &pl = *dp.host;
     const int n = pl.n, ns = in.n;
     const bool trans = sys != 0, cj = sys == 1;
     const int64_t wstride = pl.fv_rows * (int64_t)nrhs;
-    const double nr = nrhs, nz = ns, es = sizeof(cx<T>);
+    const double nr = nreal, nz = ns, es = sizeof(cx<T>);
     const size_t blk = blk_smem_bytes<T>();
     const int rt = (nrhs + 63) / 64;
     const bool prof = prof_enabled();
@@ -1798,7 +1804,7 @@ void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::
             for (int r = 0; r < nrhs; r++) {
                 const int64_t off = ((int64_t)sel.s[z] * nrhs + r) * n;
                 {
-                    ProfScope ps(K_PERMUTE, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
+                    ProfScope ps(K_PERMUTE_IN, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
                     hipLaunchKernelGGL((k_vec_in<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, B + off, Xp, dp.perm,
                                        n);
                 }
@@ -1820,7 +1826,7 @@ void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::
                                        dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj, dp.single[l]);
                 }
                 {
-                    ProfScope ps(K_PERMUTE, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
+                    ProfScope ps(K_PERMUTE_OUT, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
                     hipLaunchKernelGGL((k_vec_out<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, Xp, X + off,
                                        iout + f * mstride, n, (int)accumulate);
                 }
@@ -1832,24 +1838,26 @@ void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::
     // internal panels are indexed by launch slot: slot z of Xp/Wp serves frequency sel.f[z]
     Sel in = sel;
     for (int z = 0; z < ns; z++) in.s[z] = z;
-    const int64_t wstride = pl.fv_rows * (int64_t)nrhs;
-    cx<T> *Xp = (cx<T> *)ws_x.get(sizeof(cx<T>) * (size_t)n * nrhs * ns);
-    cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)wstride * ns);
+    // internal row stride: whole 128-byte lines, so every row segment the sweep
+    // kernels move is line-aligned (zero padding columns)
+    const int ld = (nrhs + (int)(128 / sizeof(cx<T>)) - 1) / (int)(128 / sizeof(cx<T>)) * (int)(128 / sizeof(cx<T>));
+    cx<T> *Xp = (cx<T> *)ws_x.get(sizeof(cx<T>) * (size_t)n * ld * ns);
+    cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)pl.fv_rows * ld * ns);
     const double nr = nrhs, nz = ns;
     (void)nslot_buf;
     {
         // B slot sel.s[z] -> internal slot z
-        ProfScope ps(K_PERMUTE, st, 0.0, nz * ((double)n * nr * (sizeof(cx<TIO>) + es) + 4.0 * n));
-        const int tiles = ((n + PT_K - 1) / PT_K) * ((nrhs + PT_R - 1) / PT_R);
+        ProfScope ps(K_PERMUTE_IN, st, 0.0, nz * ((double)n * nr * (sizeof(cx<TIO>) + es) + 4.0 * n));
+        const int tiles = ((n + PT_K - 1) / PT_K) * ((ld + PT_R - 1) / PT_R);
         hipLaunchKernelGGL((k_permute_in_slots<T, TIO>), dim3(tiles, ns), dim3(256), 0, st, B, Xp, dp.iperm, n, nrhs,
-                           sel);
+                           ld, sel);
     }
-    sweeps<T>(F, in, sys, nrhs, Xp, Wp, st);
+    sweeps<T>(F, in, sys, ld, Xp, Wp, st, nrhs);
     {
-        ProfScope ps(K_PERMUTE, st, 0.0, nz * ((double)n * nr * (sizeof(cx<TIO>) + es) + 4.0 * n));
+        ProfScope ps(K_PERMUTE_OUT, st, 0.0, nz * ((double)n * nr * (sizeof(cx<TIO>) + es) + 4.0 * n));
         const int tiles = ((n + PT_K - 1) / PT_K) * ((nrhs + PT_R - 1) / PT_R);
         hipLaunchKernelGGL((k_permute_out_slots<T, TIO>), dim3(tiles, ns), dim3(256), 0, st, Xp, X, iout, mstride, n,
-                           nrhs, (int)accumulate, sel);
+                           nrhs, ld, (int)accumulate, sel);
     }
     GFDFD_HIP(hipGetLastError());
 }

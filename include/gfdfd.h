@@ -179,7 +179,7 @@ int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, co
  * scatter_orig, small_lu, panel, panel_apply, trsm_u, gemm_update, diag_inv,
  * fwd_gather, fwd_small1, fwd_front1, fwd_front, fwd_blk, bwd_small1,
  * bwd_front1, bwd_front, bwd_contrib, bwd_blk, piv_unswap, fwd_vec, bwd_vec,
- * permute, residual, axpy.  No counterpart in the reference (it times whole
+ * permute_in, permute_out, residual, axpy.  No counterpart in the reference (it times whole
  * phases with MPI_Wtime, src/AC/forward_shot_AC.c:54-65).                   */
 int gfdfd_profile(int enable);
 void gfdfd_profile_reset(void);
