@@ -28,11 +28,11 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp64_mfma": 78.6e12, "fp32_mfma": 157.3e12, "hbm": 8.0e12}
 KERNELS = ["assemble", "front_assemble", "scatter_orig", "small_lu", "panel", "panel_apply", "trsm_u", "gemm_update", "diag_inv",
-           "fwd_gather", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk", "bwd_small1", "bwd_front1", "bwd_front",
+           "fwd_gather", "fwd_leaf", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk", "bwd_small1", "bwd_front1", "bwd_front",
            "bwd_contrib", "bwd_blk",
-           "piv_unswap", "fwd_vec", "bwd_vec", "permute_in", "permute_out", "residual", "axpy"]
+           "piv_unswap", "fwd_vec", "bwd_vec", "permute_in", "permute_out", "residual", "norm_reduce", "axpy"]
 # kernels whose arithmetic runs on the matrix cores (at the factor precision)
-MFMA_KERNELS = {"small_lu", "panel", "trsm_u", "gemm_update", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk",
+MFMA_KERNELS = {"small_lu", "panel", "trsm_u", "gemm_update", "fwd_leaf", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk",
                 "bwd_small1", "bwd_front1", "bwd_front", "bwd_contrib", "bwd_blk"}
 
 
@@ -66,12 +66,28 @@ def roofline(name, d, mfma_peak):
                 flops_per_launch=fl, bytes_per_launch=by)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(kernel, workload):
+    """Copy-calibrated FETCH_SIZE + WRITE_SIZE bytes per launch of a kernel class,
+    recorded from rocprofv3 pmc runs of this bench on the same workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("workload_name") != workload:
+        return None
+    return dict(rec, source=os.path.relpath(TRAFFIC_FILE, ROOT))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg3")
+    ap.add_argument("--config", default="ac_1601x401_8f")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-plain", action="store_true", help="skip the per-shot plain FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -207,6 +223,12 @@ def main():
         roof = roofline(dom, kst[dom], PEAK["fp32_mfma"])
         roof["launches_per_step"] = kst[dom]["launches"] / args.steps
         roof["ms_per_step"] = kst[dom]["ms"] / args.steps
+        # HBM bytes per launch of this kernel from the committed FETCH_SIZE/WRITE_SIZE
+        # pmc runs of this bench (copy-calibrated); null when they cover another kernel
+        tr = measured_traffic(dom, w.name)
+        if tr is not None:
+            roof["traffic"] = tr["traffic_bytes_per_launch"]
+            roof["traffic_source"] = tr["source"]
 
     # --- CPU baseline ---------------------------------------------------------------------------
     cpu = None

@@ -1672,7 +1672,9 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
         if (dp.single[l] && dp.mmax_all[l] <= kSmallM) {
             const WaveLaunch wl =
                 small_launch(nfr, nrhs, ns, 2 * sizeof(T) * dp.pmax_all[l] * small_stride(dp.mmax_all[l]) + 12 * kSmallM);
-            ProfScope ps(K_FWD_SMALL1, st, nz * 8.0 * nr * c.tri_elems, nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
+            // levels without children (the leaves) are a kernel of their own
+            ProfScope ps(dp.lv_children[l] ? K_FWD_SMALL1 : K_FWD_LEAF, st, nz * 8.0 * nr * c.tri_elems,
+                         nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
             if (dp.lv_children[l])
                 hipLaunchKernelGGL((k_fwd_small1<T, true>), wl.grid, wl.block, wl.lds, st, F.arena, F.fstride,
                                    dp.lv_fr[l], nfr, trans ? dp.gsrc_id : F.gsrc, trans ? (int64_t)0 : pl.fv_rows * 3,
@@ -1894,9 +1896,10 @@ static void residual_sel(const DevPlan &dp, const cx<double> *vals, int sys, int
         hipLaunchKernelGGL(k_residual, dim3(nblk, sel.n), dim3(256), fused ? 4 * sizeof(double) * nrhs : 0, st, vals,
                            dp.colptr, dp.rowind, dp.tq, n, nrhs, sys, dp.host->nnz, B, X, R, partial, sel);
     }
-    if (fused)
+    if (fused) {
+        ProfScope ps(K_NORM_REDUCE, st, sel.n * (double)nblk * nrhs, sel.n * 8.0 * (nblk + 1.0) * nrhs);
         hipLaunchKernelGGL(k_norm_reduce, dim3(nrhs, sel.n), dim3(256), 0, st, partial, nblk, nrhs, nrm, sel);
-    else if (nrm)
+    } else if (nrm)
         for (int z = 0; z < sel.n; z++)
             hipLaunchKernelGGL(k_colnorm2, dim3(nrhs, 1), dim3(256), 0, st, R + (int64_t)sel.s[z] * n * nrhs, n, nrhs,
                                nrm + (int64_t)sel.s[z] * nrhs);

@@ -110,29 +110,29 @@ def te_interior(nxi, nyi, seed, n_ellipses=20):
 
 def config(name):
     """The five benchmark workloads (grid sizes include the PML)."""
-    if name == "cfg1":      # 201x101, h=10, PML 20, 5 Hz, 10 sources, 100 receivers on row 5
+    if name == "ac_201x101_inclusion":      # 201x101, h=10, PML 20, 5 Hz, 10 sources, 100 receivers on row 5
         nx, ny, npml, dh = 201, 101, 20, 10.0
         vp, rho = inclusion_interior(nx - 2 * npml, ny - 2 * npml)
         return Workload(name, 1, nx, ny, npml, dh, [5.0], extend_pml(vp, npml), extend_pml(rho, npml),
                         acquisition_row(nx, npml, dh, 5, 10), acquisition_row(nx, npml, dh, 5, 100))
-    if name == "cfg2":      # 801x187, h=25, PML 20, 3/5/7/10 Hz, 100 shots, 400 receivers
+    if name == "ac_801x187_4f":      # 801x187, h=25, PML 20, 3/5/7/10 Hz, 100 shots, 400 receivers
         nx, ny, npml, dh = 801, 187, 20, 25.0
         vp, rho = layered_fault_interior(nx - 2 * npml, ny - 2 * npml, seed=2)
         return Workload(name, 1, nx, ny, npml, dh, [3.0, 5.0, 7.0, 10.0], extend_pml(vp, npml),
                         extend_pml(rho, npml), acquisition_row(nx, npml, dh, 1, 100),
                         acquisition_row(nx, npml, dh, 1, 400))
-    if name == "cfg3":      # 1601x401, h=12.5, PML 30, 2..16 Hz, 200 shots
+    if name == "ac_1601x401_8f":      # 1601x401, h=12.5, PML 30, 2..16 Hz, 200 shots
         nx, ny, npml, dh = 1601, 401, 30, 12.5
         vp, rho = layered_fault_interior(nx - 2 * npml, ny - 2 * npml, seed=3)
         return Workload(name, 1, nx, ny, npml, dh, [2.0 * k for k in range(1, 9)], extend_pml(vp, npml),
                         extend_pml(rho, npml), acquisition_row(nx, npml, dh, 1, 200), [])
-    if name == "cfg4":      # 801x201, h=20, PML 20, 3..9 Hz, 80 shots, 320 receivers; smoothed start model
+    if name == "ac_801x201_fwi":      # 801x201, h=20, PML 20, 3..9 Hz, 80 shots, 320 receivers; smoothed start model
         nx, ny, npml, dh = 801, 201, 20, 20.0
         vp, rho = layered_fault_interior(nx - 2 * npml, ny - 2 * npml, seed=4)
         return Workload(name, 1, nx, ny, npml, dh, [3.0, 5.0, 7.0, 9.0], extend_pml(vp, npml),
                         extend_pml(rho, npml), acquisition_row(nx, npml, dh, 1, 80),
                         acquisition_row(nx, npml, dh, 1, 320))
-    if name == "cfg5":      # TE 2401x801, h=0.04, PML 40, 100..600 MHz, 150 shots, 600 receivers
+    if name == "te_2401x801_6f":      # TE 2401x801, h=0.04, PML 40, 100..600 MHz, 150 shots, 600 receivers
         nx, ny, npml, dh = 2401, 801, 40, 0.04
         eps, sig = te_interior(nx - 2 * npml, ny - 2 * npml, seed=5)
         return Workload(name, 3, nx, ny, npml, dh, [1e8 * k for k in range(1, 7)], extend_pml(eps, npml),

@@ -175,12 +175,14 @@ int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, co
 /* ---- measurement ------------------------------------------------------------
  * Per-kernel-class timing with HIP events on the launch stream, plus the
  * algorithmic flops and HBM bytes of each launch derived from the plan.
- * Classes (one kernel each, piv_unswap two): assemble, front_assemble,
+ * Classes (one kernel symbol per precision each; piv_unswap two, fwd_front
+ * and bwd_front one per tile width): assemble, front_assemble,
  * scatter_orig, small_lu, panel, panel_apply, trsm_u, gemm_update, diag_inv,
- * fwd_gather, fwd_small1, fwd_front1, fwd_front, fwd_blk, bwd_small1,
- * bwd_front1, bwd_front, bwd_contrib, bwd_blk, piv_unswap, fwd_vec, bwd_vec,
- * permute_in, permute_out, residual, axpy.  No counterpart in the reference (it times whole
- * phases with MPI_Wtime, src/AC/forward_shot_AC.c:54-65).                   */
+ * fwd_gather, fwd_leaf, fwd_small1, fwd_front1, fwd_front, fwd_blk,
+ * bwd_small1, bwd_front1, bwd_front, bwd_contrib, bwd_blk, piv_unswap,
+ * fwd_vec, bwd_vec, permute_in, permute_out, residual, norm_reduce, axpy.
+ * No counterpart in the reference (it times whole phases with MPI_Wtime,
+ * src/AC/forward_shot_AC.c:54-65).                                          */
 int gfdfd_profile(int enable);
 void gfdfd_profile_reset(void);
 int gfdfd_profile_read(const char *kernel, double *ms, long long *launches, double *flops, double *bytes);

@@ -1,0 +1,128 @@
+"""GPU parity of the FWI gradient path against the oracle.
+
+grad_obj on the device (batched assembly and factorisation, forward solves,
+receiver sampling, residual with the reference's signed mute, adjoint
+sources conj(res)/DH^2, adjoint solves with M for AC and M^T for TE, and the
+gradient kernels) against the oracle's per-shot restatement of
+src/AC/grad_obj_AC.c:50-168 and src/TE/grad_obj_TE.c:53-172 with FP64 band-LU
+solves.  The reference accumulates the gradient and the misfit in float,
+the device in double, so the bar is a relative error of 1e-5 on the gradient
+(FP32 rounding of the summands) and on the misfit -- in FP64 and with FP32
+factors refined in FP64 to a relative residual of 1e-10.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import germaine_amd as G
+from germaine_amd import fdfd
+from oracle import oracle as O
+from tests._grids import smooth_model
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+NX, NY, NPML = 52, 38, 8
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def models(phys):
+    j, i = np.mgrid[0:NY, 0:NX]
+    bump = np.exp(-(((i - 26) / 6.0) ** 2 + ((j - 22) / 5.0) ** 2))
+    if phys == G.AC:
+        m1, m2 = smooth_model(NX, NY, seed=13)
+        true1 = (m1 + 180.0 * bump).astype(np.float32)
+        return dict(dh=10.0, freqs=[6.0, 11.5], m1=m1, m2=m2, t1=true1, t2=m2, adj=0)
+    eps0 = 8.854187817e-12
+    rng = np.random.default_rng(3)
+    m1 = ((5.0 + rng.random((NY, NX))) * eps0).astype(np.float32)
+    m2 = (0.002 + 0.002 * rng.random((NY, NX))).astype(np.float32)
+    true1 = (m1 + 2.0 * eps0 * bump).astype(np.float32)
+    return dict(dh=0.04, freqs=[1.2e8, 2.6e8], m1=m1, m2=m2, t1=true1, t2=m2, adj=2)
+
+
+def oracle_csc(phys, dh, f, m1, m2):
+    px, py = O.pml_profiles(NX, NY, NPML, dh, 90.0, f, 0.0, 0)
+    return O.csc_of_M(NX, NY, *O.assemble(phys, NX, NY, dh, f, 0.0, m1, m2, px, py))
+
+
+def oracle_fields(phys, dh, f, m1, m2, srcs, rhs=None, sysno=0):
+    csc = oracle_csc(phys, dh, f, m1, m2)
+    n = NX * NY
+    if rhs is None:
+        rhs = np.zeros((n, len(srcs)), complex)
+        for s, k in enumerate(srcs):
+            rhs[k, s] = O.source_amp(phys, f, 0.0, dh)
+    X, info = O.band_solve(*csc, rhs, sysno, O.grid_transpose_perm(NX, NY))
+    assert info == 0
+    return csc, X
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+@pytest.mark.parametrize("phys", [G.AC, G.TE])
+def test_gradient_matches_oracle(phys, precision):
+    say("gradient", phys, precision)
+    m = models(phys)
+    dh, freqs, n = m["dh"], m["freqs"], NX * NY
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    # observed data from the true model, with two muted samples per frequency:
+    # one zero, one with both parts negative (the signed mute, src/AC/calc_res_AC.c:82-85)
+    pobs = []
+    for f in freqs:
+        _, Ut = oracle_fields(phys, dh, f, m["t1"], m["t2"], srcs)
+        p = Ut[recs, :].T.astype(np.complex64)
+        p[0, 3] = 0
+        p[2, 7] = -abs(p[2, 7].real) - 1e-3j * abs(p[2, 7])
+        pobs.append(p)
+
+    pl = fdfd.Plan(NX, NY)
+    tol = None if precision == G.FP64 else 1e-10
+    l2_dev, grad = fdfd.grad_obj(pl, phys, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs,
+                                 [dev(p) for p in pobs], npml=NPML, precision=precision, tol=tol)
+    torch.cuda.synchronize()
+
+    g1 = np.zeros(n, np.float32)
+    g2 = np.zeros(n, np.float32)
+    l2 = 0.0
+    for fi, f in enumerate(freqs):
+        csc, U = oracle_fields(phys, dh, f, m["m1"], m["m2"], srcs)
+        rhs = np.zeros((n, len(srcs)), complex)
+        for s in range(len(srcs)):
+            res, l2s = O.residual(U[recs, s].astype(np.complex64), pobs[fi][s])
+            assert res[3] == 0 or s != 0
+            l2 += l2s
+            rhs[:, s] = O.adjoint_rhs(n, recs, res, dh)
+        _, LAM = oracle_fields(phys, dh, f, m["m1"], m["m2"], srcs, rhs, m["adj"])
+        for s in range(len(srcs)):
+            # vec2mat's one-cell shift reads x[n], which no solve writes: the
+            # device leaves that cell unchanged, the oracle adds zero there
+            if phys == G.AC:
+                O.grad_ac_accumulate(NX, NY, f, 0.0, U[:, s], 0j, LAM[:, s], 0j, m["m1"], m["m2"], g1)
+            else:
+                O.grad_te_accumulate(NX, NY, f, 0.0, U[:, s], 0j, LAM[:, s], 0j, g1, g2)
+    l2 *= 0.5
+
+    def relerr(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    assert abs(l2_dev - l2) <= 1e-5 * abs(l2), (l2_dev, l2)
+    if phys == G.AC:
+        e = relerr(grad.cpu().numpy(), g1.astype(np.float64))
+        say("  misfit", l2_dev, "gradient rel err", e)
+        assert e <= 1e-5
+    else:
+        es = relerr(grad[0].cpu().numpy(), g1.astype(np.float64))
+        ee = relerr(grad[1].cpu().numpy(), g2.astype(np.float64))
+        say("  misfit", l2_dev, "gradient rel err sigma", es, "eps", ee)
+        assert es <= 1e-5 and ee <= 1e-5

@@ -128,9 +128,9 @@ def test_batched_frequencies_equal_single():
 def test_umfpack_abi_config_201x101():
     """201x101 grid, h=10 m, 20-cell PML, inclusion model, 5 Hz, 10 sources / 100
     receivers on row 5: the reference's per-shot loop through the drop-in ABI."""
-    w = models.config("cfg1")
+    w = models.config("ac_201x101_inclusion")
     nx, ny, n = w.nx, w.ny, w.nx * w.ny
-    say("umfpack ABI cfg1")
+    say("umfpack ABI ac_201x101_inclusion")
     tri, csc = oracle_system(G.AC, nx, ny, w.npml, w.dh, w.freqs[0], w.m1, w.m2)
     irow, icol, ar, ai = tri
     nnz = len(ar)
@@ -172,8 +172,8 @@ def test_umfpack_abi_config_201x101():
 
 
 def test_forward_shots_fp32_refined_config_201x101():
-    w = models.config("cfg1")
-    say("forward_shots cfg1 fp32+IR")
+    w = models.config("ac_201x101_inclusion")
+    say("forward_shots ac_201x101_inclusion fp32+IR")
     pl = fdfd.Plan(w.nx, w.ny)
     out = fdfd.forward_shots(pl, w.physics, w.dh, w.freqs, dev(w.m1), dev(w.m2), w.src, w.rec, npml=w.npml,
                              precision=G.FP32, tol=1e-10, keep_fields=True)
