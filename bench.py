@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--fused-mmax", type=int, default=0,
                     help="largest front order of a multi-block level on the fused solve kernels "
                          "(0: none, -1: as far as LDS allows)")
+    ap.add_argument("--refine-layout", type=int, default=0,
+                    help="panel refinement layout: 0 elimination order (default), 1 caller's layout")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed region, one device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -116,7 +118,7 @@ def main():
     from germaine_amd import fdfd, models
 
     L = G.lib()
-    fdfd.solve_tuning(args.fused_rw, args.fused_mmax)
+    fdfd.solve_tuning(args.fused_rw, args.fused_mmax, args.refine_layout)
     w = models.config(args.config)
     nf, ns = len(w.freqs), len(w.src)
     log(f"bench: {w.name} {w.nx}x{w.ny} freqs={w.freqs} shots={ns}")
@@ -257,7 +259,7 @@ def main():
                                f"{nf} freqs {w.freqs[0]:g}-{w.freqs[-1]:g} Hz batched, {ns}-shot panel",
                    "n": n, "nnz": pl.nnz, "fronts": info.fronts, "factor_gflop_per_freq": info.factor_flops / 1e9,
                    "nnz_lu": info.nnz_lu, "refine_tol": args.tol, "fused_rw": args.fused_rw,
-                   "fused_mmax": args.fused_mmax},
+                   "fused_mmax": args.fused_mmax, "refine_layout": args.refine_layout},
         "fp64_batched": None if t64 is None else {
             "ms_per_step": round(t64 * 1e3, 3), "value": round(nf * ns / t64, 2),
             "kernels_ms_per_step": {k: round(v["ms"] / n64, 3) for k, v in kst64.items()}},

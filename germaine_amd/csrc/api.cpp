@@ -273,9 +273,9 @@ int gfdfd_factor(gfdfd_plan plan, int nfreq, const double *d_vals, int precision
     GUARD_END
 }
 
-int gfdfd_solve_tuning(int rw_cap, int fused_mmax) {
+int gfdfd_solve_tuning(int rw_cap, int fused_mmax, int refine_layout) {
     GUARD_BEGIN
-    set_solve_tuning(rw_cap, fused_mmax);
+    set_solve_tuning(rw_cap, fused_mmax, refine_layout);
     return GFDFD_OK;
     GUARD_END
 }

@@ -86,10 +86,11 @@ void axpy_fp64(int64_t count, const cx<double> *D, cx<double> *X, hipStream_t st
 void cast_c128_to_c64(int64_t count, const cx<double> *a, cx<float> *b, hipStream_t st);
 void cast_c64_to_c128(int64_t count, const cx<float> *a, cx<double> *b, hipStream_t st);
 
-// panel-solve kernel selection: widest fused right-hand-side tile (16/32/64) and
+// panel-solve kernel selection: widest fused right-hand-side tile (16/32/64),
 // the largest front order of a multi-block level taking the fused kernels
-// (negative: no limit beyond LDS)
-void set_solve_tuning(int rw_cap, int fused_mmax);
+// (negative: no limit beyond LDS), and the layout of the panel refinement
+// (0: elimination order, 1: user layout with a transpose per correction)
+void set_solve_tuning(int rw_cap, int fused_mmax, int refine_layout);
 
 struct RefineStats {
     int iterations = 0;

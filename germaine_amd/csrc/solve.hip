@@ -1429,6 +1429,201 @@ __global__ void __launch_bounds__(256) k_norm_reduce(const double *__restrict__ 
     if (threadIdx.x == 0) out[(int64_t)sel.s[z] * nrhs + r] = s[0];
 }
 
+// ---- row-major panel refinement ----------------------------------------------------
+// The panel refinement keeps b and the FP64 iterate row-major in grid order
+// (the right-hand sides of a point contiguous, row stride ld64), next to the
+// factor-precision panels the sweeps work on (row-major in elimination order,
+// row stride ld).  A refinement step is then one pass that reads x, the latest
+// correction d and b and writes x + d and the next right-hand side
+// r = b - op(M)(x + d) straight into the sweeps' layout: every access moves
+// whole row segments, and no panel is transposed between the steps.
+
+// B (user layout) -> Bg (FP64, grid order: row k, stride ld64) and P (factor
+// precision, elimination order: row iperm[k], stride ld) on slot sel.s[z];
+// also the tile's sums of |b|^2 per column in bpart[(z * ktiles + ktile) *
+// nrhs + r].  grid (tiles, slots), 64 x 32 tiles.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ir_in(const cx<double> *__restrict__ B, cx<double> *__restrict__ Bp,
+                                               cx<T> *__restrict__ P, const int *__restrict__ iperm, int n, int nrhs,
+                                               int ld64, int ld, double *__restrict__ bpart, Sel sel) {
+    __shared__ cx<double> S[PT_R][PT_K + 1];
+    const int ktiles = (n + PT_K - 1) / PT_K, kt = blockIdx.x % ktiles;
+    const int k0 = kt * PT_K, r0 = (blockIdx.x / ktiles) * PT_R;
+    const int s = sel.s[blockIdx.y];
+    B += (int64_t)s * n * nrhs;
+    Bp += (int64_t)s * n * ld64;
+    P += (int64_t)s * n * ld;
+    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
+        const int kk = e % PT_K, rr = e / PT_K;
+        cx<double> v = {0.0, 0.0};
+        if (k0 + kk < n && r0 + rr < nrhs) v = B[k0 + kk + (int64_t)(r0 + rr) * n];
+        S[rr][kk] = v;
+    }
+    __syncthreads();
+    {
+        // 8 lanes per column
+        const int rr = threadIdx.x >> 3, part = threadIdx.x & 7;
+        double acc = 0.0;
+        for (int kk = part; kk < PT_K; kk += 8) acc += S[rr][kk].x * S[rr][kk].x + S[rr][kk].y * S[rr][kk].y;
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        acc += __shfl_xor(acc, 4);
+        if (part == 0 && r0 + rr < nrhs) bpart[((int64_t)blockIdx.y * ktiles + kt) * nrhs + r0 + rr] = acc;
+    }
+    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
+        const int rr = e % PT_R, kk = e / PT_R;
+        if (k0 + kk >= n) continue;
+        const cx<double> v = S[rr][kk];
+        if (r0 + rr < ld64) Bp[(int64_t)(k0 + kk) * ld64 + r0 + rr] = v;
+        if (r0 + rr < ld) P[(int64_t)iperm[k0 + kk] * ld + r0 + rr] = {(T)v.x, (T)v.y};
+    }
+}
+
+// One refinement step of launch slot z (frequency sel.f[z], slot s = sel.s[z]):
+// x = Xa + D (D alone on the first step: Xa null), r = b - op(M) x;  Xb <- x,
+// R <- r in the factor precision (zero padding columns), and the sum of |r|^2
+// over the workgroup's points of every column in
+// partial[(z * gridDim.x + block) * nrhs + c].  b, Xa and Xb are in grid order
+// (row k = i + j*nx, stride ld64), D and R in elimination order (row iperm[k],
+// stride ld).  Xa/Xb and D/R must be distinct buffers: other workgroups read a
+// point's x while its own workgroup writes.  Row k of M has the entries of CSC
+// column k (the pattern is symmetric), all within the 3x3 neighbourhood.
+// Tiles of kIrTx consecutive points of one grid row: per 16-column chunk the
+// x of the 3 x (kIrTx + 2) neighbourhood points is staged in LDS once (each
+// point's x is used by up to nine rows), then one thread per (point, column)
+// applies the stencil from LDS.  grid (blocks, slots), grid-stride over tiles.
+constexpr int kIrTx = 16, kIrCw = 16, kIrStage = 3 * (kIrTx + 2);
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
+                                                 const int *__restrict__ rowind, const int64_t *__restrict__ tq,
+                                                 const int *__restrict__ iperm, int nx, int ny, int nrhs, int ld64,
+                                                 int ld, int sys, int64_t nnz, const cx<double> *__restrict__ Bg,
+                                                 const cx<double> *__restrict__ Xa, const cx<T> *__restrict__ D,
+                                                 cx<double> *__restrict__ Xb, cx<T> *__restrict__ R,
+                                                 double *__restrict__ partial, Sel sel) {
+    extern __shared__ double wsum[];   // [4][nrhs]
+    __shared__ cx<double> sx[kIrStage][kIrCw];
+    __shared__ cx<double> scoef[kIrTx][9];
+    __shared__ int sidx[kIrTx][9];     // staged row of each stencil entry
+    __shared__ int sg[kIrStage];       // grid row of each staged point (-1: outside the grid)
+    __shared__ int se[kIrStage];       // its elimination position
+    const int z = blockIdx.y, s = sel.s[z];
+    const int64_t n = (int64_t)nx * ny;
+    vals += sel.f[z] * nnz;
+    const int64_t o64 = s * n * ld64, o32 = s * n * ld;
+    Bg += o64;
+    Xb += o64;
+    const bool first = Xa == nullptr;
+    if (!first) Xa += o64;
+    D += o32;
+    R += o32;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double *ws = wsum + wave * nrhs;
+    for (int c = lane; c < nrhs; c += 64) ws[c] = 0.0;
+    // XCD-aware tile order: workgroup b runs on XCD b % 8 (gridDim.x is a
+    // multiple of 8); each XCD takes a contiguous range of tiles in column-major
+    // order (j fastest), so the tiles above and below a tile -- which stage the
+    // same grid rows -- run on the same XCD and meet that XCD's L2.
+    const int tpr = (nx + kIrTx - 1) / kIrTx, ntiles = tpr * ny;
+    const int xcd = blockIdx.x % 8, per = (ntiles + 7) / 8, t1 = min(ntiles, (xcd + 1) * per);
+    for (int tt = xcd * per + blockIdx.x / 8; tt < t1; tt += gridDim.x / 8) {
+        const int j = tt % ny, i0 = (tt / ny) * kIrTx, w = min(kIrTx, nx - i0);
+        __syncthreads();   // the previous tile's staging is no longer read
+        if (tid < kIrStage) {
+            const int jj = j - 1 + tid / (kIrTx + 2), ii = i0 - 1 + tid % (kIrTx + 2);
+            const bool in = jj >= 0 && jj < ny && ii >= 0 && ii < nx;
+            sg[tid] = in ? jj * nx + ii : -1;
+            se[tid] = in ? iperm[jj * nx + ii] : 0;
+        } else if (tid >= 64 && tid < 64 + kIrTx * 9) {
+            const int r = (tid - 64) / 9, t = (tid - 64) % 9;
+            int li = (kIrTx + 2) + r + 1;   // missing entries: zero value on the point itself
+            cx<double> a = {0.0, 0.0};
+            if (r < w) {
+                const int k = j * nx + i0 + r, q0 = colptr[k];
+                if (t < colptr[k + 1] - q0) {
+                    const int q = q0 + t;
+                    a = sys == 0 ? vals[tq[q]] : vals[q];
+                    if (sys == 1) a.y = -a.y;
+                    const int kk = rowind[q], jj = kk / nx, ii = kk - jj * nx;
+                    li = (jj - j + 1) * (kIrTx + 2) + (ii - i0 + 1);
+                }
+            }
+            sidx[r][t] = li;
+            scoef[r][t] = a;
+        }
+        __syncthreads();
+        const int r = tid >> 4, cc = tid & 15;
+        const int64_t k = (int64_t)j * nx + i0 + r;
+        const int64_t dst = r < w ? (int64_t)se[(kIrTx + 2) + r + 1] * ld : 0;
+        // this thread's row of the stencil, kept in registers for the tile
+        cx<double> a[9];
+        int li[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            a[t] = scoef[r][t];
+            li[t] = sidx[r][t];
+        }
+        for (int c0 = 0; c0 < ld; c0 += kIrCw) {
+            for (int e = tid; e < kIrStage * kIrCw; e += 256) {
+                const int row = e / kIrCw, ec = e % kIrCw, c = c0 + ec;
+                cx<double> x = {0.0, 0.0};
+                if (sg[row] >= 0 && c < nrhs) {
+                    const cx<T> d = D[(int64_t)se[row] * ld + c];
+                    x = {(double)d.x, (double)d.y};
+                    if (!first) x = cadd(x, Xa[(int64_t)sg[row] * ld64 + c]);
+                }
+                sx[row][ec] = x;
+            }
+            __syncthreads();
+            const int c = c0 + cc;
+            cx<double> acc = {0.0, 0.0};
+            if (r < w && c < nrhs) {
+                acc = Bg[k * ld64 + c];
+#pragma unroll
+                for (int t = 0; t < 9; t++) acc = cmsub(acc, a[t], sx[li[t]][cc]);
+                Xb[k * ld64 + c] = sx[(kIrTx + 2) + r + 1][cc];
+                R[dst + c] = {(T)acc.x, (T)acc.y};
+            } else if (r < w && c < ld) {
+                R[dst + c] = {T(0), T(0)};
+            }
+            // the wave's four points of a column: sum, then one LDS update per column
+            double v = acc.x * acc.x + acc.y * acc.y;
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (lane < 16 && c < nrhs) ws[c] += v;
+            __syncthreads();   // sx is restaged for the next chunk
+        }
+    }
+    __syncthreads();
+    for (int c = tid; c < nrhs; c += blockDim.x)
+        partial[((int64_t)z * gridDim.x + blockIdx.x) * nrhs + c] =
+            wsum[c] + wsum[nrhs + c] + wsum[2 * nrhs + c] + wsum[3 * nrhs + c];
+}
+
+// Final iterates (FP64, grid order, stride ld64) -> X (user layout) on slot
+// sel.s[z]; the slot's iterate sits in X1 when bit z of `in1` is set, else in
+// X0.  grid (tiles, slots), 64 x 32 tiles.
+__global__ void __launch_bounds__(256) k_ir_out(const cx<double> *__restrict__ X0, const cx<double> *__restrict__ X1,
+                                                uint64_t in1, cx<double> *__restrict__ X, int n, int nrhs, int ld64,
+                                                Sel sel) {
+    __shared__ cx<double> S[PT_R][PT_K + 1];
+    const int ktiles = (n + PT_K - 1) / PT_K;
+    const int k0 = (blockIdx.x % ktiles) * PT_K, r0 = (blockIdx.x / ktiles) * PT_R;
+    const int z = blockIdx.y, s = sel.s[z];
+    const cx<double> *Xp = (((in1 >> z) & 1) ? X1 : X0) + (int64_t)s * n * ld64;
+    X += (int64_t)s * n * nrhs;
+    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
+        const int rr = e % PT_R, kk = e / PT_R;
+        if (k0 + kk < n && r0 + rr < nrhs) S[rr][kk] = Xp[(int64_t)(k0 + kk) * ld64 + r0 + rr];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
+        const int kk = e % PT_K, rr = e / PT_K;
+        if (k0 + kk < n && r0 + rr < nrhs) X[k0 + kk + (int64_t)(r0 + rr) * n] = S[rr][kk];
+    }
+}
+
 // backward error of one column: residual and max_i |r_i| / (|op(M)||x| + |b|)_i
 __global__ void __launch_bounds__(256) k_backerr(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
                                                  const int *__restrict__ rowind, const int64_t *__restrict__ tq, int n,
@@ -1527,6 +1722,7 @@ struct SolveTuning {
     // multi-block levels take the per-block kernels by default: measured faster
     // than the fused per-front kernels on the 1601x401 workload in FP32
     int rw_cap = 64, fused_mmax = 0;
+    int refine_layout = 0;   // panel refinement: 0 row-major (k_ir_step), 1 user layout
     SolveTuning() {
         if (const char *e = std::getenv("GFDFD_FUSED_RW")) {
             const int v = std::atoi(e);
@@ -1538,6 +1734,7 @@ SolveTuning &tuning() {
     static SolveTuning t;
     return t;
 }
+int refine_mode() { return tuning().refine_layout; }
 
 template <typename T>
 int fused_rw(int mmax, int max_children, bool single) {
@@ -1630,9 +1827,11 @@ void set_smem_attrs() {
 
 }  // namespace
 
-void set_solve_tuning(int rw_cap, int fused_mmax) {
+void set_solve_tuning(int rw_cap, int fused_mmax, int refine_layout) {
     if (rw_cap != 16 && rw_cap != 32 && rw_cap != 64) throw std::invalid_argument("tile width must be 16, 32 or 64");
+    if (refine_layout != 0 && refine_layout != 1) throw std::invalid_argument("refinement layout must be 0 or 1");
     tuning().rw_cap = rw_cap;
+    tuning().refine_layout = refine_layout;
     tuning().fused_mmax = fused_mmax < 0 ? (1 << 30) : fused_mmax;
 }
 
@@ -1942,15 +2141,122 @@ void cast_c64_to_c128(int64_t count, const cx<float> *a, cx<double> *b, hipStrea
     hipLaunchKernelGGL((k_cast<float, double>), dim3(grid_for(count)), dim3(256), 0, st, count, a, b);
 }
 
+// Host side of the stopping test: h holds the squared column norms of the
+// residuals (h[s * nrhs + r]) and of b (h[(ns + s) * nrhs + r]).  Records the
+// worst relative residual of every active slot and keeps those above tol.
+static void keep_unconverged(const std::vector<double> &h, int ns, int nrhs, int it, double tol, int max_iter,
+                             std::vector<int> &act_f, std::vector<int> &act_s, std::vector<RefineStats> &rs) {
+    std::vector<int> nf, nsl;
+    for (size_t z = 0; z < act_s.size(); z++) {
+        const int s = act_s[z];
+        double worst = 0;
+        for (int r = 0; r < nrhs; r++) {
+            const double b = std::sqrt(h[((size_t)ns + s) * nrhs + r]), rr = std::sqrt(h[(size_t)s * nrhs + r]);
+            worst = std::max(worst, b > 0 ? rr / b : rr);
+        }
+        rs[s].max_rel_residual = worst;
+        rs[s].iterations = it;
+        if (worst > tol && it < max_iter) {
+            nf.push_back(act_f[z]);
+            nsl.push_back(s);
+        }
+    }
+    act_f = nf;
+    act_s = nsl;
+}
+
+thread_local Workspace ws_b64, ws_x64a, ws_x64b, ws_p2;
+constexpr int kIrMaxRhs = 1024;   // widest panel of the elimination-order refinement (LDS column sums)
+
+// Row-major panel refinement (grid plans, kVecMaxRhs < nrhs <= kIrMaxRhs): b
+// is transposed once (FP64 in grid order, and the factor-precision copy in
+// elimination order the first solve runs on), every step is one k_ir_step
+// pass plus, while a slot has not converged, one correction solve in place on
+// the new right-hand side, and the final iterates are transposed back once.
+// Slot s of every panel serves frequency freqs[s]; converged slots leave the
+// batch.
+template <typename T>
+std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<int> &freqs, int sys, int nrhs,
+                                       const cx<double> *B, cx<double> *X, double tol, int max_iter,
+                                       hipStream_t st) {
+    const DevPlan &dp = *F.plan;
+    const Plan &pl = *dp.host;
+    const int n = pl.n, ns = (int)freqs.size();
+    set_smem_attrs<T>();
+    const int cl = (int)(128 / sizeof(cx<T>));
+    const int ld = (nrhs + cl - 1) / cl * cl, ld64 = (nrhs + 7) / 8 * 8;
+    const size_t e64 = (size_t)n * ld64 * ns, e32 = (size_t)n * ld * ns;
+    cx<double> *Bp = (cx<double> *)ws_b64.get(sizeof(cx<double>) * e64);
+    cx<double> *Xv[2] = {(cx<double> *)ws_x64a.get(sizeof(cx<double>) * e64),
+                         (cx<double> *)ws_x64b.get(sizeof(cx<double>) * e64)};
+    cx<T> *P[2] = {(cx<T> *)ws_x.get(sizeof(cx<T>) * e32), (cx<T> *)ws_p2.get(sizeof(cx<T>) * e32)};
+    cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)pl.fv_rows * ld * ns);
+    double *nrm = (double *)ws_n.get(sizeof(double) * 2 * nrhs * ns);
+    const int ktiles = (n + PT_K - 1) / PT_K;
+    const int nblk = std::min(((pl.nx + kIrTx - 1) / kIrTx) * pl.ny + 7, 2048) / 8 * 8;   // a multiple of 8
+    double *partial = (double *)ws_p.get(sizeof(double) * (size_t)std::max(ktiles, nblk) * nrhs * ns);
+    std::vector<int> slots(ns);
+    for (int z = 0; z < ns; z++) slots[z] = z;
+    const Sel all = make_sel(freqs, slots);
+    const double nr = nrhs, es = sizeof(cx<T>), nnz = (double)pl.nnz;
+    {
+        ProfScope ps(K_PERMUTE_IN, st, 0.0, ns * ((double)n * nr * (16.0 + 16.0 + es) + 4.0 * n));
+        const int tiles = ktiles * ((std::max(ld, ld64) + PT_R - 1) / PT_R);
+        hipLaunchKernelGGL(k_ir_in<T>, dim3(tiles, ns), dim3(256), 0, st, B, Bp, P[0], dp.iperm, n, nrhs, ld64, ld,
+                           partial, all);
+    }
+    hipLaunchKernelGGL(k_norm_reduce, dim3(nrhs, ns), dim3(256), 0, st, partial, ktiles, nrhs,
+                       nrm + (int64_t)nrhs * ns, all);
+    sweeps<T>(F, all, sys, ld, P[0], Wp, st, nrhs);
+    std::vector<RefineStats> rs(ns);
+    std::vector<double> h(2 * (size_t)nrhs * ns);
+    std::vector<int> act_f = freqs, act_s = slots;
+    uint64_t in1 = 0;   // bit s: slot s's latest iterate is in Xv[1]
+    for (int it = 0;; it++) {
+        const int a = it & 1;
+        const Sel sel = make_sel(act_f, act_s);
+        {
+            // per slot: b, x (not on the first step), d, the neighbours' x and d
+            // from cache, x + d and r written; the stencil values and indices
+            const double panel = (double)n * nr * (16.0 + (it ? 16.0 : 0.0) + es + 16.0 + es);
+            ProfScope ps(K_RESIDUAL, st, sel.n * (8.0 * nnz * nr + 2.0 * n * nr), sel.n * (panel + nnz * 28.0));
+            hipLaunchKernelGGL(k_ir_step<T>, dim3(nblk, sel.n), dim3(256), 4 * sizeof(double) * nrhs, st, F.vals,
+                               dp.colptr, dp.rowind, dp.tq, dp.iperm, pl.nx, pl.ny, nrhs, ld64, ld, sys, pl.nnz, Bp,
+                               it ? Xv[a] : nullptr, P[a], Xv[1 - a], P[1 - a], partial, sel);
+        }
+        {
+            ProfScope ps(K_NORM_REDUCE, st, sel.n * (double)nblk * nrhs, sel.n * 8.0 * (nblk + 1.0) * nrhs);
+            hipLaunchKernelGGL(k_norm_reduce, dim3(nrhs, sel.n), dim3(256), 0, st, partial, nblk, nrhs, nrm, sel);
+        }
+        for (int s : act_s) in1 = a ? (in1 & ~(uint64_t(1) << s)) : (in1 | (uint64_t(1) << s));
+        GFDFD_HIP(hipMemcpyAsync(h.data(), nrm, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+        GFDFD_HIP(hipStreamSynchronize(st));
+        keep_unconverged(h, ns, nrhs, it, tol, max_iter, act_f, act_s, rs);
+        if (act_f.empty()) break;
+        sweeps<T>(F, make_sel(act_f, act_s), sys, ld, P[1 - a], Wp, st, nrhs);   // d = op(M)^-1 r in place
+    }
+    {
+        ProfScope ps(K_PERMUTE_OUT, st, 0.0, ns * ((double)n * nr * 32.0 + 4.0 * n));
+        const int tiles = ktiles * ((nrhs + PT_R - 1) / PT_R);
+        hipLaunchKernelGGL(k_ir_out, dim3(tiles, ns), dim3(256), 0, st, Xv[0], Xv[1], in1, X, n, nrhs, ld64, all);
+    }
+    GFDFD_HIP(hipGetLastError());
+    return rs;
+}
+
 // FP64 refinement of nsel frequencies at once; frequencies leave the batch as
-// they converge (the correction solves batch the remaining ones).  The
-// residual kernel also leaves the column norms, and each correction is added
-// to X by the solve's final transpose.
+// they converge (the correction solves batch the remaining ones).  Panels take
+// the elimination-order refinement above; up to kVecMaxRhs right-hand sides
+// (or more than kIrMaxRhs) run the residual in the user layout, with the
+// column norms from the residual kernel and each correction added to X by the
+// solve's final transpose.
 template <typename T>
 std::vector<RefineStats> solve_refined_batch(const Factors<T> &F, const std::vector<int> &freqs, int sys, int nrhs,
                                              const cx<double> *B, cx<double> *X, double tol, int max_iter,
                                              hipStream_t st) {
     const DevPlan &dp = *F.plan;
+    if (nrhs > kVecMaxRhs && nrhs <= kIrMaxRhs && dp.tq && dp.host->grid && refine_mode() == 0)
+        return refine_panels<T>(F, freqs, sys, nrhs, B, X, tol, max_iter, st);
     const int n = dp.host->n, ns = (int)freqs.size();
     const int64_t panel = (int64_t)n * nrhs;
     cx<double> *R = (cx<double> *)ws_r.get(sizeof(cx<double>) * panel * ns);
@@ -1967,24 +2273,8 @@ std::vector<RefineStats> solve_refined_batch(const Factors<T> &F, const std::vec
         residual_sel(dp, F.vals, sys, nrhs, B, X, R, sel, st, nrm);
         GFDFD_HIP(hipMemcpyAsync(h.data(), nrm, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
         GFDFD_HIP(hipStreamSynchronize(st));
-        std::vector<int> nf, nsl;
-        for (size_t z = 0; z < act_s.size(); z++) {
-            const int s = act_s[z];
-            double worst = 0;
-            for (int r = 0; r < nrhs; r++) {
-                double b = std::sqrt(h[(size_t)nrhs * ns + (size_t)s * nrhs + r]), rr = std::sqrt(h[(size_t)s * nrhs + r]);
-                worst = std::max(worst, b > 0 ? rr / b : rr);
-            }
-            rs[s].max_rel_residual = worst;
-            rs[s].iterations = it;
-            if (worst > tol && it < max_iter) {
-                nf.push_back(act_f[z]);
-                nsl.push_back(s);
-            }
-        }
-        if (nf.empty()) break;
-        act_f = nf;
-        act_s = nsl;
+        keep_unconverged(h, ns, nrhs, it, tol, max_iter, act_f, act_s, rs);
+        if (act_f.empty()) break;
         solve_batch<T, double>(F, act_f, act_s, ns, sys, nrhs, R, X, st, true);   // X += op(M)^-1 r
     }
     return rs;

@@ -73,12 +73,13 @@ def grid_index(x, y, dh, npml, fsshift, nx):
 
 # ---- plan / factors ------------------------------------------------------------
 
-def solve_tuning(rw_cap=64, fused_mmax=0):
+def solve_tuning(rw_cap=64, fused_mmax=0, refine_layout=0):
     """Panel-solve kernel selection (process-wide): widest fused right-hand-side
-    tile, and the largest front order of a multi-block level that takes the
+    tile, the largest front order of a multi-block level that takes the
     fused per-front kernels (0, the default: all such levels take the
-    per-block kernels; negative: as far as LDS allows)."""
-    check(lib().gfdfd_solve_tuning(rw_cap, fused_mmax), "gfdfd_solve_tuning")
+    per-block kernels; negative: as far as LDS allows), and the layout of the
+    panel refinement (0: elimination order; 1: the caller's layout)."""
+    check(lib().gfdfd_solve_tuning(rw_cap, fused_mmax, refine_layout), "gfdfd_solve_tuning")
 
 
 class Plan:

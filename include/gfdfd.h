@@ -144,8 +144,11 @@ int gfdfd_residual(gfdfd_plan plan, const double *d_vals, int sys, int nrhs, con
  * reference): the widest right-hand-side tile of the fused per-front kernels
  * (16, 32 or 64; default 64 or GFDFD_FUSED_RW) and the largest front order of
  * a level with multi-block fronts that still takes them (negative: as far as
- * LDS allows; 0, the default: such levels take the per-block kernels).      */
-int gfdfd_solve_tuning(int rw_cap, int fused_mmax);
+ * LDS allows; 0, the default: such levels take the per-block kernels), and
+ * the layout the refinement of 5 to 1024 right-hand sides works in (0, the
+ * default: elimination order, one fused residual-and-update pass per step;
+ * 1: the caller's column-major layout, two transposes per correction).      */
+int gfdfd_solve_tuning(int rw_cap, int fused_mmax, int refine_layout);
 
 /* ---- shot kernels --------------------------------------------------------------
  * RHS_source_AC / RHS_source_TE for all shots (src/AC/RHS_source_AC.c:46-53):

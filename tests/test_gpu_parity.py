@@ -221,6 +221,46 @@ def test_batched_solves_match_oracle(precision, sysno):
         assert rel(X[z], Xr.T) <= tol, (z, k)
 
 
+@pytest.mark.parametrize("precision", [G.FP32, G.FP64])
+@pytest.mark.parametrize("sysno", [0, 1, 2])
+def test_refinement_layouts_agree(precision, sysno):
+    """The panel refinement in elimination order (one fused residual-and-update
+    pass per step) against the same refinement in the caller's layout (residual
+    there, a transpose before and after every correction): both round x, d and
+    r to the same precisions and sum the stencil in the same order, so with the
+    same number of steps the iterates agree to rounding (the compiler may fuse
+    the stencil products differently in the two kernels): 1e-13; both within
+    1e-8 of the oracle.  66 right-hand sides leave padding columns in both
+    internal row strides."""
+    nx, ny, npml, dh = 41, 30, 6, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=4)
+    freqs = [2.5, 7.0, 13.0]
+    say("refinement layouts", precision, sysno)
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    F = pl.factor(pl.assemble(G.AC, dh, freqs, dev(m1), dev(m2), npml=npml), precision)
+    rng = np.random.default_rng(17 + sysno)
+    B = dev(rng.standard_normal((3, 66, nx * ny)) + 1j * rng.standard_normal((3, 66, nx * ny)))
+    order = [1, 2, 0]
+    out = {}
+    try:
+        for layout in (0, 1):
+            fdfd.solve_tuning(refine_layout=layout)
+            X, its, rr = F.solve_refine_batch(order, B, sys=sysno, tol=1e-12, max_iter=30)
+            out[layout] = (X.cpu().numpy(), list(its), list(rr))
+            say("  layout", layout, "iterations", its, rr)
+    finally:
+        fdfd.solve_tuning()
+    (X0, it0, rr0), (X1, it1, rr1) = out[0], out[1]
+    assert max(rr0) <= 1e-12 and max(rr1) <= 1e-12
+    if it0 == it1:
+        assert rel(X0, X1) <= 1e-13
+    Bh = B.cpu().numpy()
+    for z, k in enumerate(order):
+        _, csc = oracle_system(G.AC, nx, ny, npml, dh, freqs[k], m1, m2)
+        Xr, _ = O.band_solve(*csc, Bh[z].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+        assert rel(X0[z], Xr.T) <= 1e-8 and rel(X1[z], Xr.T) <= 1e-8, (z, k)
+
+
 def random_grid_csc(nx, ny, seed):
     """Random complex values on the 9-point pattern (no diagonal dominance): the
     in-front partial pivoting swaps rows, which the assembled Helmholtz matrices
