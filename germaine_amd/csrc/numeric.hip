@@ -625,7 +625,7 @@ __global__ void k_gsrc(const FrontDev *__restrict__ fronts, int nfronts, const i
 // forward and backward substitutions -- for M as for M^T -- into one product
 // with no dependent phase:  [y; w2 - L21 y] = [inv(L11); L21 inv(L11)] w1  and
 // x1 = [inv(U11), -inv(U11) U12] [y1; x2].  Computed in FP64 for both factor
-// precisions.  One workgroup per front (thread j builds column j of both
+// precisions.  One workgroup per front (eight lanes build column j of both
 // inverses; the products run over all threads, 32 rows or columns at a time),
 // grid (fronts, nfreq).
 // ---------------------------------------------------------------------------
@@ -635,7 +635,6 @@ __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int
                                                   const int *__restrict__ lv_single) {
     __shared__ cx<double> D[32][33];
     __shared__ cx<double> R[32][33];
-    __shared__ cx<double> V[32][33];
     const FrontDev F = fronts[blockIdx.x];
     cx<T> *G = arena + blockIdx.y * fstride + F.buf_off;
     const int m = F.m, p = F.p, tid = threadIdx.x;
@@ -647,21 +646,35 @@ __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int
             D[i][t] = {(double)v.x, (double)v.y};
         }
         __syncthreads();
-        if (tid < kb) {
-            const int j = tid;
+        {
+            // eight adjacent lanes (one wave) per column j: each sums every eighth
+            // term of a row's dot product, xor-shuffles add the partial sums, and
+            // lane 0 of the eight stores the entry the next row reads
+            const int j = tid >> 3, part = tid & 7;
+            auto sum8 = [](cx<double> a) {
+                for (int o = 1; o < 8; o <<= 1) {
+                    a.x += __shfl_xor(a.x, o);
+                    a.y += __shfl_xor(a.y, o);
+                }
+                return a;
+            };
             // column j of inv(L_bb), L_bb unit lower: x_j = 1, x_i = -sum_{j<=t<i} L[i][t] x_t
-            R[j][j] = {1.0, 0.0};
-            for (int i = j + 1; i < kb; i++) {
+            if (part == 0 && j < kb) R[j][j] = {1.0, 0.0};
+            for (int i = 1; i < kb; i++) {
                 cx<double> acc = {0.0, 0.0};
-                for (int t = j; t < i; t++) acc = cmsub(acc, D[i][t], R[t][j]);
-                R[i][j] = acc;
+                if (j < i)
+                    for (int t = j + part; t < i; t += 8) acc = cmsub(acc, D[i][t], R[t][j]);
+                acc = sum8(acc);
+                if (part == 0 && j < i) R[i][j] = acc;
             }
             // column j of inv(U_bb): x_j = 1/U[j][j], x_i = -(sum_{i<t<=j} U[i][t] x_t) / U[i][i]
-            R[j][j] = crecip(D[j][j]);
-            for (int i = j - 1; i >= 0; i--) {
+            if (part == 0 && j < kb) R[j][j] = crecip(D[j][j]);
+            for (int i = kb - 2; i >= 0; i--) {
                 cx<double> acc = {0.0, 0.0};
-                for (int t = i + 1; t <= j; t++) acc = cmsub(acc, D[i][t], R[t][j]);
-                R[i][j] = cdiv(acc, D[i][i]);
+                if (i < j && j < kb)
+                    for (int t = i + 1 + part; t <= j; t += 8) acc = cmsub(acc, D[i][t], R[t][j]);
+                acc = sum8(acc);
+                if (part == 0 && i < j && j < kb) R[i][j] = cdiv(acc, D[i][i]);
             }
         }
         __syncthreads();
@@ -672,6 +685,7 @@ __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int
         __syncthreads();
     }
     if (!lv_single[F.level] || p > 32) return;
+    cx<double>(*V)[33] = D;   // the block's entries are no longer needed: D stages the products' operands
     // L21 <- L21 inv(L11), 32 rows at a time staged in V (all reads before any write)
     for (int r0 = p; r0 < m; r0 += 32) {
         const int rows = min(32, m - r0);
