@@ -14,8 +14,11 @@ Also reported: the same step with FP64 fronts (no refinement), the plain FP64
 path the drop-in ABI executes (per frequency a factorisation and one
 single-RHS solve per shot, device-resident), the FP32+IR wavefield error
 against the FP64 solution, per-kernel event timings with algorithmic
-flops/bytes (the dominant kernel's roofline), and the CPU baseline (the
-oracle on the host cores, on a bounded sample).
+flops/bytes (every class over the warm-up step; the dominant class over the
+timed steps, which gives the roofline), and the CPU baseline (the oracle on
+the host cores, on a bounded sample).  Kernel selection comes from the
+autotuner (germaine_amd/autotune.py): its cached choice for this workload and
+device, or a search over the candidates before anything is timed.
 """
 import argparse
 import json
@@ -94,6 +97,11 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true", help="skip the batched FP64 comparison step")
+    ap.add_argument("--tune", choices=["auto", "search", "off"], default="auto",
+                    help="kernel selection: auto = the autotuner's cached choice for this workload and device, "
+                         "searching (before the timed region) when there is none; search = always search; "
+                         "off = the switches below")
+    ap.add_argument("--small-max", type=int, default=64, help="largest front order factored wholly in LDS")
     ap.add_argument("--fused-rw", type=int, default=64, help="widest fused solve tile (16, 32, 64)")
     ap.add_argument("--fused-mmax", type=int, default=0,
                     help="largest front order of a multi-block level on the fused solve kernels "
@@ -118,16 +126,17 @@ def main():
     from germaine_amd import fdfd, models
 
     L = G.lib()
-    fdfd.solve_tuning(args.fused_rw, args.fused_mmax, args.refine_layout)
     w = models.config(args.config)
     nf, ns = len(w.freqs), len(w.src)
     log(f"bench: {w.name} {w.nx}x{w.ny} freqs={w.freqs} shots={ns}")
     m1 = torch.as_tensor(w.m1, device="cuda")
     m2 = torch.as_tensor(w.m2, device="cuda")
     ksrc = torch.as_tensor(np.asarray(w.src, np.int32), device="cuda")
-    pl = fdfd.Plan(w.nx, w.ny)
+    cfg = dict(small_max=args.small_max, rw_cap=args.fused_rw, fused_mmax=args.fused_mmax,
+               refine_layout=args.refine_layout)
+    fdfd.solve_tuning(cfg["rw_cap"], cfg["fused_mmax"], cfg["refine_layout"])
+    pl = fdfd.Plan(w.nx, w.ny, small_max=cfg["small_max"])
     n = pl.n
-    info = pl.info
     st = torch.cuda.current_stream()
     amps = [fdfd.source_amp(w.physics, f, w.s, w.dh) for f in w.freqs]
     Bs = torch.empty((nf, ns, n), dtype=torch.complex128, device="cuda")
@@ -152,26 +161,62 @@ def main():
             its = []
         return F, its
 
+    def profiling(on, cls=None):
+        if on:
+            G.check(L.gfdfd_profile_select(cls.encode() if cls else None))
+            L.gfdfd_profile_reset()
+        L.gfdfd_profile(1 if on else 0)
+
     def timed(precision, refine, steps, warmup, profile=False):
+        """Warm-up steps (every kernel class event-timed when profile: the per-class
+        breakdown and the dominant class), then the timed steps, in which only the
+        dominant class records events (two per launch), so the headline time carries
+        no per-launch profiling of the other classes."""
+        kw, dom = {}, None
+        if profile and warmup:
+            profiling(True)
         for _ in range(warmup):
             step(precision, refine)
         torch.cuda.synchronize()
+        if profile and warmup:
+            profiling(False)
+            kw = kernel_stats(L, ctypes)
+            dom = max(kw, key=lambda k: kw[k]["ms"]) if kw else None
         if profile:
-            L.gfdfd_profile_reset()
-            L.gfdfd_profile(1)
+            profiling(True, dom)
         t0 = time.perf_counter()
         for _ in range(steps):
             _, its = step(precision, refine)
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / steps
+        kt = {}
         if profile:
-            L.gfdfd_profile(0)
+            profiling(False)
+            kt = kernel_stats(L, ctypes)
+            G.check(L.gfdfd_profile_select(None))
         store.pop(precision, None)                  # release this precision's factors
-        return t, its
+        return t, its, (kw or kt), (kt if dom else {})
+
+    # --- kernel selection: autotuner (before any timed region) ----------------------------
+    chosen_by = "flags"
+    if args.tune != "off":
+        from germaine_amd import autotune
+
+        def measure(c):
+            nonlocal pl
+            fdfd.solve_tuning(c["rw_cap"], c["fused_mmax"], c["refine_layout"])
+            pl = fdfd.Plan(w.nx, w.ny, small_max=c["small_max"])
+            return timed(G.FP32, True, 1, 1)[0]
+
+        k = autotune.key(w.nx, w.ny, nf, ns, G.FP32, torch.cuda.get_device_name(0))
+        cfg, chosen_by, _ = autotune.tune(k, measure, log=log, use_cache=args.tune == "auto")
+        log(f"autotune ({chosen_by}): {cfg}")
+        fdfd.solve_tuning(cfg["rw_cap"], cfg["fused_mmax"], cfg["refine_layout"])
+        pl = fdfd.Plan(w.nx, w.ny, small_max=cfg["small_max"])
+    info = pl.info
 
     # --- headline: FP32 fronts + FP64 refinement, batched -----------------------------
-    t32, its32 = timed(G.FP32, True, args.steps, args.warmup, profile=True)
-    kst = kernel_stats(L, ctypes)
+    t32, its32, kst, kdom = timed(G.FP32, True, args.steps, args.warmup, profile=True)
     log(f"fp32+IR batched: {t32 * 1e3:.1f} ms/step, refinement (iters, relres) per freq {its32}")
     X32 = Xs.clone() if not args.no_check else None
 
@@ -179,8 +224,7 @@ def main():
     n64 = max(1, args.steps // 2)
     t64, kst64 = None, {}
     if not args.no_fp64:
-        t64, _ = timed(G.FP64, False, n64, 1, profile=True)
-        kst64 = kernel_stats(L, ctypes)
+        t64, _, kst64, _ = timed(G.FP64, False, n64, 1, profile=True)
         log(f"fp64 batched: {t64 * 1e3:.1f} ms/step")
     err = None
     if X32 is not None and t64 is not None:
@@ -218,13 +262,16 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # --- roofline of the dominant kernel -----------------------------------------------------
-    dom = max(kst, key=lambda k: kst[k]["ms"]) if kst else None
+    # --- roofline of the dominant kernel, event-timed over the timed steps ------------------
+    # (kst: every class over the warm-up steps; kdom: the dominant class over the timed steps)
+    kst_steps = args.warmup if args.warmup else args.steps
+    src = kdom if kdom else kst
+    dom = max(src, key=lambda k: src[k]["ms"]) if src else None
     roof = None
     if dom:
-        roof = roofline(dom, kst[dom], PEAK["fp32_mfma"])
-        roof["launches_per_step"] = kst[dom]["launches"] / args.steps
-        roof["ms_per_step"] = kst[dom]["ms"] / args.steps
+        roof = roofline(dom, src[dom], PEAK["fp32_mfma"])
+        roof["launches_per_step"] = src[dom]["launches"] / args.steps
+        roof["ms_per_step"] = src[dom]["ms"] / args.steps
         # HBM bytes per launch of this kernel from the committed FETCH_SIZE/WRITE_SIZE
         # pmc runs of this bench (copy-calibrated); null when they cover another kernel
         tr = measured_traffic(dom, w.name)
@@ -258,18 +305,19 @@ def main():
         "config": {"workload": f"{w.name}: AC {w.nx}x{w.ny} incl. {w.npml}-cell PML, h={w.dh} m, "
                                f"{nf} freqs {w.freqs[0]:g}-{w.freqs[-1]:g} Hz batched, {ns}-shot panel",
                    "n": n, "nnz": pl.nnz, "fronts": info.fronts, "factor_gflop_per_freq": info.factor_flops / 1e9,
-                   "nnz_lu": info.nnz_lu, "refine_tol": args.tol, "fused_rw": args.fused_rw,
-                   "fused_mmax": args.fused_mmax, "refine_layout": args.refine_layout},
+                   "nnz_lu": info.nnz_lu, "refine_tol": args.tol,
+                   "tuning": dict(cfg, chosen_by=chosen_by)},
         "fp64_batched": None if t64 is None else {
             "ms_per_step": round(t64 * 1e3, 3), "value": round(nf * ns / t64, 2),
-            "kernels_ms_per_step": {k: round(v["ms"] / n64, 3) for k, v in kst64.items()}},
+            "kernels_ms_per_step": {k: round(v["ms"], 3) for k, v in kst64.items()}},
         "plain_fp64_per_shot": None if tplain is None else {"ms": round(tplain * 1e3, 3),
                                                              "value": round(nf * ns / tplain, 2)},
         "speedup_vs_plain_fp64": None if tplain is None else round(tplain / t32, 3),
         "refinement": [{"iters": a, "relres": b} for a, b in its32],
         "max_rel_err_vs_fp64": None if err is None else max(err),
         "roofline": roof,
-        "kernels": {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] / args.steps,
+        "kernels_note": "every kernel class event-timed over the warm-up step(s), per step",
+        "kernels": {k: {"ms_per_step": round(v["ms"] / kst_steps, 3), "launches_per_step": v["launches"] / kst_steps,
                         "tflops": round(v["flops"] / max(v["ms"], 1e-9) * 1e-9, 3),
                         "gbps": round(v["bytes"] / max(v["ms"], 1e-9) * 1e-6, 1)} for k, v in kst.items()},
         "cpu_baseline": cpu,

@@ -75,6 +75,7 @@ SIGNATURES = {
     "gfdfd_grad_ac": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_grad_te": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_profile": (_I, [_I]),
+    "gfdfd_profile_select": (_I, [C.c_char_p]),
     "gfdfd_profile_reset": (_V, []),
     "gfdfd_profile_read": (_I, [C.c_char_p, _DP, _LLP, _DP, _DP]),
     "umfpack_zi_triplet_to_col": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

@@ -27,6 +27,8 @@ struct Totals {
 };
 std::mutex mu;
 bool enabled = false;
+bool selected[K_NCLASS] = {};   // classes that record events while enabled
+bool select_all = true;
 std::vector<Rec> pending;
 std::vector<hipEvent_t> pool;
 Totals totals[K_NCLASS];
@@ -62,7 +64,7 @@ void drain() {
 bool prof_enabled() { return enabled; }
 
 ProfScope::ProfScope(KClass c, hipStream_t st, double flops, double bytes) : st_(st) {
-    if (!enabled) return;
+    if (!enabled || !(select_all || selected[c])) return;
     std::lock_guard<std::mutex> lk(mu);
     Rec r{c, get_event(), get_event(), flops, bytes};
     if (!r.a || !r.b) return;
@@ -89,6 +91,22 @@ int gfdfd_profile(int enable) {
     if (!enable) drain();
     enabled = enable != 0;
     return GFDFD_OK;
+}
+
+int gfdfd_profile_select(const char *kernel) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!kernel || !*kernel) {
+        select_all = true;
+        return GFDFD_OK;
+    }
+    for (int c = 0; c < K_NCLASS; c++)
+        if (std::strcmp(kernel, kClassNames[c]) == 0) {
+            for (bool &b : selected) b = false;
+            selected[c] = true;
+            select_all = false;
+            return GFDFD_OK;
+        }
+    return GFDFD_ERR_ARG;
 }
 
 void gfdfd_profile_reset(void) {

@@ -187,6 +187,9 @@ int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, co
  * No counterpart in the reference (it times whole phases with MPI_Wtime,
  * src/AC/forward_shot_AC.c:54-65).                                          */
 int gfdfd_profile(int enable);
+/* record only the named class (NULL or "": every class), e.g. to time one
+ * kernel inside a timed region at the cost of two events per launch       */
+int gfdfd_profile_select(const char *kernel);
 void gfdfd_profile_reset(void);
 int gfdfd_profile_read(const char *kernel, double *ms, long long *launches, double *flops, double *bytes);
 
