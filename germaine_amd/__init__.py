@@ -61,6 +61,7 @@ SIGNATURES = {
     "gfdfd_assemble": (_I, [_P, _I, _F, _I, C.POINTER(FreqParams), _P, _P, _P, _P]),
     "gfdfd_factor": (_I, [_P, _I, _P, _I, _P, C.POINTER(_P)]),
     "gfdfd_refactor": (_I, [_P, _P, _P]),
+    "gfdfd_refactor_graph": (_I, [_P, _P, _P]),
     "gfdfd_solve_tuning": (_I, [_I, _I, _I]),
     "gfdfd_factors_free": (_V, [C.POINTER(_P)]),
     "gfdfd_factors_singular": (_I, [_P, _P]),

@@ -8,6 +8,7 @@
 #include "../../include/gfdfd.h"
 #include "assemble.hpp"
 #include "engine.hpp"
+#include "prof.hpp"
 #include "runtime.hpp"
 #include "shot.hpp"
 
@@ -103,6 +104,39 @@ void refactor(Factors<T> &F, const cx<double> *vals, hipStream_t st) {
     if (!F.own_vals) throw std::invalid_argument("factors do not own their values");
     GFDFD_HIP(hipMemcpyAsync(F.vals, vals, sizeof(cx<double>) * pl.nnz * F.nfreq, hipMemcpyDeviceToDevice, st));
     factor_numeric<T>(*F.plan, F, F.vals, pl.nnz, st);
+}
+
+// the same as one graph launch: the first call with a given source buffer
+// captures the copy and every factorisation kernel on a private stream; later
+// calls with the same buffer replay the graph on st
+template <typename T>
+void refactor_graph(Factors<T> &F, const cx<double> *vals, hipStream_t st) {
+    if (!F.graph || F.graph_src != vals) {
+        ProfPause no_events;   // captured launches carry no profiling events
+        if (F.graph) {
+            GFDFD_HIP(hipGraphExecDestroy(F.graph));
+            F.graph = nullptr;
+        }
+        hipStream_t cs;
+        GFDFD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        GFDFD_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        try {
+            refactor<T>(F, vals, cs);
+        } catch (...) {
+            (void)hipStreamEndCapture(cs, &g);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        GFDFD_HIP(hipStreamEndCapture(cs, &g));
+        const hipError_t e = hipGraphInstantiate(&F.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cs);
+        GFDFD_HIP(e);
+        F.graph_src = vals;
+    }
+    GFDFD_HIP(hipGraphLaunch(F.graph, st));
 }
 
 template std::unique_ptr<Factors<double>> make_factors<double>(std::shared_ptr<DevPlan>, int, const cx<double> *, bool,
@@ -295,6 +329,16 @@ void gfdfd_factors_free(gfdfd_factors *f) {
         delete *f;
         *f = nullptr;
     }
+}
+
+int gfdfd_refactor_graph(gfdfd_factors f, const double *d_vals, void *stream) {
+    if (!f || !d_vals) return report(GFDFD_ERR_ARG, "gfdfd_refactor_graph: NULL argument");
+    GUARD_BEGIN
+    const cx<double> *v = reinterpret_cast<const cx<double> *>(d_vals);
+    if (f->f64) refactor_graph<double>(*f->f64, v, S(stream));
+    else refactor_graph<float>(*f->f32, v, S(stream));
+    return GFDFD_OK;
+    GUARD_END
 }
 
 int gfdfd_factors_singular(gfdfd_factors f, int *flags) {

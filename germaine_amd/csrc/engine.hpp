@@ -65,6 +65,8 @@ struct Factors {
     int *gsrc = nullptr;            // nfreq * fv_rows * 3: gather sources after pivoting (forward sweep of M)
     cx<double> *vals = nullptr;     // nfreq * nnz FP64 values of M (CSC order) kept for refinement
     bool own_vals = false;
+    hipGraphExec_t graph = nullptr; // captured refactorisation (gfdfd_refactor_graph)
+    const void *graph_src = nullptr;   // the values buffer it reads
     ~Factors();
 };
 

@@ -190,6 +190,7 @@ DevPlan::~DevPlan() {
 
 template <typename T>
 Factors<T>::~Factors() {
+    if (graph) (void)hipGraphExecDestroy(graph);
     dev_free(arena);
     dev_free(piv);
     dev_free(rowperm);
@@ -578,6 +579,13 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
 // row permutation of every front from its swap sequence: rowperm[pos0+i] is
 // the local row that ends at position i
 // ---------------------------------------------------------------------------
+// zero-pivot flags cleared by a kernel rather than a memset node, so the
+// factorisation can be captured into a graph and replayed
+__global__ void k_clear_flags(int *__restrict__ flags, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) flags[i] = 0;
+}
+
 __global__ void k_rowperm(const FrontDev *__restrict__ fronts, int nfronts, const int *__restrict__ piv, int n,
                           int *__restrict__ rowperm) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -729,7 +737,7 @@ template <typename T>
 void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, int64_t vstride, hipStream_t st) {
     const Plan &pl = *dp.host;
     const int nf = F.nfreq;
-    GFDFD_HIP(hipMemsetAsync(F.sing, 0, sizeof(int) * nf, st));
+    hipLaunchKernelGGL(k_clear_flags, dim3((nf + 63) / 64), dim3(64), 0, st, F.sing, nf);
     const size_t lds_small_max = 160 * 1024 - 256;
     const size_t panel_lds_max = 160 * 1024 - 1024;   // panels of up to 640 c64 / 320 c128 candidate rows
     static bool attr_set = false;

@@ -61,10 +61,15 @@ void drain() {
 }
 }  // namespace
 
-bool prof_enabled() { return enabled; }
+static thread_local int paused = 0;   // ProfPause depth of this thread
+
+bool prof_enabled() { return enabled && paused == 0; }
+
+ProfPause::ProfPause() { paused++; }
+ProfPause::~ProfPause() { paused--; }
 
 ProfScope::ProfScope(KClass c, hipStream_t st, double flops, double bytes) : st_(st) {
-    if (!enabled || !(select_all || selected[c])) return;
+    if (!enabled || paused || !(select_all || selected[c])) return;
     std::lock_guard<std::mutex> lk(mu);
     Rec r{c, get_event(), get_event(), flops, bytes};
     if (!r.a || !r.b) return;

@@ -17,6 +17,14 @@ extern const char *const kClassNames[K_NCLASS];
 
 bool prof_enabled();
 
+// While alive, this thread's launches record no events (graph capture).
+struct ProfPause {
+    ProfPause();
+    ~ProfPause();
+    ProfPause(const ProfPause &) = delete;
+    ProfPause &operator=(const ProfPause &) = delete;
+};
+
 // Records a start event on construction and an end event on destruction
 // (both on `st`) when profiling is enabled; otherwise does nothing.
 class ProfScope {

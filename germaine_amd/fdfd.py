@@ -120,9 +120,11 @@ class Plan:
         check(lib().gfdfd_plan_index(self.h, idx.ctypes.data), "gfdfd_plan_index")
         return dict(p=p, m=m, pos0=pos0, parent=parent, level=level, idx_off=idx_off, idx=idx)
 
-    def assemble(self, physics, dh, freqs, m1, m2, stream=None, s=0.0, npml=0, a0_pml=90.0, free_surf=0):
+    def assemble(self, physics, dh, freqs, m1, m2, stream=None, s=0.0, npml=0, a0_pml=90.0, free_surf=0,
+                 out=None):
         """Device assembly for a list of frequencies; m1/m2 float32 cuda [ny, nx].
-        Returns complex128 cuda [nfreq, nnz] (values of M in CSC order)."""
+        Returns complex128 cuda [nfreq, nnz] (values of M in CSC order), written
+        into `out` when given."""
         import torch
         nf = len(freqs)
         profs, fp = [], (FreqParams * nf)()
@@ -133,7 +135,8 @@ class Plan:
             fp[k].omega2[0], fp[k].omega2[1] = float(o2[0]), float(o2[1])
             fp[k].omega[0], fp[k].omega[1] = float(om[0]), float(om[1])
             fp[k].prof = prof.ctypes.data_as(C.POINTER(C.c_float))
-        vals = torch.empty((nf, self.nnz), dtype=torch.complex128, device="cuda")
+        vals = torch.empty((nf, self.nnz), dtype=torch.complex128, device="cuda") if out is None else out
+        assert vals.shape == (nf, self.nnz) and vals.dtype == torch.complex128 and vals.is_contiguous()
         m1 = m1.contiguous()
         m2 = m2.contiguous()
         check(lib().gfdfd_assemble(self.h, physics, dh, nf, fp, _ptr(m1), _ptr(m2), _ptr(vals), _stream(stream)),
@@ -170,6 +173,13 @@ class Factors:
         vals = vals.contiguous()
         assert vals.shape[0] == self.nfreq
         check(lib().gfdfd_refactor(self.h, _ptr(vals), _stream(stream)), "gfdfd_refactor")
+        return self
+
+    def refactor_graph(self, vals, stream=None):
+        """refactor as one hipGraph launch (captured on the first call with this
+        values buffer, replayed afterwards: refill the same buffer between calls)."""
+        assert vals.is_contiguous() and vals.shape[0] == self.nfreq
+        check(lib().gfdfd_refactor_graph(self.h, _ptr(vals), _stream(stream)), "gfdfd_refactor_graph")
         return self
 
     def __del__(self):

@@ -112,6 +112,15 @@ int gfdfd_factor(gfdfd_plan plan, int nfreq, const double *d_vals, int precision
  * line-search trial without a new allocation (src/AC/fwi_FD_AC.c:229-409
  * refactors every frequency with umfpack_zi_numeric each time)              */
 int gfdfd_refactor(gfdfd_factors f, const double *d_vals, void *stream);
+/* gfdfd_refactor as one hipGraph launch: the first call with a given d_vals
+ * captures the value copy and every factorisation kernel (all frequencies,
+ * all levels) into a graph; later calls with the same d_vals pointer replay
+ * it -- write the next values into that buffer (e.g. with gfdfd_assemble)
+ * and call again.  A new pointer recaptures.  Graph launches record no
+ * gfdfd_profile events.
+ * The FWI loop's per-iteration refactorisation (src/AC/fwi_FD_AC.c:229-409)
+ * without per-kernel launch cost.                                           */
+int gfdfd_refactor_graph(gfdfd_factors f, const double *d_vals, void *stream);
 void gfdfd_factors_free(gfdfd_factors *f);
 /* per-frequency zero-pivot flags (nfreq ints) */
 int gfdfd_factors_singular(gfdfd_factors f, int *flags);

@@ -343,3 +343,36 @@ def test_refactor_equals_fresh_factorisation():
         Xa = F.solve_batch([0, 1], B).cpu().numpy()
         Xb = pl.factor(vb, precision).solve_batch([0, 1], B).cpu().numpy()
         assert rel(Xa, Xb) <= 1e-13
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_refactor_graph_replays_match_eager(precision):
+    """The refactorisation captured as one graph and replayed: every replay, with
+    eager work and synchronisations between replays and the values buffer
+    refilled in between, equals the eager refactorisation of the same values
+    bit for bit (same kernels, same order), and the oracle to 1e-10 (FP64)."""
+    nx, ny, npml, dh = 61, 41, 6, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=9)
+    say("refactor graph", precision)
+    pl = fdfd.Plan(nx, ny)
+    fa, fb = [3.0, 8.0], [5.0, 12.0]
+    va = pl.assemble(G.AC, dh, fa, dev(m1), dev(m2), npml=npml)
+    vb = pl.assemble(G.AC, dh, fb, dev(m1), dev(m2), npml=npml)
+    rng = np.random.default_rng(4)
+    B = dev(rng.standard_normal((2, 8, nx * ny)) + 1j * rng.standard_normal((2, 8, nx * ny)))
+    ref = {}
+    for name, v in (("a", va), ("b", vb)):
+        ref[name] = pl.factor(v, precision).solve_batch([0, 1], B).cpu().numpy()
+    buf = torch.empty_like(va)
+    F = pl.factor(va, precision)
+    for k, name in enumerate(["b", "a", "b", "b", "a"]):
+        buf.copy_(vb if name == "b" else va)
+        F.refactor_graph(buf)
+        X = F.solve_batch([0, 1], B).cpu().numpy()          # eager solves between the replays
+        torch.cuda.synchronize()
+        assert np.array_equal(X, ref[name]), (k, name)
+    if precision == G.FP64:
+        for z, f in enumerate(fa):
+            _, csc = oracle_system(G.AC, nx, ny, npml, dh, f, m1, m2)
+            Xr, _ = O.band_solve(*csc, B[z].cpu().numpy().T, perm=O.grid_transpose_perm(nx, ny))
+            assert rel(X[z], Xr.T) <= 1e-10
