@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 
 #include "engine.hpp"
@@ -1493,16 +1494,20 @@ __global__ void __launch_bounds__(256) k_ir_in(const cx<double> *__restrict__ B,
 // point's x is used by up to nine rows), then one thread per (point, column)
 // applies the stencil from LDS.  grid (blocks, slots), grid-stride over tiles.
 constexpr int kIrTx = 16, kIrCw = 16, kIrStage = 3 * (kIrTx + 2);
+constexpr int kIrThreads = kIrTx * kIrCw, kIrWaves = kIrThreads / 64;
+constexpr int kIrCoef0 = (kIrStage + 63) / 64 * 64;   // first thread of the stencil-entry gather
+static_assert(kIrStage <= kIrCoef0 && kIrCoef0 + 9 * kIrTx <= kIrThreads,
+              "one thread per staged point and per stencil entry of the tile");
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
+__global__ void __launch_bounds__(kIrThreads) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
                                                  const int *__restrict__ rowind, const int64_t *__restrict__ tq,
                                                  const int *__restrict__ iperm, int nx, int ny, int nrhs, int ld64,
                                                  int ld, int sys, int64_t nnz, const cx<double> *__restrict__ Bg,
                                                  const cx<double> *__restrict__ Xa, const cx<T> *__restrict__ D,
                                                  cx<double> *__restrict__ Xb, cx<T> *__restrict__ R,
                                                  double *__restrict__ partial, Sel sel) {
-    extern __shared__ double wsum[];   // [4][nrhs]
+    extern __shared__ double wsum[];   // [kIrWaves][nrhs]
     __shared__ cx<double> sx[kIrStage][kIrCw];
     __shared__ cx<double> scoef[kIrTx][9];
     __shared__ int sidx[kIrTx][9];     // staged row of each stencil entry
@@ -1535,8 +1540,8 @@ __global__ void __launch_bounds__(256) k_ir_step(const cx<double> *__restrict__ 
             const bool in = jj >= 0 && jj < ny && ii >= 0 && ii < nx;
             sg[tid] = in ? jj * nx + ii : -1;
             se[tid] = in ? iperm[jj * nx + ii] : 0;
-        } else if (tid >= 64 && tid < 64 + kIrTx * 9) {
-            const int r = (tid - 64) / 9, t = (tid - 64) % 9;
+        } else if (tid >= kIrCoef0 && tid < kIrCoef0 + kIrTx * 9) {
+            const int r = (tid - kIrCoef0) / 9, t = (tid - kIrCoef0) % 9;
             int li = (kIrTx + 2) + r + 1;   // missing entries: zero value on the point itself
             cx<double> a = {0.0, 0.0};
             if (r < w) {
@@ -1565,7 +1570,7 @@ __global__ void __launch_bounds__(256) k_ir_step(const cx<double> *__restrict__ 
             li[t] = sidx[r][t];
         }
         for (int c0 = 0; c0 < ld; c0 += kIrCw) {
-            for (int e = tid; e < kIrStage * kIrCw; e += 256) {
+            for (int e = tid; e < kIrStage * kIrCw; e += kIrThreads) {
                 const int row = e / kIrCw, ec = e % kIrCw, c = c0 + ec;
                 cx<double> x = {0.0, 0.0};
                 if (sg[row] >= 0 && c < nrhs) {
@@ -1596,9 +1601,11 @@ __global__ void __launch_bounds__(256) k_ir_step(const cx<double> *__restrict__ 
         }
     }
     __syncthreads();
-    for (int c = tid; c < nrhs; c += blockDim.x)
-        partial[((int64_t)z * gridDim.x + blockIdx.x) * nrhs + c] =
-            wsum[c] + wsum[nrhs + c] + wsum[2 * nrhs + c] + wsum[3 * nrhs + c];
+    for (int c = tid; c < nrhs; c += blockDim.x) {
+        double v = 0.0;
+        for (int q = 0; q < kIrWaves; q++) v += wsum[q * nrhs + c];
+        partial[((int64_t)z * gridDim.x + blockIdx.x) * nrhs + c] = v;
+    }
 }
 
 // Final iterates (FP64, grid order, stride ld64) -> X (user layout) on slot
@@ -2146,6 +2153,22 @@ void cast_c64_to_c128(int64_t count, const cx<float> *a, cx<double> *b, hipStrea
 // worst relative residual of every active slot and keeps those above tol.
 static void keep_unconverged(const std::vector<double> &h, int ns, int nrhs, int it, double tol, int max_iter,
                              std::vector<int> &act_f, std::vector<int> &act_s, std::vector<RefineStats> &rs) {
+    static const bool trace = std::getenv("GFDFD_IR_TRACE") != nullptr;
+    if (trace)
+        for (size_t z = 0; z < act_s.size(); z++) {
+            // per step and frequency: columns above tol, and the relative residual's quantiles
+            const int s = act_s[z];
+            std::vector<double> rel(nrhs);
+            for (int r = 0; r < nrhs; r++) {
+                const double b = std::sqrt(h[((size_t)ns + s) * nrhs + r]), rr = std::sqrt(h[(size_t)s * nrhs + r]);
+                rel[r] = b > 0 ? rr / b : rr;
+            }
+            std::sort(rel.begin(), rel.end());
+            const int above = (int)(rel.end() - std::upper_bound(rel.begin(), rel.end(), tol));
+            std::fprintf(stderr, "refine step %d freq %d: %d/%d columns above %.1e; rel residual min %.2e median %.2e "
+                         "p90 %.2e max %.2e\n", it, act_f[z], above, nrhs, tol, rel[0], rel[nrhs / 2],
+                         rel[(nrhs * 9) / 10], rel[nrhs - 1]);
+        }
     std::vector<int> nf, nsl;
     for (size_t z = 0; z < act_s.size(); z++) {
         const int s = act_s[z];
@@ -2166,7 +2189,11 @@ static void keep_unconverged(const std::vector<double> &h, int ns, int nrhs, int
 }
 
 thread_local Workspace ws_b64, ws_x64a, ws_x64b, ws_p2;
-constexpr int kIrMaxRhs = 1024;   // widest panel of the elimination-order refinement (LDS column sums)
+// widest panel of the row-major refinement: its per-wave column sums and the
+// staged neighbourhood share the default 64 KB of LDS
+constexpr int kIrMaxRhs = 1024;
+static_assert(kIrWaves * 8 * kIrMaxRhs + (kIrStage * kIrCw + 9 * kIrTx) * 16 + (9 * kIrTx + 2 * kIrStage) * 4 <= 65536,
+              "refinement-step LDS");
 
 // Row-major panel refinement (grid plans, kVecMaxRhs < nrhs <= kIrMaxRhs): b
 // is transposed once (FP64 in grid order, and the factor-precision copy in
@@ -2220,7 +2247,8 @@ std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<in
             // from cache, x + d and r written; the stencil values and indices
             const double panel = (double)n * nr * (16.0 + (it ? 16.0 : 0.0) + es + 16.0 + es);
             ProfScope ps(K_RESIDUAL, st, sel.n * (8.0 * nnz * nr + 2.0 * n * nr), sel.n * (panel + nnz * 28.0));
-            hipLaunchKernelGGL(k_ir_step<T>, dim3(nblk, sel.n), dim3(256), 4 * sizeof(double) * nrhs, st, F.vals,
+            hipLaunchKernelGGL(k_ir_step<T>, dim3(nblk, sel.n), dim3(kIrThreads), kIrWaves * sizeof(double) * nrhs, st,
+                               F.vals,
                                dp.colptr, dp.rowind, dp.tq, dp.iperm, pl.nx, pl.ny, nrhs, ld64, ld, sys, pl.nnz, Bp,
                                it ? Xv[a] : nullptr, P[a], Xv[1 - a], P[1 - a], partial, sel);
         }
