@@ -1498,6 +1498,7 @@ constexpr int kIrThreads = kIrTx * kIrCw, kIrWaves = kIrThreads / 64;
 constexpr int kIrCoef0 = (kIrStage + 63) / 64 * 64;   // first thread of the stencil-entry gather
 static_assert(kIrStage <= kIrCoef0 && kIrCoef0 + 9 * kIrTx <= kIrThreads,
               "one thread per staged point and per stencil entry of the tile");
+constexpr int kIrPer = (kIrStage * kIrCw + kIrThreads - 1) / kIrThreads;   // staging entries per thread
 
 template <typename T>
 __global__ void __launch_bounds__(kIrThreads) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
@@ -1570,21 +1571,31 @@ __global__ void __launch_bounds__(kIrThreads) k_ir_step(const cx<double> *__rest
             li[t] = sidx[r][t];
         }
         for (int c0 = 0; c0 < ld; c0 += kIrCw) {
-            for (int e = tid; e < kIrStage * kIrCw; e += kIrThreads) {
-                const int row = e / kIrCw, ec = e % kIrCw, c = c0 + ec;
-                cx<double> x = {0.0, 0.0};
-                if (sg[row] >= 0 && c < nrhs) {
-                    const cx<T> d = D[(int64_t)se[row] * ld + c];
-                    x = {(double)d.x, (double)d.y};
-                    if (!first) x = cadd(x, Xa[(int64_t)sg[row] * ld64 + c]);
+            const int c = c0 + cc;
+            const bool own = r < w && c < nrhs;
+            // every load of the chunk is issued before the first is used: b, then the
+            // neighbourhood's d and x (a fixed number of staging entries per thread)
+            cx<double> acc = own ? Bg[k * ld64 + c] : cx<double>{0.0, 0.0};
+            cx<T> dv[kIrPer];
+            cx<double> xv[kIrPer];
+#pragma unroll
+            for (int q = 0; q < kIrPer; q++) {
+                const int e = tid + kIrThreads * q, row = e / kIrCw, cs = c0 + e % kIrCw;
+                dv[q] = {T(0), T(0)};
+                xv[q] = {0.0, 0.0};
+                if (e < kIrStage * kIrCw && sg[row] >= 0 && cs < nrhs) {
+                    dv[q] = D[(int64_t)se[row] * ld + cs];
+                    if (!first) xv[q] = Xa[(int64_t)sg[row] * ld64 + cs];
                 }
-                sx[row][ec] = x;
+            }
+#pragma unroll
+            for (int q = 0; q < kIrPer; q++) {
+                const int e = tid + kIrThreads * q;
+                if (e < kIrStage * kIrCw)
+                    sx[e / kIrCw][e % kIrCw] = cadd(xv[q], cx<double>{(double)dv[q].x, (double)dv[q].y});
             }
             __syncthreads();
-            const int c = c0 + cc;
-            cx<double> acc = {0.0, 0.0};
-            if (r < w && c < nrhs) {
-                acc = Bg[k * ld64 + c];
+            if (own) {
 #pragma unroll
                 for (int t = 0; t < 9; t++) acc = cmsub(acc, a[t], sx[li[t]][cc]);
                 Xb[k * ld64 + c] = sx[(kIrTx + 2) + r + 1][cc];
