@@ -246,6 +246,40 @@ __device__ __forceinline__ void sub_tile(cx<T> *w, int nrhs, int row0, int rowen
             }
 }
 
+// the same, with the tile's current values of w loaded beforehand into c
+// (load_tile), so the read-modify-write waits on nothing
+template <typename T>
+__device__ __forceinline__ void load_tile(const cx<T> *w, int nrhs, int row0, int rowend, int r0, cx<T> (&c)[2][2][4]) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = row0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = r0 + wc + b * 16 + lr;
+                c[a][b][v] = (row < rowend && col < nrhs) ? w[(int64_t)row * nrhs + col] : cx<T>{T(0), T(0)};
+            }
+}
+template <typename T>
+__device__ __forceinline__ void sub_tile_from(cx<T> *w, int nrhs, int row0, int rowend, int r0,
+                                              const cx<T> (&c)[2][2][4], typename mfma_t<T>::acc (&cr)[2][2],
+                                              typename mfma_t<T>::acc (&ci)[2][2]) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = row0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = r0 + wc + b * 16 + lr;
+                if (row < rowend && col < nrhs)
+                    w[(int64_t)row * nrhs + col] = {c[a][b][v].x - cr[a][b][v], c[a][b][v].y - ci[a][b][v]};
+            }
+}
+
 // Y[0:nb] = D Y[0:nb] in LDS (D: the block operator, zero outside nb x nb):
 // waves own 16 rows x 32 right-hand sides each
 template <typename T>
@@ -280,6 +314,9 @@ __device__ __forceinline__ void diag_apply(const BlkSmem<T> &s, int nb, int ncol
 
 // forward block b0 of every front of a level: y_b = Dop_bb w_b (written to X
 // by the row-tile-0 workgroup), then rows [b0+nb, m) of W -= Lop[:, b] y_b.
+// Everything the workgroup reads -- the diagonal operator, y's rows, the
+// operator block below and the rows it updates -- is requested before the
+// first barrier, so the workgroup waits on global memory once.
 // grid: (row tiles, fronts, right-hand-side tiles)
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena, int64_t fstride,
@@ -293,21 +330,54 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
     const int m = F.m, nb = min(BK, F.p - b0), tid = threadIdx.x;
     const int row0 = b0 + nb + blockIdx.x * 64;
     if (blockIdx.x > 0 && row0 >= m) return;
+    const bool upd = row0 < m;
     const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64, ncol = min(64, nrhs - r0);
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
     auto Lop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
-    for (int e = tid; e < BK * BK; e += 256) {
-        const int i = e % BK, t = e / BK;
-        s.D[i * (BK + 1) + t] = (i < nb && t < nb) ? dop(G, m, b0, i, t, true, trans, cj) : cx<T>{T(0), T(0)};
-    }
-    for (int e = tid; e < BK * 64; e += 256) {
-        int k = e >> 6, j = e & 63;
-        cx<T> v = {T(0), T(0)};
-        if (k < nb && j < ncol) v = w[(int64_t)(b0 + k) * nrhs + r0 + j];
-        s.Yr[e] = v.x;
-        s.Yi[e] = v.y;
+    cx<T> wold[2][2][4];
+    if (upd) load_tile<T>(w, nrhs, row0, m, r0, wold);
+    {
+        cx<T> dv[BK * BK / 256], yv[BK * 64 / 256], av[BK * 64 / 256];
+#pragma unroll
+        for (int u = 0; u < BK * BK / 256; u++) {
+            const int e = tid + 256 * u, i = e % BK, t = e / BK;
+            dv[u] = (i < nb && t < nb) ? dop(G, m, b0, i, t, true, trans, cj) : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < BK * 64 / 256; u++) {
+            const int e = tid + 256 * u, k = e >> 6, j = e & 63;
+            yv[u] = (k < nb && j < ncol) ? w[(int64_t)(b0 + k) * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < BK * 64 / 256; u++) {
+            const int e = tid + 256 * u;
+            int k, i;
+            if (!trans) { i = e & 63; k = e >> 6; }        // column-major L: coalesced along rows
+            else { k = e % BK; i = e / BK; }               // U^T: coalesced along k
+            av[u] = (upd && k < nb && row0 + i < m) ? Lop(row0 + i, b0 + k) : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < BK * BK / 256; u++) {
+            const int e = tid + 256 * u;
+            s.D[(e % BK) * (BK + 1) + e / BK] = dv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < BK * 64 / 256; u++) {
+            const int e = tid + 256 * u;
+            s.Yr[e] = yv[u].x;
+            s.Yi[e] = yv[u].y;
+        }
+#pragma unroll
+        for (int u = 0; u < BK * 64 / 256; u++) {
+            const int e = tid + 256 * u;
+            int k, i;
+            if (!trans) { i = e & 63; k = e >> 6; }
+            else { k = e % BK; i = e / BK; }
+            s.Ar[k * 64 + i] = av[u].x;
+            s.Ai[k * 64 + i] = av[u].y;
+        }
     }
     __syncthreads();
     diag_apply<T>(s, nb, ncol);
@@ -316,20 +386,10 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
             int k = e >> 6, j = e & 63;
             if (j < ncol) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
         }
-    if (row0 >= m) return;
-    for (int e = tid; e < BK * 64; e += 256) {
-        int k, i;
-        if (!trans) { i = e & 63; k = e >> 6; }        // column-major L: coalesced along rows
-        else { k = e % BK; i = e / BK; }               // U^T: coalesced along k
-        cx<T> v = {T(0), T(0)};
-        if (k < nb && row0 + i < m) v = Lop(row0 + i, b0 + k);
-        s.Ar[k * 64 + i] = v.x;
-        s.Ai[k * 64 + i] = v.y;
-    }
-    __syncthreads();
+    if (!upd) return;
     typename mfma_t<T>::acc cr[2][2], ci[2][2];
     mma_tile<T>(s, nb, ncol, cr, ci);
-    sub_tile<T>(w, nrhs, row0, m, r0, cr, ci);
+    sub_tile_from<T>(w, nrhs, row0, m, r0, wold, cr, ci);
 }
 
 // backward, ancestors: W[f][0:p] = y1 - Uop12 x2, x2 gathered from X at the
