@@ -197,19 +197,21 @@ __device__ __forceinline__ cx<T> dop(const cx<T> *G, int m, int b0, int i, int t
     return trans ? ld(G + b0 + t + (int64_t)(b0 + i) * m, cj) : ld(G + b0 + i + (int64_t)(b0 + t) * m, cj);
 }
 
-// acc = A[64 x kb] * Y[kb x 64] from LDS (4 waves in 2x2, 32x32 each); column
-// sub-tiles at or beyond ncol (right-hand sides past the end) are skipped
+// acc = A[64 x kb] * Y[kb x 64] from LDS (4 waves in 2x2, 32x32 each), or
+// acc += that when accumulate; column sub-tiles at or beyond ncol (right-hand
+// sides past the end) are skipped
 template <typename T>
 __device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, int ncol, typename mfma_t<T>::acc (&cr)[2][2],
-                                         typename mfma_t<T>::acc (&ci)[2][2]) {
+                                         typename mfma_t<T>::acc (&ci)[2][2], bool accumulate = false) {
     typedef typename mfma_t<T>::acc acc_t;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) {
-            cr[a][b] = acc_t{0, 0, 0, 0};
-            ci[a][b] = acc_t{0, 0, 0, 0};
-        }
+    if (!accumulate)
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                cr[a][b] = acc_t{0, 0, 0, 0};
+                ci[a][b] = acc_t{0, 0, 0, 0};
+            }
     const int kend = (kb + 3) & ~3;
     for (int kk = 0; kk < kend; kk += 4) {
         const int k = kk + lk;
@@ -393,7 +395,9 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
 }
 
 // backward, ancestors: W[f][0:p] = y1 - Uop12 x2, x2 gathered from X at the
-// contribution positions.  grid: (row tiles over p, fronts, rhs tiles)
+// contribution positions.  Each 32-column chunk of Uop12 and x2 is loaded into
+// registers while the previous chunk's products run.
+// grid: (row tiles over p, fronts, rhs tiles)
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ arena, int64_t fstride,
                                                      const FrontDev *__restrict__ fronts, const int *__restrict__ list,
@@ -412,47 +416,53 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
     auto Uop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
     typedef typename mfma_t<T>::acc acc_t;
-    acc_t cr[2][2], ci[2][2], tr[2][2], ti[2][2];
+    acc_t cr[2][2], ci[2][2];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) {
             cr[a][b] = acc_t{0, 0, 0, 0};
             ci[a][b] = acc_t{0, 0, 0, 0};
         }
-    for (int s0 = p; s0 < m; s0 += BK) {
+    constexpr int PER = BK * 64 / 256;
+    cx<T> av[PER], yv[PER];
+    auto fetch = [&](int s0) {
         const int kb = min(BK, m - s0);
-        __syncthreads();
-        for (int e = tid; e < BK * 64; e += 256) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = tid + 256 * u;
             int k, i;
             if (!trans) { i = e & 63; k = e >> 6; }
             else { k = e % BK; i = e / BK; }
-            cx<T> v = {T(0), T(0)};
-            if (k < kb && row0 + i < p) v = Uop(row0 + i, s0 + k);
-            s.Ar[k * 64 + i] = v.x;
-            s.Ai[k * 64 + i] = v.y;
+            av[u] = (k < kb && row0 + i < p) ? Uop(row0 + i, s0 + k) : cx<T>{T(0), T(0)};
         }
-        for (int e = tid; e < BK * 64; e += 256) {
-            int k = e >> 6, j = e & 63;
-            cx<T> v = {T(0), T(0)};
-            if (k < kb && j < ncol) v = X[(int64_t)idx[F.idx_off + s0 + k] * nrhs + r0 + j];
-            s.Yr[e] = v.x;
-            s.Yi[e] = v.y;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = tid + 256 * u, k = e >> 6, j = e & 63;
+            yv[u] = (k < kb && j < ncol) ? X[(int64_t)idx[F.idx_off + s0 + k] * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+        }
+    };
+    if (p < m) fetch(p);
+    for (int s0 = p; s0 < m; s0 += BK) {
+        const int kb = min(BK, m - s0);
+        __syncthreads();   // the previous chunk's operands are no longer read
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = tid + 256 * u;
+            int k, i;
+            if (!trans) { i = e & 63; k = e >> 6; }
+            else { k = e % BK; i = e / BK; }
+            s.Ar[k * 64 + i] = av[u].x;
+            s.Ai[k * 64 + i] = av[u].y;
+            s.Yr[e] = yv[u].x;
+            s.Yi[e] = yv[u].y;
         }
         __syncthreads();
-        mma_tile<T>(s, kb, ncol, tr, ti);
-        for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) {
-                cr[a][b] += tr[a][b];
-                ci[a][b] += ti[a][b];
-            }
+        if (s0 + BK < m) fetch(s0 + BK);   // in flight during this chunk's products
+        mma_tile<T>(s, kb, ncol, cr, ci, true);
     }
-    // w[0:p] = y1 (from X) - accumulated product
-    for (int e = tid; e < 64 * 64; e += 256) {
-        int i = e >> 6, j = e & 63;
-        if (row0 + i < p && j < ncol)
-            w[(int64_t)(row0 + i) * nrhs + r0 + j] = X[(int64_t)(F.pos0 + row0 + i) * nrhs + r0 + j];
-    }
-    __syncthreads();
-    sub_tile<T>(w, nrhs, row0, p, r0, cr, ci);
+    // w[0:p] = y1 - accumulated product
+    cx<T> y1[2][2][4];
+    load_tile<T>(X + (int64_t)F.pos0 * nrhs, nrhs, row0, p, r0, y1);
+    sub_tile_from<T>(w, nrhs, row0, p, r0, y1, cr, ci);
 }
 
 // backward block b0 (blocks from the bottom): x_b = Dop_bb z_b (to X by the
@@ -474,16 +484,46 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
     auto Uop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
-    for (int e = tid; e < BK * BK; e += 256) {
-        const int i = e % BK, t = e / BK;
-        s.D[i * (BK + 1) + t] = (i < nb && t < nb) ? dop(G, m, b0, i, t, false, trans, cj) : cx<T>{T(0), T(0)};
-    }
-    for (int e = tid; e < BK * 64; e += 256) {
-        int k = e >> 6, j = e & 63;
-        cx<T> v = {T(0), T(0)};
-        if (k < nb && j < ncol) v = w[(int64_t)(b0 + k) * nrhs + r0 + j];
-        s.Yr[e] = v.x;
-        s.Yi[e] = v.y;
+    const bool upd = row0 < b0;
+    // every load before the first barrier (as in k_fwd_blk)
+    cx<T> wold[2][2][4];
+    if (upd) load_tile<T>(w, nrhs, row0, b0, r0, wold);
+    {
+        cx<T> dv[BK * BK / 256], yv[BK * 64 / 256], av[BK * 64 / 256];
+#pragma unroll
+        for (int u = 0; u < BK * BK / 256; u++) {
+            const int e = tid + 256 * u, i = e % BK, t = e / BK;
+            dv[u] = (i < nb && t < nb) ? dop(G, m, b0, i, t, false, trans, cj) : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < BK * 64 / 256; u++) {
+            const int e = tid + 256 * u, k = e >> 6, j = e & 63;
+            yv[u] = (k < nb && j < ncol) ? w[(int64_t)(b0 + k) * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < BK * 64 / 256; u++) {
+            const int e = tid + 256 * u;
+            int k, i;
+            if (!trans) { i = e & 63; k = e >> 6; }
+            else { k = e % BK; i = e / BK; }
+            av[u] = (upd && k < nb && row0 + i < b0) ? Uop(row0 + i, b0 + k) : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < BK * BK / 256; u++) {
+            const int e = tid + 256 * u;
+            s.D[(e % BK) * (BK + 1) + e / BK] = dv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < BK * 64 / 256; u++) {
+            const int e = tid + 256 * u;
+            int k, i;
+            if (!trans) { i = e & 63; k = e >> 6; }
+            else { k = e % BK; i = e / BK; }
+            s.Yr[e] = yv[u].x;
+            s.Yi[e] = yv[u].y;
+            s.Ar[k * 64 + i] = av[u].x;
+            s.Ai[k * 64 + i] = av[u].y;
+        }
     }
     __syncthreads();
     diag_apply<T>(s, nb, ncol);
@@ -492,20 +532,10 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
             int k = e >> 6, j = e & 63;
             if (j < ncol) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
         }
-    if (row0 >= b0) return;
-    for (int e = tid; e < BK * 64; e += 256) {
-        int k, i;
-        if (!trans) { i = e & 63; k = e >> 6; }
-        else { k = e % BK; i = e / BK; }
-        cx<T> v = {T(0), T(0)};
-        if (k < nb && row0 + i < b0) v = Uop(row0 + i, b0 + k);
-        s.Ar[k * 64 + i] = v.x;
-        s.Ai[k * 64 + i] = v.y;
-    }
-    __syncthreads();
+    if (!upd) return;
     typename mfma_t<T>::acc cr[2][2], ci[2][2];
     mma_tile<T>(s, nb, ncol, cr, ci);
-    sub_tile<T>(w, nrhs, row0, b0, r0, cr, ci);
+    sub_tile_from<T>(w, nrhs, row0, b0, r0, wold, cr, ci);
 }
 
 // ---- fused per-front substitutions (fronts whose panel fits in LDS) ---------
