@@ -522,19 +522,39 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
     const int r0 = base + tl.ti * 64, c0 = base + tl.tj * 64;
     cx<T> *G = arena + blockIdx.y * fstride + F.buf_off;
     const int tid = threadIdx.x;
-    for (int e = tid; e < KB * 64; e += 256) {
-        int i = e & 63, k = e >> 6;
-        cx<T> v = {T(0), T(0)};
-        if (k < kb && r0 + i < m) v = G[(r0 + i) + (int64_t)(k0 + k) * m];
-        Ar[k][i] = v.x;
-        Ai[k][i] = v.y;
-    }
-    for (int e = tid; e < KB * 64; e += 256) {
-        int k = e % KB, j = e / KB;
-        cx<T> v = {T(0), T(0)};
-        if (k < kb && c0 + j < m) v = G[(k0 + k) + (int64_t)(c0 + j) * m];
-        Br[k][j] = v.x;
-        Bi[k][j] = v.y;
+    const int lane = tid & 63, w = tid >> 6, wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    // every load up front: the operand panels and the lane's 16 entries of C
+    cx<T> c[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = r0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = c0 + wc + b * 16 + lr;
+                c[a][b][v] = (row < m && col < m) ? G[row + (int64_t)col * m] : cx<T>{T(0), T(0)};
+            }
+    {
+        cx<T> av[KB * 64 / 256], bv[KB * 64 / 256];
+#pragma unroll
+        for (int u = 0; u < KB * 64 / 256; u++) {
+            const int e = tid + 256 * u, i = e & 63, k = e >> 6;
+            av[u] = (k < kb && r0 + i < m) ? G[(r0 + i) + (int64_t)(k0 + k) * m] : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < KB * 64 / 256; u++) {
+            const int e = tid + 256 * u, k = e % KB, j = e / KB;
+            bv[u] = (k < kb && c0 + j < m) ? G[(k0 + k) + (int64_t)(c0 + j) * m] : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < KB * 64 / 256; u++) {
+            const int e = tid + 256 * u;
+            Ar[e >> 6][e & 63] = av[u].x;
+            Ai[e >> 6][e & 63] = av[u].y;
+            Br[e % KB][e / KB] = bv[u].x;
+            Bi[e % KB][e / KB] = bv[u].y;
+        }
     }
     __syncthreads();
     typedef typename mfma_t<T>::acc acc_t;
@@ -544,8 +564,6 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
             cr[a][b] = acc_t{0, 0, 0, 0};
             ci[a][b] = acc_t{0, 0, 0, 0};
         }
-    const int lane = tid & 63, w = tid >> 6, wr = (w >> 1) * 32, wc = (w & 1) * 32;
-    const int lr = lane & 15, lk = lane >> 4;
     const int kend = (kb + 3) & ~3;
     for (int kk = 0; kk < kend; kk += 4) {
         const int k = kk + lk;
@@ -563,15 +581,15 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
                 cmma<T>(ar[a], ai[a], br[b], bi[b], cr[a][b], ci[a][b]);
             }
     }
+#pragma unroll
     for (int a = 0; a < 2; a++)
+#pragma unroll
         for (int b = 0; b < 2; b++)
+#pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = r0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = c0 + wc + b * 16 + lr;
-                if (row < m && col < m) {
-                    cx<T> &g = G[row + (int64_t)col * m];
-                    g.x -= cr[a][b][v];
-                    g.y -= ci[a][b][v];
-                }
+                if (row < m && col < m)
+                    G[row + (int64_t)col * m] = {c[a][b][v].x - cr[a][b][v], c[a][b][v].y - ci[a][b][v]};
             }
 }
 
