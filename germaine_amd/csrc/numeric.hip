@@ -214,6 +214,8 @@ __global__ void __launch_bounds__(256) k_front_assemble(cx<T> *__restrict__ aren
     cx<T> *G = A + F.buf_off;
     const int64_t total = (int64_t)F.m * F.m;
     const int64_t step = (int64_t)gridDim.y * blockDim.x;
+    // unrolled: several entries' index and value loads in flight at once
+#pragma unroll 4
     for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < total; e += step) {
         int s = (int)(e / F.m), r = (int)(e - (int64_t)s * F.m);
         cx<T> v = {T(0), T(0)};
@@ -481,10 +483,13 @@ __global__ void __launch_bounds__(64) k_trsm_u(cx<T> *__restrict__ arena, int64_
     if (c0 >= m) return;
     cx<T> *G = arena + blockIdx.z * fstride + F.buf_off;
     const int tid = threadIdx.x;
+    // unrolled so that a batch of loads is in flight at once
+#pragma unroll 8
     for (int e = tid; e < kb * kb; e += 64) {
         int i = e % kb, t = e / kb;
         Ls[i][t] = G[(k0 + i) + (int64_t)(k0 + t) * m];
     }
+#pragma unroll 8
     for (int e = tid; e < kb * 64; e += 64) {
         int i = e % kb, c = e / kb, col = c0 + c;
         Xs[i][c] = col < m ? G[(k0 + i) + (int64_t)col * m] : cx<T>{T(0), T(0)};
