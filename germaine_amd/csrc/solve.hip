@@ -940,14 +940,17 @@ __global__ void __launch_bounds__(256) k_fwd_front1(const cx<T> *__restrict__ ar
 }
 
 // Backward: x1 = [Uop_bb, -Uop12] [y1; x2] for the p <= 32 pivot rows (two
-// 16-row strips, K = m in chunks of 32).  LDS per wave: the m x 16 panel and its rows' sources.
+// 16-row strips, K = m in chunks of 32).  No LDS: each lane loads its operand
+// rows (k = k0 + 4s + lane/16, column lane%16) straight from X into MFMA layout;
+// a chunk's factor entries and operand rows are requested together (the rows'
+// sources one chunk ahead), so each chunk waits on memory once; occupancy is
+// bounded by registers only.
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ arena, int64_t fstride,
                                                     const FrontDev *__restrict__ lvF, int nfr,
                                                     const int *__restrict__ idx, const int *__restrict__ rowperm,
-                                                    cx<T> *__restrict__ X, int n, int nrhs, int rtg, int wave_lds,
-                                                    int trans, int cj, Sel sel) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+                                                    cx<T> *__restrict__ X, int n, int nrhs, int rtg, int trans, int cj,
+                                                    Sel sel) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
     int fi, tg;
     xcd_front_tile(blockIdx.x, rtg, fi, tg);
@@ -956,59 +959,50 @@ __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ ar
     const int z = blockIdx.y, ncol = min(WT, nrhs - r0);
     const FrontDev F = lvF[fi];
     const int m = F.m, p = F.p;
-    T *Pr = reinterpret_cast<T *>(smem_raw + wave * wave_lds), *Pi = Pr + m * WT;
-    int *srow = reinterpret_cast<int *>(Pi + m * WT);
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
-    const bool two = p > 16;
+    const bool two = p > 16, cin = lr < ncol;
     const AOp o0{ars, acs, p, m, 0, 0, 2, p, (bool)trans, (bool)cj};
     const AOp o1{ars, acs, p - 16, m, 16, 0, 2, p, (bool)trans, (bool)cj};
+    // operand row k: pivot row k of the front, or the ancestors' row idx[k]
+    auto rows_of = [&](int k0, int (&r)[8]) {
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int k = k0 + 4 * s + lk;
+            r[s] = k >= m ? -1 : k < p ? F.pos0 + k : idx[F.idx_off + k];
+        }
+    };
     cx<T> a0[8], a1[8];
-    load_a<T>(G, o0, 0, a0);
-    if (two) load_a<T>(G + 16 * ars, o1, 0, a1);
-    for (int i = lane; i < m; i += 64) srow[i] = i < p ? F.pos0 + i : idx[F.idx_off + i];
+    int rows[8];
+    rows_of(0, rows);
     int dst[8];   // output positions of the lane's accumulator rows
 #pragma unroll
     for (int v = 0; v < 8; v++) {
         const int row = (v >> 2) * 16 + mfma_t<T>::row(lk, v & 3);
         dst[v] = row < p ? F.pos0 + (trans ? rowperm[(int64_t)sel.f[z] * n + F.pos0 + row] : row) : -1;
     }
-    wave_sync();
-    for (int e0 = 0; e0 < m * WT; e0 += 64 * 8) {
-        cx<T> v[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int e = e0 + lane + 64 * u, i = e >> 4, j = e & 15;
-            v[u] = {T(0), T(0)};
-            if (i < m && j < ncol) v[u] = X[(int64_t)srow[i] * nrhs + r0 + j];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int e = e0 + lane + 64 * u, i = e >> 4, j = e & 15;
-            if (i < m) {
-                Pr[i * WT + j] = v[u].x;
-                Pi[i * WT + j] = v[u].y;
-            }
-        }
-    }
-    wave_sync();
-    typename mfma_t<T>::acc c0r[1], c0i[1], c1r[1], c1i[1];
-    acc_zero<T, 1>(c0r, c0i);
-    acc_zero<T, 1>(c1r, c1i);
+    typename mfma_t<T>::acc c0r = {0, 0, 0, 0}, c0i = {0, 0, 0, 0}, c1r = {0, 0, 0, 0}, c1i = {0, 0, 0, 0};
     for (int k0 = 0; k0 < m; k0 += 32) {
-        if (k0 > 0) {   // (chunk 0 was requested before the gather)
-            load_a<T>(G, o0, k0, a0);
-            if (two) load_a<T>(G + 16 * ars, o1, k0, a1);
+        cx<T> b[8];
+        load_a<T>(G, o0, k0, a0);
+        if (two) load_a<T>(G + 16 * ars, o1, k0, a1);
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+            b[s] = (rows[s] >= 0 && cin) ? X[(int64_t)rows[s] * nrhs + r0 + lr] : cx<T>{T(0), T(0)};
+        if (k0 + 32 < m) rows_of(k0 + 32, rows);
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            if (k0 + 4 * s >= m) break;
+            cmma<T>(a0[s].x, a0[s].y, b[s].x, b[s].y, c0r, c0i);
+            if (two) cmma<T>(a1[s].x, a1[s].y, b[s].x, b[s].y, c1r, c1i);
         }
-        mma_a<T, 1>(a0, k0, m, Pr, Pi, 0, WT, c0r, c0i);
-        if (two) mma_a<T, 1>(a1, k0, m, Pr, Pi, 0, WT, c1r, c1i);
     }
-    if (lr < ncol) {
+    if (cin) {
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             if (dst[v] < 0) continue;
-            const T xr = v < 4 ? c0r[0][v] : c1r[0][v - 4], xi = v < 4 ? c0i[0][v] : c1i[0][v - 4];
+            const T xr = v < 4 ? c0r[v] : c1r[v - 4], xi = v < 4 ? c0i[v] : c1i[v - 4];
             X[(int64_t)dst[v] * nrhs + r0 + lr] = {xr, xi};
         }
     }
@@ -1876,9 +1870,9 @@ WaveLaunch wave_launch(int nfr, int nrhs, int nslots, size_t per_wave) {
 
 // staged small-front launch: the workgroup's LDS is the staged operator and
 // sources only; up to 8 waves (16-column tiles), as many workgroups as fit
-inline WaveLaunch small_launch(int nfr, int nrhs, int nslots, size_t lds) {
+inline WaveLaunch small_launch(int nfr, int nrhs, int nslots, size_t lds, int max_waves = 8) {
     const int rt = (nrhs + WT - 1) / WT;
-    const int wpg = std::min(8, rt);
+    const int wpg = std::min(max_waves, rt);
     const int rtg = (rt + wpg - 1) / wpg;
     return {dim3(((nfr + 7) / 8) * 8 * rtg, nslots), dim3(64 * wpg), (lds + 15) & ~size_t(15), rtg, 0};
 }
@@ -1927,8 +1921,6 @@ void set_smem_attrs() {
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_small1<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, fl));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_front1<T>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
-    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_front1<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, fl));
     done = true;
 }
@@ -2042,11 +2034,11 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
             continue;
         }
         if (dp.single[l]) {
-            const WaveLaunch wl = wave_launch<T>(nfr, nrhs, ns, dp.mmax_all[l] * (WT * 2 * sizeof(T) + 4));
+            const WaveLaunch wl = small_launch(nfr, nrhs, ns, 0, 4);
             ProfScope ps(K_BWD_FRONT1, st, nz * 8.0 * nr * c.tri_elems,
                          nz * (c.tri_elems + (c.sum_m + c.sum_p) * nr) * es);
-            hipLaunchKernelGGL(k_bwd_front1<T>, wl.grid, wl.block, wl.lds, st, F.arena, F.fstride, dp.lv_fr[l], nfr,
-                               dp.idx, F.rowperm, Xp, n, nrhs, wl.rtg, wl.wave_lds, (int)trans, (int)cj, in);
+            hipLaunchKernelGGL(k_bwd_front1<T>, wl.grid, wl.block, 0, st, F.arena, F.fstride, dp.lv_fr[l], nfr, dp.idx,
+                               F.rowperm, Xp, n, nrhs, wl.rtg, (int)trans, (int)cj, in);
             continue;
         }
         const int rw = fused_rw<T>(dp.mmax_all[l], dp.max_children, false);
