@@ -278,32 +278,48 @@ __global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int
                                                   int *__restrict__ piv, int n, int *__restrict__ sing) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *S = reinterpret_cast<cx<T> *>(smem_raw);
-    __shared__ T sb[4];
-    __shared__ int si[4];
+    __shared__ int si[1];
     const FrontDev F = fronts[list[blockIdx.x]];
     const int fq = blockIdx.y;
     cx<T> *G = arena + fq * fstride + F.buf_off;
     const int m = F.m, p = F.p, mm = m * m, tid = threadIdx.x, bs = blockDim.x;
+    __shared__ cx<T> pv[2];   // pivot entry and the entry it swaps with, published with the pivot row
+#pragma unroll 4
     for (int e = tid; e < mm; e += bs) S[e] = G[e];
     __syncthreads();
     for (int j = 0; j < p; j++) {
-        T best = T(-1);
-        int bi = j;
-        for (int r = j + tid; r < p; r += bs) better(best, bi, cabs1(S[r + j * m]), r);
-        const int pr = block_argmax(best, bi, sb, si);
-        if (tid == 0) {
-            piv[(int64_t)fq * n + F.pos0 + j] = pr;
-            if (sb[0] == T(0)) atomicOr(sing + fq, 1);
+        // pivot search by wave 0 alone (p <= 64 candidates: one per lane)
+        if (tid < 64) {
+            T best = T(-1);
+            int bi = j;
+            for (int r = j + tid; r < p; r += 64) better(best, bi, cabs1(S[r + j * m]), r);
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                better(best, bi, ob, oi);
+            }
+            if (tid == 0) {
+                si[0] = bi;
+                pv[0] = S[bi + j * m];
+                pv[1] = S[j + j * m];
+                piv[(int64_t)fq * n + F.pos0 + j] = bi;
+                if (best == T(0)) atomicOr(sing + fq, 1);
+            }
         }
+        __syncthreads();
+        // row swap in the other columns and the scaled column j, in one pass:
+        // column j's swapped pair comes from pv, read before anyone wrote
+        const int pr = si[0];
+        const cx<T> d = crecip(pv[0]);
         if (pr != j)
             for (int c = tid; c < m; c += bs) {
+                if (c == j) continue;
                 cx<T> t = S[j + c * m];
                 S[j + c * m] = S[pr + c * m];
                 S[pr + c * m] = t;
             }
-        __syncthreads();
-        const cx<T> d = crecip(S[j + j * m]);
-        for (int r = j + 1 + tid; r < m; r += bs) S[r + j * m] = cmul(S[r + j * m], d);
+        for (int r = j + 1 + tid; r < m; r += bs) S[r + j * m] = cmul(r == pr ? pv[1] : S[r + j * m], d);
+        if (tid == 0) S[j + j * m] = pv[0];
         __syncthreads();
         const int t = m - j - 1;
         for (int e = tid; e < t * t; e += bs) {
