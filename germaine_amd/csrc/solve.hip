@@ -1345,6 +1345,7 @@ __global__ void __launch_bounds__(256) k_fwd_vec(const cx<T> *__restrict__ arena
         for (int u = 0; u < 2; u++) {
             const int i = tid + 256 * u;
             if (i < m)
+#pragma unroll 8
                 for (int t = 0; t < p; t++) {
                     const cx<T> a = i < p ? dop(G, m, 0, i, t, true, trans, cj) : Lop(i, t);
                     acc[u] = cadd(acc[u], cmul(a, sw[t]));
@@ -1367,8 +1368,10 @@ __global__ void __launch_bounds__(256) k_fwd_vec(const cx<T> *__restrict__ arena
         __syncthreads();
         if (tid < nb) sw[tb + tid] = y;
         __syncthreads();
+        // the factor loads of a row are unrolled so that several are in flight at once
         for (int i = tb + nb + tid; i < m; i += 256) {
             cx<T> acc = sw[i];
+#pragma unroll 8
             for (int t = 0; t < nb; t++) acc = cmsub(acc, Lop(i, tb + t), sw[tb + t]);
             sw[i] = acc;
         }
@@ -1400,6 +1403,7 @@ __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena
         const int i = tid >> 3, part = tid & 7;
         cx<T> acc = {T(0), T(0)};
         if (i < p)
+#pragma unroll 4
             for (int t = part; t < m; t += 8) {
                 const cx<T> a = t < p ? dop(G, m, 0, i, t, false, trans, cj) : Uop(i, t);
                 acc = t < p ? cadd(acc, cmul(a, sw[t])) : csub(acc, cmul(a, sw[t]));
@@ -1413,6 +1417,7 @@ __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena
     }
     for (int i = tid; i < p; i += 256) {
         cx<T> acc = sw[i];
+#pragma unroll 8
         for (int s = p; s < m; s++) acc = cmsub(acc, Uop(i, s), sw[s]);
         sw[i] = acc;
     }
@@ -1429,6 +1434,7 @@ __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena
         __syncthreads();
         for (int i = tid; i < tb; i += 256) {
             cx<T> acc = sw[i];
+#pragma unroll 8
             for (int t = 0; t < nb; t++) acc = cmsub(acc, Uop(i, tb + t), sw[tb + t]);
             sw[i] = acc;
         }
