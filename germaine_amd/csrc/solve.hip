@@ -1009,7 +1009,7 @@ __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ ar
 }
 
 // ---- single-block levels with small fronts (m <= 64): staged operator -------
-// The workgroup (up to 8 waves = 8 consecutive 16-column tiles of one front)
+// The workgroup (up to 16 waves = consecutive 16-column tiles of one front)
 // stages the front's whole operator and its rows' gather sources in LDS, with
 // all loads in flight at once; then every wave loads its right-hand-side
 // operand straight from global memory into registers in MFMA layout (k = 4s +
@@ -1018,6 +1018,13 @@ __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ ar
 // task, and the only LDS is the operator, so many tasks stay resident.
 // Operator element (i, k) sits at A[k * as + i].
 constexpr int kSmallM = 64;
+// forward workgroup size limit: one workgroup covers a front's whole panel
+// (13 tiles of 16 at 200 right-hand sides) in FP32, so the operator is staged
+// once per front; FP64 keeps 8 waves, within its registers. The backward
+// kernel stays at 8 waves (measured faster: more workgroups per CU).
+template <typename T>
+constexpr int kSmallThreads = sizeof(T) == 4 ? 1024 : 512;
+constexpr int kSmallBwdThreads = 512;
 
 // operator row stride: rows rounded to 16, then to 16 or 48 mod 64, so the four
 // k-rows of one operand read start in different banks
@@ -1031,7 +1038,7 @@ __host__ __device__ __forceinline__ int small_stride(int rows) {
 // the leaves' level has none and keeps no registers for them.
 // grid: (ceil(fronts/8)*8 * tile groups, frequency slots)
 template <typename T, bool CH>
-__global__ void __launch_bounds__(512) k_fwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
+__global__ void __launch_bounds__(kSmallThreads<T>) k_fwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
                                                     const FrontDev *__restrict__ lvF, int nfr,
                                                     const int *__restrict__ gsrc, int64_t gstride,
                                                     cx<T> *__restrict__ X, cx<T> *__restrict__ W, int n,
@@ -1138,7 +1145,7 @@ __global__ void __launch_bounds__(512) k_fwd_small1(const cx<T> *__restrict__ ar
 // Backward.  Operator [p][m] staged as A[k * as + i] with as = p rounded to 16;
 // right-hand-side operand: all m rows (K = m <= 64, 16 per lane) in registers.
 template <typename T>
-__global__ void __launch_bounds__(512) k_bwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
+__global__ void __launch_bounds__(kSmallBwdThreads) k_bwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
                                                     const FrontDev *__restrict__ lvF, int nfr,
                                                     const int *__restrict__ idx, const int *__restrict__ rowperm,
                                                     cx<T> *__restrict__ X, int n, int nrhs, int rtg, int trans, int cj,
@@ -1875,8 +1882,8 @@ WaveLaunch wave_launch(int nfr, int nrhs, int nslots, size_t per_wave) {
 }
 
 // staged small-front launch: the workgroup's LDS is the staged operator and
-// sources only; up to 8 waves (16-column tiles), as many workgroups as fit
-inline WaveLaunch small_launch(int nfr, int nrhs, int nslots, size_t lds, int max_waves = 8) {
+// sources only; up to max_waves waves (16-column tiles), as many workgroups as fit
+inline WaveLaunch small_launch(int nfr, int nrhs, int nslots, size_t lds, int max_waves) {
     const int rt = (nrhs + WT - 1) / WT;
     const int wpg = std::min(max_waves, rt);
     const int rtg = (rt + wpg - 1) / wpg;
@@ -1976,7 +1983,8 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
         const int nfr = dp.n_all[l];
         if (dp.single[l] && dp.mmax_all[l] <= kSmallM) {
             const WaveLaunch wl =
-                small_launch(nfr, nrhs, ns, 2 * sizeof(T) * dp.pmax_all[l] * small_stride(dp.mmax_all[l]) + 12 * kSmallM);
+                small_launch(nfr, nrhs, ns, 2 * sizeof(T) * dp.pmax_all[l] * small_stride(dp.mmax_all[l]) + 12 * kSmallM,
+                             kSmallThreads<T> / 64);
             // levels without children (the leaves) are a kernel of their own
             ProfScope ps(dp.lv_children[l] ? K_FWD_SMALL1 : K_FWD_LEAF, st, nz * 8.0 * nr * c.tri_elems,
                          nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
@@ -2032,7 +2040,8 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
         const int nfr = dp.n_all[l];
         if (dp.single[l] && dp.mmax_all[l] <= kSmallM) {
             const WaveLaunch wl = small_launch(nfr, nrhs, ns,
-                                               2 * sizeof(T) * dp.mmax_all[l] * 32 + 4 * (kSmallM + 32));
+                                               2 * sizeof(T) * dp.mmax_all[l] * 32 + 4 * (kSmallM + 32),
+                                               kSmallBwdThreads / 64);
             ProfScope ps(K_BWD_SMALL1, st, nz * 8.0 * nr * c.tri_elems,
                          nz * (c.tri_elems + (c.sum_m + c.sum_p) * nr) * es);
             hipLaunchKernelGGL(k_bwd_small1<T>, wl.grid, wl.block, wl.lds, st, F.arena, F.fstride, dp.lv_fr[l], nfr,
