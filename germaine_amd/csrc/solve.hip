@@ -1598,7 +1598,7 @@ static_assert(kIrStage <= kIrCoef0 && kIrCoef0 + 9 * kIrTx <= kIrThreads,
 constexpr int kIrPer = (kIrStage * kIrCw + kIrThreads - 1) / kIrThreads;   // staging entries per thread
 
 template <typename T>
-__global__ void __launch_bounds__(kIrThreads) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
+__global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 5))) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
                                                  const int *__restrict__ rowind, const int64_t *__restrict__ tq,
                                                  const int *__restrict__ iperm, int nx, int ny, int nrhs, int ld64,
                                                  int ld, int sys, int64_t nnz, const cx<double> *__restrict__ Bg,
@@ -1659,14 +1659,8 @@ __global__ void __launch_bounds__(kIrThreads) k_ir_step(const cx<double> *__rest
         const int r = tid >> 4, cc = tid & 15;
         const int64_t k = (int64_t)j * nx + i0 + r;
         const int64_t dst = r < w ? (int64_t)se[(kIrTx + 2) + r + 1] * ld : 0;
-        // this thread's row of the stencil, kept in registers for the tile
-        cx<double> a[9];
-        int li[9];
-#pragma unroll
-        for (int t = 0; t < 9; t++) {
-            a[t] = scoef[r][t];
-            li[t] = sidx[r][t];
-        }
+        // the stencil row is read from LDS per chunk (16 lanes share each
+        // address), not held in registers: fewer registers, more waves resident
         for (int c0 = 0; c0 < ld; c0 += kIrCw) {
             const int c = c0 + cc;
             const bool own = r < w && c < nrhs;
@@ -1694,7 +1688,7 @@ __global__ void __launch_bounds__(kIrThreads) k_ir_step(const cx<double> *__rest
             __syncthreads();
             if (own) {
 #pragma unroll
-                for (int t = 0; t < 9; t++) acc = cmsub(acc, a[t], sx[li[t]][cc]);
+                for (int t = 0; t < 9; t++) acc = cmsub(acc, scoef[r][t], sx[sidx[r][t]][cc]);
                 Xb[k * ld64 + c] = sx[(kIrTx + 2) + r + 1][cc];
                 R[dst + c] = {(T)acc.x, (T)acc.y};
             } else if (r < w && c < ld) {
