@@ -672,56 +672,67 @@ __global__ void k_gsrc(const FrontDev *__restrict__ fronts, int nfronts, const i
 // forward and backward substitutions -- for M as for M^T -- into one product
 // with no dependent phase:  [y; w2 - L21 y] = [inv(L11); L21 inv(L11)] w1  and
 // x1 = [inv(U11), -inv(U11) U12] [y1; x2].  Computed in FP64 for both factor
-// precisions.  One workgroup per front (eight lanes build column j of both
-// inverses; the products run over all threads, 32 rows or columns at a time),
-// grid (fronts, nfreq).
+// precisions.  One workgroup per front: waves 0-1 build inv(L_bb) and waves
+// 2-3 inv(U_bb) at the same time, four lanes per column; the products run over
+// all threads, 32 rows or columns at a time.  The block is staged in the
+// factor precision (exact in FP64), the inverses in FP64.  grid (fronts, nfreq).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int64_t fstride,
                                                   const FrontDev *__restrict__ fronts,
                                                   const int *__restrict__ lv_single) {
-    __shared__ cx<double> D[32][33];
+    __shared__ cx<T> D[32][33];
     __shared__ cx<double> R[32][33];
+    __shared__ cx<double> rd[32];   // 1 / U[i][i]
     const FrontDev F = fronts[blockIdx.x];
     cx<T> *G = arena + blockIdx.y * fstride + F.buf_off;
     const int m = F.m, p = F.p, tid = threadIdx.x;
+    auto up = [](cx<T> v) { return cx<double>{(double)v.x, (double)v.y}; };
     for (int b0 = 0; b0 < p; b0 += 32) {
         const int kb = min(32, p - b0);
         for (int e = tid; e < kb * kb; e += 256) {
             const int i = e % kb, t = e / kb;
-            const cx<T> v = G[b0 + i + (int64_t)(b0 + t) * m];
-            D[i][t] = {(double)v.x, (double)v.y};
+            D[i][t] = G[b0 + i + (int64_t)(b0 + t) * m];
         }
         __syncthreads();
+        if (tid < kb) rd[tid] = crecip(up(D[tid][tid]));
+        __syncthreads();
         {
-            // eight adjacent lanes (one wave) per column j: each sums every eighth
+            // four adjacent lanes (one wave) per column j: each sums every fourth
             // term of a row's dot product, xor-shuffles add the partial sums, and
-            // lane 0 of the eight stores the entry the next row reads
-            const int j = tid >> 3, part = tid & 7;
-            auto sum8 = [](cx<double> a) {
-                for (int o = 1; o < 8; o <<= 1) {
+            // lane 0 of the four stores the entry the next row reads.  The two
+            // triangles of R are disjoint, so both inverses proceed at once.
+            const bool lower = tid < 128;
+            const int j = (tid & 127) >> 2, part = tid & 3;
+            auto sum4 = [](cx<double> a) {
+                for (int o = 1; o < 4; o <<= 1) {
                     a.x += __shfl_xor(a.x, o);
                     a.y += __shfl_xor(a.y, o);
                 }
                 return a;
             };
-            // column j of inv(L_bb), L_bb unit lower: x_j = 1, x_i = -sum_{j<=t<i} L[i][t] x_t
-            if (part == 0 && j < kb) R[j][j] = {1.0, 0.0};
-            for (int i = 1; i < kb; i++) {
-                cx<double> acc = {0.0, 0.0};
-                if (j < i)
-                    for (int t = j + part; t < i; t += 8) acc = cmsub(acc, D[i][t], R[t][j]);
-                acc = sum8(acc);
-                if (part == 0 && j < i) R[i][j] = acc;
-            }
-            // column j of inv(U_bb): x_j = 1/U[j][j], x_i = -(sum_{i<t<=j} U[i][t] x_t) / U[i][i]
-            if (part == 0 && j < kb) R[j][j] = crecip(D[j][j]);
-            for (int i = kb - 2; i >= 0; i--) {
-                cx<double> acc = {0.0, 0.0};
-                if (i < j && j < kb)
-                    for (int t = i + 1 + part; t <= j; t += 8) acc = cmsub(acc, D[i][t], R[t][j]);
-                acc = sum8(acc);
-                if (part == 0 && i < j && j < kb) R[i][j] = cdiv(acc, D[i][i]);
+            if (lower) {
+                // column j of inv(L_bb), L_bb unit lower: x_j = 1,
+                // x_i = -L[i][j] - sum_{j<t<i} L[i][t] x_t  (R[j][j] belongs to the U half)
+                for (int i = 1; i < kb; i++) {
+                    cx<double> acc = {0.0, 0.0};
+                    if (j < i) {
+                        if (part == 0) acc = {-(double)D[i][j].x, -(double)D[i][j].y};
+                        for (int t = j + 1 + part; t < i; t += 4) acc = cmsub(acc, up(D[i][t]), R[t][j]);
+                    }
+                    acc = sum4(acc);
+                    if (part == 0 && j < i) R[i][j] = acc;
+                }
+            } else {
+                // column j of inv(U_bb): x_j = 1/U[j][j], x_i = -(sum_{i<t<=j} U[i][t] x_t) / U[i][i]
+                if (part == 0 && j < kb) R[j][j] = rd[j];
+                for (int i = kb - 2; i >= 0; i--) {
+                    cx<double> acc = {0.0, 0.0};
+                    if (i < j && j < kb)
+                        for (int t = i + 1 + part; t <= j; t += 4) acc = cmsub(acc, up(D[i][t]), R[t][j]);
+                    acc = sum4(acc);
+                    if (part == 0 && i < j && j < kb) R[i][j] = cmul(acc, rd[i]);
+                }
             }
         }
         __syncthreads();
@@ -732,20 +743,19 @@ __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int
         __syncthreads();
     }
     if (!lv_single[F.level] || p > 32) return;
-    cx<double>(*V)[33] = D;   // the block's entries are no longer needed: D stages the products' operands
+    cx<T>(*V)[33] = D;   // the block's entries are no longer needed: D stages the products' operands
     // L21 <- L21 inv(L11), 32 rows at a time staged in V (all reads before any write)
     for (int r0 = p; r0 < m; r0 += 32) {
         const int rows = min(32, m - r0);
         for (int e = tid; e < rows * p; e += 256) {
             const int i = e % rows, s = e / rows;
-            const cx<T> v = G[r0 + i + (int64_t)s * m];
-            V[i][s] = {(double)v.x, (double)v.y};
+            V[i][s] = G[r0 + i + (int64_t)s * m];
         }
         __syncthreads();
         for (int e = tid; e < rows * p; e += 256) {
             const int i = e % rows, t = e / rows;
-            cx<double> acc = V[i][t];
-            for (int s = t + 1; s < p; s++) acc = cadd(acc, cmul(V[i][s], R[s][t]));
+            cx<double> acc = up(V[i][t]);
+            for (int s = t + 1; s < p; s++) acc = cadd(acc, cmul(up(V[i][s]), R[s][t]));
             G[r0 + i + (int64_t)t * m] = {(T)acc.x, (T)acc.y};
         }
         __syncthreads();
@@ -755,14 +765,13 @@ __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int
         const int cols = min(32, m - c0);
         for (int e = tid; e < cols * p; e += 256) {
             const int s = e % p, c = e / p;
-            const cx<T> v = G[s + (int64_t)(c0 + c) * m];
-            V[c][s] = {(double)v.x, (double)v.y};
+            V[c][s] = G[s + (int64_t)(c0 + c) * m];
         }
         __syncthreads();
         for (int e = tid; e < cols * p; e += 256) {
             const int i = e % p, c = e / p;
             cx<double> acc = {0.0, 0.0};
-            for (int s = i; s < p; s++) acc = cadd(acc, cmul(R[i][s], V[c][s]));
+            for (int s = i; s < p; s++) acc = cadd(acc, cmul(R[i][s], up(V[c][s])));
             G[i + (int64_t)(c0 + c) * m] = {(T)acc.x, (T)acc.y};
         }
         __syncthreads();
