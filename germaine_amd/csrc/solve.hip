@@ -1038,7 +1038,8 @@ __host__ __device__ __forceinline__ int small_stride(int rows) {
 // the leaves' level has none and keeps no registers for them.
 // grid: (ceil(fronts/8)*8 * tile groups, frequency slots)
 template <typename T, bool CH>
-__global__ void __launch_bounds__(kSmallThreads<T>) k_fwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
+__global__ void __launch_bounds__(kSmallThreads<T>)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
                                                     const FrontDev *__restrict__ lvF, int nfr,
                                                     const int *__restrict__ gsrc, int64_t gstride,
                                                     cx<T> *__restrict__ X, cx<T> *__restrict__ W, int n,
@@ -1059,50 +1060,66 @@ __global__ void __launch_bounds__(kSmallThreads<T>) k_fwd_small1(const cx<T> *__
     W += sel.s[z] * wstride;
     const int *gs = gsrc + sel.f[z] * gstride + F.fv_off * 3;
     const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
-    // stage the operator [inv(L11); L21 inv(L11)] (or its transposed-solve form) and the sources
-    for (int e0 = 0; e0 < m * p; e0 += 8 * nth) {
-        cx<T> v[8];
+    const bool cin = lr < ncol;
+    cx<T> b[8], w2[CH ? 16 : 1];
+    if (!CH) {
+        // no children: pivot row k's right-hand side is row pos0 + pi(k) of X, pi
+        // the front's row pivoting (src[3k] = pos0 + pi(k)); the operator's
+        // column k is staged at pi(k) instead, so the rows pos0.. are read in
+        // place, before and alongside the staging loads
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int s = 0; s < 8; s++) {
+            const int k = 4 * s + lk;
+            b[s] = {T(0), T(0)};
+            if (k < p && cin) b[s] = X[(int64_t)(F.pos0 + k) * nrhs + r0 + lr];
+        }
+    }
+    // stage the operator [inv(L11); L21 inv(L11)] (or its transposed-solve form) and the sources
+    // m * p <= 2048 entries: one pass at 512 threads or more
+    for (int e0 = 0; e0 < m * p; e0 += 4 * nth) {
+        cx<T> v[4];
+        int col[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
             const int e = e0 + u * nth + threadIdx.x, k = e / m, i = e - k * m;
             v[u] = {T(0), T(0)};
+            col[u] = k;
             if (e < m * p) {
                 v[u] = G[i * ars + k * acs];
+                if (!CH) col[u] = gs[3 * k] - F.pos0;
                 if (cj) v[u].y = -v[u].y;
                 if (i < p && k > i) v[u] = {T(0), T(0)};
                 if (i == k && !trans) v[u] = {T(1), T(0)};
             }
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int e = e0 + u * nth + threadIdx.x, k = e / m, i = e - k * m;
+        for (int u = 0; u < 4; u++) {
+            const int e = e0 + u * nth + threadIdx.x, i = e - (e / m) * m;
             if (e < m * p) {
-                Ar[k * as + i] = v[u].x;
-                Ai[k * as + i] = v[u].y;
+                Ar[col[u] * as + i] = v[u].x;
+                Ai[col[u] * as + i] = v[u].y;
             }
         }
     }
-    for (int e = threadIdx.x; e < 3 * m; e += nth) src[e] = gs[e];
+    if (CH)
+        for (int e = threadIdx.x; e < 3 * m; e += nth) src[e] = gs[e];
     __syncthreads();
     if (r0 >= nrhs) return;
-    const bool cin = lr < ncol;
     // pivot rows in operand layout (k = 4s + lk), and the contribution rows the
     // lane's accumulators will meet in the epilogue (rows >= p; m <= 64: 4 strips)
-    cx<T> b[8], w2[CH ? 16 : 1];
+    if (CH)
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-        const int k = 4 * s + lk;
-        b[s] = {T(0), T(0)};
-        if (k < p && cin) {
-            const int a = src[3 * k];
-            if (a >= 0) b[s] = X[(int64_t)a * nrhs + r0 + lr];
-            if (CH) {
+        for (int s = 0; s < 8; s++) {
+            const int k = 4 * s + lk;
+            b[s] = {T(0), T(0)};
+            if (k < p && cin) {
+                const int a = src[3 * k];
+                if (a >= 0) b[s] = X[(int64_t)a * nrhs + r0 + lr];
                 const int u = src[3 * k + 1], w = src[3 * k + 2];
                 if (u >= 0) b[s] = cadd(b[s], W[(int64_t)u * nrhs + r0 + lr]);
                 if (w >= 0) b[s] = cadd(b[s], W[(int64_t)w * nrhs + r0 + lr]);
             }
         }
-    }
     if (CH)
 #pragma unroll
         for (int q = 0; q < 4; q++)
