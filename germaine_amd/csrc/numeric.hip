@@ -59,7 +59,9 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
                 for (int c = 0; c < F.nchild; c++) {
                     const Front &C = pl.fronts[pl.children[F.child_off + c]];
                     const int a = pl.inv[C.inv_off + i];
-                    if (a >= 0 && k < 3) r[k++] = (int)(C.fv_off + C.p + a);
+                    if (a < 0) continue;
+                    if (k == 3) throw std::logic_error("front row with more than two child contributions");
+                    r[k++] = (int)(C.fv_off + C.p + a);
                 }
             }
         gsrc_id = dev_upload(g);

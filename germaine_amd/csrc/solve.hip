@@ -145,23 +145,45 @@ __device__ __forceinline__ cx<T> front_row(const FrontDev &F, int i, int r, int 
     return v;
 }
 
-// forward gather: W[f] rows; for M solves the pivot rows are taken through the
-// front's row permutation (P applied on the fly)
+// forward gather: W[f] rows from the gather table (per front row: X position
+// or -1, then up to two children's contribution rows in W or -1; for M solves
+// the table already follows the front's row permutation).  Four entries per
+// thread, every load of the four issued before the sums.
+// grid (fronts, chunks, slots)
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_gather(const cx<T> *__restrict__ X, cx<T> *__restrict__ W,
                                                     const FrontDev *__restrict__ fronts, const int *__restrict__ list,
-                                                    const int *__restrict__ children, const int *__restrict__ inv,
-                                                    const int *__restrict__ rowperm, int n, int64_t wstride, int nrhs,
-                                                    Sel sel) {
+                                                    const int *__restrict__ gsrc, int64_t gstride, int n,
+                                                    int64_t wstride, int nrhs, Sel sel) {
     const FrontDev F = fronts[list[blockIdx.x]];
+    const int z = blockIdx.z;
     const int64_t total = (int64_t)F.m * nrhs;
-    X += (int64_t)sel.s[blockIdx.z] * n * nrhs;
-    W += (int64_t)sel.s[blockIdx.z] * wstride;
-    if (rowperm) rowperm += (int64_t)sel.f[blockIdx.z] * n;
-    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.y * blockDim.x) {
-        int i = (int)(e / nrhs), r = (int)(e - (int64_t)i * nrhs);
-        int src = (rowperm && i < F.p) ? rowperm[F.pos0 + i] : i;
-        W[(F.fv_off + i) * nrhs + r] = front_row(F, src, r, nrhs, X, W, fronts, children, inv);
+    X += (int64_t)sel.s[z] * n * nrhs;
+    W += (int64_t)sel.s[z] * wstride;
+    const int *gs = gsrc + sel.f[z] * gstride + F.fv_off * 3;
+    const int64_t step = (int64_t)gridDim.y * blockDim.x;
+    for (int64_t e0 = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e0 < total; e0 += 4 * step) {
+        cx<T> v[4], c1[4], c2[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t e = e0 + u * step;
+            v[u] = c1[u] = c2[u] = {T(0), T(0)};
+            if (e < total) {
+                const int i = (int)(e / nrhs), r = (int)(e - (int64_t)i * nrhs);
+                const int a = gs[3 * i], b = gs[3 * i + 1], c = gs[3 * i + 2];
+                if (a >= 0) v[u] = X[(int64_t)a * nrhs + r];
+                if (b >= 0) c1[u] = W[(int64_t)b * nrhs + r];
+                if (c >= 0) c2[u] = W[(int64_t)c * nrhs + r];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t e = e0 + u * step;
+            if (e < total) {
+                const int i = (int)(e / nrhs), r = (int)(e - (int64_t)i * nrhs);
+                W[(F.fv_off + i) * nrhs + r] = cadd(v[u], cadd(c1[u], c2[u]));
+            }
+        }
     }
 }
 
@@ -1182,11 +1204,12 @@ __global__ void __launch_bounds__(kSmallBwdThreads) k_bwd_small1(const cx<T> *__
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
-    // stage [inv(U11), -inv(U11) U12] (or its transposed-solve form), rows, output positions
-    for (int e0 = 0; e0 < m * p; e0 += 8 * nth) {
-        cx<T> v[8];
+    // stage [inv(U11), -inv(U11) U12] (or its transposed-solve form), rows, output
+    // positions; m * p <= 2048 entries: one pass at 512 threads
+    for (int e0 = 0; e0 < m * p; e0 += 4 * nth) {
+        cx<T> v[4];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < 4; u++) {
             const int e = e0 + u * nth + threadIdx.x, k = e / p, i = e - k * p;
             v[u] = {T(0), T(0)};
             if (e < m * p) {
@@ -1198,7 +1221,7 @@ __global__ void __launch_bounds__(kSmallBwdThreads) k_bwd_small1(const cx<T> *__
             }
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < 4; u++) {
             const int e = e0 + u * nth + threadIdx.x, k = e / p, i = e - k * p;
             if (e < m * p) {
                 Ar[k * as + i] = v[u].x;
@@ -2033,7 +2056,8 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
             int64_t work = (int64_t)dp.mmax_all[l] * nrhs;
             int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, work / 1024), 256);
             hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nfr, chunks, ns), dim3(256), 0, st, Xp, Wp, dp.fronts,
-                               dp.lv_all[l], dp.children, dp.inv, trans ? nullptr : F.rowperm, n, wstride, nrhs, in);
+                               dp.lv_all[l], trans ? dp.gsrc_id : F.gsrc, trans ? (int64_t)0 : pl.fv_rows * 3, n,
+                               wstride, nrhs, in);
         }
         const int pmax = dp.pmax_all[l];
         for (int b0 = 0; b0 < pmax; b0 += BK) {
