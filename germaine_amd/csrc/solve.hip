@@ -417,9 +417,11 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
 }
 
 // backward, ancestors: W[f][0:p] = y1 - Uop12 x2, x2 gathered from X at the
-// contribution positions.  Each 32-column chunk of Uop12 and x2 is loaded into
-// registers while the previous chunk's products run.
-// grid: (row tiles over p, fronts, rhs tiles)
+// contribution positions.  The positions are staged in LDS first (after the
+// block tiles, m - p ints), so each 32-column chunk of Uop12 and x2 is one
+// memory round trip, loaded into registers while the previous chunk's products
+// run.  grid: (row tiles over p, fronts, rhs tiles)
+constexpr int kContribRowsMax = 16384;   // m - p of any front on this path
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ arena, int64_t fstride,
                                                      const FrontDev *__restrict__ fronts, const int *__restrict__ list,
@@ -428,10 +430,13 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
                                                      int trans, int cj, Sel sel) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     BlkSmem<T> s(smem_raw);
+    int *srow = reinterpret_cast<int *>(smem_raw + blk_smem_bytes<T>());
     const FrontDev F = fronts[list[blockIdx.y]];
     const int m = F.m, p = F.p, tid = threadIdx.x;
     const int row0 = blockIdx.x * 64;
     if (row0 >= p) return;
+    for (int e = tid; e < m - p; e += 256) srow[e] = idx[F.idx_off + p + e];
+    __syncthreads();
     const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64, ncol = min(64, nrhs - r0);
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
@@ -459,7 +464,7 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int e = tid + 256 * u, k = e >> 6, j = e & 63;
-            yv[u] = (k < kb && j < ncol) ? X[(int64_t)idx[F.idx_off + s0 + k] * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+            yv[u] = (k < kb && j < ncol) ? X[(int64_t)srow[s0 - p + k] * nrhs + r0 + j] : cx<T>{T(0), T(0)};
         }
     };
     if (p < m) fetch(p);
@@ -1947,7 +1952,7 @@ void set_smem_attrs() {
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_blk<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, blk));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_contrib<T>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, blk));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, blk + 4 * kContribRowsMax));
     const int fl = (int)kFusedLdsMax;
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_front<T, 64>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, fl));
@@ -2106,7 +2111,9 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
         {
             ProfScope ps(K_BWD_CONTRIB, st, nz * 8.0 * nr * c.contrib_elems,
                          nz * (c.contrib_elems + (c.sum_m + c.sum_p) * nr) * es);
-            hipLaunchKernelGGL(k_bwd_contrib<T>, dim3((pmax + 63) / 64, nfr, rt * ns), dim3(256), blk, st, F.arena,
+            if (dp.mmax_all[l] > kContribRowsMax) throw std::logic_error("front too large for the contribution kernel");
+            hipLaunchKernelGGL(k_bwd_contrib<T>, dim3((pmax + 63) / 64, nfr, rt * ns), dim3(256),
+                               blk + 4 * (size_t)dp.mmax_all[l], st, F.arena,
                                F.fstride, dp.fronts, dp.lv_all[l], dp.idx, Xp, Wp, n, wstride, nrhs, rt, (int)trans,
                                (int)cj, in);
         }
