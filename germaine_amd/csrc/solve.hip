@@ -188,23 +188,28 @@ __global__ void __launch_bounds__(256) k_fwd_gather(const cx<T> *__restrict__ X,
 }
 
 // ---- panel substitutions on MFMA -------------------------------------------
-// Shared-memory layout of the block kernels (dynamic): A tile [BK][64] re/im,
-// right-hand-side block [BK][64] re/im, diagonal block [BK][BK+1] complex.
+// Shared-memory layout of the block kernels (dynamic): right-hand-side block
+// [BK][64] re/im, then the A tile [BK][64] re/im, whose space the diagonal
+// block [BK][BK+1] complex shares: the diagonal product runs (and ends with a
+// barrier) before the A tile is stored.
 template <typename T>
 struct BlkSmem {
     T *Ar, *Ai, *Yr, *Yi;
     cx<T> *D;
     __device__ BlkSmem(unsigned char *raw) {
-        Ar = reinterpret_cast<T *>(raw);
-        Ai = Ar + BK * 64;
-        Yr = Ai + BK * 64;
+        Yr = reinterpret_cast<T *>(raw);
         Yi = Yr + BK * 64;
-        D = reinterpret_cast<cx<T> *>(Yi + BK * 64);
+        Ar = Yi + BK * 64;
+        Ai = Ar + BK * 64;
+        D = reinterpret_cast<cx<T> *>(Ar);
     }
 };
+static_assert(sizeof(cx<double>) * BK * (BK + 1) <= 2 * sizeof(double) * BK * 64 &&
+                  sizeof(cx<float>) * BK * (BK + 1) <= 2 * sizeof(float) * BK * 64,
+              "the diagonal block fits in the A tile's space");
 template <typename T>
 constexpr size_t blk_smem_bytes() {
-    return sizeof(T) * 4 * BK * 64 + sizeof(cx<T>) * BK * (BK + 1);
+    return sizeof(T) * 4 * BK * 64;
 }
 
 // element (i, t) of the operator that solves 32-row diagonal block b0 in the
@@ -393,6 +398,8 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
             s.Yr[e] = yv[u].x;
             s.Yi[e] = yv[u].y;
         }
+        __syncthreads();
+        diag_apply<T>(s, nb, ncol);   // ends with a barrier: D's space is free
 #pragma unroll
         for (int u = 0; u < BK * 64 / 256; u++) {
             const int e = tid + 256 * u;
@@ -404,7 +411,6 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
         }
     }
     __syncthreads();
-    diag_apply<T>(s, nb, ncol);
     if (blockIdx.x == 0)
         for (int e = tid; e < nb * 64; e += 256) {
             int k = e >> 6, j = e & 63;
@@ -543,17 +549,22 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
 #pragma unroll
         for (int u = 0; u < BK * 64 / 256; u++) {
             const int e = tid + 256 * u;
+            s.Yr[e] = yv[u].x;
+            s.Yi[e] = yv[u].y;
+        }
+        __syncthreads();
+        diag_apply<T>(s, nb, ncol);   // ends with a barrier: D's space is free
+#pragma unroll
+        for (int u = 0; u < BK * 64 / 256; u++) {
+            const int e = tid + 256 * u;
             int k, i;
             if (!trans) { i = e & 63; k = e >> 6; }
             else { k = e % BK; i = e / BK; }
-            s.Yr[e] = yv[u].x;
-            s.Yi[e] = yv[u].y;
             s.Ar[k * 64 + i] = av[u].x;
             s.Ai[k * 64 + i] = av[u].y;
         }
     }
     __syncthreads();
-    diag_apply<T>(s, nb, ncol);
     if (blockIdx.x == 0)
         for (int e = tid; e < nb * 64; e += 256) {
             int k = e >> 6, j = e & 63;
