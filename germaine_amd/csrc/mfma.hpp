@@ -41,4 +41,22 @@ __device__ __forceinline__ void cmma(T ar, T ai, T br, T bi, typename mfma_t<T>:
     ci = mfma_t<T>::op(ai, br, ci);
 }
 
+// Three real products per complex product (Gauss): t1 += ar br, t2 += ai bi,
+// t3 += (ar + ai)(br + bi); then re = t1 - t2, im = t3 - t1 - t2.  bs = br + bi
+// is passed in, as the caller reuses it across strips.
+template <typename T>
+__device__ __forceinline__ void cmma3(T ar, T ai, T br, T bi, T bs, typename mfma_t<T>::acc &t1,
+                                      typename mfma_t<T>::acc &t2, typename mfma_t<T>::acc &t3) {
+    t1 = mfma_t<T>::op(ar, br, t1);
+    t2 = mfma_t<T>::op(ai, bi, t2);
+    t3 = mfma_t<T>::op(ar + ai, bs, t3);
+}
+template <typename T>
+__device__ __forceinline__ void cmma3_finish(const typename mfma_t<T>::acc &t1, const typename mfma_t<T>::acc &t2,
+                                             const typename mfma_t<T>::acc &t3, typename mfma_t<T>::acc &cr,
+                                             typename mfma_t<T>::acc &ci) {
+    cr = t1 - t2;
+    ci = t3 - t1 - t2;
+}
+
 }  // namespace gfdfd

@@ -1172,16 +1172,31 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1
                     if (w >= 0) t = cadd(t, W[(int64_t)w * nrhs + r0 + lr]);
                 }
             }
+    T bs[8];   // br + bi of every operand entry, for the three-product complex MMA
+#pragma unroll
+    for (int s = 0; s < 8; s++) bs[s] = b[s].x + b[s].y;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         if (16 * q >= m) break;
         typename mfma_t<T>::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+        if (CH) {   // four real products (registers are short with the contribution rows held)
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
-            if (4 * s >= p) break;
-            const int k = 4 * s + lk;
-            const T ar = k < p ? Ar[k * as + 16 * q + lr] : T(0), ai = k < p ? Ai[k * as + 16 * q + lr] : T(0);
-            cmma<T>(ar, ai, b[s].x, b[s].y, cr, ci);
+            for (int s = 0; s < 8; s++) {
+                if (4 * s >= p) break;
+                const int k = 4 * s + lk;
+                const T ar = k < p ? Ar[k * as + 16 * q + lr] : T(0), ai = k < p ? Ai[k * as + 16 * q + lr] : T(0);
+                cmma<T>(ar, ai, b[s].x, b[s].y, cr, ci);
+            }
+        } else {
+            typename mfma_t<T>::acc t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                if (4 * s >= p) break;
+                const int k = 4 * s + lk;
+                const T ar = k < p ? Ar[k * as + 16 * q + lr] : T(0), ai = k < p ? Ai[k * as + 16 * q + lr] : T(0);
+                cmma3<T>(ar, ai, b[s].x, b[s].y, bs[s], t1, t2, t3);
+            }
+            cmma3_finish<T>(t1, t2, t3, cr, ci);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {

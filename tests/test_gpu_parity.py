@@ -313,10 +313,14 @@ def test_solve_paths_match_oracle(precision, path, sysno):
     fdfd.solve_tuning(*path)
     try:
         F = pl.factor(vals, precision)
+        X1 = None
         if precision == G.FP64:
             X = F.solve_batch([0, 1, 2], dev(B), sys=sysno).cpu().numpy()
             tol = 1e-10
         else:
+            # the unrefined FP32 solve is checked on its own as well: refinement
+            # would hide a wrong substitution behind extra iterations
+            X1 = F.solve_batch([0, 1, 2], dev(B), sys=sysno).cpu().numpy()
             X, its, rr = F.solve_refine_batch([0, 1, 2], dev(B), sys=sysno, tol=1e-12, max_iter=30)
             X = X.cpu().numpy()
             say("  iterations", its, rr)
@@ -327,6 +331,10 @@ def test_solve_paths_match_oracle(precision, path, sysno):
         Xr, info = O.band_solve(*csc, B[z].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
         assert info == 0
         assert rel(X[z], Xr.T) <= (tol if z < 2 else 100 * tol), z
+        if X1 is not None and z < 2:
+            e1 = rel(X1[z], Xr.T)
+            say("  unrefined FP32 error", z, e1)
+            assert e1 <= 1e-3, (z, e1)
 
 
 def test_refactor_equals_fresh_factorisation():
