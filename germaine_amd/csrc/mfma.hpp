@@ -51,6 +51,19 @@ __device__ __forceinline__ void cmma3(T ar, T ai, T br, T bi, T bs, typename mfm
     t2 = mfma_t<T>::op(ai, bi, t2);
     t3 = mfma_t<T>::op(ar + ai, bs, t3);
 }
+// the three accumulators of one 16x16 complex block
+template <typename T>
+struct cacc3 {
+    typename mfma_t<T>::acc t1, t2, t3;
+    __device__ __forceinline__ void zero() {
+        t1 = typename mfma_t<T>::acc{0, 0, 0, 0};
+        t2 = t1;
+        t3 = t1;
+    }
+    __device__ __forceinline__ void mma(T ar, T ai, T br, T bi, T bs) { cmma3<T>(ar, ai, br, bi, bs, t1, t2, t3); }
+    __device__ __forceinline__ T re(int v) const { return t1[v] - t2[v]; }
+    __device__ __forceinline__ T im(int v) const { return t3[v] - t1[v] - t2[v]; }
+};
 template <typename T>
 __device__ __forceinline__ void cmma3_finish(const typename mfma_t<T>::acc &t1, const typename mfma_t<T>::acc &t2,
                                              const typename mfma_t<T>::acc &t3, typename mfma_t<T>::acc &cr,

@@ -228,17 +228,13 @@ __device__ __forceinline__ cx<T> dop(const cx<T> *G, int m, int b0, int i, int t
 // acc += that when accumulate; column sub-tiles at or beyond ncol (right-hand
 // sides past the end) are skipped
 template <typename T>
-__device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, int ncol, typename mfma_t<T>::acc (&cr)[2][2],
-                                         typename mfma_t<T>::acc (&ci)[2][2], bool accumulate = false) {
-    typedef typename mfma_t<T>::acc acc_t;
+__device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, int ncol, cacc3<T> (&c)[2][2],
+                                         bool accumulate = false) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
     if (!accumulate)
         for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) {
-                cr[a][b] = acc_t{0, 0, 0, 0};
-                ci[a][b] = acc_t{0, 0, 0, 0};
-            }
+            for (int b = 0; b < 2; b++) c[a][b].zero();
     const int kend = (kb + 3) & ~3;
     for (int kk = 0; kk < kend; kk += 4) {
         const int k = kk + lk;
@@ -253,7 +249,7 @@ __device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, int ncol, 
         }
         for (int b = 0; b < 2; b++)
             if (wc + b * 16 < ncol)
-                for (int a = 0; a < 2; a++) cmma<T>(ar[a], ai[a], br[b], bi[b], cr[a][b], ci[a][b]);
+                for (int a = 0; a < 2; a++) c[a][b].mma(ar[a], ai[a], br[b], bi[b], br[b] + bi[b]);
     }
 }
 
@@ -293,8 +289,7 @@ __device__ __forceinline__ void load_tile(const cx<T> *w, int nrhs, int row0, in
 }
 template <typename T>
 __device__ __forceinline__ void sub_tile_from(cx<T> *w, int nrhs, int row0, int rowend, int r0,
-                                              const cx<T> (&c)[2][2][4], typename mfma_t<T>::acc (&cr)[2][2],
-                                              typename mfma_t<T>::acc (&ci)[2][2]) {
+                                              const cx<T> (&c)[2][2][4], const cacc3<T> (&acc)[2][2]) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
@@ -305,7 +300,7 @@ __device__ __forceinline__ void sub_tile_from(cx<T> *w, int nrhs, int row0, int 
             for (int v = 0; v < 4; v++) {
                 const int row = row0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = r0 + wc + b * 16 + lr;
                 if (row < rowend && col < nrhs)
-                    w[(int64_t)row * nrhs + col] = {c[a][b][v].x - cr[a][b][v], c[a][b][v].y - ci[a][b][v]};
+                    w[(int64_t)row * nrhs + col] = {c[a][b][v].x - acc[a][b].re(v), c[a][b][v].y - acc[a][b].im(v)};
             }
 }
 
@@ -313,29 +308,27 @@ __device__ __forceinline__ void sub_tile_from(cx<T> *w, int nrhs, int row0, int 
 // waves own 16 rows x 32 right-hand sides each
 template <typename T>
 __device__ __forceinline__ void diag_apply(const BlkSmem<T> &s, int nb, int ncol) {
-    typedef typename mfma_t<T>::acc acc_t;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rw = (w >> 1) * 16, wc = (w & 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
-    acc_t cr[2], ci[2];
-    for (int b = 0; b < 2; b++) {
-        cr[b] = acc_t{0, 0, 0, 0};
-        ci[b] = acc_t{0, 0, 0, 0};
-    }
+    cacc3<T> c[2];
+    for (int b = 0; b < 2; b++) c[b].zero();
     const int kend = (nb + 3) & ~3;
     for (int kk = 0; kk < kend; kk += 4) {
         const int k = kk + lk;
         const cx<T> a = s.D[(rw + lr) * (BK + 1) + k];
         for (int b = 0; b < 2; b++)
-            if (wc + b * 16 < ncol)
-                cmma<T>(a.x, a.y, s.Yr[k * 64 + wc + b * 16 + lr], s.Yi[k * 64 + wc + b * 16 + lr], cr[b], ci[b]);
+            if (wc + b * 16 < ncol) {
+                const T br = s.Yr[k * 64 + wc + b * 16 + lr], bi = s.Yi[k * 64 + wc + b * 16 + lr];
+                c[b].mma(a.x, a.y, br, bi, br + bi);
+            }
     }
     __syncthreads();
     for (int b = 0; b < 2; b++)
         for (int v = 0; v < 4; v++) {
             const int row = rw + mfma_t<T>::row(lk, v), col = wc + b * 16 + lr;
             if (row < nb) {
-                s.Yr[row * 64 + col] = cr[b][v];
-                s.Yi[row * 64 + col] = ci[b][v];
+                s.Yr[row * 64 + col] = c[b].re(v);
+                s.Yi[row * 64 + col] = c[b].im(v);
             }
         }
     __syncthreads();
@@ -417,9 +410,9 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
             if (j < ncol) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
         }
     if (!upd) return;
-    typename mfma_t<T>::acc cr[2][2], ci[2][2];
-    mma_tile<T>(s, nb, ncol, cr, ci);
-    sub_tile_from<T>(w, nrhs, row0, m, r0, wold, cr, ci);
+    cacc3<T> acc[2][2];
+    mma_tile<T>(s, nb, ncol, acc);
+    sub_tile_from<T>(w, nrhs, row0, m, r0, wold, acc);
 }
 
 // backward, ancestors: W[f][0:p] = y1 - Uop12 x2, x2 gathered from X at the
@@ -448,13 +441,9 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
     auto Uop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
-    typedef typename mfma_t<T>::acc acc_t;
-    acc_t cr[2][2], ci[2][2];
+    cacc3<T> acc[2][2];
     for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) {
-            cr[a][b] = acc_t{0, 0, 0, 0};
-            ci[a][b] = acc_t{0, 0, 0, 0};
-        }
+        for (int b = 0; b < 2; b++) acc[a][b].zero();
     constexpr int PER = BK * 64 / 256;
     cx<T> av[PER], yv[PER];
     auto fetch = [&](int s0) {
@@ -490,12 +479,12 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
         }
         __syncthreads();
         if (s0 + BK < m) fetch(s0 + BK);   // in flight during this chunk's products
-        mma_tile<T>(s, kb, ncol, cr, ci, true);
+        mma_tile<T>(s, kb, ncol, acc, true);
     }
     // w[0:p] = y1 - accumulated product
     cx<T> y1[2][2][4];
     load_tile<T>(X + (int64_t)F.pos0 * nrhs, nrhs, row0, p, r0, y1);
-    sub_tile_from<T>(w, nrhs, row0, p, r0, y1, cr, ci);
+    sub_tile_from<T>(w, nrhs, row0, p, r0, y1, acc);
 }
 
 // backward block b0 (blocks from the bottom): x_b = Dop_bb z_b (to X by the
@@ -571,9 +560,9 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
             if (j < ncol) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
         }
     if (!upd) return;
-    typename mfma_t<T>::acc cr[2][2], ci[2][2];
-    mma_tile<T>(s, nb, ncol, cr, ci);
-    sub_tile_from<T>(w, nrhs, row0, b0, r0, wold, cr, ci);
+    cacc3<T> acc[2][2];
+    mma_tile<T>(s, nb, ncol, acc);
+    sub_tile_from<T>(w, nrhs, row0, b0, r0, wold, acc);
 }
 
 // ---- fused per-front substitutions (fronts whose panel fits in LDS) ---------
@@ -1020,7 +1009,9 @@ __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ ar
         const int row = (v >> 2) * 16 + mfma_t<T>::row(lk, v & 3);
         dst[v] = row < p ? F.pos0 + (trans ? rowperm[(int64_t)sel.f[z] * n + F.pos0 + row] : row) : -1;
     }
-    typename mfma_t<T>::acc c0r = {0, 0, 0, 0}, c0i = {0, 0, 0, 0}, c1r = {0, 0, 0, 0}, c1i = {0, 0, 0, 0};
+    cacc3<T> c0, c1;
+    c0.zero();
+    c1.zero();
     for (int k0 = 0; k0 < m; k0 += 32) {
         cx<T> b[8];
         load_a<T>(G, o0, k0, a0);
@@ -1032,15 +1023,16 @@ __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ ar
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             if (k0 + 4 * s >= m) break;
-            cmma<T>(a0[s].x, a0[s].y, b[s].x, b[s].y, c0r, c0i);
-            if (two) cmma<T>(a1[s].x, a1[s].y, b[s].x, b[s].y, c1r, c1i);
+            const T bs = b[s].x + b[s].y;
+            c0.mma(a0[s].x, a0[s].y, b[s].x, b[s].y, bs);
+            if (two) c1.mma(a1[s].x, a1[s].y, b[s].x, b[s].y, bs);
         }
     }
     if (cin) {
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             if (dst[v] < 0) continue;
-            const T xr = v < 4 ? c0r[v] : c1r[v - 4], xi = v < 4 ? c0i[v] : c1i[v - 4];
+            const T xr = v < 4 ? c0.re(v) : c1.re(v - 4), xi = v < 4 ? c0.im(v) : c1.im(v - 4);
             X[(int64_t)dst[v] * nrhs + r0 + lr] = {xr, xi};
         }
     }
@@ -1276,18 +1268,19 @@ __global__ void __launch_bounds__(kSmallBwdThreads) k_bwd_small1(const cx<T> *__
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         if (16 * q >= p) break;
-        typename mfma_t<T>::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+        cacc3<T> c;
+        c.zero();
 #pragma unroll
         for (int s = 0; s < 16; s++) {
             if (4 * s >= m) break;
             const int k = 4 * s + lk;
             const T ar = k < m ? Ar[k * as + 16 * q + lr] : T(0), ai = k < m ? Ai[k * as + 16 * q + lr] : T(0);
-            cmma<T>(ar, ai, b[s].x, b[s].y, cr, ci);
+            c.mma(ar, ai, b[s].x, b[s].y, b[s].x + b[s].y);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int row = 16 * q + mfma_t<T>::row(lk, v);
-            if (row < p && cin) X[(int64_t)dstl[row] * nrhs + r0 + lr] = {cr[v], ci[v]};
+            if (row < p && cin) X[(int64_t)dstl[row] * nrhs + r0 + lr] = {c.re(v), c.im(v)};
         }
     }
 }
