@@ -51,18 +51,32 @@ __device__ __forceinline__ void cmma3(T ar, T ai, T br, T bi, T bs, typename mfm
     t2 = mfma_t<T>::op(ai, bi, t2);
     t3 = mfma_t<T>::op(ar + ai, bs, t3);
 }
-// the three accumulators of one 16x16 complex block
-template <typename T>
+// Accumulators of one 16x16 complex block: the three-product form (three
+// accumulators) by default in FP32; four real products into (re, im) by
+// default in FP64, where the three-product form measured slower in the
+// LDS-tiled products (it stays faster where the operands sit in registers:
+// those kernels ask for it with Three = true).
+template <typename T, bool Three = sizeof(T) == 4>
 struct cacc3 {
+    static constexpr bool three = Three;
     typename mfma_t<T>::acc t1, t2, t3;
     __device__ __forceinline__ void zero() {
         t1 = typename mfma_t<T>::acc{0, 0, 0, 0};
         t2 = t1;
         t3 = t1;
     }
-    __device__ __forceinline__ void mma(T ar, T ai, T br, T bi, T bs) { cmma3<T>(ar, ai, br, bi, bs, t1, t2, t3); }
-    __device__ __forceinline__ T re(int v) const { return t1[v] - t2[v]; }
-    __device__ __forceinline__ T im(int v) const { return t3[v] - t1[v] - t2[v]; }
+    __device__ __forceinline__ void mma(T ar, T ai, T br, T bi, T bs) {
+        if constexpr (three) cmma3<T>(ar, ai, br, bi, bs, t1, t2, t3);
+        else cmma<T>(ar, ai, br, bi, t1, t2);
+    }
+    __device__ __forceinline__ T re(int v) const {
+        if constexpr (three) return t1[v] - t2[v];
+        else return t1[v];
+    }
+    __device__ __forceinline__ T im(int v) const {
+        if constexpr (three) return t3[v] - t1[v] - t2[v];
+        else return t2[v];
+    }
 };
 template <typename T>
 __device__ __forceinline__ void cmma3_finish(const typename mfma_t<T>::acc &t1, const typename mfma_t<T>::acc &t2,

@@ -580,13 +580,9 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
         }
     }
     __syncthreads();
-    typedef typename mfma_t<T>::acc acc_t;
-    acc_t cr[2][2], ci[2][2];
+    cacc3<T> acc[2][2];   // three real products per complex product
     for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) {
-            cr[a][b] = acc_t{0, 0, 0, 0};
-            ci[a][b] = acc_t{0, 0, 0, 0};
-        }
+        for (int b = 0; b < 2; b++) acc[a][b].zero();
     const int kend = (kb + 3) & ~3;
     for (int kk = 0; kk < kend; kk += 4) {
         const int k = kk + lk;
@@ -599,10 +595,10 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
             br[b] = Br[k][wc + b * 16 + lr];
             bi[b] = Bi[k][wc + b * 16 + lr];
         }
-        for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) {
-                cmma<T>(ar[a], ai[a], br[b], bi[b], cr[a][b], ci[a][b]);
-            }
+        for (int b = 0; b < 2; b++) {
+            const T bs = br[b] + bi[b];
+            for (int a = 0; a < 2; a++) acc[a][b].mma(ar[a], ai[a], br[b], bi[b], bs);
+        }
     }
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -612,7 +608,7 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
             for (int v = 0; v < 4; v++) {
                 const int row = r0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = c0 + wc + b * 16 + lr;
                 if (row < m && col < m)
-                    G[row + (int64_t)col * m] = {c[a][b][v].x - cr[a][b][v], c[a][b][v].y - ci[a][b][v]};
+                    G[row + (int64_t)col * m] = {c[a][b][v].x - acc[a][b].re(v), c[a][b][v].y - acc[a][b].im(v)};
             }
 }
 

@@ -1268,7 +1268,7 @@ __global__ void __launch_bounds__(kSmallBwdThreads) k_bwd_small1(const cx<T> *__
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         if (16 * q >= p) break;
-        cacc3<T> c;
+        cacc3<T, true> c;
         c.zero();
 #pragma unroll
         for (int s = 0; s < 16; s++) {
