@@ -943,7 +943,6 @@ __global__ void __launch_bounds__(256) k_fwd_front1(const cx<T> *__restrict__ ar
         }
     };
     cx<T> w2[4];
-    typename mfma_t<T>::acc cr[1], ci[1];
     for (int s0 = 0; s0 < m; s0 += 16) {
         // (strip 0's factor entries were requested before the gather)
         if (s0 > 0) {
@@ -953,14 +952,22 @@ __global__ void __launch_bounds__(256) k_fwd_front1(const cx<T> *__restrict__ ar
             load_a<T>(G + s0 * ars, o, 0, av);
         }
         load_w2(s0, w2);
-        acc_zero<T, 1>(cr, ci);
-        mma_a<T, 1>(av, 0, p, Br, Bi, 0, WT, cr, ci);
+        cacc3<T> c;
+        c.zero();
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            if (4 * s >= p) break;
+            const int k = 4 * s + lk;
+            const bool kin = k < p;
+            const T br = kin ? Br[k * WT + lr] : T(0), bi = kin ? Bi[k * WT + lr] : T(0);
+            c.mma(av[s].x, av[s].y, br, bi, br + bi);
+        }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int row = s0 + mfma_t<T>::row(lk, v);
             if (row < m && lr < ncol) {
-                if (row < p) X[(int64_t)(F.pos0 + row) * nrhs + r0 + lr] = {cr[0][v], ci[0][v]};
-                else W[(F.fv_off + row) * nrhs + r0 + lr] = {w2[v].x - cr[0][v], w2[v].y - ci[0][v]};
+                if (row < p) X[(int64_t)(F.pos0 + row) * nrhs + r0 + lr] = {c.re(v), c.im(v)};
+                else W[(F.fv_off + row) * nrhs + r0 + lr] = {w2[v].x - c.re(v), w2[v].y - c.im(v)};
             }
         }
     }
@@ -1170,16 +1177,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         if (16 * q >= m) break;
-        typename mfma_t<T>::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-        if (CH) {   // four real products (registers are short with the contribution rows held)
-#pragma unroll
-            for (int s = 0; s < 8; s++) {
-                if (4 * s >= p) break;
-                const int k = 4 * s + lk;
-                const T ar = k < p ? Ar[k * as + 16 * q + lr] : T(0), ai = k < p ? Ai[k * as + 16 * q + lr] : T(0);
-                cmma<T>(ar, ai, b[s].x, b[s].y, cr, ci);
-            }
-        } else {
+        typename mfma_t<T>::acc cr, ci;
+        {
             typename mfma_t<T>::acc t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
 #pragma unroll
             for (int s = 0; s < 8; s++) {
