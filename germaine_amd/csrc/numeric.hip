@@ -206,25 +206,41 @@ template struct Factors<float>;
 // ---------------------------------------------------------------------------
 // 1+2: front assembly
 // ---------------------------------------------------------------------------
+constexpr int kAsmMaxChildren = 8;
 template <typename T>
 __global__ void __launch_bounds__(256) k_front_assemble(cx<T> *__restrict__ arena, int64_t fstride,
                                                         const FrontDev *__restrict__ fronts,
                                                         const int *__restrict__ list, const int *__restrict__ children,
                                                         const int *__restrict__ inv) {
+    extern __shared__ int cmap[];   // [nchild][m]: child row of each front row, or -1
+    __shared__ int64_t cbase[kAsmMaxChildren];
+    __shared__ int cstride[kAsmMaxChildren];
     const FrontDev F = fronts[list[blockIdx.x]];
     cx<T> *A = arena + blockIdx.z * fstride;
     cx<T> *G = A + F.buf_off;
-    const int64_t total = (int64_t)F.m * F.m;
+    const int m = F.m, nc = F.nchild;
+    // the children's row maps and block origins, staged once: each entry below
+    // then waits on one value load only
+    if (threadIdx.x < nc) {
+        const FrontDev C = fronts[children[F.child_off + threadIdx.x]];
+        cbase[threadIdx.x] = C.buf_off + C.p + (int64_t)C.p * C.m;   // contribution block origin
+        cstride[threadIdx.x] = C.m;
+    }
+    for (int c = 0; c < nc; c++) {
+        const int io = fronts[children[F.child_off + c]].inv_off;
+        for (int r = threadIdx.x; r < m; r += blockDim.x) cmap[c * m + r] = inv[io + r];
+    }
+    __syncthreads();
+    const int64_t total = (int64_t)m * m;
     const int64_t step = (int64_t)gridDim.y * blockDim.x;
-    // unrolled: several entries' index and value loads in flight at once
+    // unrolled: several entries' value loads in flight at once
 #pragma unroll 4
     for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < total; e += step) {
-        int s = (int)(e / F.m), r = (int)(e - (int64_t)s * F.m);
+        const int s = (int)(e / m), r = (int)(e - (int64_t)s * m);
         cx<T> v = {T(0), T(0)};
-        for (int c = 0; c < F.nchild; c++) {
-            const FrontDev C = fronts[children[F.child_off + c]];
-            int a = inv[C.inv_off + r], b = inv[C.inv_off + s];
-            if (a >= 0 && b >= 0) v = cadd(v, A[C.buf_off + (C.p + a) + (int64_t)(C.p + b) * C.m]);
+        for (int c = 0; c < nc; c++) {
+            const int a = cmap[c * m + r], b = cmap[c * m + s];
+            if (a >= 0 && b >= 0) v = cadd(v, A[cbase[c] + a + (int64_t)b * cstride[c]]);
         }
         G[e] = v;
     }
@@ -807,7 +823,10 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
             ProfScope ps(K_FRONT_ASSEMBLE, st, 0.0, cnt.asm_elems * es);
             int64_t mm = (int64_t)dp.mmax_all[l] * dp.mmax_all[l];
             int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, mm / (256 * 8)), 512);
-            hipLaunchKernelGGL(k_front_assemble<T>, dim3(dp.n_all[l], chunks, nf), dim3(256), 0, st, F.arena,
+            const size_t lds = sizeof(int) * (size_t)dp.max_children * dp.mmax_all[l];
+            if (dp.max_children > kAsmMaxChildren || lds > 64 * 1024)
+                throw std::logic_error("front assembly: too many children or too large a front for the row maps");
+            hipLaunchKernelGGL(k_front_assemble<T>, dim3(dp.n_all[l], chunks, nf), dim3(256), lds, st, F.arena,
                                F.fstride, dp.fronts, dp.lv_all[l], dp.children, dp.inv);
         }
         const int64_t ocnt = L.orig_end - L.orig_begin;
