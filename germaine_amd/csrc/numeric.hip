@@ -339,10 +339,12 @@ __global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int
         for (int r = j + 1 + tid; r < m; r += bs) S[r + j * m] = cmul(r == pr ? pv[1] : S[r + j * m], d);
         if (tid == 0) S[j + j * m] = pv[0];
         __syncthreads();
-        const int t = m - j - 1;
-        for (int e = tid; e < t * t; e += bs) {
-            int c = j + 1 + e / t, r = j + 1 + e % t;
-            S[r + c * m] = cmsub(S[r + c * m], S[r + j * m], S[j + c * m]);
+        // rank-1 update: lane -> rows (one per lane up to order 65), waves take
+        // the columns in turn; the row's multiplier stays in a register and the
+        // column's pivot-row entry is one broadcast read
+        for (int r = j + 1 + (tid & 63); r < m; r += 64) {
+            const cx<T> l = S[r + j * m];
+            for (int c = j + 1 + (tid >> 6); c < m; c += bs >> 6) S[r + c * m] = cmsub(S[r + c * m], l, S[j + c * m]);
         }
         __syncthreads();
     }
