@@ -1055,13 +1055,13 @@ __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ ar
 // task, and the only LDS is the operator, so many tasks stay resident.
 // Operator element (i, k) sits at A[k * as + i].
 constexpr int kSmallM = 64;
-// forward workgroup size limit: one workgroup covers a front's whole panel
-// (13 tiles of 16 at 200 right-hand sides) in FP32, so the operator is staged
-// once per front; FP64 keeps 8 waves, within its registers. The backward
-// kernel stays at 8 waves (measured faster: more workgroups per CU).
+// workgroup size limits (waves of 16-column tiles): 8 for the forward, 7 for
+// the backward kernel -- at 200 right-hand sides (13 tiles) a front's panel is
+// two workgroups, which measured faster than one of 13 to 16 waves (the two
+// halves of a front run on two CUs) and than smaller groups
 template <typename T>
-constexpr int kSmallThreads = sizeof(T) == 4 ? 1024 : 512;
-constexpr int kSmallBwdThreads = 512;
+constexpr int kSmallThreads = 512;
+constexpr int kSmallBwdThreads = 448;
 
 // operator row stride: rows rounded to 16, then to 16 or 48 mod 64, so the four
 // k-rows of one operand read start in different banks
