@@ -1916,8 +1916,8 @@ int fused_rw(int mmax, int max_children, bool single) {
     return 0;
 }
 
-// wave-task launch of a single-block level: up to four waves (consecutive
-// 16-column tiles of one front) per workgroup as the per-wave LDS allows
+// wave-task launch of a single-block level: one wave (a 16-column tile of one
+// front) per workgroup, with that wave's LDS
 struct WaveLaunch {
     dim3 grid, block;
     size_t lds;
@@ -1928,13 +1928,10 @@ WaveLaunch wave_launch(int nfr, int nrhs, int nslots, size_t per_wave) {
     const size_t wl = (per_wave + 15) & ~size_t(15);
     if (wl > kFusedLdsMax) throw std::logic_error("single-block front too large for a wave task");
     const int rt = (nrhs + WT - 1) / WT;
-    // waves per workgroup: the most resident waves per CU (LDS-bound), then the widest
-    int wpg = 1, best = 0;
-    for (int w = 1; w <= std::min(4, rt); w++) {
-        const int res = w * (int)(kFusedLdsMax / (w * wl));
-        if (res >= best) best = res, wpg = w;
-    }
-    const int rtg = (rt + wpg - 1) / wpg;
+    // one wave per workgroup: measured faster than grouping 2 or 4 tiles of a
+    // front (the tasks share nothing but the factor entries, which L2 serves)
+    const int wpg = 1;
+    const int rtg = rt;
     return {dim3(((nfr + 7) / 8) * 8 * rtg, nslots), dim3(64 * wpg), wpg * wl, rtg, (int)wl};
 }
 
@@ -2388,7 +2385,8 @@ std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<in
     cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)pl.fv_rows * ld * ns);
     double *nrm = (double *)ws_n.get(sizeof(double) * 2 * nrhs * ns);
     const int ktiles = (n + PT_K - 1) / PT_K;
-    const int nblk = std::min(((pl.nx + kIrTx - 1) / kIrTx) * pl.ny + 7, 2048) / 8 * 8;   // a multiple of 8
+    // up to 8192 workgroups per slot (measured faster than 1024-4096), a multiple of 8
+    const int nblk = std::min(((pl.nx + kIrTx - 1) / kIrTx) * pl.ny + 7, 8192) / 8 * 8;
     double *partial = (double *)ws_p.get(sizeof(double) * (size_t)std::max(ktiles, nblk) * nrhs * ns);
     std::vector<int> slots(ns);
     for (int z = 0; z < ns; z++) slots[z] = z;
