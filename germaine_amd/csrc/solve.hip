@@ -2104,7 +2104,8 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
             continue;
         }
         if (dp.single[l]) {
-            const WaveLaunch wl = small_launch(nfr, nrhs, ns, 0, 4);
+            // one wave per workgroup: measured faster than 2 or 4 (no LDS is shared)
+            const WaveLaunch wl = small_launch(nfr, nrhs, ns, 0, 1);
             ProfScope ps(K_BWD_FRONT1, st, nz * 8.0 * nr * c.tri_elems,
                          nz * (c.tri_elems + (c.sum_m + c.sum_p) * nr) * es);
             hipLaunchKernelGGL(k_bwd_front1<T>, wl.grid, wl.block, 0, st, F.arena, F.fstride, dp.lv_fr[l], nfr, dp.idx,
