@@ -410,8 +410,7 @@ __global__ void __launch_bounds__(256) k_panel_lu(cx<T> *__restrict__ arena, int
                                                   int *__restrict__ piv, int n, int k0, int nb, int *__restrict__ sing) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *P = reinterpret_cast<cx<T> *>(smem_raw);
-    __shared__ T sb[4];
-    __shared__ int si[4];
+    __shared__ int si[1];
     const FrontDev F = fronts[list[blockIdx.x]];
     if (F.p <= k0) return;
     const int fq = blockIdx.y;
@@ -422,23 +421,38 @@ __global__ void __launch_bounds__(256) k_panel_lu(cx<T> *__restrict__ arena, int
         P[c * np + r] = G[(k0 + r) + (int64_t)(k0 + c) * m];
     }
     __syncthreads();
+    __shared__ cx<T> pv[2];   // pivot entry and the entry it swaps with, published with the pivot row
     for (int j = 0; j < kb; j++) {
-        T best = T(-1);
-        int bi = j;
-        for (int r = j + tid; r < np; r += 256) better(best, bi, cabs1(P[j * np + r]), r);
-        const int pr = block_argmax(best, bi, sb, si);
-        if (tid == 0) {
-            piv[(int64_t)fq * n + F.pos0 + k0 + j] = k0 + pr;
-            if (sb[0] == T(0)) atomicOr(sing + fq, 1);
+        // pivot search by wave 0 alone (a few candidates per lane)
+        if (tid < 64) {
+            T best = T(-1);
+            int bi = j;
+            for (int r = j + tid; r < np; r += 64) better(best, bi, cabs1(P[j * np + r]), r);
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                better(best, bi, ob, oi);
+            }
+            if (tid == 0) {
+                si[0] = bi;
+                pv[0] = P[j * np + bi];
+                pv[1] = P[j * np + j];
+                piv[(int64_t)fq * n + F.pos0 + k0 + j] = k0 + bi;
+                if (best == T(0)) atomicOr(sing + fq, 1);
+            }
         }
-        if (pr != j && tid < kb) {
+        __syncthreads();
+        // row swap in the panel's other columns and the scaled column j, in one
+        // pass: column j's swapped pair comes from pv, read before anyone wrote
+        const int pr = si[0];
+        const cx<T> d = crecip(pv[0]);
+        if (pr != j && tid < kb && tid != j) {
             const cx<T> t = P[tid * np + j];
             P[tid * np + j] = P[tid * np + pr];
             P[tid * np + pr] = t;
         }
-        __syncthreads();
-        const cx<T> d = crecip(P[j * np + j]);
-        for (int r = j + 1 + tid; r < np; r += 256) P[j * np + r] = cmul(P[j * np + r], d);
+        for (int r = j + 1 + tid; r < np; r += 256) P[j * np + r] = cmul(r == pr ? pv[1] : P[j * np + r], d);
+        if (tid == 0) P[j * np + j] = pv[0];
         __syncthreads();
         const int rows = np - j - 1, ncol = kb - j - 1;
         for (int e = tid; e < rows * ncol; e += 256) {
