@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(64) k_panel_apply(cx<T> *__restrict__ arena, i
 }
 
 template <typename T>
-__global__ void __launch_bounds__(64) k_trsm_u(cx<T> *__restrict__ arena, int64_t fstride,
+__global__ void __launch_bounds__(256) k_trsm_u(cx<T> *__restrict__ arena, int64_t fstride,
                                                const FrontDev *__restrict__ fronts, const int *__restrict__ list,
                                                int k0, int nb) {
     __shared__ cx<T> Ls[32][33];
@@ -534,24 +534,34 @@ __global__ void __launch_bounds__(64) k_trsm_u(cx<T> *__restrict__ arena, int64_
     cx<T> *G = arena + blockIdx.z * fstride + F.buf_off;
     const int tid = threadIdx.x;
     // unrolled so that a batch of loads is in flight at once
-#pragma unroll 8
-    for (int e = tid; e < kb * kb; e += 64) {
+#pragma unroll 4
+    for (int e = tid; e < kb * kb; e += 256) {
         int i = e % kb, t = e / kb;
         Ls[i][t] = G[(k0 + i) + (int64_t)(k0 + t) * m];
     }
 #pragma unroll 8
-    for (int e = tid; e < kb * 64; e += 64) {
+    for (int e = tid; e < kb * 64; e += 256) {
         int i = e % kb, c = e / kb, col = c0 + c;
         Xs[i][c] = col < m ? G[(k0 + i) + (int64_t)col * m] : cx<T>{T(0), T(0)};
     }
     __syncthreads();
-    for (int i = 1; i < kb; i++) {
-        cx<T> acc = Xs[i][tid];
-        for (int t = 0; t < i; t++) acc = cmsub(acc, Ls[i][t], Xs[t][tid]);
-        Xs[i][tid] = acc;
+    {
+        // four adjacent lanes (one wave) per column: each sums every fourth term
+        // of row i's dot product, two xor-shuffles add them, lane 0 of the four
+        // stores the entry the next rows read
+        const int c = tid >> 2, part = tid & 3;
+        for (int i = 1; i < kb; i++) {
+            cx<T> acc = {T(0), T(0)};
+            for (int t = part; t < i; t += 4) acc = cmsub(acc, Ls[i][t], Xs[t][c]);
+            acc.x += __shfl_xor(acc.x, 1);
+            acc.y += __shfl_xor(acc.y, 1);
+            acc.x += __shfl_xor(acc.x, 2);
+            acc.y += __shfl_xor(acc.y, 2);
+            if (part == 0) Xs[i][c] = cadd(Xs[i][c], acc);
+        }
     }
     __syncthreads();
-    for (int e = tid; e < kb * 64; e += 64) {
+    for (int e = tid; e < kb * 64; e += 256) {
         int i = e % kb, c = e / kb, col = c0 + c;
         if (col < m) G[(k0 + i) + (int64_t)col * m] = Xs[i][c];
     }
@@ -882,7 +892,7 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
                 {
                 ProfScope ps(K_TRSM, st, cnt.trsm_flops[si] * nf, 0.0);
                 int colchunks = (L.large_mmax - k0 + 63) / 64;
-                hipLaunchKernelGGL(k_trsm_u<T>, dim3(colchunks, dp.n_large[l], nf), dim3(64), 0, st, F.arena,
+                hipLaunchKernelGGL(k_trsm_u<T>, dim3(colchunks, dp.n_large[l], nf), dim3(256), 0, st, F.arena,
                                    F.fstride, dp.fronts, dp.lv_large[l], k0, pl.nb);
                 }
                 const auto &sp = dp.steps[l][si];
