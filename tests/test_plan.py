@@ -63,6 +63,17 @@ def test_plan_structure(nx, ny, leaf):
     check_structure(nx, ny, leaf)
 
 
+@pytest.mark.parametrize("nx,ny", [(37, 23), (90, 64), (201, 101), (801, 187)])
+def test_grid_plans_have_at_most_two_children_per_row(nx, ny):
+    """The device gather table holds two child contribution rows per front row
+    (the factor setup refuses a plan that needs more); nested dissection of a
+    grid gives every front at most two children, hence at most two per row."""
+    pl = fdfd.Plan(nx, ny)
+    parent = np.asarray(pl.fronts()["parent"])
+    kids = np.bincount(parent[parent >= 0], minlength=len(parent))
+    assert kids.max() == 2
+
+
 def mf_solve(fr, perm, M, B, sys):
     """numpy multifrontal factor + solve following the plan."""
     n = M.shape[0]
