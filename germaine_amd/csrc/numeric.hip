@@ -848,7 +848,8 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
         if (dp.n_all[l] > 0) {
             ProfScope ps(K_FRONT_ASSEMBLE, st, 0.0, cnt.asm_elems * es);
             int64_t mm = (int64_t)dp.mmax_all[l] * dp.mmax_all[l];
-            int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, mm / (256 * 8)), 512);
+            // about 16 entries per thread (measured faster than 4 or 8)
+            int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, mm / (256 * 16)), 256);
             const size_t lds = sizeof(int) * (size_t)dp.max_children * dp.mmax_all[l];
             if (dp.max_children > kAsmMaxChildren || lds > 64 * 1024)
                 throw std::logic_error("front assembly: too many children or too large a front for the row maps");
