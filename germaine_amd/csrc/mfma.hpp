@@ -1,5 +1,6 @@
 // Complex products on gfx950 matrix cores: the real MFMA 16x16x4 instructions
-// for f32 and f64, four real products per complex product.
+// for f32 and f64, four real products per complex product (cmma) or three
+// (cmma3 / cacc3, Gauss's form).
 //
 // Operand layout (one wave): A lane l -> (row l%16, k l/16), B lane l ->
 // (k l/16, col l%16).  Accumulator register v of lane l holds column l%16 and

@@ -568,7 +568,7 @@ __global__ void __launch_bounds__(256) k_trsm_u(cx<T> *__restrict__ arena, int64
 }
 
 // ---------------------------------------------------------------------------
-// trailing update C -= A*B on MFMA (complex = 4 real products), 64x64 tiles,
+// trailing update C -= A*B on MFMA (complex = 3 real products in FP32, 4 in FP64), 64x64 tiles,
 // K <= 32.  4 waves in 2x2, each wave 32x32 = 2x2 blocks of 16x16.
 // MFMA 16x16x4 operand layout: A lane l -> (row l%16, k l/16), B lane l ->
 // (k l/16, col l%16).  Accumulator register v of lane l holds column l%16 and

@@ -1046,7 +1046,7 @@ __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ ar
 }
 
 // ---- single-block levels with small fronts (m <= 64): staged operator -------
-// The workgroup (up to 16 waves = consecutive 16-column tiles of one front)
+// The workgroup (up to 8 waves = consecutive 16-column tiles of one front)
 // stages the front's whole operator and its rows' gather sources in LDS, with
 // all loads in flight at once; then every wave loads its right-hand-side
 // operand straight from global memory into registers in MFMA layout (k = 4s +
