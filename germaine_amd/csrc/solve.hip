@@ -1227,7 +1227,7 @@ __global__ void __launch_bounds__(kSmallBwdThreads) k_bwd_small1(const cx<T> *__
     X += (int64_t)sel.s[z] * n * nrhs;
     const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
     // stage [inv(U11), -inv(U11) U12] (or its transposed-solve form), rows, output
-    // positions; m * p <= 2048 entries: one pass at 512 threads
+    // positions; m * p <= 2048 entries: one or two passes at 448 threads
     for (int e0 = 0; e0 < m * p; e0 += 4 * nth) {
         cx<T> v[4];
 #pragma unroll
