@@ -192,7 +192,8 @@ __global__ void __launch_bounds__(256) k_fwd_gather(const cx<T> *__restrict__ X,
 // [BK][64] re/im, then the A tile [BK][64] re/im, whose space the diagonal
 // block [BK][BK+1] complex shares: the diagonal product runs (and ends with a
 // barrier) before the A tile is stored.
-template <typename T>
+// RT: rows of the A tile (64, or 128 for the forward block kernel's 8 waves).
+template <typename T, int RT = 64>
 struct BlkSmem {
     T *Ar, *Ai, *Yr, *Yi;
     cx<T> *D;
@@ -200,16 +201,16 @@ struct BlkSmem {
         Yr = reinterpret_cast<T *>(raw);
         Yi = Yr + BK * 64;
         Ar = Yi + BK * 64;
-        Ai = Ar + BK * 64;
+        Ai = Ar + BK * RT;
         D = reinterpret_cast<cx<T> *>(Ar);
     }
 };
 static_assert(sizeof(cx<double>) * BK * (BK + 1) <= 2 * sizeof(double) * BK * 64 &&
                   sizeof(cx<float>) * BK * (BK + 1) <= 2 * sizeof(float) * BK * 64,
               "the diagonal block fits in the A tile's space");
-template <typename T>
+template <typename T, int RT = 64>
 constexpr size_t blk_smem_bytes() {
-    return sizeof(T) * 4 * BK * 64;
+    return sizeof(T) * (2 * BK * 64 + 2 * BK * RT);
 }
 
 // element (i, t) of the operator that solves 32-row diagonal block b0 in the
@@ -227,8 +228,8 @@ __device__ __forceinline__ cx<T> dop(const cx<T> *G, int m, int b0, int i, int t
 // acc = A[64 x kb] * Y[kb x 64] from LDS (4 waves in 2x2, 32x32 each), or
 // acc += that when accumulate; column sub-tiles at or beyond ncol (right-hand
 // sides past the end) are skipped
-template <typename T>
-__device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, int ncol, cacc3<T> (&c)[2][2],
+template <typename T, int RT = 64>
+__device__ __forceinline__ void mma_tile(const BlkSmem<T, RT> &s, int kb, int ncol, cacc3<T> (&c)[2][2],
                                          bool accumulate = false) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
@@ -240,8 +241,8 @@ __device__ __forceinline__ void mma_tile(const BlkSmem<T> &s, int kb, int ncol, 
         const int k = kk + lk;
         T ar[2], ai[2], br[2], bi[2];
         for (int a = 0; a < 2; a++) {
-            ar[a] = s.Ar[k * 64 + wr + a * 16 + lr];
-            ai[a] = s.Ai[k * 64 + wr + a * 16 + lr];
+            ar[a] = s.Ar[k * RT + wr + a * 16 + lr];
+            ai[a] = s.Ai[k * RT + wr + a * 16 + lr];
         }
         for (int b = 0; b < 2; b++) {
             br[b] = s.Yr[k * 64 + wc + b * 16 + lr];
@@ -305,14 +306,16 @@ __device__ __forceinline__ void sub_tile_from(cx<T> *w, int nrhs, int row0, int 
 }
 
 // Y[0:nb] = D Y[0:nb] in LDS (D: the block operator, zero outside nb x nb):
-// waves own 16 rows x 32 right-hand sides each
-template <typename T>
-__device__ __forceinline__ void diag_apply(const BlkSmem<T> &s, int nb, int ncol) {
+// waves 0-3 own 16 rows x 32 right-hand sides each (further waves only meet
+// the barriers)
+template <typename T, int RT = 64>
+__device__ __forceinline__ void diag_apply(const BlkSmem<T, RT> &s, int nb, int ncol) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rw = (w >> 1) * 16, wc = (w & 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
     cacc3<T> c[2];
     for (int b = 0; b < 2; b++) c[b].zero();
-    const int kend = (nb + 3) & ~3;
+    const bool act = RT == 64 || w < 4;
+    const int kend = act ? (nb + 3) & ~3 : 0;
     for (int kk = 0; kk < kend; kk += 4) {
         const int k = kk + lk;
         const cx<T> a = s.D[(rw + lr) * (BK + 1) + k];
@@ -323,14 +326,15 @@ __device__ __forceinline__ void diag_apply(const BlkSmem<T> &s, int nb, int ncol
             }
     }
     __syncthreads();
-    for (int b = 0; b < 2; b++)
-        for (int v = 0; v < 4; v++) {
-            const int row = rw + mfma_t<T>::row(lk, v), col = wc + b * 16 + lr;
-            if (row < nb) {
-                s.Yr[row * 64 + col] = c[b].re(v);
-                s.Yi[row * 64 + col] = c[b].im(v);
+    if (act)
+        for (int b = 0; b < 2; b++)
+            for (int v = 0; v < 4; v++) {
+                const int row = rw + mfma_t<T>::row(lk, v), col = wc + b * 16 + lr;
+                if (row < nb) {
+                    s.Yr[row * 64 + col] = c[b].re(v);
+                    s.Yi[row * 64 + col] = c[b].im(v);
+                }
             }
-        }
     __syncthreads();
 }
 
@@ -340,17 +344,19 @@ __device__ __forceinline__ void diag_apply(const BlkSmem<T> &s, int nb, int ncol
 // operator block below and the rows it updates -- is requested before the
 // first barrier, so the workgroup waits on global memory once.
 // grid: (row tiles, fronts, right-hand-side tiles)
+constexpr int kFwdRT = 128;   // rows per forward block workgroup (8 waves of 32x32 tiles)
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena, int64_t fstride,
+__global__ void __launch_bounds__(4 * kFwdRT) k_fwd_blk(const cx<T> *__restrict__ arena, int64_t fstride,
                                                  const FrontDev *__restrict__ fronts, const int *__restrict__ list,
                                                  cx<T> *__restrict__ X, cx<T> *__restrict__ W, int n, int64_t wstride,
                                                  int nrhs, int rt, int b0, int trans, int cj, Sel sel) {
+    constexpr int RT = kFwdRT, NT = 4 * RT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    BlkSmem<T> s(smem_raw);
+    BlkSmem<T, RT> s(smem_raw);
     const FrontDev F = fronts[list[blockIdx.y]];
     if (F.p <= b0) return;
     const int m = F.m, nb = min(BK, F.p - b0), tid = threadIdx.x;
-    const int row0 = b0 + nb + blockIdx.x * 64;
+    const int row0 = b0 + nb + blockIdx.x * RT;
     if (blockIdx.x > 0 && row0 >= m) return;
     const bool upd = row0 < m;
     const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64, ncol = min(64, nrhs - r0);
@@ -361,57 +367,57 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const cx<T> *__restrict__ arena
     cx<T> wold[2][2][4];
     if (upd) load_tile<T>(w, nrhs, row0, m, r0, wold);
     {
-        cx<T> dv[BK * BK / 256], yv[BK * 64 / 256], av[BK * 64 / 256];
+        cx<T> dv[BK * BK / NT], yv[BK * 64 / NT], av[BK * RT / NT];
 #pragma unroll
-        for (int u = 0; u < BK * BK / 256; u++) {
-            const int e = tid + 256 * u, i = e % BK, t = e / BK;
+        for (int u = 0; u < BK * BK / NT; u++) {
+            const int e = tid + NT * u, i = e % BK, t = e / BK;
             dv[u] = (i < nb && t < nb) ? dop(G, m, b0, i, t, true, trans, cj) : cx<T>{T(0), T(0)};
         }
 #pragma unroll
-        for (int u = 0; u < BK * 64 / 256; u++) {
-            const int e = tid + 256 * u, k = e >> 6, j = e & 63;
+        for (int u = 0; u < BK * 64 / NT; u++) {
+            const int e = tid + NT * u, k = e >> 6, j = e & 63;
             yv[u] = (k < nb && j < ncol) ? w[(int64_t)(b0 + k) * nrhs + r0 + j] : cx<T>{T(0), T(0)};
         }
 #pragma unroll
-        for (int u = 0; u < BK * 64 / 256; u++) {
-            const int e = tid + 256 * u;
+        for (int u = 0; u < BK * RT / NT; u++) {
+            const int e = tid + NT * u;
             int k, i;
-            if (!trans) { i = e & 63; k = e >> 6; }        // column-major L: coalesced along rows
+            if (!trans) { i = e % RT; k = e / RT; }        // column-major L: coalesced along rows
             else { k = e % BK; i = e / BK; }               // U^T: coalesced along k
             av[u] = (upd && k < nb && row0 + i < m) ? Lop(row0 + i, b0 + k) : cx<T>{T(0), T(0)};
         }
 #pragma unroll
-        for (int u = 0; u < BK * BK / 256; u++) {
-            const int e = tid + 256 * u;
+        for (int u = 0; u < BK * BK / NT; u++) {
+            const int e = tid + NT * u;
             s.D[(e % BK) * (BK + 1) + e / BK] = dv[u];
         }
 #pragma unroll
-        for (int u = 0; u < BK * 64 / 256; u++) {
-            const int e = tid + 256 * u;
+        for (int u = 0; u < BK * 64 / NT; u++) {
+            const int e = tid + NT * u;
             s.Yr[e] = yv[u].x;
             s.Yi[e] = yv[u].y;
         }
         __syncthreads();
-        diag_apply<T>(s, nb, ncol);   // ends with a barrier: D's space is free
+        diag_apply<T, RT>(s, nb, ncol);   // ends with a barrier: D's space is free
 #pragma unroll
-        for (int u = 0; u < BK * 64 / 256; u++) {
-            const int e = tid + 256 * u;
+        for (int u = 0; u < BK * RT / NT; u++) {
+            const int e = tid + NT * u;
             int k, i;
-            if (!trans) { i = e & 63; k = e >> 6; }
+            if (!trans) { i = e % RT; k = e / RT; }
             else { k = e % BK; i = e / BK; }
-            s.Ar[k * 64 + i] = av[u].x;
-            s.Ai[k * 64 + i] = av[u].y;
+            s.Ar[k * RT + i] = av[u].x;
+            s.Ai[k * RT + i] = av[u].y;
         }
     }
     __syncthreads();
     if (blockIdx.x == 0)
-        for (int e = tid; e < nb * 64; e += 256) {
+        for (int e = tid; e < nb * 64; e += NT) {
             int k = e >> 6, j = e & 63;
             if (j < ncol) X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {s.Yr[e], s.Yi[e]};
         }
     if (!upd) return;
     cacc3<T> acc[2][2];
-    mma_tile<T>(s, nb, ncol, acc);
+    mma_tile<T, RT>(s, nb, ncol, acc);
     sub_tile_from<T>(w, nrhs, row0, m, r0, wold, acc);
 }
 
@@ -1963,7 +1969,7 @@ void set_smem_attrs() {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, big));
     const int blk = (int)blk_smem_bytes<T>();
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_blk<T>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, blk));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)blk_smem_bytes<T, kFwdRT>()));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_blk<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, blk));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_contrib<T>),
@@ -2084,8 +2090,9 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
             // factor entries of the block column, and the panel rows below the block read and written
             const double fe = prof ? c.fwd_blk_elems[b0 / BK] : 0.0;
             ProfScope ps(K_FWD_BLK, st, nz * 8.0 * nr * fe, nz * (fe + 2.0 * nr * fe / BK) * es);
-            const int tiles = std::max(1, (dp.mmax_all[l] - b0 - 1) / 64 + 1);
-            hipLaunchKernelGGL(k_fwd_blk<T>, dim3(tiles, nfr, rt * ns), dim3(256), blk, st, F.arena, F.fstride,
+            const int tiles = std::max(1, (dp.mmax_all[l] - b0 - 1) / kFwdRT + 1);
+            const size_t fblk = blk_smem_bytes<T, kFwdRT>();
+            hipLaunchKernelGGL(k_fwd_blk<T>, dim3(tiles, nfr, rt * ns), dim3(4 * kFwdRT), fblk, st, F.arena, F.fstride,
                                dp.fronts, dp.lv_all[l], Xp, Wp, n, wstride, nrhs, rt, b0, (int)trans, (int)cj, in);
         }
     }
