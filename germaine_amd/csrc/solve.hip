@@ -1655,19 +1655,21 @@ __global__ void __launch_bounds__(256) k_ir_in(const cx<double> *__restrict__ B,
 // stride ld).  Xa/Xb and D/R must be distinct buffers: other workgroups read a
 // point's x while its own workgroup writes.  Row k of M has the entries of CSC
 // column k (the pattern is symmetric), all within the 3x3 neighbourhood.
-// Tiles of kIrTx consecutive points of one grid row: per 16-column chunk the
-// x of the 3 x (kIrTx + 2) neighbourhood points is staged in LDS once (each
-// point's x is used by up to nine rows), then one thread per (point, column)
-// applies the stencil from LDS.  grid (blocks, slots), grid-stride over tiles.
-constexpr int kIrTx = 16, kIrCw = 16, kIrStage = 3 * (kIrTx + 2);
-constexpr int kIrThreads = kIrTx * kIrCw, kIrWaves = kIrThreads / 64;
+// Tiles of kIrTx x kIrTy points (16 consecutive points of two grid rows): per
+// 16-column chunk the x of the 4 x (kIrTx + 2) neighbourhood points is staged
+// in LDS once (each point's x is used by up to nine rows; two rows per tile
+// stage 2.25 points per output instead of 3.4 with one), then one thread per
+// (point, column) applies the stencil from LDS.  grid (blocks, slots),
+// grid-stride over tiles.
+constexpr int kIrTx = 16, kIrTy = 2, kIrCw = 16, kIrPts = kIrTx * kIrTy, kIrStage = (kIrTy + 2) * (kIrTx + 2);
+constexpr int kIrThreads = kIrPts * kIrCw, kIrWaves = kIrThreads / 64;
 constexpr int kIrCoef0 = (kIrStage + 63) / 64 * 64;   // first thread of the stencil-entry gather
-static_assert(kIrStage <= kIrCoef0 && kIrCoef0 + 9 * kIrTx <= kIrThreads,
+static_assert(kIrStage <= kIrCoef0 && kIrCoef0 + 9 * kIrPts <= kIrThreads,
               "one thread per staged point and per stencil entry of the tile");
 constexpr int kIrPer = (kIrStage * kIrCw + kIrThreads - 1) / kIrThreads;   // staging entries per thread
 
 template <typename T>
-__global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 5))) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
+__global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4))) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
                                                  const int *__restrict__ rowind, const int64_t *__restrict__ tq,
                                                  const int *__restrict__ iperm, int nx, int ny, int nrhs, int ld64,
                                                  int ld, int sys, int64_t nnz, const cx<double> *__restrict__ Bg,
@@ -1676,8 +1678,8 @@ __global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu
                                                  double *__restrict__ partial, Sel sel) {
     extern __shared__ double wsum[];   // [kIrWaves][nrhs]
     __shared__ cx<double> sx[kIrStage][kIrCw];
-    __shared__ cx<double> scoef[kIrTx][9];
-    __shared__ int sidx[kIrTx][9];     // staged row of each stencil entry
+    __shared__ cx<double> scoef[kIrPts][9];
+    __shared__ int sidx[kIrPts][9];    // staged row of each stencil entry
     __shared__ int sg[kIrStage];       // grid row of each staged point (-1: outside the grid)
     __shared__ int se[kIrStage];       // its elimination position
     const int z = blockIdx.y, s = sel.s[z];
@@ -1697,22 +1699,22 @@ __global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu
     // multiple of 8); each XCD takes a contiguous range of tiles in column-major
     // order (j fastest), so the tiles above and below a tile -- which stage the
     // same grid rows -- run on the same XCD and meet that XCD's L2.
-    const int tpr = (nx + kIrTx - 1) / kIrTx, ntiles = tpr * ny;
+    const int tpr = (nx + kIrTx - 1) / kIrTx, nyt = (ny + kIrTy - 1) / kIrTy, ntiles = tpr * nyt;
     const int xcd = blockIdx.x % 8, per = (ntiles + 7) / 8, t1 = min(ntiles, (xcd + 1) * per);
     for (int tt = xcd * per + blockIdx.x / 8; tt < t1; tt += gridDim.x / 8) {
-        const int j = tt % ny, i0 = (tt / ny) * kIrTx, w = min(kIrTx, nx - i0);
+        const int j = (tt % nyt) * kIrTy, i0 = (tt / nyt) * kIrTx, w = min(kIrTx, nx - i0), hgt = min(kIrTy, ny - j);
         __syncthreads();   // the previous tile's staging is no longer read
         if (tid < kIrStage) {
             const int jj = j - 1 + tid / (kIrTx + 2), ii = i0 - 1 + tid % (kIrTx + 2);
             const bool in = jj >= 0 && jj < ny && ii >= 0 && ii < nx;
             sg[tid] = in ? jj * nx + ii : -1;
             se[tid] = in ? iperm[jj * nx + ii] : 0;
-        } else if (tid >= kIrCoef0 && tid < kIrCoef0 + kIrTx * 9) {
-            const int r = (tid - kIrCoef0) / 9, t = (tid - kIrCoef0) % 9;
-            int li = (kIrTx + 2) + r + 1;   // missing entries: zero value on the point itself
+        } else if (tid >= kIrCoef0 && tid < kIrCoef0 + kIrPts * 9) {
+            const int r = (tid - kIrCoef0) / 9, t = (tid - kIrCoef0) % 9, ry = r / kIrTx, rx = r % kIrTx;
+            int li = (ry + 1) * (kIrTx + 2) + rx + 1;   // missing entries: zero value on the point itself
             cx<double> a = {0.0, 0.0};
-            if (r < w) {
-                const int k = j * nx + i0 + r, q0 = colptr[k];
+            if (rx < w && ry < hgt) {
+                const int k = (j + ry) * nx + i0 + rx, q0 = colptr[k];
                 if (t < colptr[k + 1] - q0) {
                     const int q = q0 + t;
                     a = sys == 0 ? vals[tq[q]] : vals[q];
@@ -1725,14 +1727,16 @@ __global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu
             scoef[r][t] = a;
         }
         __syncthreads();
-        const int r = tid >> 4, cc = tid & 15;
-        const int64_t k = (int64_t)j * nx + i0 + r;
-        const int64_t dst = r < w ? (int64_t)se[(kIrTx + 2) + r + 1] * ld : 0;
+        const int r = tid >> 4, cc = tid & 15, ry = r / kIrTx, rx = r % kIrTx;
+        const bool pin = rx < w && ry < hgt;   // the thread's point is on the grid
+        const int own_li = (ry + 1) * (kIrTx + 2) + rx + 1;
+        const int64_t k = (int64_t)(j + ry) * nx + i0 + rx;
+        const int64_t dst = pin ? (int64_t)se[own_li] * ld : 0;
         // the stencil row is read from LDS per chunk (16 lanes share each
         // address), not held in registers: fewer registers, more waves resident
         for (int c0 = 0; c0 < ld; c0 += kIrCw) {
             const int c = c0 + cc;
-            const bool own = r < w && c < nrhs;
+            const bool own = pin && c < nrhs;
             // every load of the chunk is issued before the first is used: b, then the
             // neighbourhood's d and x (a fixed number of staging entries per thread)
             cx<double> acc = own ? Bg[k * ld64 + c] : cx<double>{0.0, 0.0};
@@ -1758,9 +1762,9 @@ __global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu
             if (own) {
 #pragma unroll
                 for (int t = 0; t < 9; t++) acc = cmsub(acc, scoef[r][t], sx[sidx[r][t]][cc]);
-                Xb[k * ld64 + c] = sx[(kIrTx + 2) + r + 1][cc];
+                Xb[k * ld64 + c] = sx[own_li][cc];
                 R[dst + c] = {(T)acc.x, (T)acc.y};
-            } else if (r < w && c < ld) {
+            } else if (pin && c < ld) {
                 R[dst + c] = {T(0), T(0)};
             }
             // the wave's four points of a column: sum, then one LDS update per column
@@ -1974,6 +1978,9 @@ void set_smem_attrs() {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, blk));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_contrib<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, blk + 4 * kContribRowsMax));
+    // the refinement step's per-wave column sums: up to 64 KB at 1024 right-hand sides
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ir_step<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kIrWaves * 8 * 1024));
     const int fl = (int)kFusedLdsMax;
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_front<T, 64>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, fl));
@@ -2365,7 +2372,8 @@ thread_local Workspace ws_b64, ws_x64a, ws_x64b, ws_p2;
 // widest panel of the row-major refinement: its per-wave column sums and the
 // staged neighbourhood share the default 64 KB of LDS
 constexpr int kIrMaxRhs = 1024;
-static_assert(kIrWaves * 8 * kIrMaxRhs + (kIrStage * kIrCw + 9 * kIrTx) * 16 + (9 * kIrTx + 2 * kIrStage) * 4 <= 65536,
+constexpr int kIrDynLdsMax = kIrWaves * 8 * kIrMaxRhs;   // per-wave column sums at the widest panel
+static_assert(kIrDynLdsMax + (kIrStage * kIrCw + 9 * kIrPts) * 16 + (9 * kIrPts + 2 * kIrStage) * 4 <= 160 * 1024,
               "refinement-step LDS");
 
 // Row-major panel refinement (grid plans, kVecMaxRhs < nrhs <= kIrMaxRhs): b
@@ -2394,7 +2402,7 @@ std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<in
     double *nrm = (double *)ws_n.get(sizeof(double) * 2 * nrhs * ns);
     const int ktiles = (n + PT_K - 1) / PT_K;
     // up to 8192 workgroups per slot (measured faster than 1024-4096), a multiple of 8
-    const int nblk = std::min(((pl.nx + kIrTx - 1) / kIrTx) * pl.ny + 7, 8192) / 8 * 8;
+    const int nblk = std::min(((pl.nx + kIrTx - 1) / kIrTx) * ((pl.ny + kIrTy - 1) / kIrTy) + 7, 8192) / 8 * 8;
     double *partial = (double *)ws_p.get(sizeof(double) * (size_t)std::max(ktiles, nblk) * nrhs * ns);
     std::vector<int> slots(ns);
     for (int z = 0; z < ns; z++) slots[z] = z;

@@ -337,6 +337,27 @@ def test_solve_paths_match_oracle(precision, path, sysno):
             assert e1 <= 1e-3, (z, e1)
 
 
+def test_refinement_wide_panel():
+    """The fused elimination-order refinement at 1000 right-hand sides (near its
+    1024 limit, where the per-wave column sums need more than the default 64 KB
+    of LDS) on an odd grid height (the two-row tiles end half empty): within
+    1e-8 of the oracle."""
+    nx, ny, npml, dh = 37, 23, 5, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=9)
+    freqs = [6.0]
+    say("wide refinement panel")
+    pl = fdfd.Plan(nx, ny)
+    F = pl.factor(pl.assemble(G.AC, dh, freqs, dev(m1), dev(m2), npml=npml), G.FP32)
+    rng = np.random.default_rng(23)
+    B = rng.standard_normal((1, 1000, nx * ny)) + 1j * rng.standard_normal((1, 1000, nx * ny))
+    X, its, rr = F.solve_refine_batch([0], dev(B), tol=1e-12, max_iter=30)
+    say("  iterations", its, rr)
+    assert max(rr) <= 1e-12
+    _, csc = oracle_system(G.AC, nx, ny, npml, dh, freqs[0], m1, m2)
+    Xr, _ = O.band_solve(*csc, B[0].T, perm=O.grid_transpose_perm(nx, ny))
+    assert rel(X.cpu().numpy()[0], Xr.T) <= 1e-8
+
+
 def test_refactor_equals_fresh_factorisation():
     nx, ny, npml, dh = 61, 41, 6, 10.0
     m1, m2 = smooth_model(nx, ny, seed=9)
