@@ -483,6 +483,16 @@ int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, con
     GUARD_END
 }
 
+int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u, const float *d_vp, const float *d_rho,
+                       double *d_hess, void *stream) {
+    if (n < 1 || nshots < 1 || !omega2 || !d_u || !d_vp || !d_rho || !d_hess)
+        return report(GFDFD_ERR_ARG, "gfdfd_hess_shin_ac: bad argument");
+    GUARD_BEGIN
+    hess_shin_ac(n, nshots, omega2, reinterpret_cast<const cx<double> *>(d_u), d_vp, d_rho, d_hess, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
 int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u, const double *d_lam,
                   double *d_grad_sigma, double *d_grad_eps, void *stream) {
     if (n < 1 || nshots < 1 || !omega || !omega2 || !d_u || !d_lam || !d_grad_sigma || !d_grad_eps)

@@ -81,6 +81,24 @@ __global__ void k_grad_ac(int n, int nshots, double o2r, double o2i, const cx<do
     }
 }
 
+// src/AC/hessian_shin_AC.c:117-124 summed over shots in FP64: Shin's
+// pseudo-Hessian diagonal, hess += |J|^2 with J = 2 Omega^2 u / (rho vp^3),
+// i.e. 4 |Omega^2|^2 |u|^2 / (rho vp^3)^2; cell q uses the solution at q+1
+// (vec2mat's shift) and the last cell is left unchanged, as in k_grad_ac.
+__global__ void k_hess_shin_ac(int n, int nshots, double o2abs2, const cx<double> *__restrict__ U,
+                               const float *__restrict__ vp, const float *__restrict__ rho, double *__restrict__ hess) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        if (q + 1 >= n) continue;
+        double acc = 0.0;
+        for (int s = 0; s < nshots; s++) {
+            const cx<double> u = U[q + 1 + (int64_t)s * n];
+            acc += u.x * u.x + u.y * u.y;
+        }
+        const double v = vp[q], d = (double)rho[q] * v * v * v;
+        hess[q] += 4.0 * o2abs2 * acc / (d * d);
+    }
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -128,6 +146,14 @@ void grad_ac(int n, int nshots, const float omega2[2], const cx<double> *U, cons
              const float *rho, double *grad, hipStream_t st) {
     hipLaunchKernelGGL(k_grad_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, (double)omega2[0],
                        (double)omega2[1], U, L, vp, rho, grad);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const float *vp, const float *rho,
+                  double *hess, hipStream_t st) {
+    const double o2r = omega2[0], o2i = omega2[1];
+    hipLaunchKernelGGL(k_hess_shin_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, o2r * o2r + o2i * o2i, U, vp,
+                       rho, hess);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -11,6 +11,8 @@ void adjoint_sources(int n, int nshots, int ntr, const int *krec, const cx<float
                      float dh, cx<double> *B, double *l2, hipStream_t st);
 void grad_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const cx<double> *L, const float *vp,
              const float *rho, double *grad, hipStream_t st);
+void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const float *vp, const float *rho,
+                  double *hess, hipStream_t st);
 void grad_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U,
              const cx<double> *L, double *gsig, double *geps, hipStream_t st);
 

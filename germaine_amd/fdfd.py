@@ -286,6 +286,29 @@ def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_
     return dict(data=None if krec is None else data, fields=fields, iters=its, relres=res, factors=F, vals=vals)
 
 
+def hessian_shin(plan, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=0.0, free_surf=0, precision=FP64, tol=1e-10,
+                 max_iter=10):
+    """Shin's pseudo-Hessian diagonal for the acoustic problem, summed over frequencies and
+    shots (src/AC/hessian_shin_AC.c:50-125, batched): per frequency one factorisation and one
+    panel solve for all shots, then sum |2 Omega^2 u / (rho vp^3)|^2 on the device.
+    Returns [n] float64 (the last cell, which the reference fills from an unwritten entry, is 0)."""
+    import torch
+    n = plan.n
+    vals = plan.assemble(AC, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
+    F = plan.factor(vals, precision)
+    hess = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for f, fr in enumerate(freqs):
+        B = spike_panel(n, ksrc, source_amp(AC, fr, s, dh))
+        if precision == FP64 and tol is None:
+            U = F.solve(f, B)
+        else:
+            U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
+        o2 = omega2_grad_ac(fr, s)
+        check(lib().gfdfd_hess_shin_ac(n, len(ksrc), o2.ctypes.data, _ptr(U), _ptr(m1.contiguous()),
+                                       _ptr(m2.contiguous()), _ptr(hess), _stream(None)), "gfdfd_hess_shin_ac")
+    return hess
+
+
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
              precision=FP64, tol=1e-10, max_iter=10):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,

@@ -180,6 +180,13 @@ int gfdfd_adjoint_sources(int n, int nshots, int ntr, const int *d_krec, const f
  * which reads past the solution in the reference, is left unchanged.        */
 int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam, const float *d_vp,
                   const float *d_rho, double *d_grad, void *stream);
+/* hessian_shin_AC (src/AC/hessian_shin_AC.c:117-124): Shin's pseudo-Hessian
+ * diagonal, hess += |J|^2 with J = 2 Omega^2 u / (rho vp^3), summed over the
+ * nshots forward fields of d_u in FP64 and added to d_hess (n doubles).
+ * omega2 from gfdfd_omega2_grad_ac (the same 2 pi f - i S as the gradient);
+ * cell q uses the solution at q+1, the last cell is left unchanged.          */
+int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u, const float *d_vp, const float *d_rho,
+                       double *d_hess, void *stream);
 /* ass_grad_TE (src/TE/ass_grad_TE.c:31-35): sigma and epsilon gradients */
 int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u, const double *d_lam,
                   double *d_grad_sigma, double *d_grad_eps, void *stream);

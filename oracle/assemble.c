@@ -429,6 +429,23 @@ void orc_grad_ac_accumulate(int nx, int ny, float freq, float s,
     }
 }
 
+/* src/AC/hessian_shin_AC.c:50-51,114-124: one shot's contribution to Shin's
+ * pseudo-Hessian diagonal, J = 2 Omega^2 u (1/(rho vp^3)) rounded to complex
+ * float and hess += Re(J conj(J)) in float; u as in orc_grad_ac_accumulate.  */
+void orc_hess_shin_ac_accumulate(int nx, int ny, float freq, float s,
+                                 const double *ur, const double *ui, double u_last_r, double u_last_i,
+                                 const float *vp, const float *rho, float *hess)
+{
+    cf omega2 = cpowf(((2.0 * M_PI * freq) - (I * s)), 2.0);
+    size_t n = (size_t)nx * ny;
+    for (size_t q = 0; q < n; q++) {
+        float fr = (q + 1 < n) ? (float)ur[q + 1] : (float)u_last_r;
+        float fi = (q + 1 < n) ? (float)ui[q + 1] : (float)u_last_i;
+        cf J = 2.0 * omega2 * (fr + fi * I) * (1.0 / (rho[q] * vp[q] * vp[q] * vp[q]));
+        hess[q] += crealf(J * conjf(J));
+    }
+}
+
 /* src/TE/ass_grad_TE.c:19-56 (STF_INV=0): sigma and epsilon gradients */
 void orc_grad_te_accumulate(int nx, int ny, float freq, float s,
                             const double *ur, const double *ui, double u_last_r, double u_last_i,

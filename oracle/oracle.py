@@ -63,6 +63,10 @@ def lib():
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
                                              f(np.float32), f(np.float32), f(np.float32)]
+        L.orc_hess_shin_ac_accumulate.restype = None
+        L.orc_hess_shin_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
+                                                  f(np.float64), f(np.float64), C.c_double, C.c_double,
+                                                  f(np.float32), f(np.float32), f(np.float32)]
         L.orc_grad_te_accumulate.restype = None
         L.orc_grad_te_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
@@ -191,6 +195,15 @@ def adjoint_rhs(n, krec, res, dh):
     lib().orc_adjoint_rhs(n, len(krec), np.ascontiguousarray(krec, np.int32),
                           np.ascontiguousarray(res.real), np.ascontiguousarray(res.imag), dh, rr, ri)
     return rr + 1j * ri
+
+
+def hess_shin_ac_accumulate(nx, ny, freq, s, u, u_last, vp, rho, hess):
+    lib().orc_hess_shin_ac_accumulate(nx, ny, freq, s,
+                                      np.ascontiguousarray(u.real), np.ascontiguousarray(u.imag),
+                                      u_last.real, u_last.imag,
+                                      np.ascontiguousarray(vp, np.float32).ravel(),
+                                      np.ascontiguousarray(rho, np.float32).ravel(), hess)
+    return hess
 
 
 def grad_ac_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, vp, rho, grad):

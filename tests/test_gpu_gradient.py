@@ -126,3 +126,26 @@ def test_gradient_matches_oracle(phys, precision):
         ee = relerr(grad[1].cpu().numpy(), g2.astype(np.float64))
         say("  misfit", l2_dev, "gradient rel err sigma", es, "eps", ee)
         assert es <= 1e-5 and ee <= 1e-5
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_shin_hessian_matches_oracle(precision):
+    """hessian_shin (src/AC/hessian_shin_AC.c:50-125) on the device against the
+    oracle's per-shot float restatement with FP64 band-LU fields: relative 1e-5
+    (the reference accumulates in float, the device in double)."""
+    say("shin hessian", precision)
+    m = models(G.AC)
+    dh, freqs, n = m["dh"], m["freqs"], NX * NY
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    pl = fdfd.Plan(NX, NY)
+    h = fdfd.hessian_shin(pl, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, npml=NPML, precision=precision,
+                          tol=None if precision == G.FP64 else 1e-10).cpu().numpy()
+    ref = np.zeros(n, np.float32)
+    for f in freqs:
+        _, U = oracle_fields(G.AC, dh, f, m["m1"], m["m2"], srcs)
+        for s in range(len(srcs)):
+            O.hess_shin_ac_accumulate(NX, NY, f, 0.0, U[:, s], 0j, m["m1"], m["m2"], ref)
+    err = float(np.max(np.abs(h[:-1] - ref[:-1])) / np.max(np.abs(ref)))
+    say("  rel err", err)
+    assert err <= 1e-5
+    assert h[-1] == 0.0

@@ -123,3 +123,34 @@ def test_adjoint_rhs_conjugates_and_scales():
     assert rhs[3] == complex(np.float32(1) / np.float32(4), -np.float32(2) / np.float32(4))
     assert rhs[11] == complex(-np.float32(3) / np.float32(4), -np.float32(0.5) / np.float32(4))
     assert np.count_nonzero(rhs) == 2
+
+
+def test_shin_hessian_oracle_pins():
+    """The pseudo-Hessian restatement against (a) the gradient restatement, which
+    the finite-difference checks above pin: with the adjoint field set to
+    conj(J) = conj(2 Omega^2 u / (rho vp^3)) cell by cell the gradient is -|J|^2,
+    i.e. minus the Hessian contribution; (b) the closed form for a constant
+    field, 4 |Omega^2|^2 |c|^2 / (rho vp^3)^2."""
+    rng = np.random.default_rng(31)
+    nx, ny, f, s = 17, 11, 7.5, 0.3
+    n = nx * ny
+    vp = (1800.0 + 600.0 * rng.random(n)).astype(np.float32)
+    rho = (1000.0 + 500.0 * rng.random(n)).astype(np.float32)
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    u_last = 0.4 - 0.7j
+    om2 = (2.0 * np.pi * f - 1j * s) ** 2
+    d = rho.astype(np.float64) * vp.astype(np.float64) ** 3
+    ushift = np.append(u[1:], u_last)            # cell q uses the field at q + 1
+    lam = np.empty(n, complex)
+    lam[1:] = np.conj(2.0 * om2 * ushift[:-1] / d[:-1])
+    lam[0] = 0.0
+    lam_last = np.conj(2.0 * om2 * ushift[-1] / d[-1])
+    g = O.grad_ac_accumulate(nx, ny, f, s, u, np.complex128(u_last), lam, np.complex128(lam_last), vp, rho,
+                             np.zeros(n, np.float32))
+    h = O.hess_shin_ac_accumulate(nx, ny, f, s, u, np.complex128(u_last), vp, rho, np.zeros(n, np.float32))
+    assert np.all(h >= 0)
+    assert np.max(np.abs(g + h)) <= 1e-5 * np.max(h)
+    c = 0.25 + 1.5j
+    hc = O.hess_shin_ac_accumulate(nx, ny, f, s, np.full(n, c), np.complex128(c), vp, rho, np.zeros(n, np.float32))
+    exact = 4.0 * abs(om2) ** 2 * abs(c) ** 2 / d ** 2
+    assert np.max(np.abs(hc - exact) / exact) <= 1e-5
