@@ -493,6 +493,17 @@ int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u
     GUARD_END
 }
 
+int gfdfd_hess_shin_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u,
+                       double *d_hess_sigma, double *d_hess_eps, void *stream) {
+    if (n < 1 || nshots < 1 || !omega || !omega2 || !d_u || !d_hess_sigma || !d_hess_eps)
+        return report(GFDFD_ERR_ARG, "gfdfd_hess_shin_te: bad argument");
+    GUARD_BEGIN
+    hess_shin_te(n, nshots, omega, omega2, reinterpret_cast<const cx<double> *>(d_u), d_hess_sigma, d_hess_eps,
+                 S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
 int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u, const double *d_lam,
                   double *d_grad_sigma, double *d_grad_eps, void *stream) {
     if (n < 1 || nshots < 1 || !omega || !omega2 || !d_u || !d_lam || !d_grad_sigma || !d_grad_eps)

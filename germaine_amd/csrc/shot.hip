@@ -99,6 +99,24 @@ __global__ void k_hess_shin_ac(int n, int nshots, double o2abs2, const cx<double
     }
 }
 
+// src/TE/hessian_shin_TE.c:124-128 summed over shots in FP64: hess_sigma +=
+// |(i Omega) u|^2 = |Omega|^2 |u|^2, hess_eps += |Omega^2 u|^2 = |Omega^2|^2 |u|^2
+// (the reference's MAT1_NORM^2 / MAT2_NORM^2 scaling of the sums, :147-148, is
+// the caller's); cell q uses the solution at q+1, the last cell is left unchanged.
+__global__ void k_hess_shin_te(int n, int nshots, double om_abs2, double o2_abs2, const cx<double> *__restrict__ U,
+                               double *__restrict__ hsig, double *__restrict__ heps) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        if (q + 1 >= n) continue;
+        double acc = 0.0;
+        for (int s = 0; s < nshots; s++) {
+            const cx<double> u = U[q + 1 + (int64_t)s * n];
+            acc += u.x * u.x + u.y * u.y;
+        }
+        hsig[q] += om_abs2 * acc;
+        heps[q] += o2_abs2 * acc;
+    }
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -154,6 +172,14 @@ void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U,
     const double o2r = omega2[0], o2i = omega2[1];
     hipLaunchKernelGGL(k_hess_shin_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, o2r * o2r + o2i * o2i, U, vp,
                        rho, hess);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void hess_shin_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U, double *hsig,
+                  double *heps, hipStream_t st) {
+    const double omr = omega[0], omi = omega[1], o2r = omega2[0], o2i = omega2[1];
+    hipLaunchKernelGGL(k_hess_shin_te, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, omr * omr + omi * omi,
+                       o2r * o2r + o2i * o2i, U, hsig, heps);
     GFDFD_HIP(hipGetLastError());
 }
 

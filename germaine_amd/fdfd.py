@@ -286,27 +286,37 @@ def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_
     return dict(data=None if krec is None else data, fields=fields, iters=its, relres=res, factors=F, vals=vals)
 
 
-def hessian_shin(plan, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=0.0, free_surf=0, precision=FP64, tol=1e-10,
-                 max_iter=10):
-    """Shin's pseudo-Hessian diagonal for the acoustic problem, summed over frequencies and
-    shots (src/AC/hessian_shin_AC.c:50-125, batched): per frequency one factorisation and one
-    panel solve for all shots, then sum |2 Omega^2 u / (rho vp^3)|^2 on the device.
-    Returns [n] float64 (the last cell, which the reference fills from an unwritten entry, is 0)."""
+def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=0.0, free_surf=0, precision=FP64,
+                 tol=1e-10, max_iter=10):
+    """Shin's pseudo-Hessian diagonal summed over frequencies and shots
+    (src/AC/hessian_shin_AC.c:50-125, src/TE/hessian_shin_TE.c:53-137, batched): per frequency
+    one factorisation and one panel solve for all shots, then the |J|^2 sums on the device.
+    Returns AC: [n] float64; TE: (hess_sigma, hess_eps), without the reference's MAT*_NORM^2
+    scaling (src/TE/hessian_shin_TE.c:147-148).  The last cell, which the reference fills from
+    an unwritten entry, is 0."""
     import torch
+    if physics not in (AC, TE):
+        raise ValueError("SH has no Hessian path in the reference")
     n = plan.n
-    vals = plan.assemble(AC, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
+    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
     F = plan.factor(vals, precision)
-    hess = torch.zeros(n, dtype=torch.float64, device="cuda")
+    h1 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    h2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     for f, fr in enumerate(freqs):
-        B = spike_panel(n, ksrc, source_amp(AC, fr, s, dh))
+        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
         if precision == FP64 and tol is None:
             U = F.solve(f, B)
         else:
             U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
-        o2 = omega2_grad_ac(fr, s)
-        check(lib().gfdfd_hess_shin_ac(n, len(ksrc), o2.ctypes.data, _ptr(U), _ptr(m1.contiguous()),
-                                       _ptr(m2.contiguous()), _ptr(hess), _stream(None)), "gfdfd_hess_shin_ac")
-    return hess
+        if physics == AC:
+            o2 = omega2_grad_ac(fr, s)
+            check(lib().gfdfd_hess_shin_ac(n, len(ksrc), o2.ctypes.data, _ptr(U), _ptr(m1.contiguous()),
+                                           _ptr(m2.contiguous()), _ptr(h1), _stream(None)), "gfdfd_hess_shin_ac")
+        else:
+            o2, om = omega(TE, fr, s)
+            check(lib().gfdfd_hess_shin_te(n, len(ksrc), om.ctypes.data, o2.ctypes.data, _ptr(U), _ptr(h1),
+                                           _ptr(h2), _stream(None)), "gfdfd_hess_shin_te")
+    return h1 if physics == AC else (h1, h2)
 
 
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,

@@ -187,6 +187,14 @@ int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, con
  * cell q uses the solution at q+1, the last cell is left unchanged.          */
 int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u, const float *d_vp, const float *d_rho,
                        double *d_hess, void *stream);
+/* hessian_shin_TE (src/TE/hessian_shin_TE.c:124-128): pseudo-Hessian
+ * diagonals, hess_sigma += |(i Omega) u|^2 and hess_eps += |Omega^2 u|^2 over
+ * the nshots columns of d_u in FP64, added to d_hess_sigma / d_hess_eps (n
+ * doubles each); omega, omega2 from gfdfd_omega(GFDFD_TE, ...).  The
+ * reference's MAT1_NORM^2 / MAT2_NORM^2 scaling of the sums (:147-148) is left
+ * to the caller.  Cell q uses the solution at q+1; the last cell is unchanged. */
+int gfdfd_hess_shin_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u,
+                       double *d_hess_sigma, double *d_hess_eps, void *stream);
 /* ass_grad_TE (src/TE/ass_grad_TE.c:31-35): sigma and epsilon gradients */
 int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u, const double *d_lam,
                   double *d_grad_sigma, double *d_grad_eps, void *stream);

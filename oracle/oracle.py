@@ -67,6 +67,10 @@ def lib():
         L.orc_hess_shin_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                                   f(np.float64), f(np.float64), C.c_double, C.c_double,
                                                   f(np.float32), f(np.float32), f(np.float32)]
+        L.orc_hess_shin_te_accumulate.restype = None
+        L.orc_hess_shin_te_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
+                                                  f(np.float64), f(np.float64), C.c_double, C.c_double,
+                                                  f(np.float32), f(np.float32)]
         L.orc_grad_te_accumulate.restype = None
         L.orc_grad_te_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
@@ -204,6 +208,13 @@ def hess_shin_ac_accumulate(nx, ny, freq, s, u, u_last, vp, rho, hess):
                                       np.ascontiguousarray(vp, np.float32).ravel(),
                                       np.ascontiguousarray(rho, np.float32).ravel(), hess)
     return hess
+
+
+def hess_shin_te_accumulate(nx, ny, freq, s, u, u_last, hess_sigma, hess_eps):
+    lib().orc_hess_shin_te_accumulate(nx, ny, freq, s,
+                                      np.ascontiguousarray(u.real), np.ascontiguousarray(u.imag),
+                                      u_last.real, u_last.imag, hess_sigma, hess_eps)
+    return hess_sigma, hess_eps
 
 
 def grad_ac_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, vp, rho, grad):

@@ -154,3 +154,27 @@ def test_shin_hessian_oracle_pins():
     hc = O.hess_shin_ac_accumulate(nx, ny, f, s, np.full(n, c), np.complex128(c), vp, rho, np.zeros(n, np.float32))
     exact = 4.0 * abs(om2) ** 2 * abs(c) ** 2 / d ** 2
     assert np.max(np.abs(hc - exact) / exact) <= 1e-5
+
+
+def test_shin_hessian_te_oracle_pins():
+    """TE pseudo-Hessians against the (finite-difference pinned) TE gradient
+    restatement: with the adjoint field conj((i Omega) u) the sigma gradient
+    equals the sigma Hessian contribution, with conj(Omega^2 u) the epsilon
+    gradient equals the epsilon one."""
+    rng = np.random.default_rng(37)
+    nx, ny, f, s = 13, 9, 2.0e8, 1.0e7
+    n = nx * ny
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    u_last = np.complex128(-0.3 + 0.8j)
+    om = 2.0 * np.pi * (f + 1j * s)
+    hs, he = O.hess_shin_te_accumulate(nx, ny, f, s, u, u_last, np.zeros(n, np.float32), np.zeros(n, np.float32))
+    ushift = np.append(u[1:], u_last)
+    for fac, h, which in ((1j * om, hs, 0), (om ** 2, he, 1)):
+        lam = np.empty(n, complex)
+        lam[1:] = np.conj(fac * ushift[:-1])
+        lam[0] = 0.0
+        lam_last = np.complex128(np.conj(fac * ushift[-1]))
+        g = O.grad_te_accumulate(nx, ny, f, s, u, u_last, lam, lam_last, np.zeros(n, np.float32),
+                                 np.zeros(n, np.float32))[which]
+        assert np.all(h >= 0)
+        assert np.max(np.abs(g - h)) <= 1e-5 * np.max(h)
