@@ -99,6 +99,32 @@ __global__ void k_hess_shin_ac(int n, int nshots, double o2abs2, const cx<double
     }
 }
 
+// src/AC/hessian_AC.c:129-150 summed over shots and receivers in FP64: the
+// Gauss-Newton diagonal, hess += |2 Omega^2 u g / (rho vp^3)|^2 for every
+// (shot field u, receiver Green's function g) pair, which factorises into
+// 4 |Omega^2|^2 (sum_s |u_s|^2) (sum_r |g_r|^2) / (rho vp^3)^2 -- so the
+// receivers' Green's functions are one panel solve per frequency, shared by
+// all shots, where the reference solves them again for every shot.  Cell q
+// uses the solutions at q+1; the last cell is left unchanged.
+__global__ void k_hess_gn_ac(int n, int nshots, int nrec, double o2abs2, const cx<double> *__restrict__ U,
+                             const cx<double> *__restrict__ Gr, const float *__restrict__ vp,
+                             const float *__restrict__ rho, double *__restrict__ hess) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        if (q + 1 >= n) continue;
+        double au = 0.0, ag = 0.0;
+        for (int s = 0; s < nshots; s++) {
+            const cx<double> u = U[q + 1 + (int64_t)s * n];
+            au += u.x * u.x + u.y * u.y;
+        }
+        for (int r = 0; r < nrec; r++) {
+            const cx<double> g = Gr[q + 1 + (int64_t)r * n];
+            ag += g.x * g.x + g.y * g.y;
+        }
+        const double v = vp[q], d = (double)rho[q] * v * v * v;
+        hess[q] += 4.0 * o2abs2 * au * ag / (d * d);
+    }
+}
+
 // src/TE/hessian_shin_TE.c:124-128 summed over shots in FP64: hess_sigma +=
 // |(i Omega) u|^2 = |Omega|^2 |u|^2, hess_eps += |Omega^2 u|^2 = |Omega^2|^2 |u|^2
 // (the reference's MAT1_NORM^2 / MAT2_NORM^2 scaling of the sums, :147-148, is
@@ -172,6 +198,14 @@ void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U,
     const double o2r = omega2[0], o2i = omega2[1];
     hipLaunchKernelGGL(k_hess_shin_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, o2r * o2r + o2i * o2i, U, vp,
                        rho, hess);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void hess_gn_ac(int n, int nshots, int nrec, const float omega2[2], const cx<double> *U, const cx<double> *Gr,
+                const float *vp, const float *rho, double *hess, hipStream_t st) {
+    const double o2r = omega2[0], o2i = omega2[1];
+    hipLaunchKernelGGL(k_hess_gn_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, nrec, o2r * o2r + o2i * o2i, U,
+                       Gr, vp, rho, hess);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -319,6 +319,34 @@ def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=
     return h1 if physics == AC else (h1, h2)
 
 
+def hessian_gn(plan, dh, freqs, m1, m2, ksrc, krec, npml=0, a0_pml=90.0, s=0.0, free_surf=0, precision=FP64,
+               tol=1e-10, max_iter=10):
+    """Gauss-Newton Hessian diagonal of the acoustic problem, summed over frequencies, shots and
+    receivers (src/AC/hessian_AC.c:50-160, batched): per frequency one factorisation, one panel
+    solve for the shots and one for the receivers' Green's functions (a 1/DH^2 spike at each
+    receiver, src/AC/RHS_source_AC_hess.c:20-33) -- shared by all shots, where the reference
+    solves them again per shot -- then the pair sums on the device.  Returns [n] float64 (the
+    last cell is 0)."""
+    import torch
+    n = plan.n
+    vals = plan.assemble(AC, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
+    F = plan.factor(vals, precision)
+    hess = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for f, fr in enumerate(freqs):
+        B = spike_panel(n, ksrc, source_amp(AC, fr, s, dh))
+        BG = spike_panel(n, krec, complex(np.float32(1.0) / np.float32(np.float32(dh) * np.float32(dh))))
+        if precision == FP64 and tol is None:
+            U, GR = F.solve(f, B), F.solve(f, BG)
+        else:
+            U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
+            GR, _, _ = F.solve_refine(f, BG, tol=tol, max_iter=max_iter)
+        o2 = omega2_grad_ac(fr, s)
+        check(lib().gfdfd_hess_gn_ac(n, len(ksrc), len(krec), o2.ctypes.data, _ptr(U), _ptr(GR),
+                                     _ptr(m1.contiguous()), _ptr(m2.contiguous()), _ptr(hess), _stream(None)),
+              "gfdfd_hess_gn_ac")
+    return hess
+
+
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
              precision=FP64, tol=1e-10, max_iter=10):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,

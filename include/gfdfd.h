@@ -187,6 +187,14 @@ int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, con
  * cell q uses the solution at q+1, the last cell is left unchanged.          */
 int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u, const float *d_vp, const float *d_rho,
                        double *d_hess, void *stream);
+/* hessian_AC (src/AC/hessian_AC.c:129-150): Gauss-Newton Hessian diagonal,
+ * hess += |2 Omega^2 u g / (rho vp^3)|^2 over every pair of the nshots forward
+ * fields d_u and the nrec receiver Green's functions d_g (solutions for a
+ * 1/DH^2 spike at each receiver, RHS_source_AC_hess.c:20-33), in FP64, added to
+ * d_hess (n doubles); omega2 from gfdfd_omega2_grad_ac.  Cell q uses the
+ * solutions at q+1; the last cell is left unchanged.                          */
+int gfdfd_hess_gn_ac(int n, int nshots, int nrec, const float *omega2, const double *d_u, const double *d_g,
+                     const float *d_vp, const float *d_rho, double *d_hess, void *stream);
 /* hessian_shin_TE (src/TE/hessian_shin_TE.c:124-128): pseudo-Hessian
  * diagonals, hess_sigma += |(i Omega) u|^2 and hess_eps += |Omega^2 u|^2 over
  * the nshots columns of d_u in FP64, added to d_hess_sigma / d_hess_eps (n

@@ -446,6 +446,27 @@ void orc_hess_shin_ac_accumulate(int nx, int ny, float freq, float s,
     }
 }
 
+/* src/AC/hessian_AC.c:144-150: one (shot field u, receiver Green's function g)
+ * pair's contribution to the Gauss-Newton Hessian diagonal, J = -2 Omega^2 u
+ * (1/(rho vp^3)) g rounded to complex float, hess += Re(J conj(J)) in float;
+ * u and g as the float fields vec2mat makes (cell q reads the solution at q+1). */
+void orc_hess_gn_ac_accumulate(int nx, int ny, float freq, float s,
+                               const double *ur, const double *ui, double u_last_r, double u_last_i,
+                               const double *gr, const double *gi, double g_last_r, double g_last_i,
+                               const float *vp, const float *rho, float *hess)
+{
+    cf omega2 = cpowf(((2.0 * M_PI * freq) - (I * s)), 2.0);
+    size_t n = (size_t)nx * ny;
+    for (size_t q = 0; q < n; q++) {
+        float fr = (q + 1 < n) ? (float)ur[q + 1] : (float)u_last_r;
+        float fi = (q + 1 < n) ? (float)ui[q + 1] : (float)u_last_i;
+        float pr = (q + 1 < n) ? (float)gr[q + 1] : (float)g_last_r;
+        float pi = (q + 1 < n) ? (float)gi[q + 1] : (float)g_last_i;
+        cf J = -2.0 * omega2 * (fr + fi * I) * (1.0 / (rho[q] * vp[q] * vp[q] * vp[q])) * (pr + pi * I);
+        hess[q] += crealf(J * conjf(J));
+    }
+}
+
 /* src/TE/hessian_shin_TE.c:57-58,124-128: one shot's contribution to the
  * sigma and epsilon pseudo-Hessian diagonals, J in complex float, accumulated
  * in float (without the MAT*_NORM scaling of :147-148).                      */

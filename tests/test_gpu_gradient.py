@@ -170,3 +170,33 @@ def test_shin_hessian_matches_oracle(phys, precision):
         hsig, heps = (x.cpu().numpy() for x in hs)
         # Omega and Omega^2 reach the kernel as floats (gfdfd_omega): agreement to their rounding
         assert np.max(np.abs(heps - om2abs * hsig)) <= 1e-5 * np.max(heps)
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_gauss_newton_hessian_matches_oracle(precision):
+    """hessian_gn (src/AC/hessian_AC.c:50-160: every shot field against every
+    receiver Green's function) on the device against the oracle's per-pair float
+    restatement with FP64 band-LU fields: relative 1e-5."""
+    say("gauss-newton hessian", precision)
+    m = models(G.AC)
+    dh, freqs, n = m["dh"], m["freqs"], NX * NY
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    recs = [O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in (3, 11, 19, 27, 35, 43)]
+    pl = fdfd.Plan(NX, NY)
+    h = fdfd.hessian_gn(pl, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs, npml=NPML, precision=precision,
+                        tol=None if precision == G.FP64 else 1e-10).cpu().numpy()
+    ref = np.zeros(n, np.float32)
+    amp = np.float32(1.0) / np.float32(np.float32(dh) * np.float32(dh))
+    for f in freqs:
+        _, U = oracle_fields(G.AC, dh, f, m["m1"], m["m2"], srcs)
+        rhs = np.zeros((n, len(recs)), complex)
+        for r, k in enumerate(recs):
+            rhs[k, r] = float(amp)
+        _, GR = oracle_fields(G.AC, dh, f, m["m1"], m["m2"], recs, rhs=rhs)
+        for s in range(len(srcs)):
+            for r in range(len(recs)):
+                O.hess_gn_ac_accumulate(NX, NY, f, 0.0, U[:, s], 0j, GR[:, r], 0j, m["m1"], m["m2"], ref)
+    err = float(np.max(np.abs(h[:-1] - ref[:-1])) / np.max(np.abs(ref)))
+    say("  rel err", err)
+    assert np.isfinite(ref).all() and err <= 1e-5
+    assert h[-1] == 0.0

@@ -178,3 +178,22 @@ def test_shin_hessian_te_oracle_pins():
                                  np.zeros(n, np.float32))[which]
         assert np.all(h >= 0)
         assert np.max(np.abs(g - h)) <= 1e-5 * np.max(h)
+
+
+def test_gauss_newton_hessian_oracle_pin():
+    """The Gauss-Newton restatement with a constant unit Green's function is the
+    (pinned) pseudo-Hessian restatement, and it scales with |g|^2."""
+    rng = np.random.default_rng(41)
+    nx, ny, f, s = 15, 10, 5.5, 0.2
+    n = nx * ny
+    vp = (1800.0 + 600.0 * rng.random(n)).astype(np.float32)
+    rho = (1000.0 + 500.0 * rng.random(n)).astype(np.float32)
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    u_last = np.complex128(0.2 + 0.1j)
+    shin = O.hess_shin_ac_accumulate(nx, ny, f, s, u, u_last, vp, rho, np.zeros(n, np.float32))
+    one = np.ones(n, complex)
+    gn1 = O.hess_gn_ac_accumulate(nx, ny, f, s, u, u_last, one, np.complex128(1.0), vp, rho, np.zeros(n, np.float32))
+    assert np.max(np.abs(gn1 - shin)) <= 1e-6 * np.max(shin)
+    g = np.full(n, 0.5 - 1.5j)
+    gng = O.hess_gn_ac_accumulate(nx, ny, f, s, u, u_last, g, np.complex128(g[0]), vp, rho, np.zeros(n, np.float32))
+    assert np.max(np.abs(gng - abs(g[0]) ** 2 * shin)) <= 1e-5 * np.max(gng)
