@@ -504,6 +504,17 @@ int gfdfd_hess_gn_ac(int n, int nshots, int nrec, const float *omega2, const dou
     GUARD_END
 }
 
+int gfdfd_hess_gn_te(int n, int nshots, int nrec, const float *omega, const float *omega2, const double *d_u,
+                     const double *d_g, double *d_hess_sigma, double *d_hess_eps, void *stream) {
+    if (n < 1 || nshots < 1 || nrec < 1 || !omega || !omega2 || !d_u || !d_g || !d_hess_sigma || !d_hess_eps)
+        return report(GFDFD_ERR_ARG, "gfdfd_hess_gn_te: bad argument");
+    GUARD_BEGIN
+    hess_gn_te(n, nshots, nrec, omega, omega2, reinterpret_cast<const cx<double> *>(d_u),
+               reinterpret_cast<const cx<double> *>(d_g), d_hess_sigma, d_hess_eps, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
 int gfdfd_hess_shin_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u,
                        double *d_hess_sigma, double *d_hess_eps, void *stream) {
     if (n < 1 || nshots < 1 || !omega || !omega2 || !d_u || !d_hess_sigma || !d_hess_eps)

@@ -125,6 +125,29 @@ __global__ void k_hess_gn_ac(int n, int nshots, int nrec, double o2abs2, const c
     }
 }
 
+// src/TE/hessian_TE.c:149-157 summed over shots and receivers in FP64: the
+// Gauss-Newton diagonals hess_sigma += |(i Omega) u g|^2, hess_eps +=
+// |Omega^2 u g|^2, factorised like k_hess_gn_ac (without the reference's
+// MAT*_NORM^2 scaling, :187-188).  Cell q uses the solutions at q+1.
+__global__ void k_hess_gn_te(int n, int nshots, int nrec, double om_abs2, double o2_abs2,
+                             const cx<double> *__restrict__ U, const cx<double> *__restrict__ Gr,
+                             double *__restrict__ hsig, double *__restrict__ heps) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        if (q + 1 >= n) continue;
+        double au = 0.0, ag = 0.0;
+        for (int s = 0; s < nshots; s++) {
+            const cx<double> u = U[q + 1 + (int64_t)s * n];
+            au += u.x * u.x + u.y * u.y;
+        }
+        for (int r = 0; r < nrec; r++) {
+            const cx<double> g = Gr[q + 1 + (int64_t)r * n];
+            ag += g.x * g.x + g.y * g.y;
+        }
+        hsig[q] += om_abs2 * au * ag;
+        heps[q] += o2_abs2 * au * ag;
+    }
+}
+
 // src/TE/hessian_shin_TE.c:124-128 summed over shots in FP64: hess_sigma +=
 // |(i Omega) u|^2 = |Omega|^2 |u|^2, hess_eps += |Omega^2 u|^2 = |Omega^2|^2 |u|^2
 // (the reference's MAT1_NORM^2 / MAT2_NORM^2 scaling of the sums, :147-148, is
@@ -206,6 +229,14 @@ void hess_gn_ac(int n, int nshots, int nrec, const float omega2[2], const cx<dou
     const double o2r = omega2[0], o2i = omega2[1];
     hipLaunchKernelGGL(k_hess_gn_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, nrec, o2r * o2r + o2i * o2i, U,
                        Gr, vp, rho, hess);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void hess_gn_te(int n, int nshots, int nrec, const float omega[2], const float omega2[2], const cx<double> *U,
+                const cx<double> *Gr, double *hsig, double *heps, hipStream_t st) {
+    const double omr = omega[0], omi = omega[1], o2r = omega2[0], o2i = omega2[1];
+    hipLaunchKernelGGL(k_hess_gn_te, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, nrec, omr * omr + omi * omi,
+                       o2r * o2r + o2i * o2i, U, Gr, hsig, heps);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -15,6 +15,8 @@ void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U,
                   double *hess, hipStream_t st);
 void hess_gn_ac(int n, int nshots, int nrec, const float omega2[2], const cx<double> *U, const cx<double> *Gr,
                 const float *vp, const float *rho, double *hess, hipStream_t st);
+void hess_gn_te(int n, int nshots, int nrec, const float omega[2], const float omega2[2], const cx<double> *U,
+                const cx<double> *Gr, double *hsig, double *heps, hipStream_t st);
 void hess_shin_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U, double *hsig,
                   double *heps, hipStream_t st);
 void grad_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U,

@@ -319,32 +319,41 @@ def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=
     return h1 if physics == AC else (h1, h2)
 
 
-def hessian_gn(plan, dh, freqs, m1, m2, ksrc, krec, npml=0, a0_pml=90.0, s=0.0, free_surf=0, precision=FP64,
-               tol=1e-10, max_iter=10):
-    """Gauss-Newton Hessian diagonal of the acoustic problem, summed over frequencies, shots and
-    receivers (src/AC/hessian_AC.c:50-160, batched): per frequency one factorisation, one panel
-    solve for the shots and one for the receivers' Green's functions (a 1/DH^2 spike at each
-    receiver, src/AC/RHS_source_AC_hess.c:20-33) -- shared by all shots, where the reference
-    solves them again per shot -- then the pair sums on the device.  Returns [n] float64 (the
-    last cell is 0)."""
+def hessian_gn(plan, physics, dh, freqs, m1, m2, ksrc, krec, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
+               precision=FP64, tol=1e-10, max_iter=10):
+    """Gauss-Newton Hessian diagonal summed over frequencies, shots and receivers
+    (src/AC/hessian_AC.c:50-160, src/TE/hessian_TE.c:53-177, batched): per frequency one
+    factorisation, one panel solve for the shots and one for the receivers' Green's functions (a
+    1/DH^2 spike at each receiver, src/AC/RHS_source_AC_hess.c:20-33,
+    src/TE/RHS_source_TE_hess.c:21-32) -- shared by all shots, where the reference solves them
+    again per shot -- then the pair sums on the device.  Returns AC: [n] float64; TE:
+    (hess_sigma, hess_eps) without the MAT*_NORM^2 scaling.  The last cell is 0."""
     import torch
+    if physics not in (AC, TE):
+        raise ValueError("SH has no Hessian path in the reference")
     n = plan.n
-    vals = plan.assemble(AC, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
+    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
     F = plan.factor(vals, precision)
     hess = torch.zeros(n, dtype=torch.float64, device="cuda")
+    h2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     for f, fr in enumerate(freqs):
-        B = spike_panel(n, ksrc, source_amp(AC, fr, s, dh))
+        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
         BG = spike_panel(n, krec, complex(np.float32(1.0) / np.float32(np.float32(dh) * np.float32(dh))))
         if precision == FP64 and tol is None:
             U, GR = F.solve(f, B), F.solve(f, BG)
         else:
             U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
             GR, _, _ = F.solve_refine(f, BG, tol=tol, max_iter=max_iter)
-        o2 = omega2_grad_ac(fr, s)
-        check(lib().gfdfd_hess_gn_ac(n, len(ksrc), len(krec), o2.ctypes.data, _ptr(U), _ptr(GR),
-                                     _ptr(m1.contiguous()), _ptr(m2.contiguous()), _ptr(hess), _stream(None)),
-              "gfdfd_hess_gn_ac")
-    return hess
+        if physics == AC:
+            o2 = omega2_grad_ac(fr, s)
+            check(lib().gfdfd_hess_gn_ac(n, len(ksrc), len(krec), o2.ctypes.data, _ptr(U), _ptr(GR),
+                                         _ptr(m1.contiguous()), _ptr(m2.contiguous()), _ptr(hess), _stream(None)),
+                  "gfdfd_hess_gn_ac")
+        else:
+            o2, om = omega(TE, fr, s)
+            check(lib().gfdfd_hess_gn_te(n, len(ksrc), len(krec), om.ctypes.data, o2.ctypes.data, _ptr(U),
+                                         _ptr(GR), _ptr(hess), _ptr(h2), _stream(None)), "gfdfd_hess_gn_te")
+    return hess if physics == AC else (hess, h2)
 
 
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,

@@ -195,6 +195,14 @@ int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u
  * solutions at q+1; the last cell is left unchanged.                          */
 int gfdfd_hess_gn_ac(int n, int nshots, int nrec, const float *omega2, const double *d_u, const double *d_g,
                      const float *d_vp, const float *d_rho, double *d_hess, void *stream);
+/* hessian_TE (src/TE/hessian_TE.c:149-157): Gauss-Newton diagonals,
+ * hess_sigma += |(i Omega) u g|^2 and hess_eps += |Omega^2 u g|^2 over every
+ * pair of the nshots forward fields d_u and the nrec receiver Green's
+ * functions d_g (1/DH^2 spikes, RHS_source_TE_hess.c:21-32), in FP64, added to
+ * d_hess_sigma / d_hess_eps; omega, omega2 from gfdfd_omega(GFDFD_TE, ...);
+ * the MAT*_NORM^2 scaling (:187-188) is the caller's.                        */
+int gfdfd_hess_gn_te(int n, int nshots, int nrec, const float *omega, const float *omega2, const double *d_u,
+                     const double *d_g, double *d_hess_sigma, double *d_hess_eps, void *stream);
 /* hessian_shin_TE (src/TE/hessian_shin_TE.c:124-128): pseudo-Hessian
  * diagonals, hess_sigma += |(i Omega) u|^2 and hess_eps += |Omega^2 u|^2 over
  * the nshots columns of d_u in FP64, added to d_hess_sigma / d_hess_eps (n

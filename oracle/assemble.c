@@ -467,6 +467,29 @@ void orc_hess_gn_ac_accumulate(int nx, int ny, float freq, float s,
     }
 }
 
+/* src/TE/hessian_TE.c:57-58,149-157: one (shot field, receiver Green's
+ * function) pair's contribution to the sigma and epsilon Gauss-Newton
+ * diagonals, J in complex float, accumulated in float.                       */
+void orc_hess_gn_te_accumulate(int nx, int ny, float freq, float s,
+                               const double *ur, const double *ui, double u_last_r, double u_last_i,
+                               const double *gr, const double *gi, double g_last_r, double g_last_i,
+                               float *hess_sigma, float *hess_eps)
+{
+    cf omega = 2.0 * M_PI * (freq + (I * s));
+    cf omega2 = cpowf(2.0 * M_PI * (freq + (I * s)), 2.0);
+    size_t n = (size_t)nx * ny;
+    for (size_t q = 0; q < n; q++) {
+        float fr = (q + 1 < n) ? (float)ur[q + 1] : (float)u_last_r;
+        float fi = (q + 1 < n) ? (float)ui[q + 1] : (float)u_last_i;
+        float pr = (q + 1 < n) ? (float)gr[q + 1] : (float)g_last_r;
+        float pi = (q + 1 < n) ? (float)gi[q + 1] : (float)g_last_i;
+        cf J = (omega * I) * (fr + fi * I) * (pr + pi * I);
+        hess_sigma[q] += crealf(J * conjf(J));
+        J = omega2 * (fr + fi * I) * (pr + pi * I);
+        hess_eps[q] += crealf(J * conjf(J));
+    }
+}
+
 /* src/TE/hessian_shin_TE.c:57-58,124-128: one shot's contribution to the
  * sigma and epsilon pseudo-Hessian diagonals, J in complex float, accumulated
  * in float (without the MAT*_NORM scaling of :147-148).                      */

@@ -72,6 +72,11 @@ def lib():
                                                 f(np.float64), f(np.float64), C.c_double, C.c_double,
                                                 f(np.float64), f(np.float64), C.c_double, C.c_double,
                                                 f(np.float32), f(np.float32), f(np.float32)]
+        L.orc_hess_gn_te_accumulate.restype = None
+        L.orc_hess_gn_te_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
+                                                f(np.float64), f(np.float64), C.c_double, C.c_double,
+                                                f(np.float64), f(np.float64), C.c_double, C.c_double,
+                                                f(np.float32), f(np.float32)]
         L.orc_hess_shin_te_accumulate.restype = None
         L.orc_hess_shin_te_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                                   f(np.float64), f(np.float64), C.c_double, C.c_double,
@@ -224,6 +229,15 @@ def hess_gn_ac_accumulate(nx, ny, freq, s, u, u_last, g, g_last, vp, rho, hess):
                                     np.ascontiguousarray(vp, np.float32).ravel(),
                                     np.ascontiguousarray(rho, np.float32).ravel(), hess)
     return hess
+
+
+def hess_gn_te_accumulate(nx, ny, freq, s, u, u_last, g, g_last, hess_sigma, hess_eps):
+    lib().orc_hess_gn_te_accumulate(nx, ny, freq, s,
+                                    np.ascontiguousarray(u.real), np.ascontiguousarray(u.imag),
+                                    u_last.real, u_last.imag,
+                                    np.ascontiguousarray(g.real), np.ascontiguousarray(g.imag),
+                                    g_last.real, g_last.imag, hess_sigma, hess_eps)
+    return hess_sigma, hess_eps
 
 
 def hess_shin_te_accumulate(nx, ny, freq, s, u, u_last, hess_sigma, hess_eps):

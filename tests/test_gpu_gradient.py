@@ -173,30 +173,49 @@ def test_shin_hessian_matches_oracle(phys, precision):
 
 
 @pytest.mark.parametrize("precision", [G.FP64, G.FP32])
-def test_gauss_newton_hessian_matches_oracle(precision):
-    """hessian_gn (src/AC/hessian_AC.c:50-160: every shot field against every
-    receiver Green's function) on the device against the oracle's per-pair float
-    restatement with FP64 band-LU fields: relative 1e-5."""
-    say("gauss-newton hessian", precision)
-    m = models(G.AC)
+@pytest.mark.parametrize("phys", [G.AC, G.TE])
+def test_gauss_newton_hessian_matches_oracle(phys, precision):
+    """hessian_gn (src/AC/hessian_AC.c:50-160, src/TE/hessian_TE.c:53-177: every
+    shot field against every receiver Green's function) on the device against
+    the oracle's per-pair float restatement with FP64 band-LU fields: relative
+    1e-5.  TE at one frequency; its epsilon sum exceeds the float range on most
+    cells (as in the reference), so it is compared where the float restatement
+    is finite and checked everywhere as |Omega|^2 times the sigma sum."""
+    say("gauss-newton hessian", phys, precision)
+    m = models(phys)
     dh, freqs, n = m["dh"], m["freqs"], NX * NY
+    if phys == G.TE:
+        freqs = freqs[:1]
     srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
     recs = [O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in (3, 11, 19, 27, 35, 43)]
     pl = fdfd.Plan(NX, NY)
-    h = fdfd.hessian_gn(pl, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs, npml=NPML, precision=precision,
-                        tol=None if precision == G.FP64 else 1e-10).cpu().numpy()
-    ref = np.zeros(n, np.float32)
+    out = fdfd.hessian_gn(pl, phys, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs, npml=NPML,
+                          precision=precision, tol=None if precision == G.FP64 else 1e-10)
+    hs = [out] if phys == G.AC else list(out)
+    refs = [np.zeros(n, np.float32) for _ in hs]
     amp = np.float32(1.0) / np.float32(np.float32(dh) * np.float32(dh))
     for f in freqs:
-        _, U = oracle_fields(G.AC, dh, f, m["m1"], m["m2"], srcs)
+        _, U = oracle_fields(phys, dh, f, m["m1"], m["m2"], srcs)
         rhs = np.zeros((n, len(recs)), complex)
         for r, k in enumerate(recs):
             rhs[k, r] = float(amp)
-        _, GR = oracle_fields(G.AC, dh, f, m["m1"], m["m2"], recs, rhs=rhs)
+        _, GR = oracle_fields(phys, dh, f, m["m1"], m["m2"], recs, rhs=rhs)
         for s in range(len(srcs)):
             for r in range(len(recs)):
-                O.hess_gn_ac_accumulate(NX, NY, f, 0.0, U[:, s], 0j, GR[:, r], 0j, m["m1"], m["m2"], ref)
-    err = float(np.max(np.abs(h[:-1] - ref[:-1])) / np.max(np.abs(ref)))
-    say("  rel err", err)
-    assert np.isfinite(ref).all() and err <= 1e-5
-    assert h[-1] == 0.0
+                if phys == G.AC:
+                    O.hess_gn_ac_accumulate(NX, NY, f, 0.0, U[:, s], 0j, GR[:, r], 0j, m["m1"], m["m2"], refs[0])
+                else:
+                    O.hess_gn_te_accumulate(NX, NY, f, 0.0, U[:, s], 0j, GR[:, r], 0j, refs[0], refs[1])
+    for i, (h, ref) in enumerate(zip(hs, refs)):
+        h = h.cpu().numpy()
+        fin = np.isfinite(ref[:-1])
+        assert fin.all() if i == 0 else True
+        if fin.any():
+            err = float(np.max(np.abs(h[:-1][fin] - ref[:-1][fin])) / np.max(np.abs(ref[:-1][fin])))
+            say("  rel err", err, "on", int(fin.sum()), "cells")
+            assert err <= 1e-5
+        assert h[-1] == 0.0
+    if phys == G.TE:
+        om2abs = abs(2.0 * np.pi * (freqs[0] + 0j)) ** 2
+        hsig, heps = (x.cpu().numpy() for x in hs)
+        assert np.max(np.abs(heps - om2abs * hsig)) <= 1e-5 * np.max(heps)

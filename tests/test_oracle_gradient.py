@@ -197,3 +197,18 @@ def test_gauss_newton_hessian_oracle_pin():
     g = np.full(n, 0.5 - 1.5j)
     gng = O.hess_gn_ac_accumulate(nx, ny, f, s, u, u_last, g, np.complex128(g[0]), vp, rho, np.zeros(n, np.float32))
     assert np.max(np.abs(gng - abs(g[0]) ** 2 * shin)) <= 1e-5 * np.max(gng)
+
+
+def test_gauss_newton_hessian_te_oracle_pin():
+    """The TE Gauss-Newton restatement with a unit Green's function is the
+    (pinned) TE pseudo-Hessian restatement."""
+    rng = np.random.default_rng(43)
+    nx, ny, f, s = 12, 9, 1.5e8, 0.0
+    n = nx * ny
+    u = 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    u_last = np.complex128(1e-3 * (0.5 - 0.5j))
+    z = lambda: np.zeros(n, np.float32)
+    ss, se = O.hess_shin_te_accumulate(nx, ny, f, s, u, u_last, z(), z())
+    gs, ge = O.hess_gn_te_accumulate(nx, ny, f, s, u, u_last, np.ones(n, complex), np.complex128(1.0), z(), z())
+    assert np.max(np.abs(gs - ss)) <= 1e-6 * np.max(ss)
+    assert np.max(np.abs(ge - se)) <= 1e-6 * np.max(se)
