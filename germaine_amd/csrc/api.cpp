@@ -493,6 +493,14 @@ int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u
     GUARD_END
 }
 
+int gfdfd_apply_hess(int n, float eps_hess, const double *d_hess, double *d_grad, void *stream) {
+    if (n < 1 || !d_hess || !d_grad) return report(GFDFD_ERR_ARG, "gfdfd_apply_hess: bad argument");
+    GUARD_BEGIN
+    apply_hess(n, eps_hess, d_hess, d_grad, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
 int gfdfd_hess_gn_ac(int n, int nshots, int nrec, const float *omega2, const double *d_u, const double *d_g,
                      const float *d_vp, const float *d_rho, double *d_hess, void *stream) {
     if (n < 1 || nshots < 1 || nrec < 1 || !omega2 || !d_u || !d_g || !d_vp || !d_rho || !d_hess)

@@ -166,6 +166,25 @@ __global__ void k_hess_shin_te(int n, int nshots, double om_abs2, double o2_abs2
     }
 }
 
+// src/AC/apply_hess_AC.c:20-37 (also the TE path's, src/TE/fwi_FD_TE.c:287-288):
+// grad *= 1 / (hess + EPS_HESS * max|hess|), in FP64.  One workgroup does the
+// whole array -- the maximum first, then the scaling -- so the call needs no
+// scratch memory; it runs once per FWI iteration, not on the hot path.
+__global__ void __launch_bounds__(1024) k_apply_hess(int n, double eps, const double *__restrict__ hess,
+                                                     double *__restrict__ grad) {
+    __shared__ double part[1024];
+    double mx = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) mx = fmax(mx, fabs(hess[q]));
+    part[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + o]);
+        __syncthreads();
+    }
+    const double shift = eps * part[0];
+    for (int q = threadIdx.x; q < n; q += blockDim.x) grad[q] *= 1.0 / (hess[q] + shift);
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -221,6 +240,11 @@ void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U,
     const double o2r = omega2[0], o2i = omega2[1];
     hipLaunchKernelGGL(k_hess_shin_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, o2r * o2r + o2i * o2i, U, vp,
                        rho, hess);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void apply_hess(int n, float eps, const double *hess, double *grad, hipStream_t st) {
+    hipLaunchKernelGGL(k_apply_hess, dim3(1), dim3(1024), 0, st, n, (double)eps, hess, grad);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -356,6 +356,14 @@ def hessian_gn(plan, physics, dh, freqs, m1, m2, ksrc, krec, npml=0, a0_pml=90.0
     return hess if physics == AC else (hess, h2)
 
 
+def apply_hess(grad, hess, eps_hess, stream=None):
+    """Diagonal-Hessian preconditioning of a gradient in place (src/AC/apply_hess_AC.c:20-37):
+    grad *= 1 / (hess + eps_hess * max|hess|); both float64 cuda tensors of n cells."""
+    check(lib().gfdfd_apply_hess(grad.numel(), float(eps_hess), _ptr(hess.contiguous()), _ptr(grad), _stream(stream)),
+          "gfdfd_apply_hess")
+    return grad
+
+
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
              precision=FP64, tol=1e-10, max_iter=10):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,

@@ -187,6 +187,10 @@ int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, con
  * cell q uses the solution at q+1, the last cell is left unchanged.          */
 int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u, const float *d_vp, const float *d_rho,
                        double *d_hess, void *stream);
+/* apply_hess_AC (src/AC/apply_hess_AC.c:20-37; the TE path applies it to each
+ * material, src/TE/fwi_FD_TE.c:287-288): d_grad *= 1 / (d_hess + eps_hess *
+ * max|d_hess|) over n cells, in FP64, without scratch memory.               */
+int gfdfd_apply_hess(int n, float eps_hess, const double *d_hess, double *d_grad, void *stream);
 /* hessian_AC (src/AC/hessian_AC.c:129-150): Gauss-Newton Hessian diagonal,
  * hess += |2 Omega^2 u g / (rho vp^3)|^2 over every pair of the nshots forward
  * fields d_u and the nrec receiver Green's functions d_g (solutions for a

@@ -446,6 +446,16 @@ void orc_hess_shin_ac_accumulate(int nx, int ny, float freq, float s,
     }
 }
 
+/* src/AC/apply_hess_AC.c:20-37: grad *= 1 / (hess + EPS_HESS * max|hess|) in
+ * float over the n cells.                                                     */
+void orc_apply_hess(int n, float eps, const float *hess, float *grad)
+{
+    float mx = 0.0f;
+    for (int q = 0; q < n; q++)
+        if (fabs(hess[q]) > mx) mx = fabs(hess[q]);
+    for (int q = 0; q < n; q++) grad[q] *= 1.0 / (hess[q] + eps * mx);
+}
+
 /* src/AC/hessian_AC.c:144-150: one (shot field u, receiver Green's function g)
  * pair's contribution to the Gauss-Newton Hessian diagonal, J = -2 Omega^2 u
  * (1/(rho vp^3)) g rounded to complex float, hess += Re(J conj(J)) in float;

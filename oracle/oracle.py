@@ -67,6 +67,8 @@ def lib():
         L.orc_hess_shin_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                                   f(np.float64), f(np.float64), C.c_double, C.c_double,
                                                   f(np.float32), f(np.float32), f(np.float32)]
+        L.orc_apply_hess.restype = None
+        L.orc_apply_hess.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32)]
         L.orc_hess_gn_ac_accumulate.restype = None
         L.orc_hess_gn_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                                 f(np.float64), f(np.float64), C.c_double, C.c_double,
@@ -218,6 +220,12 @@ def hess_shin_ac_accumulate(nx, ny, freq, s, u, u_last, vp, rho, hess):
                                       np.ascontiguousarray(vp, np.float32).ravel(),
                                       np.ascontiguousarray(rho, np.float32).ravel(), hess)
     return hess
+
+
+def apply_hess(eps, hess, grad):
+    """grad (float32, modified in place) *= 1 / (hess + eps max|hess|)"""
+    lib().orc_apply_hess(len(grad), eps, np.ascontiguousarray(hess, np.float32), grad)
+    return grad
 
 
 def hess_gn_ac_accumulate(nx, ny, freq, s, u, u_last, g, g_last, vp, rho, hess):

@@ -219,3 +219,18 @@ def test_gauss_newton_hessian_matches_oracle(phys, precision):
         om2abs = abs(2.0 * np.pi * (freqs[0] + 0j)) ** 2
         hsig, heps = (x.cpu().numpy() for x in hs)
         assert np.max(np.abs(heps - om2abs * hsig)) <= 1e-5 * np.max(heps)
+
+
+def test_apply_hess_matches_oracle():
+    """Diagonal-Hessian preconditioning (src/AC/apply_hess_AC.c) on the device
+    against the oracle's float restatement: relative 1e-6."""
+    rng = np.random.default_rng(47)
+    n = 200001                                     # more cells than the workgroup's threads
+    hess = (rng.random(n) ** 4 * 1e3).astype(np.float32).astype(np.float64)
+    grad = rng.standard_normal(n).astype(np.float32).astype(np.float64)
+    g = dev(grad.copy())
+    fdfd.apply_hess(g, dev(hess), 0.005)
+    ref = O.apply_hess(0.005, hess.astype(np.float32), grad.astype(np.float32))
+    err = float(np.max(np.abs(g.cpu().numpy() - ref) / np.abs(ref).max()))
+    say("apply hess rel err", err)
+    assert err <= 1e-6

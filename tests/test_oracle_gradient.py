@@ -212,3 +212,12 @@ def test_gauss_newton_hessian_te_oracle_pin():
     gs, ge = O.hess_gn_te_accumulate(nx, ny, f, s, u, u_last, np.ones(n, complex), np.complex128(1.0), z(), z())
     assert np.max(np.abs(gs - ss)) <= 1e-6 * np.max(ss)
     assert np.max(np.abs(ge - se)) <= 1e-6 * np.max(se)
+
+
+def test_apply_hess_known_answer():
+    """apply_hess_AC's arithmetic on values where every step is exact: max|hess|
+    = 8, EPS_HESS * 8 = 4, divisors 5, 6, 8, -4."""
+    hess = np.array([1.0, 2.0, 4.0, -8.0], np.float32)
+    grad = np.array([10.0, 12.0, 16.0, 8.0], np.float32)
+    out = O.apply_hess(0.5, hess, grad)
+    assert out.tolist() == [2.0, 2.0, 2.0, -2.0]
