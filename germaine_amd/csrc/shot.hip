@@ -185,6 +185,25 @@ __global__ void __launch_bounds__(1024) k_apply_hess(int n, double eps, const do
     for (int q = threadIdx.x; q < n; q += blockDim.x) grad[q] *= 1.0 / (hess[q] + shift);
 }
 
+// src/calc_mat_change_wolfe.c:27-46: trial model m_new = m_old + alpha * dir
+// in float, with the reference's hard constraints -- a value below lo, above
+// hi or below zero keeps m_old.  dir is the FP64 search direction rounded to
+// float, as the reference holds it; the product and the sum round separately
+// (no fused multiply-add), so the result is the reference's to the bit.
+// m_new may alias m_old.
+__global__ void k_model_update(int n, float alpha, const double *__restrict__ dir, const float *m_old,
+                               float lo, float hi, float *m_new) {
+#pragma clang fp contract(off)
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const float old = m_old[q];
+        float v = old + alpha * (float)dir[q];
+        if (v < lo) v = old;
+        if (v > hi) v = old;
+        if (v < 0.0f) v = old;
+        m_new[q] = v;
+    }
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -209,6 +228,12 @@ inline int grid_for(int64_t total) {
 }
 
 }  // namespace
+
+void model_update(int n, float alpha, const double *dir, const float *m_old, float lo, float hi, float *m_new,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_model_update, dim3(grid_for(n)), dim3(256), 0, st, n, alpha, dir, m_old, lo, hi, m_new);
+    GFDFD_HIP(hipGetLastError());
+}
 
 void spike_panel(int n, int nshots, const int *k, cx<double> amp, cx<double> *B, hipStream_t st) {
     hipLaunchKernelGGL(k_spikes, dim3(grid_for((int64_t)n * nshots)), dim3(256), 0, st, n, nshots, k, amp, B);

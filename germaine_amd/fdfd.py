@@ -11,6 +11,9 @@ of shape ``[nshots, n]``, dtype complex128.
 * :func:`grad_obj` — ``grad_obj_AC`` (src/AC/grad_obj_AC.c:10-194) /
   ``grad_obj_TE``: forward panel, residuals, adjoint panel on the same
   factors (M for AC, M^T for TE), gradient reduced over shots in FP64.
+* :func:`objective`, :func:`model_update`, :func:`parabolic_line_search`,
+  :func:`line_search` — ``obj_AC``, ``calc_mat_change_wolfe``, ``calc_opt_step``
+  and ``parabolicls_AC``: the step search of every FWI iteration on one plan.
 """
 import ctypes as C
 
@@ -362,6 +365,122 @@ def apply_hess(grad, hess, eps_hess, stream=None):
     check(lib().gfdfd_apply_hess(grad.numel(), float(eps_hess), _ptr(hess.contiguous()), _ptr(grad), _stream(stream)),
           "gfdfd_apply_hess")
     return grad
+
+
+def model_update(alpha, direction, m_old, lo, hi, out=None, stream=None):
+    """Trial model of a line-search step (src/calc_mat_change_wolfe.c:27-46):
+    m_old + alpha * direction in float, where a value below lo (MAT1_LOW), above
+    hi (MAT1_UP) or below zero keeps m_old.  m_old float32 cuda [ny, nx];
+    direction float64 cuda of as many cells.  Returns out (a new tensor if None)."""
+    import torch
+    m_old = m_old.contiguous()
+    if direction.numel() != m_old.numel():
+        raise ValueError("model_update: direction and model differ in size")
+    out = torch.empty_like(m_old) if out is None else out
+    check(lib().gfdfd_model_update(m_old.numel(), float(alpha), _ptr(direction.contiguous()), _ptr(m_old),
+                                   float(lo), float(hi), _ptr(out), _stream(stream)), "gfdfd_model_update")
+    return out
+
+
+def objective(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
+              precision=FP64, tol=1e-10, max_iter=10):
+    """Misfit only (obj_AC, src/AC/obj_AC.c:46-135): forward panel per frequency,
+    receiver samples, residual norms; 0.5 * sum |res|^2 over frequencies and shots.
+    The plan (ordering, symbolic tree, device buffers) is reused, as every
+    line-search trial of an FWI iteration does."""
+    n = plan.n
+    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
+    F = plan.factor(vals, precision)
+    l2 = 0.0
+    for f, fr in enumerate(freqs):
+        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
+        if precision == FP64 and tol is None:
+            U = F.solve(f, B)
+        else:
+            U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
+        _, l2s = adjoint_sources(n, krec, sample(U, krec), pobs[f], dh)
+        l2 += float(l2s.sum())
+    return 0.5 * l2
+
+
+def initial_step(direction, m, eps_scale):
+    """First trial step of the parabolic line search (src/AC/parabolicls_AC.c:35-37):
+    EPS_SCALE * max(vp) / max(Hgrad), signed maxima as maximum_m takes them."""
+    return np.float32(eps_scale) * np.float32(float(m.max())) / np.float32(float(direction.max()))
+
+
+def opt_step(L2t, epst):
+    """Step length at the vertex of the parabola through (epst[i], L2t[i]), i = 0..2
+    (calc_opt_step, src/calc_opt_step.c:56-102), clamped as the reference clamps it:
+    a concave fit with L2t[1] > L2t[2] takes epst[2]; a vertex beyond epst[2] takes
+    epst[2]; a negative one takes epst[0]."""
+    e = np.asarray(epst, np.float64)
+    A = np.stack([np.ones(3), e, e * e], axis=1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x = np.linalg.solve(A, np.asarray(L2t, np.float64))
+        opteps = np.float64(-x[1]) / np.float64(2.0 * x[2])
+    if 2.0 * x[2] < 0.0 and L2t[1] > L2t[2]:
+        opteps = e[2]
+    if opteps > e[2]:
+        opteps = e[2]
+    if opteps < 0.0:
+        opteps = e[0]
+    return float(opteps)
+
+
+def parabolic_line_search(obj, L2, alpha, scalefac=2.0, stepmax=6):
+    """Parabolic line search (parabolicls_AC, src/AC/parabolicls_AC.c:28-153).
+
+    obj(alpha) returns the misfit of the model stepped by alpha; L2 is the misfit
+    at alpha = 0.  The search halves the step (by scalefac) until the misfit
+    falls, then widens it until the misfit rises again, and fits a parabola
+    through the three points.  Returns (step, trials), step 0.0 when no step
+    within stepmax reductions lowers the misfit; trials lists (alpha, misfit)."""
+    f32 = np.float32
+    alpha, scalefac = f32(alpha), f32(scalefac)
+    L2t = [None, L2, 0.0, 0.0]          # 1-based, as the reference indexes them
+    epst = [None, 0.0, 0.0, 0.0]        # epst[1] stays 0: the step of L2t[1]
+    step1 = step2 = countstep = 0
+    itests = iteste = 2
+    trials = []
+    while step2 != 1 or step1 != 1:
+        for itest in range(itests, iteste + 1):
+            L2t[itest] = obj(float(alpha))
+            trials.append((float(alpha), L2t[itest]))
+        if step1 == 0 and L2t[1] <= L2t[2]:
+            alpha = alpha / scalefac
+            countstep += 1
+        if step1 == 1 and L2t[2] < L2t[3]:
+            epst[3] = float(alpha)
+            step2 = 1
+        if step1 == 1 and L2t[2] >= L2t[3]:
+            epst[3] = float(alpha)
+            alpha = alpha + alpha / scalefac
+            countstep += 1
+        if step1 == 0 and L2t[1] > L2t[2]:
+            epst[2] = float(alpha)
+            step1 = 1
+            itests = iteste = 3
+            countstep = 0
+            alpha = alpha + alpha / scalefac
+        if step1 == 0 and countstep > stepmax:
+            return 0.0, trials
+        if step1 == 1 and countstep > stepmax:
+            step2 = 1
+    return opt_step(L2t[1:], epst[1:]), trials
+
+
+def line_search(plan, dh, freqs, vp, rho, direction, ksrc, krec, pobs, L2, eps_scale, lo, hi, scalefac=2.0,
+                stepmax=6, **kw):
+    """One FWI iteration's step search for the acoustic model (src/AC/fwi_FD_AC.c
+    calling parabolicls_AC): each trial steps vp along direction on the device
+    (model_update into one reused buffer) and evaluates objective() on the same
+    plan.  Returns (step, trials); the caller applies model_update(step, ...)."""
+    trial = model_update(0.0, direction, vp, lo, hi)
+    def obj(alpha):
+        model_update(alpha, direction, vp, lo, hi, out=trial)
+        return objective(plan, AC, dh, freqs, trial, rho, ksrc, krec, pobs, **kw)
+    return parabolic_line_search(obj, L2, initial_step(direction, vp, eps_scale), scalefac, stepmax)
 
 
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,

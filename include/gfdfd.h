@@ -191,6 +191,13 @@ int gfdfd_hess_shin_ac(int n, int nshots, const float *omega2, const double *d_u
  * material, src/TE/fwi_FD_TE.c:287-288): d_grad *= 1 / (d_hess + eps_hess *
  * max|d_hess|) over n cells, in FP64, without scratch memory.               */
 int gfdfd_apply_hess(int n, float eps_hess, const double *d_hess, double *d_grad, void *stream);
+/* calc_mat_change_wolfe (src/calc_mat_change_wolfe.c:27-46), the trial
+ * model of each line-search step (src/AC/parabolicls_AC.c:58-64): d_m_new =
+ * d_m_old + alpha * d_dir in float over n cells; a value below lo (MAT1_LOW),
+ * above hi (MAT1_UP) or below zero keeps d_m_old.  d_dir is the FP64 search
+ * direction (rounded to float); d_m_new may equal d_m_old.                   */
+int gfdfd_model_update(int n, float alpha, const double *d_dir, const float *d_m_old, float lo, float hi,
+                       float *d_m_new, void *stream);
 /* hessian_AC (src/AC/hessian_AC.c:129-150): Gauss-Newton Hessian diagonal,
  * hess += |2 Omega^2 u g / (rho vp^3)|^2 over every pair of the nshots forward
  * fields d_u and the nrec receiver Green's functions d_g (solutions for a

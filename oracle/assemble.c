@@ -456,6 +456,18 @@ void orc_apply_hess(int n, float eps, const float *hess, float *grad)
     for (int q = 0; q < n; q++) grad[q] *= 1.0 / (hess[q] + eps * mx);
 }
 
+/* src/calc_mat_change_wolfe.c:27-46: vp = vp_old + eps_scale * Hgrad in float;
+ * a value below lo, above hi or below zero keeps vp_old.                      */
+void orc_model_update(int n, float alpha, const float *dir, const float *m_old, float lo, float hi, float *m_new)
+{
+    for (int q = 0; q < n; q++) {
+        m_new[q] = m_old[q] + alpha * dir[q];
+        if (m_new[q] < lo) m_new[q] = m_old[q];
+        if (m_new[q] > hi) m_new[q] = m_old[q];
+        if (m_new[q] < 0.0) m_new[q] = m_old[q];
+    }
+}
+
 /* src/AC/hessian_AC.c:144-150: one (shot field u, receiver Green's function g)
  * pair's contribution to the Gauss-Newton Hessian diagonal, J = -2 Omega^2 u
  * (1/(rho vp^3)) g rounded to complex float, hess += Re(J conj(J)) in float;

@@ -69,6 +69,9 @@ def lib():
                                                   f(np.float32), f(np.float32), f(np.float32)]
         L.orc_apply_hess.restype = None
         L.orc_apply_hess.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32)]
+        L.orc_model_update.restype = None
+        L.orc_model_update.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32), C.c_float, C.c_float,
+                                       f(np.float32)]
         L.orc_hess_gn_ac_accumulate.restype = None
         L.orc_hess_gn_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                                 f(np.float64), f(np.float64), C.c_double, C.c_double,
@@ -226,6 +229,14 @@ def apply_hess(eps, hess, grad):
     """grad (float32, modified in place) *= 1 / (hess + eps max|hess|)"""
     lib().orc_apply_hess(len(grad), eps, np.ascontiguousarray(hess, np.float32), grad)
     return grad
+
+
+def model_update(alpha, direction, m_old, lo, hi):
+    """Trial model of a line-search step, float32 (src/calc_mat_change_wolfe.c:27-46)."""
+    m_old = np.ascontiguousarray(m_old, np.float32).ravel()
+    out = np.empty_like(m_old)
+    lib().orc_model_update(len(m_old), alpha, np.ascontiguousarray(direction, np.float32).ravel(), m_old, lo, hi, out)
+    return out
 
 
 def hess_gn_ac_accumulate(nx, ny, freq, s, u, u_last, g, g_last, vp, rho, hess):

@@ -117,6 +117,10 @@ def test_gradient_matches_oracle(phys, precision):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     assert abs(l2_dev - l2) <= 1e-5 * abs(l2), (l2_dev, l2)
+    # the line search's misfit-only pass (obj_AC) on the same plan
+    l2_obj = fdfd.objective(pl, phys, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs,
+                            [dev(p) for p in pobs], npml=NPML, precision=precision, tol=tol)
+    assert abs(l2_obj - l2) <= 1e-5 * abs(l2), (l2_obj, l2)
     if phys == G.AC:
         e = relerr(grad.cpu().numpy(), g1.astype(np.float64))
         say("  misfit", l2_dev, "gradient rel err", e)
@@ -234,3 +238,46 @@ def test_apply_hess_matches_oracle():
     err = float(np.max(np.abs(g.cpu().numpy() - ref) / np.abs(ref).max()))
     say("apply hess rel err", err)
     assert err <= 1e-6
+
+
+def test_model_update_matches_oracle():
+    """Trial model of a line-search step (src/calc_mat_change_wolfe.c:27-46) on
+    the device against the oracle: bit-exact, with values crossing both bounds."""
+    rng = np.random.default_rng(53)
+    n = 100003
+    m_old = (1500.0 + 3000.0 * rng.random(n)).astype(np.float32)
+    d = (rng.standard_normal(n) * 400.0).astype(np.float64)
+    out = fdfd.model_update(1.7, dev(d), dev(m_old), 1600.0, 4400.0)
+    ref = O.model_update(1.7, d.astype(np.float32), m_old, 1600.0, 4400.0)
+    kept = int(np.count_nonzero(ref == m_old))
+    say("model update: cells kept by the bounds", kept)
+    assert kept > 1000
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_line_search_lowers_the_misfit():
+    """One acoustic FWI step search on the device (parabolicls_AC with obj_AC and
+    calc_mat_change_wolfe): along the negative gradient the chosen step is
+    positive and lowers the misfit, and every trial's misfit equals objective()
+    at that trial model."""
+    m = models(G.AC)
+    dh, freqs = m["dh"], m["freqs"]
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    pl = fdfd.Plan(NX, NY)
+    pt = fdfd.forward_shots(pl, G.AC, dh, freqs, dev(m["t1"]), dev(m["t2"]), srcs, recs, npml=NPML, tol=None)
+    pobs = pt["data"]
+    vp, rho = dev(m["m1"]), dev(m["m2"])
+    L2, grad = fdfd.grad_obj(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, tol=None)
+    step, trials = fdfd.line_search(pl, dh, freqs, vp, rho, -grad, srcs, recs, pobs, L2, 0.01, 100.0, 9000.0,
+                                    npml=NPML, tol=None)
+    say("line search: L2", L2, "trials", trials, "step", step)
+    assert step > 0.0
+    new = fdfd.model_update(step, -grad, vp, 100.0, 9000.0)
+    L2new = fdfd.objective(pl, G.AC, dh, freqs, new, rho, srcs, recs, pobs, npml=NPML, tol=None)
+    say("  misfit after the step", L2new)
+    assert L2new < L2
+    a, l2a = trials[-1]
+    again = fdfd.objective(pl, G.AC, dh, freqs, fdfd.model_update(a, -grad, vp, 100.0, 9000.0), rho, srcs, recs,
+                           pobs, npml=NPML, tol=None)
+    assert abs(again - l2a) <= 1e-12 * abs(l2a)

@@ -1,0 +1,61 @@
+"""Host logic of the FWI step search (src/AC/parabolicls_AC.c:28-153,
+src/calc_opt_step.c:56-102) and the oracle's trial-model update
+(src/calc_mat_change_wolfe.c:27-46), pinned on the CPU by closed forms: misfit
+curves that are exact parabolas, whose vertex the fit must return, with the
+sequence of trial steps worked out by hand from the reference's control flow,
+and update values that are exact in float."""
+import numpy as np
+
+from germaine_amd import fdfd
+from oracle import oracle as O
+
+
+def test_opt_step_vertex_and_clamps():
+    # exact parabola (a - 0.3)^2 through steps 0, 0.5, 0.75: the vertex
+    assert fdfd.opt_step([0.09, 0.04, 0.2025], [0.0, 0.5, 0.75]) == 0.3
+    # vertex 3 beyond the third step 2: clamped to it
+    f = lambda a: (a - 3.0) ** 2
+    assert fdfd.opt_step([f(0), f(1), f(2)], [0.0, 1.0, 2.0]) == 2.0
+    # concave fit with L2t[1] > L2t[2]: the third step
+    g = lambda a: -(a - 0.2) ** 2
+    assert fdfd.opt_step([g(0), g(1), g(2)], [0.0, 1.0, 2.0]) == 2.0
+    # convex fit whose vertex is negative: the first step
+    h = lambda a: (a + 1.0) ** 2
+    assert fdfd.opt_step([h(0.5), h(1), h(2)], [0.5, 1.0, 2.0]) == 0.5
+
+
+def test_line_search_shrinks_then_brackets():
+    f = lambda a: (a - 0.3) ** 2
+    step, trials = fdfd.parabolic_line_search(f, f(0.0), 1.0, scalefac=2.0, stepmax=6)
+    # 1 raises the misfit -> halve; 0.5 lowers it -> widen by 1/scalefac; 0.75 rises
+    assert [a for a, _ in trials] == [1.0, 0.5, 0.75]
+    assert abs(step - 0.3) < 1e-12
+
+
+def test_line_search_widens_until_the_misfit_rises():
+    f = lambda a: (a - 3.0) ** 2
+    step, trials = fdfd.parabolic_line_search(f, f(0.0), 1.0, scalefac=2.0, stepmax=6)
+    # the second point stays at step 1 (misfit 4); the third widens 1.5, 2.25,
+    # 3.375 while its misfit is below 4, and stops at 5.0625 (misfit 4.26)
+    assert [a for a, _ in trials] == [1.0, 1.5, 2.25, 3.375, 5.0625]
+    assert abs(step - 3.0) < 1e-9
+
+
+def test_line_search_gives_up():
+    # the misfit only grows: stepmax + 1 halvings, then step 0
+    step, trials = fdfd.parabolic_line_search(lambda a: 1.0 + a, 1.0, 1.0, scalefac=2.0, stepmax=2)
+    assert step == 0.0 and [a for a, _ in trials] == [1.0, 0.5, 0.25]
+    # the misfit only falls: the third step widens stepmax + 1 times and is taken
+    step, trials = fdfd.parabolic_line_search(lambda a: -a, 0.0, 1.0, scalefac=2.0, stepmax=2)
+    assert [a for a, _ in trials] == [1.0, 1.5, 2.25, 3.375]
+    assert step == 3.375
+
+
+def test_model_update_oracle_known_answer():
+    m_old = np.full(5, 1000.0, np.float32)
+    d = np.array([1.0, -1.0, 1e6, -1e6, 0.5], np.float32)
+    out = O.model_update(10.0, d, m_old, 500.0, 5000.0)
+    # above hi and below lo keep the old value
+    assert out.tolist() == [1010.0, 990.0, 1000.0, 1000.0, 1005.0]
+    # below zero keeps the old value even when lo allows it
+    assert O.model_update(2.0, np.float32([-1.0]), np.float32([1.0]), -1e9, 1e9).tolist() == [1.0]
