@@ -14,6 +14,8 @@ of shape ``[nshots, n]``, dtype complex128.
 * :func:`objective`, :func:`model_update`, :func:`parabolic_line_search`,
   :func:`line_search` — ``obj_AC``, ``calc_mat_change_wolfe``, ``calc_opt_step``
   and ``parabolicls_AC``: the step search of every FWI iteration on one plan.
+* ``grad_obj(..., rtm=True)`` — ``RTM_FD_AC``: the gradient pass with the
+  observed data as the residual, i.e. the migrated image.
 """
 import ctypes as C
 
@@ -484,10 +486,14 @@ def line_search(plan, dh, freqs, vp, rho, direction, ksrc, krec, pobs, L2, eps_s
 
 
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-             precision=FP64, tol=1e-10, max_iter=10):
+             precision=FP64, tol=1e-10, max_iter=10, rtm=False):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,
     src/TE/grad_obj_TE.c:53-172).  pobs: list per frequency of complex64 cuda [nshots, ntr].
-    Returns (L2 = 0.5 * sum |res|^2, grad (AC: [n] float64; TE: (grad_sigma, grad_eps)))."""
+    Returns (L2 = 0.5 * sum |res|^2, grad (AC: [n] float64; TE: (grad_sigma, grad_eps))).
+
+    rtm=True is the reverse-time-migration pass (RTM_FD_AC, src/AC/RTM_FD_AC.c:190):
+    the residual is the observed data itself (src/AC/calc_res_AC.c:77), with the
+    same mute, so the returned gradient is the image before precond()."""
     import torch
     n = plan.n
     vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
@@ -504,6 +510,8 @@ def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=
         else:
             U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
         pcal = sample(U, krec)
+        if rtm:                              # res = pobs - 0 = pobs exactly
+            pcal = torch.zeros_like(pcal)
         BA, l2s = adjoint_sources(n, krec, pcal, pobs[f], dh)
         l2 += float(l2s.sum())
         if precision == FP64 and tol is None:

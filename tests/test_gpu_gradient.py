@@ -68,10 +68,14 @@ def oracle_fields(phys, dh, f, m1, m2, srcs, rhs=None, sysno=0):
     return csc, X
 
 
+@pytest.mark.parametrize("rtm", [False, True])
 @pytest.mark.parametrize("precision", [G.FP64, G.FP32])
 @pytest.mark.parametrize("phys", [G.AC, G.TE])
-def test_gradient_matches_oracle(phys, precision):
-    say("gradient", phys, precision)
+def test_gradient_matches_oracle(phys, precision, rtm):
+    """rtm: the migration pass (src/AC/RTM_FD_AC.c:190), whose residual is the
+    observed data (src/AC/calc_res_AC.c:77): the oracle's residual with zero
+    predicted data."""
+    say("gradient", phys, precision, "rtm" if rtm else "fwi")
     m = models(phys)
     dh, freqs, n = m["dh"], m["freqs"], NX * NY
     srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
@@ -89,7 +93,7 @@ def test_gradient_matches_oracle(phys, precision):
     pl = fdfd.Plan(NX, NY)
     tol = None if precision == G.FP64 else 1e-10
     l2_dev, grad = fdfd.grad_obj(pl, phys, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs,
-                                 [dev(p) for p in pobs], npml=NPML, precision=precision, tol=tol)
+                                 [dev(p) for p in pobs], npml=NPML, precision=precision, tol=tol, rtm=rtm)
     torch.cuda.synchronize()
 
     g1 = np.zeros(n, np.float32)
@@ -99,7 +103,8 @@ def test_gradient_matches_oracle(phys, precision):
         csc, U = oracle_fields(phys, dh, f, m["m1"], m["m2"], srcs)
         rhs = np.zeros((n, len(srcs)), complex)
         for s in range(len(srcs)):
-            res, l2s = O.residual(U[recs, s].astype(np.complex64), pobs[fi][s])
+            pcal = np.zeros(len(recs), np.complex64) if rtm else U[recs, s].astype(np.complex64)
+            res, l2s = O.residual(pcal, pobs[fi][s])
             assert res[3] == 0 or s != 0
             l2 += l2s
             rhs[:, s] = O.adjoint_rhs(n, recs, res, dh)
@@ -117,10 +122,14 @@ def test_gradient_matches_oracle(phys, precision):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     assert abs(l2_dev - l2) <= 1e-5 * abs(l2), (l2_dev, l2)
-    # the line search's misfit-only pass (obj_AC) on the same plan
-    l2_obj = fdfd.objective(pl, phys, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs,
-                            [dev(p) for p in pobs], npml=NPML, precision=precision, tol=tol)
-    assert abs(l2_obj - l2) <= 1e-5 * abs(l2), (l2_obj, l2)
+    if rtm:
+        # the misfit of the migration pass is the data's energy, muted samples included
+        assert abs(l2 - 0.5 * sum(float(np.sum(np.abs(p.astype(np.complex128)) ** 2)) for p in pobs)) <= 1e-5 * l2
+    else:
+        # the line search's misfit-only pass (obj_AC) on the same plan
+        l2_obj = fdfd.objective(pl, phys, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs,
+                                [dev(p) for p in pobs], npml=NPML, precision=precision, tol=tol)
+        assert abs(l2_obj - l2) <= 1e-5 * abs(l2), (l2_obj, l2)
     if phys == G.AC:
         e = relerr(grad.cpu().numpy(), g1.astype(np.float64))
         say("  misfit", l2_dev, "gradient rel err", e)
