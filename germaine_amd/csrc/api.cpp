@@ -510,6 +510,16 @@ int gfdfd_model_update(int n, float alpha, const double *d_dir, const float *d_m
     GUARD_END
 }
 
+int gfdfd_model_update_norm(int n, float alpha, const double *d_dir, const float *d_m_old, float norm, float lo,
+                            float hi, float *d_m_new, void *stream) {
+    if (n < 1 || !(norm != 0.0f) || !d_dir || !d_m_old || !d_m_new)
+        return report(GFDFD_ERR_ARG, "gfdfd_model_update_norm: bad argument");
+    GUARD_BEGIN
+    model_update_norm(n, alpha, d_dir, d_m_old, norm, lo, hi, d_m_new, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
 int gfdfd_hess_gn_ac(int n, int nshots, int nrec, const float *omega2, const double *d_u, const double *d_g,
                      const float *d_vp, const float *d_rho, double *d_hess, void *stream) {
     if (n < 1 || nshots < 1 || nrec < 1 || !omega2 || !d_u || !d_g || !d_vp || !d_rho || !d_hess)

@@ -204,6 +204,23 @@ __global__ void k_model_update(int n, float alpha, const double *__restrict__ di
     }
 }
 
+// The TE line search's trial model (src/TE/parabolicls_TE.c:30-31,64-69 with
+// src/calc_mat_change_wolfe_multi_para.c:22-50 and src/scale_grad.c:16-22):
+// the material is normalised, r = (1/norm) * m_old, stepped, r + alpha * dir,
+// clamped to [lo/norm, hi/norm], and scaled back, m_new = norm * r -- each
+// operation rounded in float as the reference rounds it.  m_new may alias m_old.
+__global__ void k_model_update_norm(int n, float alpha, const double *__restrict__ dir, const float *m_old,
+                                    float norm, float lo, float hi, float *m_new) {
+#pragma clang fp contract(off)
+    const float inv = (float)(1.0 / (double)norm), lo_r = lo / norm, hi_r = hi / norm;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        float v = inv * m_old[q] + alpha * (float)dir[q];
+        if (v < lo_r) v = lo_r;
+        if (v > hi_r) v = hi_r;
+        m_new[q] = norm * v;
+    }
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -232,6 +249,13 @@ inline int grid_for(int64_t total) {
 void model_update(int n, float alpha, const double *dir, const float *m_old, float lo, float hi, float *m_new,
                   hipStream_t st) {
     hipLaunchKernelGGL(k_model_update, dim3(grid_for(n)), dim3(256), 0, st, n, alpha, dir, m_old, lo, hi, m_new);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void model_update_norm(int n, float alpha, const double *dir, const float *m_old, float norm, float lo, float hi,
+                       float *m_new, hipStream_t st) {
+    hipLaunchKernelGGL(k_model_update_norm, dim3(grid_for(n)), dim3(256), 0, st, n, alpha, dir, m_old, norm, lo, hi,
+                       m_new);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -13,7 +13,8 @@ of shape ``[nshots, n]``, dtype complex128.
   factors (M for AC, M^T for TE), gradient reduced over shots in FP64.
 * :func:`objective`, :func:`model_update`, :func:`parabolic_line_search`,
   :func:`line_search` — ``obj_AC``, ``calc_mat_change_wolfe``, ``calc_opt_step``
-  and ``parabolicls_AC``: the step search of every FWI iteration on one plan.
+  and ``parabolicls_AC``: the step search of every FWI iteration on one plan;
+  :func:`model_update_norm`, :func:`line_search_te` — ``parabolicls_TE``.
 * ``grad_obj(..., rtm=True)`` — ``RTM_FD_AC``: the gradient pass with the
   observed data as the residual, i.e. the migrated image.
 """
@@ -384,6 +385,21 @@ def model_update(alpha, direction, m_old, lo, hi, out=None, stream=None):
     return out
 
 
+def model_update_norm(alpha, direction, m_old, norm, lo, hi, out=None, stream=None):
+    """The TE line search's trial model (src/TE/parabolicls_TE.c:30-31,64-69):
+    the normalised material (1/norm) * m_old stepped by alpha * direction,
+    clamped to [lo/norm, hi/norm] and scaled back by norm, in float."""
+    import torch
+    m_old = m_old.contiguous()
+    if direction.numel() != m_old.numel():
+        raise ValueError("model_update_norm: direction and model differ in size")
+    out = torch.empty_like(m_old) if out is None else out
+    check(lib().gfdfd_model_update_norm(m_old.numel(), float(alpha), _ptr(direction.contiguous()), _ptr(m_old),
+                                        float(norm), float(lo), float(hi), _ptr(out), _stream(stream)),
+          "gfdfd_model_update_norm")
+    return out
+
+
 def objective(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
               precision=FP64, tol=1e-10, max_iter=10):
     """Misfit only (obj_AC, src/AC/obj_AC.c:46-135): forward panel per frequency,
@@ -483,6 +499,31 @@ def line_search(plan, dh, freqs, vp, rho, direction, ksrc, krec, pobs, L2, eps_s
         model_update(alpha, direction, vp, lo, hi, out=trial)
         return objective(plan, AC, dh, freqs, trial, rho, ksrc, krec, pobs, **kw)
     return parabolic_line_search(obj, L2, initial_step(direction, vp, eps_scale), scalefac, stepmax)
+
+
+def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, pobs, L2, eps_scale, norm,
+                   low, up, scalefac=2.0, stepmax=6, **kw):
+    """One FWI iteration's step search for the TE model (parabolicls_TE,
+    src/TE/parabolicls_TE.c:28-160): both materials step by the same alpha on
+    their normalised scales (norm, low, up: (MAT1 for sigma, MAT2 for epsilon)
+    pairs; dir_* are the search directions of the normalised materials), the
+    first trial step is EPS_SCALE * max(sigma/MAT1_NORM) / max(dir_sigma), and
+    each trial's misfit comes from objective() on the same plan.  The
+    Tikhonov term the reference adds to each trial (Tikhonov_cost_TE) is not
+    included: the search is the reference's when that term's weights are zero.
+    Returns (step, trials)."""
+    f32 = np.float32
+    if not (norm[0] > 0 and norm[1] > 0):
+        raise ValueError("line_search_te: the normalisations must be positive")
+    ts = model_update_norm(0.0, dir_sigma, sigma, norm[0], low[0], up[0])
+    te = model_update_norm(0.0, dir_eps, eps, norm[1], low[1], up[1])
+    def obj(alpha):
+        model_update_norm(alpha, dir_sigma, sigma, norm[0], low[0], up[0], out=ts)
+        model_update_norm(alpha, dir_eps, eps, norm[1], low[1], up[1], out=te)
+        return objective(plan, TE, dh, freqs, te, ts, ksrc, krec, pobs, **kw)
+    # max(a * sigma) = a * max(sigma) for a > 0: float multiplication is monotone
+    alpha0 = f32(eps_scale) * (f32(1.0 / norm[0]) * f32(float(sigma.max()))) / f32(float(dir_sigma.max()))
+    return parabolic_line_search(obj, L2, alpha0, scalefac, stepmax)
 
 
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,

@@ -198,6 +198,13 @@ int gfdfd_apply_hess(int n, float eps_hess, const double *d_hess, double *d_grad
  * direction (rounded to float); d_m_new may equal d_m_old.                   */
 int gfdfd_model_update(int n, float alpha, const double *d_dir, const float *d_m_old, float lo, float hi,
                        float *d_m_new, void *stream);
+/* The TE line search's trial model (src/TE/parabolicls_TE.c:30-31,64-69,
+ * calc_mat_change_wolfe_multi_para, src/calc_mat_change_wolfe_multi_para.c:22-50):
+ * r = (1/norm) * d_m_old (MAT1_NORM or MAT2_NORM), r += alpha * d_dir, r
+ * clamped to [lo/norm, hi/norm] (MATx_LOW, MATx_UP), d_m_new = norm * r, in
+ * float over n cells.  norm must be nonzero; d_m_new may equal d_m_old.      */
+int gfdfd_model_update_norm(int n, float alpha, const double *d_dir, const float *d_m_old, float norm, float lo,
+                            float hi, float *d_m_new, void *stream);
 /* hessian_AC (src/AC/hessian_AC.c:129-150): Gauss-Newton Hessian diagonal,
  * hess += |2 Omega^2 u g / (rho vp^3)|^2 over every pair of the nshots forward
  * fields d_u and the nrec receiver Green's functions d_g (solutions for a

@@ -468,6 +468,23 @@ void orc_model_update(int n, float alpha, const float *dir, const float *m_old, 
     }
 }
 
+/* src/TE/parabolicls_TE.c:30-31,64-69: scale_grad(m, 1.0/NORM) (src/scale_grad.c:
+ * 16-22, the factor passed as float), calc_mat_change_wolfe_multi_para's step
+ * and clamp to [LOW/NORM, UP/NORM] (src/calc_mat_change_wolfe_multi_para.c:
+ * 22-50), scale_grad(r, NORM) back -- all in float.                           */
+void orc_model_update_norm(int n, float alpha, const float *dir, const float *m_old, float norm, float lo, float hi,
+                           float *m_new)
+{
+    float a = 1.0 / norm, lo_r = lo / norm, hi_r = hi / norm;
+    for (int q = 0; q < n; q++) {
+        float r = a * m_old[q];
+        r = r + alpha * dir[q];
+        if (r < lo_r) r = lo_r;
+        if (r > hi_r) r = hi_r;
+        m_new[q] = norm * r;
+    }
+}
+
 /* src/AC/hessian_AC.c:144-150: one (shot field u, receiver Green's function g)
  * pair's contribution to the Gauss-Newton Hessian diagonal, J = -2 Omega^2 u
  * (1/(rho vp^3)) g rounded to complex float, hess += Re(J conj(J)) in float;

@@ -69,6 +69,9 @@ def lib():
                                                   f(np.float32), f(np.float32), f(np.float32)]
         L.orc_apply_hess.restype = None
         L.orc_apply_hess.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32)]
+        L.orc_model_update_norm.restype = None
+        L.orc_model_update_norm.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32), C.c_float, C.c_float,
+                                            C.c_float, f(np.float32)]
         L.orc_model_update.restype = None
         L.orc_model_update.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32), C.c_float, C.c_float,
                                        f(np.float32)]
@@ -236,6 +239,16 @@ def model_update(alpha, direction, m_old, lo, hi):
     m_old = np.ascontiguousarray(m_old, np.float32).ravel()
     out = np.empty_like(m_old)
     lib().orc_model_update(len(m_old), alpha, np.ascontiguousarray(direction, np.float32).ravel(), m_old, lo, hi, out)
+    return out
+
+
+def model_update_norm(alpha, direction, m_old, norm, lo, hi):
+    """TE trial model of a line-search step on normalised material, float32
+    (src/TE/parabolicls_TE.c:30-31,64-69)."""
+    m_old = np.ascontiguousarray(m_old, np.float32).ravel()
+    out = np.empty_like(m_old)
+    lib().orc_model_update_norm(len(m_old), alpha, np.ascontiguousarray(direction, np.float32).ravel(), m_old, norm,
+                                lo, hi, out)
     return out
 
 

@@ -290,3 +290,49 @@ def test_line_search_lowers_the_misfit():
     again = fdfd.objective(pl, G.AC, dh, freqs, fdfd.model_update(a, -grad, vp, 100.0, 9000.0), rho, srcs, recs,
                            pobs, npml=NPML, tol=None)
     assert abs(again - l2a) <= 1e-12 * abs(l2a)
+
+
+def test_model_update_norm_matches_oracle():
+    """The TE line search's trial model (src/TE/parabolicls_TE.c:30-31,64-69) on
+    the device against the oracle: bit-exact, with a normalisation that is not
+    a power of two and values clamped at both bounds."""
+    rng = np.random.default_rng(59)
+    n = 100003
+    m_old = (0.002 + 0.002 * rng.random(n)).astype(np.float32)
+    d = rng.standard_normal(n) * 0.4
+    out = fdfd.model_update_norm(0.9, dev(d), dev(m_old), 0.003, 0.0021, 0.0039)
+    ref = O.model_update_norm(0.9, d.astype(np.float32), m_old, 0.003, 0.0021, 0.0039)
+    hit = int(np.count_nonzero((ref == ref.min()) | (ref == ref.max())))
+    say("model update norm: cells at a bound", hit)
+    assert hit > 1000
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_line_search_te_lowers_the_misfit():
+    """One TE step search on the device (parabolicls_TE without the Tikhonov
+    term): along the negative gradients of the normalised materials, each
+    scaled to a maximum of one, the step is positive and lowers the misfit."""
+    m = models(G.TE)
+    dh, freqs = m["dh"], m["freqs"]
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    pl = fdfd.Plan(NX, NY)
+    pt = fdfd.forward_shots(pl, G.TE, dh, freqs, dev(m["t1"]), dev(m["t2"]), srcs, recs, npml=NPML, tol=None)
+    pobs = pt["data"]
+    eps, sig = dev(m["m1"]), dev(m["m2"])
+    L2, (g_sig, g_eps) = fdfd.grad_obj(pl, G.TE, dh, freqs, eps, sig, srcs, recs, pobs, npml=NPML, tol=None)
+    eps0 = 8.854187817e-12
+    norm, low, up = (0.003, 5.0 * eps0), (0.0, eps0), (1.0, 100.0 * eps0)
+    d_sig = -g_sig * norm[0]
+    d_sig = d_sig / d_sig.abs().max()
+    d_eps = -g_eps * norm[1]
+    d_eps = d_eps / d_eps.abs().max()
+    step, trials = fdfd.line_search_te(pl, dh, freqs, eps, sig, d_eps, d_sig, srcs, recs, pobs, L2, 0.01, norm, low,
+                                       up, npml=NPML, tol=None)
+    say("TE line search: L2", L2, "trials", trials, "step", step)
+    assert step > 0.0
+    ns = fdfd.model_update_norm(step, d_sig, sig, norm[0], low[0], up[0])
+    ne = fdfd.model_update_norm(step, d_eps, eps, norm[1], low[1], up[1])
+    L2new = fdfd.objective(pl, G.TE, dh, freqs, ne, ns, srcs, recs, pobs, npml=NPML, tol=None)
+    say("  misfit after the step", L2new)
+    assert L2new < L2

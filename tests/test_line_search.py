@@ -59,3 +59,9 @@ def test_model_update_oracle_known_answer():
     assert out.tolist() == [1010.0, 990.0, 1000.0, 1000.0, 1005.0]
     # below zero keeps the old value even when lo allows it
     assert O.model_update(2.0, np.float32([-1.0]), np.float32([1.0]), -1e9, 1e9).tolist() == [1.0]
+
+
+def test_model_update_norm_oracle_known_answer():
+    # norm 4: r = 0.25 * 8 = 2; steps 2.5, 52 -> hi/norm = 5, -48 -> lo/norm = 1
+    out = O.model_update_norm(0.5, np.float32([1.0, 100.0, -100.0]), np.float32([8.0, 8.0, 8.0]), 4.0, 4.0, 20.0)
+    assert out.tolist() == [10.0, 20.0, 4.0]
