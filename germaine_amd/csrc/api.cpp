@@ -1,5 +1,6 @@
 // gfdfd extension C-ABI (include/gfdfd.h): thin wrappers that validate
 // arguments, catch exceptions, set the status and print a diagnostic.
+#include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <map>
@@ -516,6 +517,37 @@ int gfdfd_model_update_norm(int n, float alpha, const double *d_dir, const float
         return report(GFDFD_ERR_ARG, "gfdfd_model_update_norm: bad argument");
     GUARD_BEGIN
     model_update_norm(n, alpha, d_dir, d_m_old, norm, lo, hi, d_m_new, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+// gauss_filt's window (src/gauss_filt.c:25-63), evaluated with the reference's
+// float and integer arithmetic: the wavelength VREF / FC_low in float, the
+// window width rounded half away from zero and made odd, the half width hfs,
+// sigma = hfs / 2 in integers and s = 2 sigma^2.
+static void gauss_window(float size, float vref, float fc_low, float dh, int *hfs, float *s) {
+    const float lam = vref / fc_low;
+    int w = (int)std::round((double)((size * lam) / dh));
+    if (w == 0) { *hfs = -1; *s = 0.0f; return; }
+    if (!(w % 2)) w += w > 0 ? 1 : -1;
+    *hfs = std::abs(w) / 2;
+    const float sigma = (float)(*hfs / 2);
+    *s = (float)(2.0 * sigma * sigma);
+}
+
+int gfdfd_precond(int nx, int ny, float dh, float vref, float fc_low, float filt_size_x, float filt_size_y,
+                  int spatfilter, int nzero_rows, const double *d_in, double *d_out, void *stream) {
+    if (nx < 1 || ny < 1 || !d_in || !d_out || d_in == d_out || nzero_rows < 0)
+        return report(GFDFD_ERR_ARG, "gfdfd_precond: bad argument");
+    int hfsx = -1, hfsy = -1;
+    float sx = 0.0f, sy = 0.0f;
+    if (spatfilter) {
+        gauss_window(filt_size_x, vref, fc_low, dh, &hfsx, &sx);
+        gauss_window(filt_size_y, vref, fc_low, dh, &hfsy, &sy);
+        if (hfsx < 0 || hfsy < 0) hfsx = hfsy = -1;        // a zero-width window returns unfiltered
+    }
+    GUARD_BEGIN
+    precond(nx, ny, hfsx, hfsy, sx, sy, nzero_rows, d_in, d_out, S(stream));
     return GFDFD_OK;
     GUARD_END
 }

@@ -221,6 +221,58 @@ __global__ void k_model_update_norm(int n, float alpha, const double *__restrict
     }
 }
 
+// precond (src/precond.c:17-26) for the filter and the horizontal taper:
+// gauss_filt (src/gauss_filt.c:50-145) smooths the [ny, nx] field with the
+// normalised Gaussian exp(-(ii^2/sx) - (jj^2/sy)) over a (2 hfsx + 1) x
+// (2 hfsy + 1) window, the field extended by its edge values; taper_grad_hor
+// (src/taper_grad_hor.c:22-31) then zeroes the first nzero rows.  hfsx < 0
+// skips the filter.  Every workgroup builds the weights in LDS: each thread
+// rounds exp to float, one thread sums them in the reference's order (x
+// outer, y inner) and all divide, so the weights are the reference's; each
+// output then sums its window in that order in float without fused
+// multiply-adds.  The field is read as float (the reference holds it so).
+constexpr int kPrecondMaxWeights = 16000;     // 64000 bytes of dynamic LDS
+__global__ void __launch_bounds__(256) k_precond(int nx, int ny, int hfsx, int hfsy, float sx, float sy, int nzero,
+                                                  const double *__restrict__ in, double *__restrict__ out) {
+#pragma clang fp contract(off)
+    extern __shared__ float wts[];             // [2 hfsx + 1][2 hfsy + 1], x outer
+    __shared__ float total;
+    const int wx = 2 * hfsx + 1, wy = 2 * hfsy + 1, nw = hfsx < 0 ? 0 : wx * wy;
+    for (int t = threadIdx.x; t < nw; t += blockDim.x) {
+        const int ii = t / wy - hfsx, jj = t % wy - hfsy;
+        const float a = (float)(ii * ii) / sx, b = (float)(jj * jj) / sy;
+        wts[t] = (float)exp((double)(-a - b));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && nw > 0) {
+        float sum = 0.0f;
+        for (int t = 0; t < nw; t++) sum += wts[t];
+        total = sum;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nw; t += blockDim.x) wts[t] /= total;
+    __syncthreads();
+    const int64_t n = (int64_t)nx * ny;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q / nx), i = (int)(q % nx);
+        float v;
+        if (nw == 0) {
+            v = (float)in[q];
+        } else {
+            v = 0.0f;
+            for (int ii = -hfsx; ii <= hfsx; ii++) {
+                const int ic = min(max(i + ii, 0), nx - 1);
+                const float *w = wts + (ii + hfsx) * wy + hfsy;
+                for (int jj = -hfsy; jj <= hfsy; jj++) {
+                    const int jc = min(max(j + jj, 0), ny - 1);
+                    v += (float)in[(int64_t)jc * nx + ic] * w[jj];
+                }
+            }
+        }
+        out[q] = j < nzero ? 0.0 : (double)v;
+    }
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -256,6 +308,16 @@ void model_update_norm(int n, float alpha, const double *dir, const float *m_old
                        float *m_new, hipStream_t st) {
     hipLaunchKernelGGL(k_model_update_norm, dim3(grid_for(n)), dim3(256), 0, st, n, alpha, dir, m_old, norm, lo, hi,
                        m_new);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void precond(int nx, int ny, int hfsx, int hfsy, float sx, float sy, int nzero, const double *in, double *out,
+             hipStream_t st) {
+    const int nw = hfsx < 0 ? 0 : (2 * hfsx + 1) * (2 * hfsy + 1);
+    if (nw > kPrecondMaxWeights) throw std::invalid_argument("precond: filter window larger than 16000 weights");
+    const size_t lds = sizeof(float) * std::max(nw, 1);
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)nx * ny + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_precond, dim3(blocks), dim3(256), lds, st, nx, ny, hfsx, hfsy, sx, sy, nzero, in, out);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -16,6 +16,8 @@ void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U,
 void apply_hess(int n, float eps, const double *hess, double *grad, hipStream_t st);
 void model_update(int n, float alpha, const double *dir, const float *m_old, float lo, float hi, float *m_new,
                   hipStream_t st);
+void precond(int nx, int ny, int hfsx, int hfsy, float sx, float sy, int nzero, const double *in, double *out,
+             hipStream_t st);
 void model_update_norm(int n, float alpha, const double *dir, const float *m_old, float norm, float lo, float hi,
                        float *m_new, hipStream_t st);
 void hess_gn_ac(int n, int nshots, int nrec, const float omega2[2], const cx<double> *U, const cx<double> *Gr,

@@ -16,7 +16,8 @@ of shape ``[nshots, n]``, dtype complex128.
   and ``parabolicls_AC``: the step search of every FWI iteration on one plan;
   :func:`model_update_norm`, :func:`line_search_te` — ``parabolicls_TE``.
 * ``grad_obj(..., rtm=True)`` — ``RTM_FD_AC``: the gradient pass with the
-  observed data as the residual, i.e. the migrated image.
+  observed data as the residual, i.e. the migrated image; :func:`precond`
+  smooths and tapers it (or an FWI gradient) as ``precond`` does.
 """
 import ctypes as C
 
@@ -524,6 +525,22 @@ def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, 
     # max(a * sigma) = a * max(sigma) for a > 0: float multiplication is monotone
     alpha0 = f32(eps_scale) * (f32(1.0 / norm[0]) * f32(float(sigma.max()))) / f32(float(dir_sigma.max()))
     return parabolic_line_search(obj, L2, alpha0, scalefac, stepmax)
+
+
+def precond(grad, nx, ny, dh, vref, fc_low, filt_size_x, filt_size_y, spatfilter=1, nzero_rows=0, stream=None):
+    """precond (src/precond.c:17-26) without the taper file: Gaussian smoothing
+    over filt_size_x / filt_size_y wavelengths vref / fc_low (gauss_filt), then
+    the first nzero_rows rows zeroed (taper_grad_hor with GRADT2 + FSSHIFT).
+    grad: float64 cuda of nx * ny cells; returns a new tensor."""
+    import torch
+    grad = grad.contiguous()
+    if grad.numel() != nx * ny:
+        raise ValueError("precond: field size differs from nx * ny")
+    out = torch.empty_like(grad)
+    check(lib().gfdfd_precond(nx, ny, float(dh), float(vref), float(fc_low), float(filt_size_x), float(filt_size_y),
+                              int(spatfilter), int(nzero_rows), _ptr(grad), _ptr(out), _stream(stream)),
+          "gfdfd_precond")
+    return out
 
 
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,

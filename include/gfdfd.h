@@ -205,6 +205,17 @@ int gfdfd_model_update(int n, float alpha, const double *d_dir, const float *d_m
  * float over n cells.  norm must be nonzero; d_m_new may equal d_m_old.      */
 int gfdfd_model_update_norm(int n, float alpha, const double *d_dir, const float *d_m_old, float norm, float lo,
                             float hi, float *d_m_new, void *stream);
+/* precond (src/precond.c:17-26) without the taper file: with spatfilter,
+ * gauss_filt (src/gauss_filt.c:25-145) smooths the [ny, nx] field d_in with a
+ * normalised Gaussian whose window spans filt_size_x (FILT_SIZE_GRAD) and
+ * filt_size_y (FILT_SIZE_GRAD1) wavelengths vref / fc_low (VREF, FC_low) on a
+ * grid of spacing dh, the field extended by its edge values; then
+ * taper_grad_hor (src/taper_grad_hor.c:22-31) zeroes the first nzero_rows
+ * rows (GRADT2 + FSSHIFT when SWS_TAPER_GRAD_HOR, else 0).  Arithmetic in
+ * float as the reference's; results in d_out (not d_in).  The window may hold
+ * at most 16000 weights.                                                     */
+int gfdfd_precond(int nx, int ny, float dh, float vref, float fc_low, float filt_size_x, float filt_size_y,
+                  int spatfilter, int nzero_rows, const double *d_in, double *d_out, void *stream);
 /* hessian_AC (src/AC/hessian_AC.c:129-150): Gauss-Newton Hessian diagonal,
  * hess += |2 Omega^2 u g / (rho vp^3)|^2 over every pair of the nshots forward
  * fields d_u and the nrec receiver Green's functions d_g (solutions for a

@@ -485,6 +485,61 @@ void orc_model_update_norm(int n, float alpha, const float *dir, const float *m_
     }
 }
 
+/* src/gauss_filt.c:25-145 then src/taper_grad_hor.c:22-31 on a float [ny, nx]
+ * field: window from FILT_SIZE_GRAD(1) wavelengths VREF/FC_low, edge-extended
+ * copy, Gaussian weights normalised by their float sum, window sums in float;
+ * then the first nzero rows set to zero.                                       */
+static int orc_gauss_half(float size, float vref, float fc_low, float dh, float *s)
+{
+    float lam = vref / fc_low;
+    int w = round((size * lam) / dh);
+    if (w == 0) return -1;
+    if (!(w % 2)) { if (w > 0) w += 1; else w -= 1; }
+    int h = abs(w) / 2;
+    float sigma = h / 2;
+    *s = 2.0 * sigma * sigma;
+    return h;
+}
+
+void orc_precond(int nx, int ny, float dh, float vref, float fc_low, float fsx, float fsy, int spatfilter,
+                 int nzero, const float *in, float *out)
+{
+    float sx = 0.0f, sy = 0.0f;
+    int hx = spatfilter ? orc_gauss_half(fsx, vref, fc_low, dh, &sx) : -1;
+    int hy = spatfilter ? orc_gauss_half(fsy, vref, fc_low, dh, &sy) : -1;
+    memcpy(out, in, sizeof(float) * (size_t)nx * ny);
+    if (hx >= 0 && hy >= 0) {
+        int px = nx + 2 * hx, py = ny + 2 * hy, wy = 2 * hy + 1;
+        float *tmp = malloc(sizeof(float) * (size_t)px * py);
+        float *k = malloc(sizeof(float) * (size_t)(2 * hx + 1) * wy);
+        for (int j = 0; j < py; j++)
+            for (int i = 0; i < px; i++) {
+                int jc = j - hy < 0 ? 0 : (j - hy >= ny ? ny - 1 : j - hy);
+                int ic = i - hx < 0 ? 0 : (i - hx >= nx ? nx - 1 : i - hx);
+                tmp[(size_t)j * px + i] = in[(size_t)jc * nx + ic];
+            }
+        float sum = 0.0f;
+        for (int ii = -hx; ii <= hx; ii++)
+            for (int jj = -hy; jj <= hy; jj++) {
+                k[(ii + hx) * wy + jj + hy] = exp(-((ii * ii) / sx) - ((jj * jj) / sy));
+                sum += k[(ii + hx) * wy + jj + hy];
+            }
+        for (int t = 0; t < (2 * hx + 1) * wy; t++) k[t] /= sum;
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                float conv = 0.0f;
+                for (int ii = -hx; ii <= hx; ii++)
+                    for (int jj = -hy; jj <= hy; jj++)
+                        conv += tmp[(size_t)(j + jj + hy) * px + i + ii + hx] * k[(ii + hx) * wy + jj + hy];
+                out[(size_t)j * nx + i] = conv;
+            }
+        free(tmp);
+        free(k);
+    }
+    for (int j = 0; j < nzero && j < ny; j++)
+        for (int i = 0; i < nx; i++) out[(size_t)j * nx + i] = 0.0f;
+}
+
 /* src/AC/hessian_AC.c:144-150: one (shot field u, receiver Green's function g)
  * pair's contribution to the Gauss-Newton Hessian diagonal, J = -2 Omega^2 u
  * (1/(rho vp^3)) g rounded to complex float, hess += Re(J conj(J)) in float;

@@ -72,6 +72,9 @@ def lib():
         L.orc_model_update_norm.restype = None
         L.orc_model_update_norm.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32), C.c_float, C.c_float,
                                             C.c_float, f(np.float32)]
+        L.orc_precond.restype = None
+        L.orc_precond.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                  C.c_int, f(np.float32), f(np.float32)]
         L.orc_model_update.restype = None
         L.orc_model_update.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32), C.c_float, C.c_float,
                                        f(np.float32)]
@@ -249,6 +252,15 @@ def model_update_norm(alpha, direction, m_old, norm, lo, hi):
     out = np.empty_like(m_old)
     lib().orc_model_update_norm(len(m_old), alpha, np.ascontiguousarray(direction, np.float32).ravel(), m_old, norm,
                                 lo, hi, out)
+    return out
+
+
+def precond(field, dh, vref, fc_low, fsx, fsy, spatfilter=1, nzero=0):
+    """gauss_filt and taper_grad_hor on a float32 [ny, nx] field (src/precond.c:17-26)."""
+    field = np.ascontiguousarray(field, np.float32)
+    ny, nx = field.shape
+    out = np.empty_like(field)
+    lib().orc_precond(nx, ny, dh, vref, fc_low, fsx, fsy, spatfilter, nzero, field.ravel(), out.ravel())
     return out
 
 

@@ -246,7 +246,7 @@ def test_apply_hess_matches_oracle():
     ref = O.apply_hess(0.005, hess.astype(np.float32), grad.astype(np.float32))
     err = float(np.max(np.abs(g.cpu().numpy() - ref) / np.abs(ref).max()))
     say("apply hess rel err", err)
-    assert err <= 1e-6
+    assert np.array_equal(got, ref), err
 
 
 def test_model_update_matches_oracle():
@@ -336,3 +336,21 @@ def test_line_search_te_lowers_the_misfit():
     L2new = fdfd.objective(pl, G.TE, dh, freqs, ne, ns, srcs, recs, pobs, npml=NPML, tol=None)
     say("  misfit after the step", L2new)
     assert L2new < L2
+
+
+@pytest.mark.parametrize("sizes", [(1.0, 0.6, 0), (2.3, 1.7, 6), (0.0, 1.0, 5)])
+def test_precond_matches_oracle(sizes):
+    """precond's Gaussian filter and horizontal taper (src/precond.c:17-26) on the
+    device against the oracle on a random field, including windows that reach
+    past every edge and a zero-width window: bit-exact (the weights round exp
+    to float on both sides; the sums follow the reference's order in float)."""
+    fsx, fsy, nzero = sizes
+    ny, nx = 61, 83
+    f = np.random.default_rng(61).standard_normal((ny, nx)).astype(np.float32)
+    out = fdfd.precond(dev(f.astype(np.float64).ravel()), nx, ny, 10.0, 1500.0, 15.0, fsx, fsy, 1, nzero)
+    ref = O.precond(f, 10.0, 1500.0, 15.0, fsx, fsy, 1, nzero).ravel()
+    got = out.cpu().numpy()
+    exact = float(np.mean(got == ref))
+    err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    say("precond", sizes, "rel err", err, "bit-exact fraction", exact)
+    assert np.array_equal(got, ref), err

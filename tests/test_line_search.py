@@ -65,3 +65,30 @@ def test_model_update_norm_oracle_known_answer():
     # norm 4: r = 0.25 * 8 = 2; steps 2.5, 52 -> hi/norm = 5, -48 -> lo/norm = 1
     out = O.model_update_norm(0.5, np.float32([1.0, 100.0, -100.0]), np.float32([8.0, 8.0, 8.0]), 4.0, 4.0, 20.0)
     assert out.tolist() == [10.0, 20.0, 4.0]
+
+
+def test_precond_oracle_pins():
+    """gauss_filt + taper_grad_hor (src/gauss_filt.c:25-145, src/taper_grad_hor.c:22-31):
+    a unit spike comes out as the normalised Gaussian exp(-(ii^2/sx) - (jj^2/sy)),
+    sx = 2 (hfsx // 2)^2, over an 11 x 7 window (x by y: VREF / FC_low = 100 m,
+    1.0 and 0.6 wavelengths at DH = 10 m, widths rounded and made odd); a constant
+    field stays constant (edge extension, normalisation); the taper zeroes rows."""
+    ny, nx = 40, 50
+    f = np.zeros((ny, nx), np.float32)
+    f[20, 25] = 1.0
+    o = O.precond(f, 10.0, 1500.0, 15.0, 1.0, 0.6)
+    hx, hy = 5, 3
+    ii, jj = np.meshgrid(np.arange(-hx, hx + 1), np.arange(-hy, hy + 1))
+    sx, sy = 2.0 * (hx // 2) ** 2, 2.0 * (hy // 2) ** 2
+    w = np.exp(-(ii * ii) / sx - (jj * jj) / sy)
+    w /= w.sum()
+    win = o[20 - hy:20 + hy + 1, 25 - hx:25 + hx + 1]
+    assert np.count_nonzero(o) == np.count_nonzero(win) == win.size     # nothing outside the window
+    assert np.max(np.abs(win - w[::-1, ::-1])) <= 1e-6 * w.max()
+    c = O.precond(np.full((ny, nx), 3.0, np.float32), 10.0, 1500.0, 15.0, 1.0, 0.6)
+    assert np.max(np.abs(c - 3.0)) <= 1e-5
+    t = O.precond(np.full((ny, nx), 3.0, np.float32), 10.0, 1500.0, 15.0, 1.0, 0.6, nzero=4)
+    assert (t[:4] == 0).all() and (t[4:] == c[4:]).all()
+    # a zero-width window leaves the field unfiltered
+    r = np.random.default_rng(2).random((ny, nx)).astype(np.float32)
+    assert (O.precond(r, 10.0, 1500.0, 15.0, 0.0, 0.6) == r).all()
