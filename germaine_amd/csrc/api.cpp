@@ -552,6 +552,19 @@ int gfdfd_precond(int nx, int ny, float dh, float vref, float fc_low, float filt
     GUARD_END
 }
 
+int gfdfd_tikhonov_te(int nx, int ny, float dh, float beta_sigma, float beta_eps, float norm_sigma, float norm_eps,
+                      float lambda_sigma, float lambda_eps, const float *d_sigma, const float *d_eps,
+                      double *d_grad_sigma, double *d_grad_eps, double *d_cost, void *stream) {
+    if (nx < 3 || ny < 3 || !d_sigma || !d_eps || !d_cost || !(norm_sigma != 0.0f) || !(norm_eps != 0.0f))
+        return report(GFDFD_ERR_ARG, "gfdfd_tikhonov_te: bad argument");
+    GUARD_BEGIN
+    tikhonov_te(nx, ny, dh, beta_sigma / norm_sigma, beta_eps / norm_eps, beta_sigma, beta_eps,
+                lambda_sigma * beta_sigma, lambda_eps * beta_eps, d_sigma, d_eps, d_grad_sigma, d_grad_eps, d_cost,
+                S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
 int gfdfd_hess_gn_ac(int n, int nshots, int nrec, const float *omega2, const double *d_u, const double *d_g,
                      const float *d_vp, const float *d_rho, double *d_hess, void *stream) {
     if (n < 1 || nshots < 1 || nrec < 1 || !omega2 || !d_u || !d_g || !d_vp || !d_rho || !d_hess)

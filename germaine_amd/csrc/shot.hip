@@ -273,6 +273,57 @@ __global__ void __launch_bounds__(256) k_precond(int nx, int ny, int hfsx, int h
     }
 }
 
+// Tikhonov regularisation of the TE materials (src/TE/Tikhonov_cost_TE.c:21-45,
+// src/TE/Tikhonov_grad_TE.c:20-37): on the normalised fields r = a * m
+// (a = BETA / NORM, in float) the five-point Laplacian of every interior cell,
+// (four neighbours summed in float - 4 r) / DH^2 rounded to float as the
+// reference rounds it; cost[0..1] = sum 0.5 BETA lap^2 per material (FP64
+// here, float in the reference), and, when grad is given, grad += (LAMBDA *
+// BETA) * lap.  One workgroup: it runs once per objective evaluation and
+// needs no scratch memory.
+__global__ void __launch_bounds__(1024) k_tikhonov_te(int nx, int ny, float dh, float a_sig, float a_eps,
+                                                      float beta_sig, float beta_eps, float lb_sig, float lb_eps,
+                                                      const float *__restrict__ sig, const float *__restrict__ eps,
+                                                      double *__restrict__ g_sig, double *__restrict__ g_eps,
+                                                      double *__restrict__ cost) {
+#pragma clang fp contract(off)
+    __shared__ double part[2][1024];
+    const double h2 = (double)(dh * dh);
+    double cs = 0.0, ce = 0.0;
+    const int iw = nx - 2, cells = iw * (ny - 2);
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+        const int j = c / iw + 1, i = c % iw + 1;
+        const int64_t q = (int64_t)j * nx + i;
+        const float *m[2] = {sig, eps};
+        const float av[2] = {a_sig, a_eps};
+        float lap[2];
+        for (int k = 0; k < 2; k++) {
+            const float *f = m[k];
+            const float w = av[k] * f[q - 1], e = av[k] * f[q + 1], s = av[k] * f[q - nx], nn = av[k] * f[q + nx];
+            const float sum4 = w + e + s + nn;
+            lap[k] = (float)(((double)sum4 - 4.0 * (double)(av[k] * f[q])) / h2);
+        }
+        cs += 0.5 * (double)beta_sig * (double)lap[0] * (double)lap[0];
+        ce += 0.5 * (double)beta_eps * (double)lap[1] * (double)lap[1];
+        if (g_sig) g_sig[q] += (double)(lb_sig * lap[0]);
+        if (g_eps) g_eps[q] += (double)(lb_eps * lap[1]);
+    }
+    part[0][threadIdx.x] = cs;
+    part[1][threadIdx.x] = ce;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            part[0][threadIdx.x] += part[0][threadIdx.x + o];
+            part[1][threadIdx.x] += part[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cost[0] = part[0][0];
+        cost[1] = part[1][0];
+    }
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -318,6 +369,14 @@ void precond(int nx, int ny, int hfsx, int hfsy, float sx, float sy, int nzero, 
     const size_t lds = sizeof(float) * std::max(nw, 1);
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)nx * ny + 255) / 256, 4096));
     hipLaunchKernelGGL(k_precond, dim3(blocks), dim3(256), lds, st, nx, ny, hfsx, hfsy, sx, sy, nzero, in, out);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void tikhonov_te(int nx, int ny, float dh, float a_sig, float a_eps, float beta_sig, float beta_eps, float lb_sig,
+                 float lb_eps, const float *sig, const float *eps, double *g_sig, double *g_eps, double *cost,
+                 hipStream_t st) {
+    hipLaunchKernelGGL(k_tikhonov_te, dim3(1), dim3(1024), 0, st, nx, ny, dh, a_sig, a_eps, beta_sig, beta_eps, lb_sig,
+                       lb_eps, sig, eps, g_sig, g_eps, cost);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -16,6 +16,9 @@ void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U,
 void apply_hess(int n, float eps, const double *hess, double *grad, hipStream_t st);
 void model_update(int n, float alpha, const double *dir, const float *m_old, float lo, float hi, float *m_new,
                   hipStream_t st);
+void tikhonov_te(int nx, int ny, float dh, float a_sig, float a_eps, float beta_sig, float beta_eps, float lb_sig,
+                 float lb_eps, const float *sig, const float *eps, double *g_sig, double *g_eps, double *cost,
+                 hipStream_t st);
 void precond(int nx, int ny, int hfsx, int hfsy, float sx, float sy, int nzero, const double *in, double *out,
              hipStream_t st);
 void model_update_norm(int n, float alpha, const double *dir, const float *m_old, float norm, float lo, float hi,

@@ -502,17 +502,35 @@ def line_search(plan, dh, freqs, vp, rho, direction, ksrc, krec, pobs, L2, eps_s
     return parabolic_line_search(obj, L2, initial_step(direction, vp, eps_scale), scalefac, stepmax)
 
 
+def tikhonov_te(sigma, eps, dh, beta, norm, lam, grad_sigma=None, grad_eps=None, stream=None):
+    """Tikhonov regularisation of the TE materials (Tikhonov_cost_TE,
+    src/TE/Tikhonov_cost_TE.c:21-45, and Tikhonov_grad_TE, src/TE/Tikhonov_grad_TE.c:20-37):
+    beta, norm, lam are (sigma, epsilon) pairs of BETA_MAT*, MAT*_NORM, LAMBDA_*.
+    Adds LAMBDA * BETA * Laplacian to the gradients that are given and returns
+    the two costs (tikh_sigma, tikh_epsilon); the misfit the reference compares
+    is L2 + lam[0] * tikh_sigma + lam[1] * tikh_epsilon when a lam is positive."""
+    import torch
+    ny, nx = sigma.shape
+    cost = torch.empty(2, dtype=torch.float64, device="cuda")
+    gp = lambda g: None if g is None else _ptr(g)
+    check(lib().gfdfd_tikhonov_te(nx, ny, float(dh), float(beta[0]), float(beta[1]), float(norm[0]), float(norm[1]),
+                                  float(lam[0]), float(lam[1]), _ptr(sigma.contiguous()), _ptr(eps.contiguous()),
+                                  gp(grad_sigma), gp(grad_eps), _ptr(cost), _stream(stream)), "gfdfd_tikhonov_te")
+    c = cost.cpu().numpy()
+    return float(c[0]), float(c[1])
+
+
 def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, pobs, L2, eps_scale, norm,
-                   low, up, scalefac=2.0, stepmax=6, **kw):
+                   low, up, scalefac=2.0, stepmax=6, tikhonov=None, **kw):
     """One FWI iteration's step search for the TE model (parabolicls_TE,
     src/TE/parabolicls_TE.c:28-160): both materials step by the same alpha on
     their normalised scales (norm, low, up: (MAT1 for sigma, MAT2 for epsilon)
     pairs; dir_* are the search directions of the normalised materials), the
     first trial step is EPS_SCALE * max(sigma/MAT1_NORM) / max(dir_sigma), and
-    each trial's misfit comes from objective() on the same plan.  The
-    Tikhonov term the reference adds to each trial (Tikhonov_cost_TE) is not
-    included: the search is the reference's when that term's weights are zero.
-    Returns (step, trials)."""
+    each trial's misfit comes from objective() on the same plan, plus the
+    Tikhonov term (Tikhonov_cost_TE) when tikhonov = (beta, lam) pairs are
+    given and a lam is positive.  L2 is the start model's misfit with that
+    term.  Returns (step, trials)."""
     f32 = np.float32
     if not (norm[0] > 0 and norm[1] > 0):
         raise ValueError("line_search_te: the normalisations must be positive")
@@ -521,7 +539,11 @@ def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, 
     def obj(alpha):
         model_update_norm(alpha, dir_sigma, sigma, norm[0], low[0], up[0], out=ts)
         model_update_norm(alpha, dir_eps, eps, norm[1], low[1], up[1], out=te)
-        return objective(plan, TE, dh, freqs, te, ts, ksrc, krec, pobs, **kw)
+        l2 = objective(plan, TE, dh, freqs, te, ts, ksrc, krec, pobs, **kw)
+        if tikhonov is not None and (tikhonov[1][0] > 0 or tikhonov[1][1] > 0):
+            cs, ce = tikhonov_te(ts, te, dh, tikhonov[0], norm, tikhonov[1])
+            l2 += tikhonov[1][0] * cs + tikhonov[1][1] * ce
+        return l2
     # max(a * sigma) = a * max(sigma) for a > 0: float multiplication is monotone
     alpha0 = f32(eps_scale) * (f32(1.0 / norm[0]) * f32(float(sigma.max()))) / f32(float(dir_sigma.max()))
     return parabolic_line_search(obj, L2, alpha0, scalefac, stepmax)

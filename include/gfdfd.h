@@ -216,6 +216,17 @@ int gfdfd_model_update_norm(int n, float alpha, const double *d_dir, const float
  * at most 16000 weights.                                                     */
 int gfdfd_precond(int nx, int ny, float dh, float vref, float fc_low, float filt_size_x, float filt_size_y,
                   int spatfilter, int nzero_rows, const double *d_in, double *d_out, void *stream);
+/* Tikhonov regularisation of the TE materials (src/TE/Tikhonov_cost_TE.c:21-45,
+ * src/TE/Tikhonov_grad_TE.c:20-37): with r = (BETA / NORM) * m for sigma and
+ * epsilon ([ny, nx] float), the five-point Laplacian lap of each interior cell
+ * over spacing dh gives d_cost[0] = sum 0.5 BETA_MAT1 lap_sigma^2 and d_cost[1]
+ * = sum 0.5 BETA_MAT2 lap_eps^2 (two device doubles, overwritten), and, for
+ * each gradient pointer that is not null, grad += LAMBDA * BETA * lap.  The
+ * caller adds LAMBDA_1 cost[0] + LAMBDA_2 cost[1] to the misfit when either
+ * LAMBDA is positive, as Tikhonov_cost_TE does.  No scratch memory.          */
+int gfdfd_tikhonov_te(int nx, int ny, float dh, float beta_sigma, float beta_eps, float norm_sigma, float norm_eps,
+                      float lambda_sigma, float lambda_eps, const float *d_sigma, const float *d_eps,
+                      double *d_grad_sigma, double *d_grad_eps, double *d_cost, void *stream);
 /* hessian_AC (src/AC/hessian_AC.c:129-150): Gauss-Newton Hessian diagonal,
  * hess += |2 Omega^2 u g / (rho vp^3)|^2 over every pair of the nshots forward
  * fields d_u and the nrec receiver Green's functions d_g (solutions for a

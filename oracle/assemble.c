@@ -540,6 +540,33 @@ void orc_precond(int nx, int ny, float dh, float vref, float fc_low, float fsx, 
         for (int i = 0; i < nx; i++) out[(size_t)j * nx + i] = 0.0f;
 }
 
+/* src/TE/Tikhonov_cost_TE.c:21-45 and src/TE/Tikhonov_grad_TE.c:20-37 on float
+ * [ny, nx] fields, x outer and y inner as the reference loops: cost[0..1] are
+ * the float sums tikh_sigma and tikh_epsilon; grad_* (float, may be NULL)
+ * receive LAMBDA * BETA * lap.                                                 */
+void orc_tikhonov_te(int nx, int ny, float dh, float beta1, float beta2, float norm1, float norm2, float lambda1,
+                     float lambda2, const float *sig, const float *eps, float *gs, float *ge, float *cost)
+{
+    float a1 = beta1 / norm1, a2 = beta2 / norm2, ts = 0.0f, te = 0.0f;
+    size_t n = (size_t)nx * ny;
+    float *rs = malloc(sizeof(float) * n), *re = malloc(sizeof(float) * n);
+    for (size_t q = 0; q < n; q++) { rs[q] = a1 * sig[q]; re[q] = a2 * eps[q]; }
+    for (int i = 1; i < nx - 1; i++)
+        for (int j = 1; j < ny - 1; j++) {
+            size_t q = (size_t)j * nx + i;
+            float ls = (rs[q - 1] + rs[q + 1] + rs[q - nx] + rs[q + nx] - 4.0 * rs[q]) / (dh * dh);
+            float le = (re[q - 1] + re[q + 1] + re[q - nx] + re[q + nx] - 4.0 * re[q]) / (dh * dh);
+            ts += 0.5 * beta1 * ls * ls;
+            te += 0.5 * beta2 * le * le;
+            if (gs) gs[q] += lambda1 * beta1 * ls;
+            if (ge) ge[q] += lambda2 * beta2 * le;
+        }
+    cost[0] = ts;
+    cost[1] = te;
+    free(rs);
+    free(re);
+}
+
 /* src/AC/hessian_AC.c:144-150: one (shot field u, receiver Green's function g)
  * pair's contribution to the Gauss-Newton Hessian diagonal, J = -2 Omega^2 u
  * (1/(rho vp^3)) g rounded to complex float, hess += Re(J conj(J)) in float;

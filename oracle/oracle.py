@@ -75,6 +75,8 @@ def lib():
         L.orc_precond.restype = None
         L.orc_precond.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
                                   C.c_int, f(np.float32), f(np.float32)]
+        L.orc_tikhonov_te.restype = None
+        L.orc_tikhonov_te.argtypes = [C.c_int, C.c_int] + [C.c_float] * 7 + [f(np.float32)] * 5
         L.orc_model_update.restype = None
         L.orc_model_update.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32), C.c_float, C.c_float,
                                        f(np.float32)]
@@ -262,6 +264,17 @@ def precond(field, dh, vref, fc_low, fsx, fsy, spatfilter=1, nzero=0):
     out = np.empty_like(field)
     lib().orc_precond(nx, ny, dh, vref, fc_low, fsx, fsy, spatfilter, nzero, field.ravel(), out.ravel())
     return out
+
+
+def tikhonov_te(sig, eps, dh, beta, norm, lam, gs, ge):
+    """Tikhonov cost (tikh_sigma, tikh_epsilon) of float32 [ny, nx] fields; gs, ge
+    (float32, cells like the fields) receive the gradient terms in place."""
+    sig = np.ascontiguousarray(sig, np.float32)
+    ny, nx = sig.shape
+    cost = np.zeros(2, np.float32)
+    lib().orc_tikhonov_te(nx, ny, dh, beta[0], beta[1], norm[0], norm[1], lam[0], lam[1], sig.ravel(),
+                          np.ascontiguousarray(eps, np.float32).ravel(), gs, ge, cost)
+    return float(cost[0]), float(cost[1])
 
 
 def hess_gn_ac_accumulate(nx, ny, freq, s, u, u_last, g, g_last, vp, rho, hess):

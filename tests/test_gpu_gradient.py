@@ -354,3 +354,25 @@ def test_precond_matches_oracle(sizes):
     err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
     say("precond", sizes, "rel err", err, "bit-exact fraction", exact)
     assert np.array_equal(got, ref), err
+
+
+def test_tikhonov_te_matches_oracle():
+    """Tikhonov regularisation of the TE materials on the device against the
+    oracle: the gradient terms bit-exact (each is one float product added to
+    zero), the costs to 1e-4 (the reference sums them in float, the device in
+    double)."""
+    rng = np.random.default_rng(67)
+    ny, nx = 38, 52
+    eps0 = 8.854187817e-12
+    sig = (0.002 + 0.002 * rng.random((ny, nx))).astype(np.float32)
+    eps = ((5.0 + rng.random((ny, nx))) * eps0).astype(np.float32)
+    beta, norm, lam = (0.7, 1.3), (0.003, 5.0 * eps0), (0.01, 0.02)
+    gs, ge = torch.zeros(nx * ny, dtype=torch.float64, device="cuda"), torch.zeros(nx * ny, dtype=torch.float64,
+                                                                                   device="cuda")
+    cs, ce = fdfd.tikhonov_te(dev(sig), dev(eps), 0.04, beta, norm, lam, gs, ge)
+    rs, re_ = np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.float32)
+    os_, oe = O.tikhonov_te(sig, eps, 0.04, beta, norm, lam, rs, re_)
+    say("tikhonov costs", cs, os_, ce, oe)
+    assert abs(cs - os_) <= 1e-4 * abs(os_) and abs(ce - oe) <= 1e-4 * abs(oe)
+    assert np.array_equal(gs.cpu().numpy(), rs.astype(np.float64))
+    assert np.array_equal(ge.cpu().numpy(), re_.astype(np.float64))

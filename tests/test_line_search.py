@@ -92,3 +92,23 @@ def test_precond_oracle_pins():
     # a zero-width window leaves the field unfiltered
     r = np.random.default_rng(2).random((ny, nx)).astype(np.float32)
     assert (O.precond(r, 10.0, 1500.0, 15.0, 0.0, 0.6) == r).all()
+
+
+def test_tikhonov_oracle_closed_form():
+    """Tikhonov_cost_TE / Tikhonov_grad_TE (src/TE/Tikhonov_cost_TE.c:21-45,
+    src/TE/Tikhonov_grad_TE.c:20-37): for sigma = i^2 + j^2 (cells) the
+    five-point Laplacian is 4 everywhere inside; with BETA / NORM = 1 and DH = 2
+    it is 1, so the cost is 0.5 BETA per interior cell and the gradient gains
+    LAMBDA BETA there; a linear epsilon has a zero Laplacian."""
+    ny, nx = 9, 13
+    j, i = np.mgrid[0:ny, 0:nx]
+    sig = (i * i + j * j).astype(np.float32)
+    eps = (3 * i - 2 * j + 7).astype(np.float32)
+    gs = np.zeros(nx * ny, np.float32)
+    ge = np.zeros(nx * ny, np.float32)
+    cs, ce = O.tikhonov_te(sig, eps, 2.0, (0.5, 2.0), (0.5, 2.0), (3.0, 5.0), gs, ge)
+    assert cs == 0.5 * 0.5 * (nx - 2) * (ny - 2) and ce == 0.0
+    inner = np.zeros((ny, nx), bool)
+    inner[1:-1, 1:-1] = True
+    assert (gs.reshape(ny, nx)[inner] == 3.0 * 0.5).all() and (gs.reshape(ny, nx)[~inner] == 0).all()
+    assert (ge == 0).all()
