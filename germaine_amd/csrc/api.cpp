@@ -565,6 +565,20 @@ int gfdfd_tikhonov_te(int nx, int ny, float dh, float beta_sigma, float beta_eps
     GUARD_END
 }
 
+int gfdfd_taper_file(int nx0, int ny0, int npml, int free_surf, const float *d_taper, const double *d_in,
+                     double *d_out, void *stream) {
+    if (nx0 < 1 || ny0 < 1 || npml < 0 || !d_taper || !d_in || !d_out || d_in == d_out)
+        return report(GFDFD_ERR_ARG, "gfdfd_taper_file: bad argument");
+    const int fsshift = free_surf ? 0 : npml;
+    if (free_surf && npml >= fsshift + ny0)
+        return report(GFDFD_ERR_ARG, "gfdfd_taper_file: the model has fewer rows than the PML");
+    GUARD_BEGIN
+    taper_file(nx0 + 2 * npml, ny0 + npml + fsshift, nx0, ny0, npml, fsshift, free_surf ? 1 : 0, d_taper, d_in, d_out,
+               S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
 int gfdfd_hess_gn_ac(int n, int nshots, int nrec, const float *omega2, const double *d_u, const double *d_g,
                      const float *d_vp, const float *d_rho, double *d_hess, void *stream) {
     if (n < 1 || nshots < 1 || nrec < 1 || !omega2 || !d_u || !d_g || !d_vp || !d_rho || !d_hess)

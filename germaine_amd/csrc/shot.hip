@@ -324,6 +324,29 @@ __global__ void __launch_bounds__(1024) k_tikhonov_te(int nx, int ny, float dh, 
     }
 }
 
+// taper_grad_file (src/taper_grad_file.c:22-66): the model part of the [ny, nx]
+// field (rows fsshift .. fsshift+ny0-1, columns npml .. npml+nx0-1) times the
+// [ny0, nx0] taper in float, then the PML filled by the reference's copies --
+// columns from the nearest model column, the rows below the model from its
+// last row and, only with a free surface (the reference's condition), the
+// first npml rows from row npml.  Those copies compose to one gather: cell
+// (j, i) takes the tapered value at (row map of j, clamped i).
+__global__ void k_taper_file(int nx, int ny, int nx0, int ny0, int npml, int fsshift, int free_surf,
+                             const float *__restrict__ taper, const double *__restrict__ in,
+                             double *__restrict__ out) {
+    const int64_t n = (int64_t)nx * ny;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q / nx), i = (int)(q % nx);
+        const int ic = min(max(i, npml), npml + nx0 - 1);
+        int jr = j;
+        if (free_surf && j < npml) jr = npml;
+        if (j >= fsshift + ny0) jr = fsshift + ny0 - 1;
+        const double g = in[(int64_t)jr * nx + ic];
+        const bool model = jr >= fsshift && jr < fsshift + ny0;
+        out[q] = model ? (double)((float)g * taper[(int64_t)(jr - fsshift) * nx0 + (ic - npml)]) : g;
+    }
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -377,6 +400,13 @@ void tikhonov_te(int nx, int ny, float dh, float a_sig, float a_eps, float beta_
                  hipStream_t st) {
     hipLaunchKernelGGL(k_tikhonov_te, dim3(1), dim3(1024), 0, st, nx, ny, dh, a_sig, a_eps, beta_sig, beta_eps, lb_sig,
                        lb_eps, sig, eps, g_sig, g_eps, cost);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void taper_file(int nx, int ny, int nx0, int ny0, int npml, int fsshift, int free_surf, const float *taper,
+                const double *in, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_taper_file, dim3(grid_for((int64_t)nx * ny)), dim3(256), 0, st, nx, ny, nx0, ny0, npml,
+                       fsshift, free_surf, taper, in, out);
     GFDFD_HIP(hipGetLastError());
 }
 

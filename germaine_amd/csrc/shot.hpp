@@ -19,6 +19,8 @@ void model_update(int n, float alpha, const double *dir, const float *m_old, flo
 void tikhonov_te(int nx, int ny, float dh, float a_sig, float a_eps, float beta_sig, float beta_eps, float lb_sig,
                  float lb_eps, const float *sig, const float *eps, double *g_sig, double *g_eps, double *cost,
                  hipStream_t st);
+void taper_file(int nx, int ny, int nx0, int ny0, int npml, int fsshift, int free_surf, const float *taper,
+                const double *in, double *out, hipStream_t st);
 void precond(int nx, int ny, int hfsx, int hfsy, float sx, float sy, int nzero, const double *in, double *out,
              hipStream_t st);
 void model_update_norm(int n, float alpha, const double *dir, const float *m_old, float norm, float lo, float hi,

@@ -565,6 +565,23 @@ def precond(grad, nx, ny, dh, vref, fc_low, filt_size_x, filt_size_y, spatfilter
     return out
 
 
+def taper_file(grad, taper, npml, free_surf, stream=None):
+    """taper_grad_file (src/taper_grad_file.c:22-66): grad (float64 cuda of the
+    grid with PML) times taper (float32 cuda [ny0, nx0], taper.bin as
+    formats.read_model_file reads it) on the model part, PML refilled by the
+    reference's edge copies.  Returns a new tensor."""
+    import torch
+    ny0, nx0 = taper.shape
+    fs = 0 if free_surf else npml
+    if grad.numel() != (nx0 + 2 * npml) * (ny0 + npml + fs):
+        raise ValueError("taper_file: field size differs from the grid with PML")
+    grad = grad.contiguous()
+    out = torch.empty_like(grad)
+    check(lib().gfdfd_taper_file(nx0, ny0, npml, int(free_surf), _ptr(taper.contiguous()), _ptr(grad), _ptr(out),
+                                 _stream(stream)), "gfdfd_taper_file")
+    return out
+
+
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
              precision=FP64, tol=1e-10, max_iter=10, rtm=False):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,

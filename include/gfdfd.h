@@ -227,6 +227,14 @@ int gfdfd_precond(int nx, int ny, float dh, float vref, float fc_low, float filt
 int gfdfd_tikhonov_te(int nx, int ny, float dh, float beta_sigma, float beta_eps, float norm_sigma, float norm_eps,
                       float lambda_sigma, float lambda_eps, const float *d_sigma, const float *d_eps,
                       double *d_grad_sigma, double *d_grad_eps, double *d_cost, void *stream);
+/* taper_grad_file (src/taper_grad_file.c:22-66), precond's last step when
+ * SWS_TAPER_FILE: the field d_in on the grid with PML ([ny, nx], nx = nx0 +
+ * 2 npml, ny = ny0 + npml + (free_surf ? 0 : npml)) times d_taper (the
+ * taper.bin values, [ny0, nx0] float) on the model part, the PML then filled
+ * by the reference's edge copies (its top rows only with a free surface, as
+ * the reference does).  Result in d_out (not d_in).                          */
+int gfdfd_taper_file(int nx0, int ny0, int npml, int free_surf, const float *d_taper, const double *d_in,
+                     double *d_out, void *stream);
 /* hessian_AC (src/AC/hessian_AC.c:129-150): Gauss-Newton Hessian diagonal,
  * hess += |2 Omega^2 u g / (rho vp^3)|^2 over every pair of the nshots forward
  * fields d_u and the nrec receiver Green's functions d_g (solutions for a

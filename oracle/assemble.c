@@ -567,6 +567,25 @@ void orc_tikhonov_te(int nx, int ny, float dh, float beta1, float beta2, float n
     free(re);
 }
 
+/* src/taper_grad_file.c:22-66 in the reference's loop order, 0-based: the
+ * model part times the taper, then the left, right, top (free surface only)
+ * and bottom copies, each reading what the earlier ones wrote.                */
+void orc_taper_file(int nx0, int ny0, int npml, int free_surf, const float *taper, float *g)
+{
+    int fs = free_surf ? 0 : npml, nx = nx0 + 2 * npml, ny = ny0 + npml + fs;
+    for (int i = 0; i < nx0; i++)
+        for (int j = 0; j < ny0; j++) g[(size_t)(j + fs) * nx + i + npml] *= taper[(size_t)j * nx0 + i];
+    for (int i = 0; i < npml; i++)
+        for (int j = 0; j < ny; j++) g[(size_t)j * nx + i] = g[(size_t)j * nx + npml];
+    for (int i = npml + nx0; i < nx; i++)
+        for (int j = 0; j < ny; j++) g[(size_t)j * nx + i] = g[(size_t)j * nx + npml + nx0 - 1];
+    if (free_surf)
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < npml; j++) g[(size_t)j * nx + i] = g[(size_t)npml * nx + i];
+    for (int i = 0; i < nx; i++)
+        for (int j = fs + ny0; j < ny; j++) g[(size_t)j * nx + i] = g[(size_t)(fs + ny0 - 1) * nx + i];
+}
+
 /* src/AC/hessian_AC.c:144-150: one (shot field u, receiver Green's function g)
  * pair's contribution to the Gauss-Newton Hessian diagonal, J = -2 Omega^2 u
  * (1/(rho vp^3)) g rounded to complex float, hess += Re(J conj(J)) in float;

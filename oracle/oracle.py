@@ -77,6 +77,8 @@ def lib():
                                   C.c_int, f(np.float32), f(np.float32)]
         L.orc_tikhonov_te.restype = None
         L.orc_tikhonov_te.argtypes = [C.c_int, C.c_int] + [C.c_float] * 7 + [f(np.float32)] * 5
+        L.orc_taper_file.restype = None
+        L.orc_taper_file.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f(np.float32), f(np.float32)]
         L.orc_model_update.restype = None
         L.orc_model_update.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32), C.c_float, C.c_float,
                                        f(np.float32)]
@@ -275,6 +277,15 @@ def tikhonov_te(sig, eps, dh, beta, norm, lam, gs, ge):
     lib().orc_tikhonov_te(nx, ny, dh, beta[0], beta[1], norm[0], norm[1], lam[0], lam[1], sig.ravel(),
                           np.ascontiguousarray(eps, np.float32).ravel(), gs, ge, cost)
     return float(cost[0]), float(cost[1])
+
+
+def taper_file(g, taper, npml, free_surf):
+    """taper_grad_file on a float32 [ny, nx] field (a tapered copy is returned)."""
+    taper = np.ascontiguousarray(taper, np.float32)
+    ny0, nx0 = taper.shape
+    out = np.ascontiguousarray(g, np.float32).copy()
+    lib().orc_taper_file(nx0, ny0, npml, free_surf, taper.ravel(), out.ravel())
+    return out
 
 
 def hess_gn_ac_accumulate(nx, ny, freq, s, u, u_last, g, g_last, vp, rho, hess):

@@ -376,3 +376,18 @@ def test_tikhonov_te_matches_oracle():
     assert abs(cs - os_) <= 1e-4 * abs(os_) and abs(ce - oe) <= 1e-4 * abs(oe)
     assert np.array_equal(gs.cpu().numpy(), rs.astype(np.float64))
     assert np.array_equal(ge.cpu().numpy(), re_.astype(np.float64))
+
+
+@pytest.mark.parametrize("free_surf", [0, 1])
+def test_taper_file_matches_oracle(free_surf):
+    """taper_grad_file (src/taper_grad_file.c:22-66) on the device against the
+    oracle's loop restatement: bit-exact, with and without a free surface."""
+    nx0, ny0, npml = 45, 30, 8
+    fs = 0 if free_surf else npml
+    nx, ny = nx0 + 2 * npml, ny0 + npml + fs
+    rng = np.random.default_rng(71)
+    g = rng.standard_normal((ny, nx)).astype(np.float32)
+    taper = rng.random((ny0, nx0)).astype(np.float32)
+    out = fdfd.taper_file(dev(g.astype(np.float64).ravel()), dev(taper), npml, free_surf)
+    ref = O.taper_file(g, taper, npml, free_surf)
+    assert np.array_equal(out.cpu().numpy(), ref.ravel().astype(np.float64))

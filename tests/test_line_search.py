@@ -5,6 +5,7 @@ curves that are exact parabolas, whose vertex the fit must return, with the
 sequence of trial steps worked out by hand from the reference's control flow,
 and update values that are exact in float."""
 import numpy as np
+import pytest
 
 from germaine_amd import fdfd
 from oracle import oracle as O
@@ -112,3 +113,26 @@ def test_tikhonov_oracle_closed_form():
     inner[1:-1, 1:-1] = True
     assert (gs.reshape(ny, nx)[inner] == 3.0 * 0.5).all() and (gs.reshape(ny, nx)[~inner] == 0).all()
     assert (ge == 0).all()
+
+
+@pytest.mark.parametrize("free_surf", [0, 1])
+def test_taper_file_oracle_pins(free_surf):
+    """taper_grad_file (src/taper_grad_file.c:22-66): a taper of 2 doubles the
+    model part; the PML then holds copies of the nearest model column and of
+    the last model row; with a free surface the reference also overwrites the
+    first npml rows (model rows there) with row npml, which it does not do
+    without one (the top PML keeps its values)."""
+    nx0, ny0, npml = 9, 7, 2
+    fs = 0 if free_surf else npml
+    nx, ny = nx0 + 2 * npml, ny0 + npml + fs
+    g = np.random.default_rng(7).random((ny, nx)).astype(np.float32)
+    out = O.taper_file(g, np.full((ny0, nx0), 2.0, np.float32), npml, free_surf)
+    model = 2.0 * g[fs:fs + ny0, npml:npml + nx0]
+    top = npml if free_surf else 0
+    assert (out[fs + top:fs + ny0, npml:npml + nx0] == model[top:]).all()
+    assert (out[:, :npml] == out[:, [npml]]).all() and (out[:, npml + nx0:] == out[:, [npml + nx0 - 1]]).all()
+    assert (out[fs + ny0:] == out[fs + ny0 - 1]).all()
+    if free_surf:
+        assert (out[:npml] == out[npml]).all()
+    else:
+        assert (out[:npml, npml:npml + nx0] == g[:npml, npml:npml + nx0]).all()
