@@ -391,3 +391,37 @@ def test_taper_file_matches_oracle(free_surf):
     out = fdfd.taper_file(dev(g.astype(np.float64).ravel()), dev(taper), npml, free_surf)
     ref = O.taper_file(g, taper, npml, free_surf)
     assert np.array_equal(out.cpu().numpy(), ref.ravel().astype(np.float64))
+
+
+def test_fwi_iteration_ac():
+    """One acoustic FWI iteration in the order of src/AC/fwi_FD_AC.c:263-353 with
+    steepest descent: gradient and misfit (grad_obj_AC), Shin pseudo-Hessian
+    (hessian_shin_AC), Hessian preconditioning (apply_hess_AC), smoothing and
+    horizontal taper (precond), descent direction -grad (descent), parabolic step
+    search (parabolicls_AC) and the model update (calc_mat_change_wolfe); every
+    stage on the device.  The iteration must lower the misfit, and a second
+    iteration from the updated model must lower it again."""
+    m = models(G.AC)
+    dh, freqs = m["dh"], m["freqs"]
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    pl = fdfd.Plan(NX, NY)
+    pobs = fdfd.forward_shots(pl, G.AC, dh, freqs, dev(m["t1"]), dev(m["t2"]), srcs, recs, npml=NPML,
+                              tol=None)["data"]
+    vp, rho = dev(m["m1"]), dev(m["m2"])
+    misfits = []
+    for it in range(2):
+        L2, grad = fdfd.grad_obj(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, tol=None)
+        hess = fdfd.hessian_shin(pl, G.AC, dh, freqs, vp, rho, srcs, npml=NPML, tol=None)
+        fdfd.apply_hess(grad, hess, 0.005)
+        grad = fdfd.precond(grad, NX, NY, dh, 1800.0, freqs[0], 0.5, 0.5, 1, NPML)
+        direction = -grad
+        step, trials = fdfd.line_search(pl, dh, freqs, vp, rho, direction, srcs, recs, pobs, L2, 0.01, 100.0,
+                                        9000.0, npml=NPML, tol=None)
+        say("FWI iteration", it, "misfit", L2, "step", step, "trials", len(trials))
+        assert step > 0.0
+        vp = fdfd.model_update(step, direction, vp, 100.0, 9000.0)
+        misfits.append(L2)
+    final = fdfd.objective(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, tol=None)
+    say("  misfits", misfits, "final", final)
+    assert misfits[1] < misfits[0] and final < misfits[1]
