@@ -83,6 +83,7 @@ SIGNATURES = {
     "gfdfd_model_update_norm": (_I, [_I, _F, _P, _P, _F, _F, _F, _P, _P]),
     "gfdfd_precond": (_I, [_I, _I, _F, _F, _F, _F, _F, _I, _I, _P, _P, _P]),
     "gfdfd_taper_file": (_I, [_I, _I, _I, _I, _P, _P, _P, _P]),
+    "gfdfd_grad_frame": (_I, [_I, _I, _I, _I, _P, _P]),
     "gfdfd_tikhonov_te": (_I, [_I, _I, _F, _F, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P]),
     "gfdfd_grad_te": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_profile": (_I, [_I]),

@@ -579,6 +579,18 @@ int gfdfd_taper_file(int nx0, int ny0, int npml, int free_surf, const float *d_t
     GUARD_END
 }
 
+int gfdfd_grad_frame(int nx0, int ny0, int npml, int free_surf, double *d_grad, void *stream) {
+    if (nx0 < 1 || ny0 < 1 || npml < 2 || !d_grad) return report(GFDFD_ERR_ARG, "gfdfd_grad_frame: bad argument");
+    const int fsshift = free_surf ? 0 : npml;
+    // src/cp_grad_frame.c:112-116: DAMPING = 60 %, amp = 1 - 0.6 in float, a from the PML width
+    const float amp = 1.0 - 60.0 / 100.0;
+    const float a = std::sqrt(-std::log((double)amp) / ((npml - 1) * (npml - 1)));
+    GUARD_BEGIN
+    grad_frame(nx0 + 2 * npml, ny0 + npml + fsshift, nx0, ny0, npml, fsshift, free_surf ? 1 : 0, a, d_grad, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
 int gfdfd_hess_gn_ac(int n, int nshots, int nrec, const float *omega2, const double *d_u, const double *d_g,
                      const float *d_vp, const float *d_rho, double *d_hess, void *stream) {
     if (n < 1 || nshots < 1 || nrec < 1 || !omega2 || !d_u || !d_g || !d_vp || !d_rho || !d_hess)

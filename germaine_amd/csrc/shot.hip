@@ -347,6 +347,42 @@ __global__ void k_taper_file(int nx, int ny, int nx0, int ny0, int npml, int fss
     }
 }
 
+// cp_grad_frame (src/cp_grad_frame.c:102-208, the PML_grad = 3 branch it
+// always takes): the gradient in the PML is damped by coeff[b] = exp(-(a^2
+// (npml - b)^2)) of the band b (1-based, 1 at the outer edge).  The
+// reference's coefficient map is written band by band -- left, right, top
+// (without a free surface), bottom, each band b spanning b .. len - b + 1 of
+// its edge, later ones overwriting earlier ones -- and then every PML column
+// and every PML row multiplies by it, so a corner cell is multiplied twice.
+// Both compose to a rule per cell: the coefficient of the last band that
+// covers it, applied once per PML column/row band it lies in, each product
+// rounded to float.  (The reference's replacement of a NaN product by the
+// value above it is not reproduced.)
+__global__ void k_grad_frame(int nx, int ny, int nx0, int ny0, int npml, int fsshift, int free_surf, float a,
+                             double *__restrict__ g) {
+    const int64_t n = (int64_t)nx * ny;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int J = (int)(q / nx) + 1, I = (int)(q % nx) + 1;
+        const int times = (I <= npml) + (I >= npml + nx0 + 1) + (!free_surf && J <= npml) + (J >= fsshift + ny0 + 1);
+        if (times == 0) continue;
+        int band = 0;                                   // 0: coefficient 1
+        if (I <= npml && J >= I && J <= ny - I + 1) band = I;
+        const int ir = nx - I + 1;
+        if (ir <= npml && J >= ir && J <= ny - ir + 1) band = ir;
+        if (!free_surf && J <= npml && I >= J && I <= nx - J + 1) band = J;
+        const int jr = ny - J + 1;
+        if (jr <= npml && I >= jr && I <= nx - jr + 1) band = jr;
+        float c = 1.0f;
+        if (band) {
+            const float d = (float)(npml - band);
+            c = (float)exp((double)(-(a * a * d * d)));
+        }
+        float v = (float)g[q];
+        for (int t = 0; t < times; t++) v *= c;
+        g[q] = v;
+    }
+}
+
 // src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
 // grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
 __global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
@@ -407,6 +443,13 @@ void taper_file(int nx, int ny, int nx0, int ny0, int npml, int fsshift, int fre
                 const double *in, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_taper_file, dim3(grid_for((int64_t)nx * ny)), dim3(256), 0, st, nx, ny, nx0, ny0, npml,
                        fsshift, free_surf, taper, in, out);
+    GFDFD_HIP(hipGetLastError());
+}
+
+void grad_frame(int nx, int ny, int nx0, int ny0, int npml, int fsshift, int free_surf, float a, double *g,
+                hipStream_t st) {
+    hipLaunchKernelGGL(k_grad_frame, dim3(grid_for((int64_t)nx * ny)), dim3(256), 0, st, nx, ny, nx0, ny0, npml,
+                       fsshift, free_surf, a, g);
     GFDFD_HIP(hipGetLastError());
 }
 

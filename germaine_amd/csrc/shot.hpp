@@ -21,6 +21,8 @@ void tikhonov_te(int nx, int ny, float dh, float a_sig, float a_eps, float beta_
                  hipStream_t st);
 void taper_file(int nx, int ny, int nx0, int ny0, int npml, int fsshift, int free_surf, const float *taper,
                 const double *in, double *out, hipStream_t st);
+void grad_frame(int nx, int ny, int nx0, int ny0, int npml, int fsshift, int free_surf, float a, double *g,
+                hipStream_t st);
 void precond(int nx, int ny, int hfsx, int hfsy, float sx, float sy, int nzero, const double *in, double *out,
              hipStream_t st);
 void model_update_norm(int n, float alpha, const double *dir, const float *m_old, float norm, float lo, float hi,

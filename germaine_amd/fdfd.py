@@ -582,6 +582,16 @@ def taper_file(grad, taper, npml, free_surf, stream=None):
     return out
 
 
+def grad_frame(grad, nx0, ny0, npml, free_surf, stream=None):
+    """cp_grad_frame (src/cp_grad_frame.c:102-208): damp the gradient (float64
+    cuda, grid with PML) in the PML with the reference's taper, in place."""
+    fs = 0 if free_surf else npml
+    if grad.numel() != (nx0 + 2 * npml) * (ny0 + npml + fs) or not grad.is_contiguous():
+        raise ValueError("grad_frame: the gradient must be the contiguous grid with PML")
+    check(lib().gfdfd_grad_frame(nx0, ny0, npml, int(free_surf), _ptr(grad), _stream(stream)), "gfdfd_grad_frame")
+    return grad
+
+
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
              precision=FP64, tol=1e-10, max_iter=10, rtm=False):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,

@@ -235,6 +235,12 @@ int gfdfd_tikhonov_te(int nx, int ny, float dh, float beta_sigma, float beta_eps
  * the reference does).  Result in d_out (not d_in).                          */
 int gfdfd_taper_file(int nx0, int ny0, int npml, int free_surf, const float *d_taper, const double *d_in,
                      double *d_out, void *stream);
+/* cp_grad_frame (src/cp_grad_frame.c:102-208, its PML_grad = 3 branch): damps
+ * the gradient d_grad (in place, [ny, nx] of the grid with PML, nx = nx0 +
+ * 2 npml, ny = ny0 + npml + (free_surf ? 0 : npml)) in the PML with the
+ * reference's 60 % Gaussian taper, corners twice as the reference does, in
+ * float.  npml >= 2 (the taper divides by npml - 1).                        */
+int gfdfd_grad_frame(int nx0, int ny0, int npml, int free_surf, double *d_grad, void *stream);
 /* hessian_AC (src/AC/hessian_AC.c:129-150): Gauss-Newton Hessian diagonal,
  * hess += |2 Omega^2 u g / (rho vp^3)|^2 over every pair of the nshots forward
  * fields d_u and the nrec receiver Green's functions d_g (solutions for a

@@ -586,6 +586,45 @@ void orc_taper_file(int nx0, int ny0, int npml, int free_surf, const float *tape
         for (int j = fs + ny0; j < ny; j++) g[(size_t)j * nx + i] = g[(size_t)(fs + ny0 - 1) * nx + i];
 }
 
+/* src/cp_grad_frame.c:102-208 (PML_grad = 3) in the reference's loop order,
+ * 0-based: the coefficient table, the coefficient map band by band (left,
+ * right, top without a free surface, bottom), then the four multiplying
+ * loops.  The NaN substitution is left out.                                  */
+void orc_grad_frame(int nx0, int ny0, int npml, int free_surf, float *A)
+{
+    int fs = free_surf ? 0 : npml, nx = nx0 + 2 * npml, ny = ny0 + npml + fs;
+    float amp = 1.0 - 60.0 / 100.0;
+    float a = sqrt(-log(amp) / ((npml - 1) * (npml - 1)));
+    float *coeff = malloc(sizeof(float) * (npml + 1));
+    float *ab = malloc(sizeof(float) * (size_t)nx * ny);
+    for (int i = 1; i <= npml; i++) coeff[i] = exp(-(a * a * (npml - i) * (npml - i)));
+    for (size_t q = 0; q < (size_t)nx * ny; q++) ab[q] = 1.0f;
+#define AB(j, i) ab[(size_t)((j) - 1) * nx + (i) - 1]
+#define AA(j, i) A[(size_t)((j) - 1) * nx + (i) - 1]
+    for (int i = 1; i <= npml; i++)
+        for (int j = i; j <= ny - i + 1; j++) AB(j, i) = coeff[i];
+    for (int i = 1; i <= npml; i++)
+        for (int j = i; j <= ny - i + 1; j++) AB(j, nx - i + 1) = coeff[i];
+    if (!free_surf)
+        for (int j = 1; j <= npml; j++)
+            for (int i = j; i <= nx - j + 1; i++) AB(j, i) = coeff[j];
+    for (int j = 1; j <= npml; j++)
+        for (int i = j; i <= nx - j + 1; i++) AB(ny - j + 1, i) = coeff[j];
+    for (int i = 1; i <= npml; i++)
+        for (int j = 1; j <= ny; j++) AA(j, i) *= AB(j, i);
+    for (int i = npml + nx0 + 1; i <= nx; i++)
+        for (int j = 1; j <= ny; j++) AA(j, i) *= AB(j, i);
+    if (!free_surf)
+        for (int i = 1; i <= nx; i++)
+            for (int j = 1; j <= npml; j++) AA(j, i) *= AB(j, i);
+    for (int i = 1; i <= nx; i++)
+        for (int j = fs + ny0 + 1; j <= ny; j++) AA(j, i) *= AB(j, i);
+#undef AB
+#undef AA
+    free(coeff);
+    free(ab);
+}
+
 /* src/AC/hessian_AC.c:144-150: one (shot field u, receiver Green's function g)
  * pair's contribution to the Gauss-Newton Hessian diagonal, J = -2 Omega^2 u
  * (1/(rho vp^3)) g rounded to complex float, hess += Re(J conj(J)) in float;

@@ -79,6 +79,8 @@ def lib():
         L.orc_tikhonov_te.argtypes = [C.c_int, C.c_int] + [C.c_float] * 7 + [f(np.float32)] * 5
         L.orc_taper_file.restype = None
         L.orc_taper_file.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f(np.float32), f(np.float32)]
+        L.orc_grad_frame.restype = None
+        L.orc_grad_frame.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f(np.float32)]
         L.orc_model_update.restype = None
         L.orc_model_update.argtypes = [C.c_int, C.c_float, f(np.float32), f(np.float32), C.c_float, C.c_float,
                                        f(np.float32)]
@@ -285,6 +287,13 @@ def taper_file(g, taper, npml, free_surf):
     ny0, nx0 = taper.shape
     out = np.ascontiguousarray(g, np.float32).copy()
     lib().orc_taper_file(nx0, ny0, npml, free_surf, taper.ravel(), out.ravel())
+    return out
+
+
+def grad_frame(g, nx0, ny0, npml, free_surf):
+    """cp_grad_frame's PML damping on a float32 [ny, nx] field (a damped copy is returned)."""
+    out = np.ascontiguousarray(g, np.float32).copy()
+    lib().orc_grad_frame(nx0, ny0, npml, free_surf, out.ravel())
     return out
 
 

@@ -425,3 +425,18 @@ def test_fwi_iteration_ac():
     final = fdfd.objective(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, tol=None)
     say("  misfits", misfits, "final", final)
     assert misfits[1] < misfits[0] and final < misfits[1]
+
+
+@pytest.mark.parametrize("free_surf", [0, 1])
+def test_grad_frame_matches_oracle(free_surf):
+    """cp_grad_frame's PML damping (src/cp_grad_frame.c:102-208) on the device
+    against the oracle's loop restatement: relative 1e-6 (the npml coefficients
+    round a double exp to float on both sides)."""
+    nx0, ny0, npml = 45, 30, 8
+    fs = 0 if free_surf else npml
+    nx, ny = nx0 + 2 * npml, ny0 + npml + fs
+    g = np.random.default_rng(73).standard_normal((ny, nx)).astype(np.float32)
+    out = fdfd.grad_frame(dev(g.astype(np.float64).ravel()), nx0, ny0, npml, free_surf).cpu().numpy()
+    ref = O.grad_frame(g, nx0, ny0, npml, free_surf).ravel()
+    say("grad frame", free_surf, "bit-exact fraction", float(np.mean(out == ref)))
+    assert np.max(np.abs(out - ref)) <= 1e-6 * np.max(np.abs(ref))

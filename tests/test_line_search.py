@@ -136,3 +136,28 @@ def test_taper_file_oracle_pins(free_surf):
         assert (out[:npml] == out[npml]).all()
     else:
         assert (out[:npml, npml:npml + nx0] == g[:npml, npml:npml + nx0]).all()
+
+
+@pytest.mark.parametrize("free_surf", [0, 1])
+def test_grad_frame_oracle_closed_form(free_surf):
+    """cp_grad_frame's PML damping (src/cp_grad_frame.c:102-208) on a field of
+    ones: coeff[b] = exp(-a^2 (npml - b)^2) with a^2 = -ln(0.4) / (npml - 1)^2,
+    so the outer band is 0.4 and band b is 0.4^(((npml - b) / (npml - 1))^2);
+    the model part is untouched, an edge band away from the corners is damped
+    once, a corner cell twice; with a free surface the top rows are not
+    damped."""
+    nx0, ny0, npml = 12, 9, 4
+    fs = 0 if free_surf else npml
+    nx, ny = nx0 + 2 * npml, ny0 + npml + fs
+    out = O.grad_frame(np.ones((ny, nx), np.float32), nx0, ny0, npml, free_surf)
+    coeff = lambda b: 0.4 ** (((npml - b) / (npml - 1)) ** 2)
+    mid = fs + ny0 // 2                                   # a model row, away from the corners
+    for b in range(1, npml + 1):
+        assert abs(out[mid, b - 1] - coeff(b)) <= 1e-6 and abs(out[mid, nx - b] - coeff(b)) <= 1e-6
+        assert abs(out[ny - b, nx // 2] - coeff(b)) <= 1e-6           # bottom band
+    assert (out[fs:fs + ny0, npml:npml + nx0] == 1.0).all()
+    assert abs(out[ny - 1, 0] - 0.4 * 0.4) <= 1e-6                  # bottom-left corner: twice
+    if free_surf:
+        assert (out[0, npml:npml + nx0] == 1.0).all()
+    else:
+        assert abs(out[0, nx // 2] - 0.4) <= 1e-6 and abs(out[0, 0] - 0.16) <= 1e-6
