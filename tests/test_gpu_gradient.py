@@ -440,3 +440,47 @@ def test_grad_frame_matches_oracle(free_surf):
     ref = O.grad_frame(g, nx0, ny0, npml, free_surf).ravel()
     say("grad frame", free_surf, "bit-exact fraction", float(np.mean(out == ref)))
     assert np.max(np.abs(out - ref)) <= 1e-6 * np.max(np.abs(ref))
+
+
+def test_fwi_iteration_te():
+    """Two TE FWI iterations in the order of src/TE/fwi_FD_TE.c:258-345 with
+    steepest descent, every stage on the device: gradient and misfit
+    (grad_obj_TE) with the Tikhonov cost and gradient, Shin pseudo-Hessians
+    and apply_hess per material, cp_grad_frame's PML damping, descent
+    directions (each scaled to a maximum of one on its normalised material),
+    and the normalised step search with the Tikhonov term.  The misfit must
+    fall at every iteration."""
+    m = models(G.TE)
+    dh, freqs = m["dh"], m["freqs"]
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    pl = fdfd.Plan(NX, NY)
+    pobs = fdfd.forward_shots(pl, G.TE, dh, freqs, dev(m["t1"]), dev(m["t2"]), srcs, recs, npml=NPML,
+                              tol=None)["data"]
+    eps, sig = dev(m["m1"]), dev(m["m2"])
+    eps0 = 8.854187817e-12
+    norm, low, up = (0.003, 5.0 * eps0), (0.0, eps0), (1.0, 100.0 * eps0)
+    beta, lam = (1.0, 1.0), (1e-9, 1e-9)
+    nx0, ny0 = NX - 2 * NPML, NY - 2 * NPML
+    misfits = []
+    for it in range(2):
+        L2, (g_sig, g_eps) = fdfd.grad_obj(pl, G.TE, dh, freqs, eps, sig, srcs, recs, pobs, npml=NPML, tol=None)
+        cs, ce = fdfd.tikhonov_te(sig, eps, dh, beta, norm, lam, g_sig, g_eps)
+        L2 += lam[0] * cs + lam[1] * ce
+        h_sig, h_eps = fdfd.hessian_shin(pl, G.TE, dh, freqs, eps, sig, srcs, npml=NPML, tol=None)
+        fdfd.apply_hess(g_sig, h_sig, 0.005)
+        fdfd.apply_hess(g_eps, h_eps, 0.005)
+        fdfd.grad_frame(g_sig, nx0, ny0, NPML, 0)
+        fdfd.grad_frame(g_eps, nx0, ny0, NPML, 0)
+        d_sig = -g_sig / g_sig.abs().max()
+        d_eps = -g_eps / g_eps.abs().max()
+        step, trials = fdfd.line_search_te(pl, dh, freqs, eps, sig, d_eps, d_sig, srcs, recs, pobs, L2, 0.01, norm,
+                                           low, up, tikhonov=(beta, lam), npml=NPML, tol=None)
+        say("TE FWI iteration", it, "misfit", L2, "step", step, "trials", len(trials))
+        assert step > 0.0
+        sig = fdfd.model_update_norm(step, d_sig, sig, norm[0], low[0], up[0])
+        eps = fdfd.model_update_norm(step, d_eps, eps, norm[1], low[1], up[1])
+        misfits.append(L2)
+    final = fdfd.objective(pl, G.TE, dh, freqs, eps, sig, srcs, recs, pobs, npml=NPML, tol=None)
+    say("  misfits", misfits, "final (without the Tikhonov term)", final)
+    assert misfits[1] < misfits[0] and final < misfits[1]
