@@ -73,7 +73,9 @@ SIGNATURES = {
     "gfdfd_spike_panel": (_I, [_I, _I, _P, _D, _D, _P, _P]),
     "gfdfd_sample": (_I, [_I, _I, _I, _P, _P, _P, _P]),
     "gfdfd_adjoint_sources": (_I, [_I, _I, _I, _P, _P, _P, _F, _P, _P, _P]),
+    "gfdfd_residual_sources": (_I, [_I, _I, _I, _P, _P, _P, _F, _I, _I, _I, _P, _P, _P, _P]),
     "gfdfd_grad_ac": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "gfdfd_grad_ac_stf": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_hess_shin_ac": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
     "gfdfd_hess_shin_te": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
     "gfdfd_hess_gn_ac": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
@@ -86,6 +88,7 @@ SIGNATURES = {
     "gfdfd_grad_frame": (_I, [_I, _I, _I, _I, _P, _P]),
     "gfdfd_tikhonov_te": (_I, [_I, _I, _F, _F, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P]),
     "gfdfd_grad_te": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "gfdfd_grad_te_stf": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_profile": (_I, [_I]),
     "gfdfd_profile_select": (_I, [C.c_char_p]),
     "gfdfd_profile_reset": (_V, []),

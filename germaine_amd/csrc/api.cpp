@@ -467,8 +467,24 @@ int gfdfd_adjoint_sources(int n, int nshots, int ntr, const int *d_krec, const f
         return report(GFDFD_ERR_ARG, "gfdfd_adjoint_sources: bad argument");
     GUARD_BEGIN
     adjoint_sources(n, nshots, ntr, d_krec, reinterpret_cast<const cx<float> *>(d_pcal),
-                    reinterpret_cast<const cx<float> *>(d_pobs), dh, reinterpret_cast<cx<double> *>(d_b), d_l2,
-                    S(stream));
+                    reinterpret_cast<const cx<float> *>(d_pobs), dh, 1, 0, 0, reinterpret_cast<cx<double> *>(d_b),
+                    d_l2, nullptr, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_residual_sources(int n, int nshots, int ntr, const int *d_krec, const float *d_pcal, const float *d_pobs,
+                           float dh, int misfit, int stf_inv, int rtm, double *d_b, double *d_l2, float *d_stf,
+                           void *stream) {
+    if (n < 1 || nshots < 1 || ntr < 1 || !d_krec || !d_pcal || !d_pobs || !d_b || !d_l2)
+        return report(GFDFD_ERR_ARG, "gfdfd_residual_sources: bad argument");
+    if (!rtm && (misfit < 1 || misfit > 3))
+        return report(GFDFD_ERR_ARG, "gfdfd_residual_sources: misfit must be 1, 2 or 3");
+    if (stf_inv != 0 && stf_inv != 1) return report(GFDFD_ERR_ARG, "gfdfd_residual_sources: stf_inv must be 0 or 1");
+    GUARD_BEGIN
+    adjoint_sources(n, nshots, ntr, d_krec, reinterpret_cast<const cx<float> *>(d_pcal),
+                    reinterpret_cast<const cx<float> *>(d_pobs), dh, misfit, stf_inv, rtm ? 1 : 0,
+                    reinterpret_cast<cx<double> *>(d_b), d_l2, reinterpret_cast<cx<float> *>(d_stf), S(stream));
     return GFDFD_OK;
     GUARD_END
 }
@@ -479,7 +495,18 @@ int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, con
         return report(GFDFD_ERR_ARG, "gfdfd_grad_ac: bad argument");
     GUARD_BEGIN
     grad_ac(n, nshots, omega2, reinterpret_cast<const cx<double> *>(d_u), reinterpret_cast<const cx<double> *>(d_lam),
-            d_vp, d_rho, d_grad, S(stream));
+            d_vp, d_rho, nullptr, d_grad, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_grad_ac_stf(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam,
+                      const float *d_vp, const float *d_rho, const float *d_stf, double *d_grad, void *stream) {
+    if (n < 1 || nshots < 1 || !omega2 || !d_u || !d_lam || !d_vp || !d_rho || !d_grad)
+        return report(GFDFD_ERR_ARG, "gfdfd_grad_ac_stf: bad argument");
+    GUARD_BEGIN
+    grad_ac(n, nshots, omega2, reinterpret_cast<const cx<double> *>(d_u), reinterpret_cast<const cx<double> *>(d_lam),
+            d_vp, d_rho, reinterpret_cast<const cx<float> *>(d_stf), d_grad, S(stream));
     return GFDFD_OK;
     GUARD_END
 }
@@ -630,7 +657,20 @@ int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, co
         return report(GFDFD_ERR_ARG, "gfdfd_grad_te: bad argument");
     GUARD_BEGIN
     grad_te(n, nshots, omega, omega2, reinterpret_cast<const cx<double> *>(d_u),
-            reinterpret_cast<const cx<double> *>(d_lam), d_grad_sigma, d_grad_eps, S(stream));
+            reinterpret_cast<const cx<double> *>(d_lam), nullptr, d_grad_sigma, d_grad_eps, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_grad_te_stf(int n, int nshots, const float *omega, const float *omega2, const double *d_u,
+                      const double *d_lam, const float *d_stf, double *d_grad_sigma, double *d_grad_eps,
+                      void *stream) {
+    if (n < 1 || nshots < 1 || !omega || !omega2 || !d_u || !d_lam || !d_grad_sigma || !d_grad_eps)
+        return report(GFDFD_ERR_ARG, "gfdfd_grad_te_stf: bad argument");
+    GUARD_BEGIN
+    grad_te(n, nshots, omega, omega2, reinterpret_cast<const cx<double> *>(d_u),
+            reinterpret_cast<const cx<double> *>(d_lam), reinterpret_cast<const cx<float> *>(d_stf), d_grad_sigma,
+            d_grad_eps, S(stream));
     return GFDFD_OK;
     GUARD_END
 }

@@ -31,21 +31,91 @@ __global__ void k_sample(int n, int nshots, int ntr, const int *__restrict__ kre
     }
 }
 
-// src/AC/calc_res_AC.c:49-95 (STF_INV=0, MISFIT=1) and src/AC/RHS_source_AC_adj.c:21-36:
-// res = pobs - pcal in float, muted where pobsr<1e-20 && pobsi<1e-20; the
-// adjoint panel holds conj(res)/DH^2 at k_rec (the last of duplicate receivers
-// wins); per-shot misfit sum |res|^2 (muted included).
-__global__ void k_adjoint(int n, int nshots, int ntr, const int *__restrict__ krec, const cx<float> *__restrict__ pcal, const cx<float> *__restrict__ pobs, float dh,
-                          cx<double> *__restrict__ B, double *__restrict__ l2) {
+// complex float arithmetic as C evaluates it (products as ac-bd, ad+bc; no
+// contraction), for the quantities the reference rounds in complex float
+struct cf32 { float r, i; };
+__device__ __forceinline__ cf32 cf_mul(cf32 a, cf32 b) {
+#pragma clang fp contract(off)
+    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+// libgcc __divsc3 for finite operands (Smith's method)
+__device__ __forceinline__ cf32 cf_div(cf32 x, cf32 y) {
+#pragma clang fp contract(off)
+    float ratio, denom;
+    cf32 q;
+    if (fabsf(y.r) < fabsf(y.i)) {
+        ratio = y.r / y.i;
+        denom = (y.r * ratio) + y.i;
+        q.r = ((x.r * ratio) + x.i) / denom;
+        q.i = ((x.i * ratio) - x.r) / denom;
+    } else {
+        ratio = y.i / y.r;
+        denom = (y.i * ratio) + y.r;
+        q.r = ((x.i * ratio) + x.r) / denom;
+        q.i = (x.i - (x.r * ratio)) / denom;
+    }
+    return q;
+}
+// cabsf / cargf / clogf, each rounded once from a double evaluation
+__device__ __forceinline__ float cf_abs(cf32 a) { return (float)sqrt((double)a.r * a.r + (double)a.i * a.i); }
+__device__ __forceinline__ float cf_arg(cf32 a) { return (float)atan2((double)a.i, (double)a.r); }
+__device__ __forceinline__ cf32 cf_log(cf32 a) {
+    return {(float)log(sqrt((double)a.r * a.r + (double)a.i * a.i)), cf_arg(a)};
+}
+
+// src/AC/calc_res_AC.c:10-95 and src/AC/RHS_source_AC_adj.c:21-36, one
+// workgroup per shot.  With stf_inv, lane 0 forms the Wiener quotient
+// sum conj(pcal) pobs / sum conj(pobs) pobs in the reference's order and
+// complex float rounding (:21-40) and publishes it (waveAC.stfr/stfi) in
+// stf[s]; else the estimate is 1.  The residual per receiver: rtm, the observed
+// datum (INVMAT=2, :77); misfit 1, (pobs - wien pcal)/|wien| (:62); misfit 2,
+// clog(pobs) - clog(pcal) (:66); misfit 3, arg(pcal) - arg(pobs) (:70); muted
+// where pobsr<1e-20 && pobsi<1e-20 (:82-85, signed).  The adjoint panel holds
+// conj(res)/DH^2 at k_rec (the last of duplicate receivers wins); per-shot
+// misfit sum |res|^2 with the muted residuals included (:88).
+__global__ void k_adjoint(int n, int nshots, int ntr, const int *__restrict__ krec, const cx<float> *__restrict__ pcal,
+                          const cx<float> *__restrict__ pobs, float dh, int misfit, int stf_inv, int rtm,
+                          cx<double> *__restrict__ B, double *__restrict__ l2, cx<float> *__restrict__ stf) {
+#pragma clang fp contract(off)
     const int s = blockIdx.x;
     cx<double> *b = B + (int64_t)s * n;
     for (int row = threadIdx.x; row < n; row += blockDim.x) b[row] = {0.0, 0.0};
+    __shared__ cf32 wien;
+    __shared__ float wabs;
+    if (threadIdx.x == 0) {
+        cf32 w = {1.0f, 0.0f};
+        if (stf_inv) {
+            cf32 nom = {0.0f, 0.0f}, den = {0.0f, 0.0f};
+            for (int t = 0; t < ntr; t++) {
+                const cx<float> o = pobs[(int64_t)s * ntr + t], c = pcal[(int64_t)s * ntr + t];
+                const cf32 a = cf_mul({c.x, -c.y}, {o.x, o.y}), d = cf_mul({o.x, -o.y}, {o.x, o.y});
+                nom = {nom.r + a.r, nom.i + a.i};
+                den = {den.r + d.r, den.i + d.i};
+            }
+            w = cf_div(nom, den);
+        }
+        wien = w;
+        wabs = cf_abs(w);
+        if (stf) stf[s] = {w.r, w.i};
+    }
     __syncthreads();
     __shared__ double part[256];
     double acc = 0.0;
     for (int t = threadIdx.x; t < ntr; t += blockDim.x) {
-        cx<float> o = pobs[(int64_t)s * ntr + t], c = pcal[(int64_t)s * ntr + t];
-        float rr = o.x - c.x, ri = o.y - c.y;
+        const cx<float> o = pobs[(int64_t)s * ntr + t], c = pcal[(int64_t)s * ntr + t];
+        cf32 res;
+        if (rtm) {
+            res = {o.x, o.y};
+        } else if (misfit == 2) {
+            const cf32 lo = cf_log({o.x, o.y}), lc = cf_log({c.x, c.y});
+            res = {lo.r - lc.r, lo.i - lc.i};
+        } else if (misfit == 3) {
+            res = {cf_arg({c.x, c.y}) - cf_arg({o.x, o.y}), 0.0f};
+        } else {
+            const cf32 wc = cf_mul(wien, {c.x, c.y});
+            res = {(o.x - wc.r) / wabs, (o.y - wc.i) / wabs};
+        }
+        float rr = res.r, ri = res.i;
         acc += (double)rr * rr + (double)ri * ri;
         if ((double)o.x < 1e-20 && (double)o.y < 1e-20) { rr = 0.0f; ri = 0.0f; }
         bool last = true;
@@ -62,22 +132,56 @@ __global__ void k_adjoint(int n, int nshots, int ntr, const int *__restrict__ kr
     if (threadIdx.x == 0) l2[s] = part[0];
 }
 
+// Per-shot weights wien/|wien| of ass_grad_AC/TE (src/AC/ass_grad_AC.c:30,
+// src/TE/ass_grad_TE.c:31-35), staged 256 shots at a time in LDS; 1 without
+// an estimate.  Each workgroup of 256 cells walks all shots this way.
+constexpr int kGradShotChunk = 256;
+__device__ __forceinline__ void stage_stf_weights(const cx<float> *__restrict__ stf, int s0, int nshots, double *wr,
+                                                  double *wi) {
+#pragma clang fp contract(off)
+    const int s = s0 + (int)threadIdx.x;
+    if (threadIdx.x < kGradShotChunk && s < nshots) {
+        if (stf) {
+            const cf32 w = {stf[s].x, stf[s].y};
+            const float a = cf_abs(w);
+            wr[threadIdx.x] = (double)(w.r / a);
+            wi[threadIdx.x] = (double)(w.i / a);
+        } else {
+            wr[threadIdx.x] = 1.0;
+            wi[threadIdx.x] = 0.0;
+        }
+    }
+}
+
 // src/AC/ass_grad_AC.c:30-31 summed over shots in FP64: cell q uses the
 // solutions at q+1 (src/vec2mat.c:19-24); the last cell reads past the
-// solution in the reference and gets 0 here.
-__global__ void k_grad_ac(int n, int nshots, double o2r, double o2i, const cx<double> *__restrict__ U,
-                          const cx<double> *__restrict__ L, const float *__restrict__ vp, const float *__restrict__ rho,
-                          double *__restrict__ grad) {
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        if (q + 1 >= n) continue;
-        double acc = 0.0;
-        for (int s = 0; s < nshots; s++) {
-            cx<double> u = U[q + 1 + (int64_t)s * n], l = L[q + 1 + (int64_t)s * n];
-            double pr = u.x * l.x - u.y * l.y, pi = u.x * l.y + u.y * l.x;
-            acc += o2r * pr - o2i * pi;
-        }
-        double v = vp[q];
-        grad[q] += -2.0 * acc / ((double)rho[q] * v * v * v);
+// solution in the reference and gets 0 here.  Shot s is weighted by
+// wien_s/|wien_s| (stf NULL: 1), so the sum is Re(Omega^2 sum_s w_s u_s lam_s).
+// One thread per cell: the grid covers all n cells.
+__global__ void __launch_bounds__(256) k_grad_ac(int n, int nshots, double o2r, double o2i,
+                                                 const cx<double> *__restrict__ U, const cx<double> *__restrict__ L,
+                                                 const float *__restrict__ vp, const float *__restrict__ rho,
+                                                 const cx<float> *__restrict__ stf, double *__restrict__ grad) {
+    __shared__ double wr[kGradShotChunk], wi[kGradShotChunk];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = q + 1 < n;
+    double ar = 0.0, ai = 0.0;
+    for (int s0 = 0; s0 < nshots; s0 += kGradShotChunk) {
+        stage_stf_weights(stf, s0, nshots, wr, wi);
+        __syncthreads();
+        const int cnt = min(kGradShotChunk, nshots - s0);
+        if (on)
+            for (int t = 0; t < cnt; t++) {
+                const cx<double> u = U[q + 1 + (int64_t)(s0 + t) * n], l = L[q + 1 + (int64_t)(s0 + t) * n];
+                const double pr = u.x * l.x - u.y * l.y, pi = u.x * l.y + u.y * l.x;
+                ar += wr[t] * pr - wi[t] * pi;
+                ai += wr[t] * pi + wi[t] * pr;
+            }
+        __syncthreads();
+    }
+    if (on) {
+        const double v = vp[q];
+        grad[q] += -2.0 * (o2r * ar - o2i * ai) / ((double)rho[q] * v * v * v);
     }
 }
 
@@ -383,19 +487,32 @@ __global__ void k_grad_frame(int nx, int ny, int nx0, int ny0, int npml, int fss
     }
 }
 
-// src/TE/ass_grad_TE.c:31-35 summed over shots in FP64:
-// grad_sigma = Re(u (i Omega) lam), grad_eps = Re(Omega^2 u lam)
-__global__ void k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
-                          const cx<double> *__restrict__ U, const cx<double> *__restrict__ L,
-                          double *__restrict__ gsig, double *__restrict__ geps) {
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        if (q + 1 >= n) continue;
-        double ar = 0.0, ai = 0.0;
-        for (int s = 0; s < nshots; s++) {
-            cx<double> u = U[q + 1 + (int64_t)s * n], l = L[q + 1 + (int64_t)s * n];
-            ar += u.x * l.x - u.y * l.y;
-            ai += u.x * l.y + u.y * l.x;
-        }
+// src/TE/ass_grad_TE.c:31-35 summed over shots in FP64, shot s weighted by
+// wien_s/|wien_s| (stf NULL: 1): with z = sum_s w_s u_s lam_s,
+// grad_sigma += Re((i Omega) z), grad_eps += Re(Omega^2 z).  Cell q uses the
+// solutions at q+1; the last cell is left unchanged.  One thread per cell.
+__global__ void __launch_bounds__(256) k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
+                                                 const cx<double> *__restrict__ U, const cx<double> *__restrict__ L,
+                                                 const cx<float> *__restrict__ stf, double *__restrict__ gsig,
+                                                 double *__restrict__ geps) {
+    __shared__ double wr[kGradShotChunk], wi[kGradShotChunk];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = q + 1 < n;
+    double ar = 0.0, ai = 0.0;
+    for (int s0 = 0; s0 < nshots; s0 += kGradShotChunk) {
+        stage_stf_weights(stf, s0, nshots, wr, wi);
+        __syncthreads();
+        const int cnt = min(kGradShotChunk, nshots - s0);
+        if (on)
+            for (int t = 0; t < cnt; t++) {
+                const cx<double> u = U[q + 1 + (int64_t)(s0 + t) * n], l = L[q + 1 + (int64_t)(s0 + t) * n];
+                const double pr = u.x * l.x - u.y * l.y, pi = u.x * l.y + u.y * l.x;
+                ar += wr[t] * pr - wi[t] * pi;
+                ai += wr[t] * pi + wi[t] * pr;
+            }
+        __syncthreads();
+    }
+    if (on) {
         // Re((i Omega) z) = -(omr*ai + omi*ar);  Re(Omega^2 z) = o2r*ar - o2i*ai
         gsig[q] += -(omr * ai + omi * ar);
         geps[q] += o2r * ar - o2i * ai;
@@ -466,15 +583,17 @@ void sample_receivers(int n, int nshots, int ntr, const int *krec, const cx<doub
 }
 
 void adjoint_sources(int n, int nshots, int ntr, const int *krec, const cx<float> *pcal, const cx<float> *pobs,
-                     float dh, cx<double> *B, double *l2, hipStream_t st) {
-    hipLaunchKernelGGL(k_adjoint, dim3(nshots), dim3(256), 0, st, n, nshots, ntr, krec, pcal, pobs, dh, B, l2);
+                     float dh, int misfit, int stf_inv, int rtm, cx<double> *B, double *l2, cx<float> *stf,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_adjoint, dim3(nshots), dim3(256), 0, st, n, nshots, ntr, krec, pcal, pobs, dh, misfit,
+                       stf_inv, rtm, B, l2, stf);
     GFDFD_HIP(hipGetLastError());
 }
 
 void grad_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const cx<double> *L, const float *vp,
-             const float *rho, double *grad, hipStream_t st) {
-    hipLaunchKernelGGL(k_grad_ac, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, (double)omega2[0],
-                       (double)omega2[1], U, L, vp, rho, grad);
+             const float *rho, const cx<float> *stf, double *grad, hipStream_t st) {
+    hipLaunchKernelGGL(k_grad_ac, dim3((n + 255) / 256), dim3(256), 0, st, n, nshots, (double)omega2[0],
+                       (double)omega2[1], U, L, vp, rho, stf, grad);
     GFDFD_HIP(hipGetLastError());
 }
 
@@ -516,9 +635,9 @@ void hess_shin_te(int n, int nshots, const float omega[2], const float omega2[2]
 }
 
 void grad_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U,
-             const cx<double> *L, double *gsig, double *geps, hipStream_t st) {
-    hipLaunchKernelGGL(k_grad_te, dim3(grid_for(n)), dim3(256), 0, st, n, nshots, (double)omega[0], (double)omega[1],
-                       (double)omega2[0], (double)omega2[1], U, L, gsig, geps);
+             const cx<double> *L, const cx<float> *stf, double *gsig, double *geps, hipStream_t st) {
+    hipLaunchKernelGGL(k_grad_te, dim3((n + 255) / 256), dim3(256), 0, st, n, nshots, (double)omega[0],
+                       (double)omega[1], (double)omega2[0], (double)omega2[1], U, L, stf, gsig, geps);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -8,9 +8,10 @@ void spike_panel(int n, int nshots, const int *k, cx<double> amp, cx<double> *B,
 void sample_receivers(int n, int nshots, int ntr, const int *krec, const cx<double> *X, cx<float> *pcal,
                       hipStream_t st);
 void adjoint_sources(int n, int nshots, int ntr, const int *krec, const cx<float> *pcal, const cx<float> *pobs,
-                     float dh, cx<double> *B, double *l2, hipStream_t st);
+                     float dh, int misfit, int stf_inv, int rtm, cx<double> *B, double *l2, cx<float> *stf,
+                     hipStream_t st);
 void grad_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const cx<double> *L, const float *vp,
-             const float *rho, double *grad, hipStream_t st);
+             const float *rho, const cx<float> *stf, double *grad, hipStream_t st);
 void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const float *vp, const float *rho,
                   double *hess, hipStream_t st);
 void apply_hess(int n, float eps, const double *hess, double *grad, hipStream_t st);
@@ -34,6 +35,6 @@ void hess_gn_te(int n, int nshots, int nrec, const float omega[2], const float o
 void hess_shin_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U, double *hsig,
                   double *heps, hipStream_t st);
 void grad_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U,
-             const cx<double> *L, double *gsig, double *geps, hipStream_t st);
+             const cx<double> *L, const cx<float> *stf, double *gsig, double *geps, hipStream_t st);
 
 }  // namespace gfdfd

@@ -258,14 +258,28 @@ def sample(X, krec, stream=None):
 
 
 def adjoint_sources(n, krec, pcal, pobs, dh, stream=None):
+    """calc_res_AC (MISFIT=1, STF_INV=0) and RHS_source_AC_adj for all shots:
+    returns (adjoint panel [nshots, n] complex128, per-shot sum |res|^2)."""
+    B, l2, _ = residual_sources(n, krec, pcal, pobs, dh, stream=stream)
+    return B, l2
+
+
+def residual_sources(n, krec, pcal, pobs, dh, misfit=1, stf_inv=0, rtm=False, stream=None):
+    """calc_res_AC in full (src/AC/calc_res_AC.c:10-95) with RHS_source_AC_adj
+    for all shots: misfit 1 (L2), 2 (logarithmic) or 3 (phase only); stf_inv=1
+    estimates each shot's source-time function by Wiener deconvolution; rtm
+    takes the observed data as the residual.  Returns (adjoint panel [nshots, n]
+    complex128, per-shot sum |res|^2 float64, per-shot estimate complex64)."""
     import torch
     nshots, ntr = pcal.shape
     k = torch.as_tensor(np.asarray(krec, np.int32), device="cuda")
     B = torch.empty((nshots, n), dtype=torch.complex128, device="cuda")
     l2 = torch.empty(nshots, dtype=torch.float64, device="cuda")
-    check(lib().gfdfd_adjoint_sources(n, nshots, ntr, _ptr(k), _ptr(pcal.contiguous()), _ptr(pobs.contiguous()), dh,
-                                      _ptr(B), _ptr(l2), _stream(stream)), "gfdfd_adjoint_sources")
-    return B, l2
+    stf = torch.empty(nshots, dtype=torch.complex64, device="cuda")
+    check(lib().gfdfd_residual_sources(n, nshots, ntr, _ptr(k), _ptr(pcal.contiguous()), _ptr(pobs.contiguous()), dh,
+                                       int(misfit), int(stf_inv), int(bool(rtm)), _ptr(B), _ptr(l2), _ptr(stf),
+                                       _stream(stream)), "gfdfd_residual_sources")
+    return B, l2, stf
 
 
 def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
@@ -593,11 +607,14 @@ def grad_frame(grad, nx0, ny0, npml, free_surf, stream=None):
 
 
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-             precision=FP64, tol=1e-10, max_iter=10, rtm=False):
+             precision=FP64, tol=1e-10, max_iter=10, rtm=False, misfit=1, stf_inv=0):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,
     src/TE/grad_obj_TE.c:53-172).  pobs: list per frequency of complex64 cuda [nshots, ntr].
     Returns (L2 = 0.5 * sum |res|^2, grad (AC: [n] float64; TE: (grad_sigma, grad_eps))).
 
+    misfit (MISFIT 1/2/3) and stf_inv (STF_INV) select calc_res_AC's residual
+    and the Wiener source-time-function estimate, whose phase weights each
+    shot's gradient (src/AC/ass_grad_AC.c:30-31, src/TE/ass_grad_TE.c:31-35).
     rtm=True is the reverse-time-migration pass (RTM_FD_AC, src/AC/RTM_FD_AC.c:190):
     the residual is the observed data itself (src/AC/calc_res_AC.c:77), with the
     same mute, so the returned gradient is the image before precond()."""
@@ -617,9 +634,7 @@ def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=
         else:
             U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
         pcal = sample(U, krec)
-        if rtm:                              # res = pobs - 0 = pobs exactly
-            pcal = torch.zeros_like(pcal)
-        BA, l2s = adjoint_sources(n, krec, pcal, pobs[f], dh)
+        BA, l2s, stf = residual_sources(n, krec, pcal, pobs[f], dh, misfit=misfit, stf_inv=stf_inv, rtm=rtm)
         l2 += float(l2s.sum())
         if precision == FP64 and tol is None:
             LAM = F.solve(f, BA, sys=adj_sys)
@@ -628,12 +643,13 @@ def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=
         nshots = len(ksrc)
         if physics == AC:
             o2 = omega2_grad_ac(fr, s)
-            check(lib().gfdfd_grad_ac(n, nshots, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(m1.contiguous()),
-                                      _ptr(m2.contiguous()), _ptr(g1), _stream(None)), "gfdfd_grad_ac")
+            check(lib().gfdfd_grad_ac_stf(n, nshots, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(m1.contiguous()),
+                                          _ptr(m2.contiguous()), _ptr(stf), _ptr(g1), _stream(None)),
+                  "gfdfd_grad_ac_stf")
         elif physics == TE:
             o2, om = omega(TE, fr, s)
-            check(lib().gfdfd_grad_te(n, nshots, om.ctypes.data, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(g1),
-                                      _ptr(g2), _stream(None)), "gfdfd_grad_te")
+            check(lib().gfdfd_grad_te_stf(n, nshots, om.ctypes.data, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(stf),
+                                          _ptr(g1), _ptr(g2), _stream(None)), "gfdfd_grad_te_stf")
         else:
             raise ValueError("SH has no gradient path in the reference (src/germaine.c:112-120)")
     grad = g1 if physics == AC else (g1, g2)

@@ -173,6 +173,17 @@ int gfdfd_sample(int n, int nshots, int ntr, const int *d_krec, const double *d_
  * d_l2[s] = sum |pobs - pcal|^2 (double).                                   */
 int gfdfd_adjoint_sources(int n, int nshots, int ntr, const int *d_krec, const float *d_pcal, const float *d_pobs,
                           float dh, double *d_b, double *d_l2, void *stream);
+/* calc_res_AC in full (src/AC/calc_res_AC.c:10-95) with RHS_source_AC_adj,
+ * for all shots.  misfit 1 (L2, :62), 2 (logarithmic, clog(pobs) -
+ * clog(pcal), :66) or 3 (phase only, arg(pcal) - arg(pobs), :70); rtm != 0
+ * is INVMAT=2 (residual = observed data, :77).  stf_inv = 1 estimates each
+ * shot's source-time function as the Wiener quotient sum conj(pcal) pobs /
+ * sum conj(pobs) pobs in complex float (:21-40; else 1) and writes it to
+ * d_stf[2s..2s+1] (complex float, may be NULL) for gfdfd_grad_*_stf.  Mute,
+ * duplicate receivers and d_l2 as gfdfd_adjoint_sources.                  */
+int gfdfd_residual_sources(int n, int nshots, int ntr, const int *d_krec, const float *d_pcal, const float *d_pobs,
+                           float dh, int misfit, int stf_inv, int rtm, double *d_b, double *d_l2, float *d_stf,
+                           void *stream);
 /* ass_grad_AC (src/AC/ass_grad_AC.c:30-31), STF_INV=0, summed over the nshots
  * columns of d_u (forward) and d_lam (adjoint) in FP64 and added to d_grad
  * (n doubles).  omega2 from gfdfd_omega2_grad_ac.  Cell q combines the
@@ -180,6 +191,11 @@ int gfdfd_adjoint_sources(int n, int nshots, int ntr, const int *d_krec, const f
  * which reads past the solution in the reference, is left unchanged.        */
 int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam, const float *d_vp,
                   const float *d_rho, double *d_grad, void *stream);
+/* ass_grad_AC with the shots' source-time functions (src/AC/ass_grad_AC.c:21,
+ * 30-31): shot s weighted by wien_s/|wien_s|, wien_s = d_stf[2s] + i
+ * d_stf[2s+1] from gfdfd_residual_sources (NULL: 1).                       */
+int gfdfd_grad_ac_stf(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam,
+                      const float *d_vp, const float *d_rho, const float *d_stf, double *d_grad, void *stream);
 /* hessian_shin_AC (src/AC/hessian_shin_AC.c:117-124): Shin's pseudo-Hessian
  * diagonal, hess += |J|^2 with J = 2 Omega^2 u / (rho vp^3), summed over the
  * nshots forward fields of d_u in FP64 and added to d_hess (n doubles).
@@ -268,6 +284,11 @@ int gfdfd_hess_shin_te(int n, int nshots, const float *omega, const float *omega
 /* ass_grad_TE (src/TE/ass_grad_TE.c:31-35): sigma and epsilon gradients */
 int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, const double *d_u, const double *d_lam,
                   double *d_grad_sigma, double *d_grad_eps, void *stream);
+/* ass_grad_TE with the shots' source-time functions (src/TE/ass_grad_TE.c:21,
+ * 31-35): shot s weighted by wien_s/|wien_s| (d_stf as gfdfd_grad_ac_stf).  */
+int gfdfd_grad_te_stf(int n, int nshots, const float *omega, const float *omega2, const double *d_u,
+                      const double *d_lam, const float *d_stf, double *d_grad_sigma, double *d_grad_eps,
+                      void *stream);
 
 /* ---- measurement ------------------------------------------------------------
  * Per-kernel-class timing with HIP events on the launch stream, plus the

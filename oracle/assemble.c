@@ -375,23 +375,48 @@ void orc_source_amp(int physics, float freq, float s, float dh, double *re, doub
     *im = amp2 / (dh * dh);
 }
 
-/* src/AC/calc_res_AC.c:49-95 with STF_INV=0, INVMAT=1, MISFIT=1:
- * res = pobs - pcal (float complex), muted where pobsr<1e-20 && pobsi<1e-20.
- * Returns sum |res|^2 over ALL receivers (muted ones included, :88), with the
- * accumulator starting at zero (the reference leaves it uninitialised).      */
-float orc_residual(int ntr, const float *pcalr, const float *pcali, const float *pobsr,
-                   const float *pobsi, float *presr, float *presi)
+/* src/AC/calc_res_AC.c:10-95 in full, for one shot.  misfit 1 (L2), 2
+ * (logarithmic, :66), 3 (phase only, :70); rtm != 0 is INVMAT=2 (res = pobs,
+ * :77); stf_inv != 0 estimates the source-time function by the Wiener quotient
+ * sum conj(pcal) pobs / sum conj(pobs) pobs in complex float (:21-40), else it
+ * is 1 (:43-45).  The estimate is returned in stf[0..1] (waveAC.stfr/stfi,
+ * :47-48, which ass_grad_AC/TE read).  Residuals are muted where
+ * pobsr<1e-20 && pobsi<1e-20 (:82-85, signed); the returned sum |res|^2 runs
+ * over ALL receivers (muted ones included, :88), starting at zero (with
+ * STF_INV=0 the reference leaves the accumulator uninitialised).             */
+float orc_residual_ex(int ntr, int misfit, int stf_inv, int rtm, const float *pcalr, const float *pcali,
+                      const float *pobsr, const float *pobsi, float *presr, float *presi, float *stf)
 {
     cf wien = 1.0 + 0.0 * I;
     float l2 = 0.0;
+    if (stf_inv) {
+        cf nom = 0.0 + 0.0 * I, den = 0.0 + 0.0 * I;
+        for (int i = 0; i < ntr; i++) {
+            nom += conj(pcalr[i] + pcali[i] * I) * (pobsr[i] + pobsi[i] * I);
+            den += conj(pobsr[i] + pobsi[i] * I) * (pobsr[i] + pobsi[i] * I);
+        }
+        wien = nom / den;
+    }
+    if (stf) { stf[0] = creal(wien); stf[1] = cimag(wien); }
     for (int i = 0; i < ntr; i++) {
-        cf res = ((pobsr[i] + pobsi[i] * I) - (wien * (pcalr[i] + pcali[i] * I))) / cabsf(wien);
+        cf obs = pobsr[i] + pobsi[i] * I, cal = pcalr[i] + pcali[i] * I, res = 0;
+        if (rtm) res = obs;
+        else if (misfit == 1) res = (obs - (wien * cal)) / cabsf(wien);
+        else if (misfit == 2) res = clogf(obs) - clogf(cal);
+        else if (misfit == 3) res = cargf(cal) - cargf(obs);
         presr[i] = creal(res);
         presi[i] = cimag(res);
         if ((pobsr[i] < 1e-20) && (pobsi[i] < 1e-20)) { presr[i] = 0.0; presi[i] = 0.0; }
         l2 += creal(res * conj(res));
     }
     return l2;
+}
+
+/* MISFIT=1, STF_INV=0, INVMAT=1: res = pobs - pcal */
+float orc_residual(int ntr, const float *pcalr, const float *pcali, const float *pobsr,
+                   const float *pobsi, float *presr, float *presi)
+{
+    return orc_residual_ex(ntr, 1, 0, 0, pcalr, pcali, pobsr, pobsi, presr, presi, NULL);
 }
 
 /* src/AC/RHS_source_AC_adj.c:21-36: conj(res)/DH^2 at k_rec (last write wins) */
@@ -405,7 +430,7 @@ void orc_adjoint_rhs(int n, int ntr, const int *krec, const float *presr, const 
     }
 }
 
-/* src/AC/ass_grad_AC.c:19-33 (STF_INV=0 so wien=1): one shot's gradient at
+/* src/AC/ass_grad_AC.c:19-33: one shot's gradient at
  * every cell, accumulated in float like :46-52.  u, lam are the float fields
  * the reference derives with vec2mat (src/vec2mat.c:17-25): cell q reads the
  * FP64 solution at q+1; the last cell reads x[n], which no solve writes, so it
@@ -413,9 +438,9 @@ void orc_adjoint_rhs(int n, int ntr, const int *krec, const float *presr, const 
 void orc_grad_ac_accumulate(int nx, int ny, float freq, float s,
                             const double *ur, const double *ui, double u_last_r, double u_last_i,
                             const double *lr, const double *li, double l_last_r, double l_last_i,
-                            const float *vp, const float *rho, float *grad)
+                            const float *vp, const float *rho, float stfr, float stfi, float *grad)
 {
-    cf wien = 1.0f + 0.0f * I;
+    cf wien = stfr + stfi * I;   /* waveAC.stfr/stfi of the shot (1 with STF_INV=0) */
     cf omega2 = cpowf(((2.0 * M_PI * freq) - (I * s)), 2.0);
     size_t n = (size_t)nx * ny;
     for (size_t q = 0; q < n; q++) {
@@ -689,13 +714,14 @@ void orc_hess_shin_te_accumulate(int nx, int ny, float freq, float s,
     }
 }
 
-/* src/TE/ass_grad_TE.c:19-56 (STF_INV=0): sigma and epsilon gradients */
+/* src/TE/ass_grad_TE.c:19-56: sigma and epsilon gradients of one shot,
+ * weighted by the shot's source-time-function phase wien/|wien|            */
 void orc_grad_te_accumulate(int nx, int ny, float freq, float s,
                             const double *ur, const double *ui, double u_last_r, double u_last_i,
                             const double *lr, const double *li, double l_last_r, double l_last_i,
-                            float *grad_sigma, float *grad_eps)
+                            float stfr, float stfi, float *grad_sigma, float *grad_eps)
 {
-    cf wien = 1.0f + 0.0f * I;
+    cf wien = stfr + stfi * I;
     cf omega = 2.0 * M_PI * (freq + (I * s));
     cf omega2 = cpowf(2.0 * M_PI * (freq + (I * s)), 2.0);
     size_t n = (size_t)nx * ny;

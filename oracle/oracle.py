@@ -55,6 +55,8 @@ def lib():
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_residual.restype = C.c_float
         L.orc_residual.argtypes = [C.c_int] + [f(np.float32)] * 6
+        L.orc_residual_ex.restype = C.c_float
+        L.orc_residual_ex.argtypes = [C.c_int] * 4 + [f(np.float32)] * 6 + [f(np.float32)]
         L.orc_adjoint_rhs.restype = None
         L.orc_adjoint_rhs.argtypes = [C.c_int, C.c_int, f(np.int32), f(np.float32), f(np.float32),
                                       C.c_float, f(np.float64), f(np.float64)]
@@ -62,7 +64,7 @@ def lib():
         L.orc_grad_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
-                                             f(np.float32), f(np.float32), f(np.float32)]
+                                             f(np.float32), f(np.float32), C.c_float, C.c_float, f(np.float32)]
         L.orc_hess_shin_ac_accumulate.restype = None
         L.orc_hess_shin_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                                   f(np.float64), f(np.float64), C.c_double, C.c_double,
@@ -102,7 +104,7 @@ def lib():
         L.orc_grad_te_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
-                                             f(np.float32), f(np.float32)]
+                                             C.c_float, C.c_float, f(np.float32), f(np.float32)]
         L.orc_band_solve.restype = C.c_int
         L.orc_band_solve.argtypes = [C.c_int, f(np.int32), f(np.int32), f(np.float64), f(np.float64),
                                      C.c_void_p, C.c_int, C.c_int, f(np.float64), f(np.float64),
@@ -219,6 +221,21 @@ def residual(pcal, pobs):
     return rr + 1j * ri, l2
 
 
+def residual_ex(pcal, pobs, misfit=1, stf_inv=0, rtm=0):
+    """calc_res_AC for one shot (src/AC/calc_res_AC.c:10-95): returns
+    (muted residuals, sum |res|^2, Wiener source-time-function estimate)."""
+    pcal = np.asarray(pcal, np.complex64)
+    pobs = np.asarray(pobs, np.complex64)
+    n = len(pcal)
+    rr = np.zeros(n, np.float32)
+    ri = np.zeros(n, np.float32)
+    stf = np.zeros(2, np.float32)
+    l2 = lib().orc_residual_ex(n, misfit, stf_inv, rtm, np.ascontiguousarray(pcal.real),
+                               np.ascontiguousarray(pcal.imag), np.ascontiguousarray(pobs.real),
+                               np.ascontiguousarray(pobs.imag), rr, ri, stf)
+    return rr + 1j * ri, l2, complex(stf[0], stf[1])
+
+
 def adjoint_rhs(n, krec, res, dh):
     res = np.asarray(res, np.complex64)
     rr = np.zeros(n)
@@ -324,21 +341,21 @@ def hess_shin_te_accumulate(nx, ny, freq, s, u, u_last, hess_sigma, hess_eps):
     return hess_sigma, hess_eps
 
 
-def grad_ac_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, vp, rho, grad):
+def grad_ac_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, vp, rho, grad, stf=1 + 0j):
     lib().orc_grad_ac_accumulate(nx, ny, freq, s,
                                  np.ascontiguousarray(u.real), np.ascontiguousarray(u.imag),
                                  u_last.real, u_last.imag,
                                  np.ascontiguousarray(lam.real), np.ascontiguousarray(lam.imag),
                                  lam_last.real, lam_last.imag,
                                  np.ascontiguousarray(vp, np.float32).ravel(),
-                                 np.ascontiguousarray(rho, np.float32).ravel(), grad)
+                                 np.ascontiguousarray(rho, np.float32).ravel(), stf.real, stf.imag, grad)
     return grad
 
 
-def grad_te_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, gsig, geps):
+def grad_te_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, gsig, geps, stf=1 + 0j):
     lib().orc_grad_te_accumulate(nx, ny, freq, s,
                                  np.ascontiguousarray(u.real), np.ascontiguousarray(u.imag),
                                  u_last.real, u_last.imag,
                                  np.ascontiguousarray(lam.real), np.ascontiguousarray(lam.imag),
-                                 lam_last.real, lam_last.imag, gsig, geps)
+                                 lam_last.real, lam_last.imag, stf.real, stf.imag, gsig, geps)
     return gsig, geps

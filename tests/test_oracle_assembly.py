@@ -6,6 +6,9 @@ neighbour index, a dropped PML factor, a sign, the transpose -- fails it:
 * an independent Kronecker-product construction of the Hustedt mixed-grid
   PML operator (rotated + standard Laplacian, lumped 9-point mass), built in
   float64 from 1-D difference operators, against the per-point triplets;
+* for SH with varying mu, a float64 flux-form construction (face and corner
+  fluxes with 2- and 4-point arithmetic averages of mu), with the reference's
+  one departure from that form (its N entry) stated explicitly;
 * the closed-form interior coefficients of Hustedt et al. (2004) for PML = 1;
 * the closed-form nonzero count 9N - 6nx - 6ny + 4;
 * a float64 closed form of the classical PML profile;
@@ -170,6 +173,119 @@ def test_sh_constant_mu_matches_kronecker():
     mu = np.float32(np.float32(2000.0) * np.float32(1500.0)) * np.float32(1500.0)
     K = kron_operator(nx, ny, dh, o2[0] + 1j * o2[1], rho.astype(np.float64), np.full((ny, nx), float(mu)), px, py)
     assert_rowwise_close(A, K, 2e-6)
+
+
+def sh_flux_operator(nx, ny, dh, omega2, rho, mu, px, py, n_uses_bm0=True):
+    """float64 variable-coefficient SH operator in flux form, built per grid point.
+
+    Standard Laplacian: face fluxes with the 2-point arithmetic averages of mu
+    at (i+-1/2, j) and (i, j+-1/2).  Rotated Laplacian: one flux per cell
+    corner (i+-1/2, j+-1/2) with the 4-point average of mu there, its x part
+    dx0*dx_c*mu_c*[(u(i+sx,j) - u(i,j)) + (u(i+sx,j+sy) - u(i,j+sy))]/(4h^2) and
+    its y part the same with the axes exchanged.  mu outside the grid is the
+    nearest edge value (src/SH/init_mat_SH.c:28-32).  Entries whose column
+    falls outside the grid are dropped; the centre keeps its whole sum.
+
+    n_uses_bm0: the reference's N entry multiplies its standard-Laplacian
+    flux by the x average mu(i-1/2, j) rather than mu(i, j-1/2)
+    (src/SH/init_A_SH_9p_pml.c:94, `(*matSH).bm0 * dym`), while the centre
+    uses mu(i, j-1/2) (:144).  The flag states that one departure from the
+    flux form explicitly; every other coefficient follows the mathematics.
+    """
+    a, b, c, d, e = weights()
+    h2i = 1.0 / (dh * dh)
+    mu = mu.astype(np.float64)
+    M = np.empty((ny + 2, nx + 2))
+    M[1:-1, 1:-1] = mu
+    M[1:-1, 0], M[1:-1, -1] = mu[:, 0], mu[:, -1]
+    M[0, :], M[-1, :] = M[1, :], M[-2, :]
+    x0, xh = px[0] + 1j * px[1], px[2] + 1j * px[3]   # [0..nx+1]
+    y0, yh = py[0] + 1j * py[1], py[2] + 1j * py[3]
+    rows, cols, vals = [], [], []
+    for j in range(1, ny + 1):
+        for i in range(1, nx + 1):
+            k = (i - 1) + (j - 1) * nx
+            ent = {}
+
+            def add(ii, jj, v):
+                ent[(ii, jj)] = ent.get((ii, jj), 0.0) + v
+
+            # standard Laplacian (weight e)
+            fx = {+1: 0.5 * (M[j, i] + M[j, i + 1]) * xh[i], -1: 0.5 * (M[j, i] + M[j, i - 1]) * xh[i - 1]}
+            fy = {+1: 0.5 * (M[j, i] + M[j + 1, i]) * yh[j], -1: 0.5 * (M[j, i] + M[j - 1, i]) * yh[j - 1]}
+            for s in (+1, -1):
+                add(i + s, j, e * h2i * x0[i] * fx[s])
+                add(i, j, -e * h2i * x0[i] * fx[s])
+                coef = fy[s]
+                if s == -1 and n_uses_bm0:
+                    coef = 0.5 * (M[j, i] + M[j, i - 1]) * yh[j - 1]
+                add(i, j + s, e * h2i * y0[j] * coef)
+                add(i, j, -e * h2i * y0[j] * fy[s])
+            # rotated Laplacian (weight d): one flux per corner
+            for sx in (+1, -1):
+                for sy in (+1, -1):
+                    mc = 0.25 * (M[j, i] + M[j, i + sx] + M[j + sy, i] + M[j + sy, i + sx])
+                    cx_ = 0.25 * h2i * x0[i] * (xh[i] if sx > 0 else xh[i - 1]) * mc
+                    cy_ = 0.25 * h2i * y0[j] * (yh[j] if sy > 0 else yh[j - 1]) * mc
+                    for (ii, jj), sg in (((i + sx, j), 1), ((i, j), -1), ((i + sx, j + sy), 1), ((i, j + sy), -1)):
+                        add(ii, jj, d * cx_ * sg)
+                    for (ii, jj), sg in (((i, j + sy), 1), ((i, j), -1), ((i + sx, j + sy), 1), ((i + sx, j), -1)):
+                        add(ii, jj, d * cy_ * sg)
+            # lumped mass of the column point
+            wts = {0: a, 1: 0.25 * b, 2: 0.25 * c}
+            for dj in (-1, 0, 1):
+                for di in (-1, 0, 1):
+                    ii, jj = i + di, j + dj
+                    if 1 <= ii <= nx and 1 <= jj <= ny:
+                        add(ii, jj, wts[abs(di) + abs(dj)] * omega2 * float(rho[jj - 1, ii - 1]))
+            for (ii, jj), v in ent.items():
+                if 1 <= ii <= nx and 1 <= jj <= ny:
+                    rows.append(k)
+                    cols.append((ii - 1) + (jj - 1) * nx)
+                    vals.append(v)
+    n = nx * ny
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+@pytest.mark.parametrize("s", [0.0, 0.4])
+def test_sh_varying_mu_matches_flux_form(s):
+    """SH with random rho and vs (so mu varies cell to cell) against the flux-form
+    construction: pins the 4-point corner and 2-point face averages of
+    src/SH/av_mu.c:12-28 and their use in src/SH/init_A_SH_9p_pml.c:68-213."""
+    nx, ny, npml, dh, f = 26, 19, 5, 10.0, 7.0
+    rng = np.random.default_rng(11)
+    vs = (1000 + 1500 * rng.random((ny, nx))).astype(np.float32)
+    rho = (1500 + 1000 * rng.random((ny, nx))).astype(np.float32)
+    px, py = O.pml_profiles(nx, ny, npml, dh, 90.0, f, s, 0)
+    A = triplets_to_csr(nx, ny, *O.assemble(O.SH, nx, ny, dh, f, s, vs, rho, px, py))
+    o2, _ = O.omega2(O.SH, f, s)
+    mu = (rho * vs) * vs   # float, as src/SH/init_mat_SH.c:22
+    K = sh_flux_operator(nx, ny, dh, o2[0] + 1j * o2[1], rho, mu, px, py)
+    assert_rowwise_close(A, K, 2e-6)
+    # without the N-entry departure the forms differ exactly in the N entries,
+    # so a wrong neighbour in any average would show
+    K0 = sh_flux_operator(nx, ny, dh, o2[0] + 1j * o2[1], rho, mu, px, py, n_uses_bm0=False)
+    D = abs(A - K0).tocoo()
+    big = D.data > 1e-4 * abs(A).max()
+    assert big.any()
+    assert np.all(D.col[big] == D.row[big] - nx)
+
+
+def test_sh_flux_form_detects_swapped_corner():
+    """A corner average taken at the wrong corner (bpm <-> bmp) in the oracle's
+    mathematics would fail the comparison above: the flux form itself changes."""
+    nx, ny, dh, f = 12, 10, 10.0, 6.0
+    rng = np.random.default_rng(3)
+    vs = (1000 + 1500 * rng.random((ny, nx))).astype(np.float32)
+    rho = np.full((ny, nx), 2000.0, np.float32)
+    px, py = O.pml_profiles(nx, ny, 0, dh, 90.0, f, 0.0, 0)
+    A = triplets_to_csr(nx, ny, *O.assemble(O.SH, nx, ny, dh, f, 0.0, vs, rho, px, py))
+    o2, _ = O.omega2(O.SH, f, 0.0)
+    mu = (rho * vs) * vs
+    # transposed mu field: the averages move to other corners, the operator changes
+    K = sh_flux_operator(nx, ny, dh, o2[0] + 1j * o2[1], rho, mu[::-1, ::-1].copy(), px, py)
+    with pytest.raises(AssertionError):
+        assert_rowwise_close(A, K, 2e-6)
 
 
 def test_interior_hustedt_coefficients():

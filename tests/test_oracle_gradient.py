@@ -221,3 +221,104 @@ def test_apply_hess_known_answer():
     grad = np.array([10.0, 12.0, 16.0, 8.0], np.float32)
     out = O.apply_hess(0.5, hess, grad)
     assert out.tolist() == [2.0, 2.0, 2.0, -2.0]
+
+
+# ---- calc_res_AC's other branches (src/AC/calc_res_AC.c:21-47, 62-77) -------------------
+def _rand_c64(rng, n, scale=1.0):
+    return (scale * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+
+
+def test_residual_log_misfit_closed_form():
+    # MISFIT=2: clog(pobs) - clog(pcal) = (ln|o| - ln|c|) + i (arg o - arg c) for
+    # fields given in polar form, with arguments inside (-pi, pi)
+    rng = np.random.default_rng(21)
+    n = 64
+    ao, ac = rng.uniform(-3, 2, n), rng.uniform(-3, 2, n)
+    bo, bc = rng.uniform(-3.0, 3.0, n), rng.uniform(-3.0, 3.0, n)
+    pobs = (np.exp(ao) * np.exp(1j * bo)).astype(np.complex64)
+    pcal = (np.exp(ac) * np.exp(1j * bc)).astype(np.complex64)
+    res, l2, stf = O.residual_ex(pcal, pobs, misfit=2)
+    want = (ao - ac) + 1j * (bo - bc)
+    muted = (pobs.real < 1e-20) & (pobs.imag < 1e-20)
+    assert muted.any() and not muted.all()
+    np.testing.assert_allclose(res, np.where(muted, 0, want), atol=2e-6 * np.max(np.abs(want)) + 2e-6)
+    assert stf == 1.0
+    np.testing.assert_allclose(l2, np.sum(np.abs(want) ** 2), rtol=2e-5)   # muted entries included
+
+
+def test_residual_phase_misfit_closed_form():
+    # MISFIT=3: arg(pcal) - arg(pobs), a real residual; amplitudes play no part
+    rng = np.random.default_rng(22)
+    n = 50
+    bo, bc = rng.uniform(-3.0, 3.0, n), rng.uniform(-3.0, 3.0, n)
+    pobs = (rng.uniform(0.1, 5, n) * np.exp(1j * bo)).astype(np.complex64)
+    pcal = (rng.uniform(0.1, 5, n) * np.exp(1j * bc)).astype(np.complex64)
+    res, l2, _ = O.residual_ex(pcal, pobs, misfit=3)
+    muted = (pobs.real < 1e-20) & (pobs.imag < 1e-20)
+    np.testing.assert_allclose(res.real, np.where(muted, 0, bc - bo), atol=2e-6)
+    assert np.all(res.imag == 0)
+    np.testing.assert_allclose(l2, np.sum((bc - bo) ** 2), rtol=2e-5)
+
+
+@pytest.mark.parametrize("wabs", [1.0, 2.5])
+def test_residual_wiener_stf_closed_form(wabs):
+    # STF_INV=1 with pobs = w pcal: the quotient sum conj(pcal) pobs / sum |pobs|^2
+    # is w/|w|^2 = 1/conj(w), and the L2 residual (pobs - wien pcal)/|wien| is
+    # pcal |w| (|w|^2 - 1)/conj(w): zero for a pure phase shift
+    rng = np.random.default_rng(23)
+    w = wabs * np.exp(0.7j)
+    pcal = _rand_c64(rng, 80)
+    pobs = (w * pcal.astype(np.complex128)).astype(np.complex64)
+    res, l2, stf = O.residual_ex(pcal, pobs, misfit=1, stf_inv=1)
+    np.testing.assert_allclose(stf, 1 / np.conj(w), rtol=1e-5)
+    want = pcal * abs(w) * (abs(w) ** 2 - 1) / np.conj(w)
+    want[(pobs.real < 1e-20) & (pobs.imag < 1e-20)] = 0
+    np.testing.assert_allclose(res, want, atol=1e-5 * max(1.0, np.max(np.abs(want))))
+
+
+def test_residual_rtm_and_mute():
+    # INVMAT=2: the residual is the observed data; muting still applies, l2 keeps
+    # the muted entries (:82-88); misfit 1 without STF is pobs - pcal exactly
+    rng = np.random.default_rng(24)
+    pcal, pobs = _rand_c64(rng, 40), _rand_c64(rng, 40)
+    pobs[3] = -1.0 - 2.0j                # both parts negative: muted (signed test)
+    res, l2, _ = O.residual_ex(pcal, pobs, rtm=1)
+    muted = (pobs.real < 1e-20) & (pobs.imag < 1e-20)
+    assert muted[3]
+    want = np.where(muted, 0, pobs).astype(np.complex64)
+    assert np.array_equal(res.astype(np.complex64), want)
+    np.testing.assert_allclose(l2, np.sum(np.abs(pobs.astype(np.complex128)) ** 2), rtol=1e-5)
+    r1, l1, _ = O.residual_ex(pcal, pobs, misfit=1)
+    r0, l0 = O.residual(pcal, pobs)
+    assert np.array_equal(r1, r0) and l1 == l0
+    d = np.where(muted, 0, pobs - pcal).astype(np.complex64)
+    assert np.array_equal(r1.astype(np.complex64), d)
+
+
+@pytest.mark.parametrize("phys", ["ac", "te"])
+def test_gradient_stf_phase_is_a_field_rotation(phys):
+    # ass_grad_AC/TE weight the shot by wien/|wien| (src/AC/ass_grad_AC.c:30-31,
+    # src/TE/ass_grad_TE.c:31-35): the gradient with STF w equals the gradient
+    # with STF 1 of the forward field rotated by w/|w|
+    nx, ny, f, s = 9, 7, 6.0, 0.0
+    n = nx * ny
+    rng = np.random.default_rng(25)
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    lam = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    w = np.complex64(0.3 - 1.1j)
+    ph = complex(w) / abs(complex(w))
+    vp = rng.uniform(1500, 3000, (ny, nx)).astype(np.float32)
+    rho = rng.uniform(1000, 2500, (ny, nx)).astype(np.float32)
+    if phys == "ac":
+        g_w = O.grad_ac_accumulate(nx, ny, f, s, u, 0j, lam, 0j, vp, rho, np.zeros(n, np.float32), stf=complex(w))
+        g_r = O.grad_ac_accumulate(nx, ny, f, s, u * ph, 0j, lam, 0j, vp, rho, np.zeros(n, np.float32))
+        np.testing.assert_allclose(g_w, g_r, rtol=1e-4, atol=1e-4 * np.max(np.abs(g_r)))
+        g1 = O.grad_ac_accumulate(nx, ny, f, s, u, 0j, lam, 0j, vp, rho, np.zeros(n, np.float32))
+        assert np.max(np.abs(g1 - g_w)) > 1e-2 * np.max(np.abs(g1))
+    else:
+        gs_w, ge_w = O.grad_te_accumulate(nx, ny, 2e8, s, u, 0j, lam, 0j, np.zeros(n, np.float32),
+                                          np.zeros(n, np.float32), stf=complex(w))
+        gs_r, ge_r = O.grad_te_accumulate(nx, ny, 2e8, s, u * ph, 0j, lam, 0j, np.zeros(n, np.float32),
+                                          np.zeros(n, np.float32))
+        np.testing.assert_allclose(gs_w, gs_r, rtol=1e-4, atol=1e-4 * np.max(np.abs(gs_r)))
+        np.testing.assert_allclose(ge_w, ge_r, rtol=1e-4, atol=1e-4 * np.max(np.abs(ge_r)))
