@@ -308,6 +308,12 @@ int gfdfd_factor(gfdfd_plan plan, int nfreq, const double *d_vals, int precision
     GUARD_END
 }
 
+int gfdfd_factor_tuning(int panel_lds_cap) {
+    if (panel_lds_cap < 0) return report(GFDFD_ERR_ARG, "gfdfd_factor_tuning: negative panel LDS cap");
+    set_factor_tuning(panel_lds_cap);
+    return GFDFD_OK;
+}
+
 int gfdfd_solve_tuning(int rw_cap, int fused_mmax, int refine_layout) {
     GUARD_BEGIN
     set_solve_tuning(rw_cap, fused_mmax, refine_layout);

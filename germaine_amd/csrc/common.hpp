@@ -38,11 +38,17 @@ struct hip_error : std::runtime_error {
                                      __FILE__ + ":" + std::to_string(__LINE__));                     \
     } while (0)
 
+// device bytes the library holds now and at most since the last reset
+// (every allocation goes through dev_alloc / dev_free)
+void mem_track_alloc(void *p, size_t bytes);
+void mem_track_free(void *p);
+
 template <typename T>
 T *dev_alloc(size_t count) {
     void *p = nullptr;
     if (count == 0) count = 1;
     GFDFD_HIP(hipMalloc(&p, count * sizeof(T)));
+    mem_track_alloc(p, count * sizeof(T));
     return static_cast<T *>(p);
 }
 
@@ -54,7 +60,10 @@ T *dev_upload(const std::vector<T> &v) {
 }
 
 inline void dev_free(void *p) {
-    if (p) (void)hipFree(p);
+    if (p) {
+        mem_track_free(p);
+        (void)hipFree(p);
+    }
 }
 
 // ---- device complex arithmetic --------------------------------------------

@@ -93,6 +93,9 @@ void cast_c64_to_c128(int64_t count, const cx<float> *a, cx<double> *b, hipStrea
 // (negative: no limit beyond LDS), and the layout of the panel refinement
 // (0: elimination order, 1: user layout with a transpose per correction)
 void set_solve_tuning(int rw_cap, int fused_mmax, int refine_layout);
+// factorisation kernel selection: panels needing more LDS than panel_lds_cap
+// bytes (0: the hardware limit) take the global-memory panel kernel
+void set_factor_tuning(int panel_lds_cap);
 
 struct RefineStats {
     int iterations = 0;

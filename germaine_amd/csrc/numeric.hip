@@ -821,13 +821,20 @@ __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int
 // ---------------------------------------------------------------------------
 // driver
 // ---------------------------------------------------------------------------
+namespace {
+size_t g_panel_lds_cap = 0;   // 0: the hardware limit
+}
+void set_factor_tuning(int panel_lds_cap) { g_panel_lds_cap = panel_lds_cap > 0 ? (size_t)panel_lds_cap : 0; }
+
 template <typename T>
 void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, int64_t vstride, hipStream_t st) {
     const Plan &pl = *dp.host;
     const int nf = F.nfreq;
     hipLaunchKernelGGL(k_clear_flags, dim3((nf + 63) / 64), dim3(64), 0, st, F.sing, nf);
     const size_t lds_small_max = 160 * 1024 - 256;
-    const size_t panel_lds_max = 160 * 1024 - 1024;   // panels of up to 640 c64 / 320 c128 candidate rows
+    const size_t panel_lds_hw = 160 * 1024 - 1024;    // panels of up to 640 c64 / 320 c128 candidate rows
+    // panels above the cap take the global-memory panel kernel
+    const size_t panel_lds_max = g_panel_lds_cap ? std::min(g_panel_lds_cap, panel_lds_hw) : panel_lds_hw;
     static bool attr_set = false;
     if (!attr_set) {
         // fronts of up to 90x90 complex doubles live in LDS (gfx950: 160 KB per CU)
@@ -836,9 +843,9 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
         GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_small_lu<float>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_small_max));
         GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_panel_lu<double>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)panel_lds_max));
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)panel_lds_hw));
         GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_panel_lu<float>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)panel_lds_max));
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)panel_lds_hw));
         attr_set = true;
     }
     for (int l = 0; l < pl.nlevels; l++) {

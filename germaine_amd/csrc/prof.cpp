@@ -2,11 +2,33 @@
 
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/gfdfd.h"
 
 namespace gfdfd {
+
+namespace {
+std::mutex mem_mu;
+std::unordered_map<void *, size_t> mem_live;
+size_t mem_now = 0, mem_peak = 0;
+}  // namespace
+
+void mem_track_alloc(void *p, size_t bytes) {
+    std::lock_guard<std::mutex> g(mem_mu);
+    mem_live[p] = bytes;
+    mem_now += bytes;
+    if (mem_now > mem_peak) mem_peak = mem_now;
+}
+
+void mem_track_free(void *p) {
+    std::lock_guard<std::mutex> g(mem_mu);
+    auto it = mem_live.find(p);
+    if (it == mem_live.end()) return;
+    mem_now -= it->second;
+    mem_live.erase(it);
+}
 
 const char *const kClassNames[K_NCLASS] = {
     "assemble",   "front_assemble", "scatter_orig", "small_lu",   "panel",       "panel_apply", "trsm_u",  "gemm_update",
@@ -90,6 +112,14 @@ ProfScope::~ProfScope() {
 using namespace gfdfd;
 
 extern "C" {
+
+int gfdfd_device_memory(long long *current_bytes, long long *peak_bytes, int reset_peak) {
+    std::lock_guard<std::mutex> g(mem_mu);
+    if (current_bytes) *current_bytes = (long long)mem_now;
+    if (peak_bytes) *peak_bytes = (long long)mem_peak;
+    if (reset_peak) mem_peak = mem_now;
+    return GFDFD_OK;
+}
 
 int gfdfd_profile(int enable) {
     std::lock_guard<std::mutex> lk(mu);

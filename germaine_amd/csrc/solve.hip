@@ -1878,7 +1878,7 @@ struct Workspace {
             dev_free(p);
             p = nullptr;
             bytes = 0;
-            GFDFD_HIP(hipMalloc(&p, b));
+            p = dev_alloc<char>(b);
             bytes = b;
         }
         return p;

@@ -89,6 +89,13 @@ def solve_tuning(rw_cap=64, fused_mmax=0, refine_layout=0):
     check(lib().gfdfd_solve_tuning(rw_cap, fused_mmax, refine_layout), "gfdfd_solve_tuning")
 
 
+def factor_tuning(panel_lds_cap=0):
+    """Factorisation kernel selection (process-wide): panels needing more than
+    panel_lds_cap bytes of LDS take the global-memory panel kernel (0: the
+    hardware limit)."""
+    check(lib().gfdfd_factor_tuning(int(panel_lds_cap)), "gfdfd_factor_tuning")
+
+
 class Plan:
     """Nested-dissection plan of an nx x ny 9-point grid (cached by size)."""
 
@@ -237,6 +244,51 @@ class Factors:
         return X, its.tolist(), rr.tolist()
 
 
+class Workspace:
+    """Factor storage reused by every call of an FWI run on one plan.
+
+    GERMAINE refactors the impedance matrices on every objective evaluation
+    (src/AC/parabolicls_AC.c:55-64 calls obj_AC per trial, src/AC/obj_AC.c:57-96
+    factors each frequency) and every iteration (src/AC/fwi_FD_AC.c:229-409).
+    Here the values buffer and the factor arena are allocated once, for
+    nfreq frequencies in one precision; each later call assembles into the
+    same buffer and refactors in place, as one hipGraph replay when graph is
+    set (the capture happens on the first refactorisation).  The factors a call
+    returns are overwritten by the next call that uses the workspace.
+    `allocations` counts the factor arenas created (1 after any number of
+    calls)."""
+
+    def __init__(self, plan, nfreq, precision=FP64, graph=True):
+        import torch
+        self.plan, self.nfreq, self.precision, self.graph = plan, nfreq, precision, graph
+        self.vals = torch.empty((nfreq, plan.nnz), dtype=torch.complex128, device="cuda")
+        self.F = None
+        self.allocations = 0
+
+    def factors(self, physics, dh, freqs, m1, m2, **kw):
+        if len(freqs) != self.nfreq:
+            raise ValueError(f"workspace holds {self.nfreq} frequencies, the call has {len(freqs)}")
+        self.plan.assemble(physics, dh, freqs, m1, m2, out=self.vals, **kw)
+        if self.F is None:
+            self.F = self.plan.factor(self.vals, self.precision)
+            self.allocations += 1
+        elif self.graph:
+            self.F.refactor_graph(self.vals)
+        else:
+            self.F.refactor(self.vals)
+        return self.F
+
+
+def _factors(plan, ws, physics, dh, freqs, m1, m2, precision, **kw):
+    """Factors of the matrices at freqs: fresh storage, or the workspace's."""
+    if ws is None:
+        vals = plan.assemble(physics, dh, freqs, m1, m2, **kw)
+        return plan.factor(vals, precision), vals
+    if ws.plan is not plan or ws.precision != precision:
+        raise ValueError("workspace belongs to another plan or precision")
+    return ws.factors(physics, dh, freqs, m1, m2, **kw), ws.vals
+
+
 # ---- shot panels and workflows ---------------------------------------------------
 
 def spike_panel(n, ksrc, amp, stream=None):
@@ -283,13 +335,14 @@ def residual_sources(n, krec, pcal, pobs, dh, misfit=1, stf_inv=0, rtm=False, st
 
 
 def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-                  precision=FP64, tol=1e-10, max_iter=10, keep_fields=False):
+                  precision=FP64, tol=1e-10, max_iter=10, keep_fields=False, ws=None):
     """All shots x all frequencies (src/AC/forward_shot_AC.c:38-101, batched).
 
     Returns dict(data=[nfreq, nshots, ntr] complex64 receiver samples or None,
-    fields=list of [nshots, n] complex128 panels when keep_fields, iters, relres)."""
-    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
-    F = plan.factor(vals, precision)
+    fields=list of [nshots, n] complex128 panels when keep_fields, iters, relres).
+    ws: a Workspace whose factor storage is reused (None: fresh storage)."""
+    F, vals = _factors(plan, ws, physics, dh, freqs, m1, m2, precision, s=s, npml=npml, a0_pml=a0_pml,
+                       free_surf=free_surf)
     data, fields, its, res = [], [], [], []
     for f, fr in enumerate(freqs):
         B = spike_panel(plan.n, ksrc, source_amp(physics, fr, s, dh))
@@ -308,7 +361,7 @@ def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_
 
 
 def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=0.0, free_surf=0, precision=FP64,
-                 tol=1e-10, max_iter=10):
+                 tol=1e-10, max_iter=10, ws=None):
     """Shin's pseudo-Hessian diagonal summed over frequencies and shots
     (src/AC/hessian_shin_AC.c:50-125, src/TE/hessian_shin_TE.c:53-137, batched): per frequency
     one factorisation and one panel solve for all shots, then the |J|^2 sums on the device.
@@ -319,8 +372,8 @@ def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=
     if physics not in (AC, TE):
         raise ValueError("SH has no Hessian path in the reference")
     n = plan.n
-    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
-    F = plan.factor(vals, precision)
+    F, _ = _factors(plan, ws, physics, dh, freqs, m1, m2, precision, s=s, npml=npml, a0_pml=a0_pml,
+                    free_surf=free_surf)
     h1 = torch.zeros(n, dtype=torch.float64, device="cuda")
     h2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     for f, fr in enumerate(freqs):
@@ -341,7 +394,7 @@ def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=
 
 
 def hessian_gn(plan, physics, dh, freqs, m1, m2, ksrc, krec, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-               precision=FP64, tol=1e-10, max_iter=10):
+               precision=FP64, tol=1e-10, max_iter=10, ws=None):
     """Gauss-Newton Hessian diagonal summed over frequencies, shots and receivers
     (src/AC/hessian_AC.c:50-160, src/TE/hessian_TE.c:53-177, batched): per frequency one
     factorisation, one panel solve for the shots and one for the receivers' Green's functions (a
@@ -353,8 +406,8 @@ def hessian_gn(plan, physics, dh, freqs, m1, m2, ksrc, krec, npml=0, a0_pml=90.0
     if physics not in (AC, TE):
         raise ValueError("SH has no Hessian path in the reference")
     n = plan.n
-    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
-    F = plan.factor(vals, precision)
+    F, _ = _factors(plan, ws, physics, dh, freqs, m1, m2, precision, s=s, npml=npml, a0_pml=a0_pml,
+                    free_surf=free_surf)
     hess = torch.zeros(n, dtype=torch.float64, device="cuda")
     h2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     for f, fr in enumerate(freqs):
@@ -416,14 +469,15 @@ def model_update_norm(alpha, direction, m_old, norm, lo, hi, out=None, stream=No
 
 
 def objective(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-              precision=FP64, tol=1e-10, max_iter=10):
+              precision=FP64, tol=1e-10, max_iter=10, misfit=1, stf_inv=0, ws=None):
     """Misfit only (obj_AC, src/AC/obj_AC.c:46-135): forward panel per frequency,
-    receiver samples, residual norms; 0.5 * sum |res|^2 over frequencies and shots.
-    The plan (ordering, symbolic tree, device buffers) is reused, as every
+    receiver samples, residual norms (calc_res_AC with misfit / stf_inv);
+    0.5 * sum |res|^2 over frequencies and shots.  The plan (ordering, symbolic
+    tree) is reused, and with a Workspace the factor storage too, as every
     line-search trial of an FWI iteration does."""
     n = plan.n
-    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
-    F = plan.factor(vals, precision)
+    F, _ = _factors(plan, ws, physics, dh, freqs, m1, m2, precision, s=s, npml=npml, a0_pml=a0_pml,
+                    free_surf=free_surf)
     l2 = 0.0
     for f, fr in enumerate(freqs):
         B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
@@ -431,7 +485,7 @@ def objective(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml
             U = F.solve(f, B)
         else:
             U, _, _ = F.solve_refine(f, B, tol=tol, max_iter=max_iter)
-        _, l2s = adjoint_sources(n, krec, sample(U, krec), pobs[f], dh)
+        _, l2s, _ = residual_sources(n, krec, sample(U, krec), pobs[f], dh, misfit=misfit, stf_inv=stf_inv)
         l2 += float(l2s.sum())
     return 0.5 * l2
 
@@ -504,16 +558,62 @@ def parabolic_line_search(obj, L2, alpha, scalefac=2.0, stepmax=6):
 
 
 def line_search(plan, dh, freqs, vp, rho, direction, ksrc, krec, pobs, L2, eps_scale, lo, hi, scalefac=2.0,
-                stepmax=6, **kw):
+                stepmax=6, ws=None, **kw):
     """One FWI iteration's step search for the acoustic model (src/AC/fwi_FD_AC.c
     calling parabolicls_AC): each trial steps vp along direction on the device
     (model_update into one reused buffer) and evaluates objective() on the same
-    plan.  Returns (step, trials); the caller applies model_update(step, ...)."""
+    plan and factor storage (ws, or one workspace for the whole search).
+    Returns (step, trials); the caller applies model_update(step, ...)."""
+    if ws is None:
+        ws = Workspace(plan, len(freqs), kw.get("precision", FP64))
     trial = model_update(0.0, direction, vp, lo, hi)
     def obj(alpha):
         model_update(alpha, direction, vp, lo, hi, out=trial)
-        return objective(plan, AC, dh, freqs, trial, rho, ksrc, krec, pobs, **kw)
+        return objective(plan, AC, dh, freqs, trial, rho, ksrc, krec, pobs, ws=ws, **kw)
     return parabolic_line_search(obj, L2, initial_step(direction, vp, eps_scale), scalefac, stepmax)
+
+
+class LBFGS:
+    """Quasi-Newton search direction of the FWI loop (GRAD_METHOD=2): the two-loop
+    recursion of src/LBFGS.c:36-160 over the last `memory` (s, y) pairs, s the
+    model change and y the gradient change of consecutive iterations, newest
+    to oldest then back, the initial inverse Hessian gamma = s.y / y.y of the
+    newest pair; the first call returns -grad (:162-170).  The memory drops its
+    oldest pair once full (rot_LBFGS_vec, src/AC/fwi_FD_AC.c:231-237), and a
+    direction that is not a descent direction clears it (check_descent,
+    src/check_descent.c:19-40).  O(n) vector work per FWI iteration, run on
+    the host in float64 (the reference runs it on MPI rank 0 in float)."""
+
+    def __init__(self, memory=5):
+        self.memory = memory
+        self.pairs = []          # (s, y), oldest first
+        self.prev = None
+
+    def direction(self, grad, model):
+        import torch
+        g = grad.detach().double().cpu().numpy().ravel()
+        x = model.detach().double().cpu().numpy().ravel()
+        if self.prev is not None:
+            self.pairs.append((x - self.prev[1], g - self.prev[0]))
+            if len(self.pairs) > self.memory:
+                self.pairs.pop(0)
+        self.prev = (g.copy(), x.copy())
+        q = -g
+        if self.pairs:
+            rho = [1.0 / float(y @ s_) if abs(float(y @ s_)) > 0.0 else 0.0 for s_, y in self.pairs]
+            alpha = [0.0] * len(self.pairs)
+            for k in range(len(self.pairs) - 1, -1, -1):
+                s_, y = self.pairs[k]
+                alpha[k] = rho[k] * float(s_ @ q)
+                q = q - alpha[k] * y
+            s_n, y_n = self.pairs[-1]
+            r = (float(y_n @ s_n) / float(y_n @ y_n)) * q
+            for k, (s_, y) in enumerate(self.pairs):
+                r = r + s_ * (alpha[k] - rho[k] * float(y @ r))
+            q = r
+        if -float(q @ g) / float(g @ g) < 0.0:
+            self.pairs.clear()
+        return torch.as_tensor(q.reshape(grad.shape), device=grad.device)
 
 
 def tikhonov_te(sigma, eps, dh, beta, norm, lam, grad_sigma=None, grad_eps=None, stream=None):
@@ -535,7 +635,7 @@ def tikhonov_te(sigma, eps, dh, beta, norm, lam, grad_sigma=None, grad_eps=None,
 
 
 def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, pobs, L2, eps_scale, norm,
-                   low, up, scalefac=2.0, stepmax=6, tikhonov=None, **kw):
+                   low, up, scalefac=2.0, stepmax=6, tikhonov=None, ws=None, **kw):
     """One FWI iteration's step search for the TE model (parabolicls_TE,
     src/TE/parabolicls_TE.c:28-160): both materials step by the same alpha on
     their normalised scales (norm, low, up: (MAT1 for sigma, MAT2 for epsilon)
@@ -548,12 +648,14 @@ def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, 
     f32 = np.float32
     if not (norm[0] > 0 and norm[1] > 0):
         raise ValueError("line_search_te: the normalisations must be positive")
+    if ws is None:
+        ws = Workspace(plan, len(freqs), kw.get("precision", FP64))
     ts = model_update_norm(0.0, dir_sigma, sigma, norm[0], low[0], up[0])
     te = model_update_norm(0.0, dir_eps, eps, norm[1], low[1], up[1])
     def obj(alpha):
         model_update_norm(alpha, dir_sigma, sigma, norm[0], low[0], up[0], out=ts)
         model_update_norm(alpha, dir_eps, eps, norm[1], low[1], up[1], out=te)
-        l2 = objective(plan, TE, dh, freqs, te, ts, ksrc, krec, pobs, **kw)
+        l2 = objective(plan, TE, dh, freqs, te, ts, ksrc, krec, pobs, ws=ws, **kw)
         if tikhonov is not None and (tikhonov[1][0] > 0 or tikhonov[1][1] > 0):
             cs, ce = tikhonov_te(ts, te, dh, tikhonov[0], norm, tikhonov[1])
             l2 += tikhonov[1][0] * cs + tikhonov[1][1] * ce
@@ -607,7 +709,7 @@ def grad_frame(grad, nx0, ny0, npml, free_surf, stream=None):
 
 
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-             precision=FP64, tol=1e-10, max_iter=10, rtm=False, misfit=1, stf_inv=0):
+             precision=FP64, tol=1e-10, max_iter=10, rtm=False, misfit=1, stf_inv=0, ws=None):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,
     src/TE/grad_obj_TE.c:53-172).  pobs: list per frequency of complex64 cuda [nshots, ntr].
     Returns (L2 = 0.5 * sum |res|^2, grad (AC: [n] float64; TE: (grad_sigma, grad_eps))).
@@ -617,11 +719,12 @@ def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=
     shot's gradient (src/AC/ass_grad_AC.c:30-31, src/TE/ass_grad_TE.c:31-35).
     rtm=True is the reverse-time-migration pass (RTM_FD_AC, src/AC/RTM_FD_AC.c:190):
     the residual is the observed data itself (src/AC/calc_res_AC.c:77), with the
-    same mute, so the returned gradient is the image before precond()."""
+    same mute, so the returned gradient is the image before precond().
+    ws: a Workspace whose factor storage is reused (None: fresh storage)."""
     import torch
     n = plan.n
-    vals = plan.assemble(physics, dh, freqs, m1, m2, s=s, npml=npml, a0_pml=a0_pml, free_surf=free_surf)
-    F = plan.factor(vals, precision)
+    F, _ = _factors(plan, ws, physics, dh, freqs, m1, m2, precision, s=s, npml=npml, a0_pml=a0_pml,
+                    free_surf=free_surf)
     g1 = torch.zeros(n, dtype=torch.float64, device="cuda")
     g2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     l2 = 0.0

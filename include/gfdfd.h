@@ -158,6 +158,12 @@ int gfdfd_residual(gfdfd_plan plan, const double *d_vals, int sys, int nrhs, con
  * default: elimination order, one fused residual-and-update pass per step;
  * 1: the caller's column-major layout, two transposes per correction).      */
 int gfdfd_solve_tuning(int rw_cap, int fused_mmax, int refine_layout);
+/* Factorisation kernel selection (process-wide, for later factorisations and
+ * graph captures): a 32-column panel whose candidate rows need more than
+ * panel_lds_cap bytes of LDS is factored by the global-memory panel kernel
+ * instead of in LDS (0: the hardware limit, 159 KB -- separators above 318
+ * rows in FP64, 637 in FP32).  Lets small systems exercise that path.        */
+int gfdfd_factor_tuning(int panel_lds_cap);
 
 /* ---- shot kernels --------------------------------------------------------------
  * RHS_source_AC / RHS_source_TE for all shots (src/AC/RHS_source_AC.c:46-53):
@@ -302,6 +308,11 @@ int gfdfd_grad_te_stf(int n, int nshots, const float *omega, const float *omega2
  * No counterpart in the reference (it times whole phases with MPI_Wtime,
  * src/AC/forward_shot_AC.c:54-65).                                          */
 int gfdfd_profile(int enable);
+/* Device memory the library holds (plans, factors, solve work buffers): the
+ * bytes allocated now and the most since the last reset; reset_peak != 0
+ * restarts the peak from the current value.  No counterpart in the reference
+ * (UMFPACK reports host memory in Info[]).                                   */
+int gfdfd_device_memory(long long *current_bytes, long long *peak_bytes, int reset_peak);
 /* record only the named class (NULL or "": every class), e.g. to time one
  * kernel inside a timed region at the cost of two events per launch       */
 int gfdfd_profile_select(const char *kernel);

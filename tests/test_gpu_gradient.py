@@ -386,6 +386,54 @@ def test_line_search_lowers_the_misfit():
     assert abs(again - l2a) <= 1e-12 * abs(l2a)
 
 
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_line_search_reuses_factor_storage(precision, monkeypatch):
+    """A full acoustic step search (parabolicls_AC, src/AC/parabolicls_AC.c:55-64,
+    obj_AC per trial) and the gradient pass of the next iteration on one
+    Workspace: one factor arena for every call, each later factorisation a
+    replay of the captured refactorisation graph, with eager work and
+    synchronisations between replays -- and every trial's misfit equal to a
+    fresh factorisation's at the same trial model."""
+    say("line search on one workspace", precision)
+    m = models(G.AC)
+    dh, freqs = m["dh"], m["freqs"]
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    tol = None if precision == G.FP64 else 1e-10
+    pl = fdfd.Plan(NX, NY)
+    pobs = fdfd.forward_shots(pl, G.AC, dh, freqs, dev(m["t1"]), dev(m["t2"]), srcs, recs, npml=NPML,
+                              tol=None)["data"]
+    vp, rho = dev(m["m1"]), dev(m["m2"])
+    L2, grad = fdfd.grad_obj(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, precision=precision, tol=tol)
+    made = []
+    real_init = fdfd.Factors.__init__
+
+    def counting_init(self, *a, **k):
+        made.append(1)
+        real_init(self, *a, **k)
+
+    monkeypatch.setattr(fdfd.Factors, "__init__", counting_init)
+    ws = fdfd.Workspace(pl, len(freqs), precision)
+    step, trials = fdfd.line_search(pl, dh, freqs, vp, rho, -grad, srcs, recs, pobs, L2, 0.01, 100.0, 9000.0,
+                                    npml=NPML, precision=precision, tol=tol, ws=ws)
+    new = fdfd.model_update(step, -grad, vp, 100.0, 9000.0)
+    L2b, grad_b = fdfd.grad_obj(pl, G.AC, dh, freqs, new, rho, srcs, recs, pobs, npml=NPML, precision=precision,
+                                tol=tol, ws=ws)
+    say("  trials", trials, "step", step, "factor arenas", len(made), "misfit after", L2b)
+    assert len(made) == 1 and ws.allocations == 1 and len(trials) >= 2
+    assert step > 0.0 and L2b < L2
+    monkeypatch.setattr(fdfd.Factors, "__init__", real_init)
+    # every trial against fresh factor storage at the same trial model
+    for a, l2a in trials:
+        fresh = fdfd.objective(pl, G.AC, dh, freqs, fdfd.model_update(a, -grad, vp, 100.0, 9000.0), rho, srcs, recs,
+                               pobs, npml=NPML, precision=precision, tol=tol)
+        assert abs(fresh - l2a) <= 1e-12 * abs(l2a), (a, fresh, l2a)
+    L2c, grad_c = fdfd.grad_obj(pl, G.AC, dh, freqs, new, rho, srcs, recs, pobs, npml=NPML, precision=precision,
+                                tol=tol)
+    assert abs(L2c - L2b) <= 1e-12 * abs(L2b)
+    assert float(torch.max(torch.abs(grad_c - grad_b))) <= 1e-12 * float(torch.max(torch.abs(grad_c)))
+
+
 def test_model_update_norm_matches_oracle():
     """The TE line search's trial model (src/TE/parabolicls_TE.c:30-31,64-69) on
     the device against the oracle: bit-exact, with a normalisation that is not
@@ -488,37 +536,44 @@ def test_taper_file_matches_oracle(free_surf):
 
 
 def test_fwi_iteration_ac():
-    """One acoustic FWI iteration in the order of src/AC/fwi_FD_AC.c:263-353 with
-    steepest descent: gradient and misfit (grad_obj_AC), Shin pseudo-Hessian
-    (hessian_shin_AC), Hessian preconditioning (apply_hess_AC), smoothing and
-    horizontal taper (precond), descent direction -grad (descent), parabolic step
-    search (parabolicls_AC) and the model update (calc_mat_change_wolfe); every
-    stage on the device.  The iteration must lower the misfit, and a second
-    iteration from the updated model must lower it again."""
+    """Two acoustic FWI iterations in the order of src/AC/fwi_FD_AC.c:263-353 with
+    steepest descent (GRAD_METHOD=3) and the pseudo-Hessian (HESSIAN=2): gradient
+    and misfit followed by cp_grad_frame (grad_obj_AC, src/AC/grad_obj_AC.c:177),
+    the Shin pseudo-Hessian at the first iteration only, also framed
+    (src/AC/hessian_shin_AC.c:140), apply_hess_AC every iteration, precond,
+    cp_grad_frame and descent (src/AC/fwi_FD_AC.c:287-288), the parabolic step
+    search and calc_mat_change_wolfe -- every stage on the device, every
+    factorisation in one Workspace.  The misfit must fall at every iteration."""
     m = models(G.AC)
     dh, freqs = m["dh"], m["freqs"]
     srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
     recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    nx0, ny0 = NX - 2 * NPML, NY - 2 * NPML
     pl = fdfd.Plan(NX, NY)
     pobs = fdfd.forward_shots(pl, G.AC, dh, freqs, dev(m["t1"]), dev(m["t2"]), srcs, recs, npml=NPML,
                               tol=None)["data"]
     vp, rho = dev(m["m1"]), dev(m["m2"])
-    misfits = []
+    ws = fdfd.Workspace(pl, len(freqs), G.FP64)
+    misfits, hess = [], None
     for it in range(2):
-        L2, grad = fdfd.grad_obj(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, tol=None)
-        hess = fdfd.hessian_shin(pl, G.AC, dh, freqs, vp, rho, srcs, npml=NPML, tol=None)
+        L2, grad = fdfd.grad_obj(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, tol=None, ws=ws)
+        fdfd.grad_frame(grad, nx0, ny0, NPML, 0)
+        if it == 0:
+            hess = fdfd.hessian_shin(pl, G.AC, dh, freqs, vp, rho, srcs, npml=NPML, tol=None, ws=ws)
+            fdfd.grad_frame(hess, nx0, ny0, NPML, 0)
         fdfd.apply_hess(grad, hess, 0.005)
         grad = fdfd.precond(grad, NX, NY, dh, 1800.0, freqs[0], 0.5, 0.5, 1, NPML)
+        fdfd.grad_frame(grad, nx0, ny0, NPML, 0)
         direction = -grad
         step, trials = fdfd.line_search(pl, dh, freqs, vp, rho, direction, srcs, recs, pobs, L2, 0.01, 100.0,
-                                        9000.0, npml=NPML, tol=None)
+                                        9000.0, npml=NPML, tol=None, ws=ws)
         say("FWI iteration", it, "misfit", L2, "step", step, "trials", len(trials))
         assert step > 0.0
         vp = fdfd.model_update(step, direction, vp, 100.0, 9000.0)
         misfits.append(L2)
-    final = fdfd.objective(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, tol=None)
-    say("  misfits", misfits, "final", final)
-    assert misfits[1] < misfits[0] and final < misfits[1]
+    final = fdfd.objective(pl, G.AC, dh, freqs, vp, rho, srcs, recs, pobs, npml=NPML, tol=None, ws=ws)
+    say("  misfits", misfits, "final", final, "factor arenas", ws.allocations)
+    assert misfits[1] < misfits[0] and final < misfits[1] and ws.allocations == 1
 
 
 @pytest.mark.parametrize("free_surf", [0, 1])
@@ -536,14 +591,98 @@ def test_grad_frame_matches_oracle(free_surf):
     assert np.max(np.abs(out - ref)) <= 1e-6 * np.max(np.abs(ref))
 
 
+def _te_direction_device(pl, dh, freqs, eps, sig, srcs, recs, pobs, norm, beta, lam, ws):
+    """fwi_FD_TE's steps from the gradient to the descent direction
+    (src/TE/fwi_FD_TE.c:262-306, GRAD_METHOD=3, HESSIAN=2, LAMBDA > 0) on the device."""
+    nx0, ny0 = NX - 2 * NPML, NY - 2 * NPML
+    L2, (g_sig, g_eps) = fdfd.grad_obj(pl, G.TE, dh, freqs, eps, sig, srcs, recs, pobs, npml=NPML, tol=None, ws=ws)
+    for g, nm in ((g_sig, norm[0]), (g_eps, norm[1])):           # src/TE/grad_obj_TE.c:178-191
+        fdfd.grad_frame(g, nx0, ny0, NPML, 0)
+        g *= float(np.float32(nm))
+    cs, ce = fdfd.tikhonov_te(sig, eps, dh, beta, norm, lam, g_sig, g_eps)
+    L2 += lam[0] * cs + lam[1] * ce
+    h_sig, h_eps = fdfd.hessian_shin(pl, G.TE, dh, freqs, eps, sig, srcs, npml=NPML, tol=None, ws=ws)
+    for h, nm in ((h_sig, norm[0]), (h_eps, norm[1])):           # src/TE/hessian_shin_TE.c:141-148
+        fdfd.grad_frame(h, nx0, ny0, NPML, 0)
+        h *= float(np.float32(nm) * np.float32(nm))
+    fdfd.apply_hess(g_sig, h_sig, 0.005)
+    fdfd.apply_hess(g_eps, h_eps, 0.005)
+    fdfd.grad_frame(g_sig, nx0, ny0, NPML, 0)
+    fdfd.grad_frame(g_eps, nx0, ny0, NPML, 0)
+    return L2, -g_sig, -g_eps
+
+
+def test_te_direction_matches_oracle_sequence():
+    """The TE descent direction of one fwi_FD_TE iteration (gradients framed and
+    scaled by MAT*_NORM, Tikhonov gradient, Shin pseudo-Hessians framed and
+    scaled by NORM^2, apply_hess per material, cp_grad_frame, descent) on the
+    device against the oracle's pieces composed in the same order with band-LU
+    fields: relative 1e-5 per material.  Low frequencies keep the float
+    pseudo-Hessian of the restatement finite; the Tikhonov weights are chosen
+    so that the regulariser moves each direction by at least 10%."""
+    m = models(G.TE)
+    dh, freqs = m["dh"], [1.5e7, 2.5e7]
+    n = NX * NY
+    nx0, ny0 = NX - 2 * NPML, NY - 2 * NPML
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
+    recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    eps0 = 8.854187817e-12
+    norm, beta = (0.003, 5.0 * eps0), (1.0, 1.0)
+    pobs = []
+    for f in freqs:
+        _, Ut = oracle_fields(G.TE, dh, f, m["t1"], m["t2"], srcs)
+        pobs.append(Ut[recs, :].T.astype(np.complex64))
+    # oracle: gradients and pseudo-Hessians per shot in float, then the sequence
+    gs = np.zeros(n, np.float32)
+    ge = np.zeros(n, np.float32)
+    hs = np.zeros(n, np.float32)
+    he = np.zeros(n, np.float32)
+    for fi, f in enumerate(freqs):
+        _, U = oracle_fields(G.TE, dh, f, m["m1"], m["m2"], srcs)
+        rhs = np.zeros((n, len(srcs)), complex)
+        for s_ in range(len(srcs)):
+            res, _ = O.residual(U[recs, s_].astype(np.complex64), pobs[fi][s_])
+            rhs[:, s_] = O.adjoint_rhs(n, recs, res, dh)
+        _, LAM = oracle_fields(G.TE, dh, f, m["m1"], m["m2"], srcs, rhs, 2)
+        for s_ in range(len(srcs)):
+            O.grad_te_accumulate(NX, NY, f, 0.0, U[:, s_], 0j, LAM[:, s_], 0j, gs, ge)
+            O.hess_shin_te_accumulate(NX, NY, f, 0.0, U[:, s_], 0j, hs, he)
+    assert np.all(np.isfinite(he)) and np.all(np.isfinite(hs))
+    frame = lambda a: O.grad_frame(a.reshape(NY, NX), nx0, ny0, NPML, 0).ravel()
+    gs, ge = np.float32(norm[0]) * frame(gs), np.float32(norm[1]) * frame(ge)
+    # Tikhonov weights from the regulariser's own gradient (lam = 1)
+    ts, te = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    O.tikhonov_te(m["m2"], m["m1"], dh, beta, norm, (1.0, 1.0), ts, te)
+    lam = (float(0.5 * np.abs(gs).max() / np.abs(ts).max()), float(0.5 * np.abs(ge).max() / np.abs(te).max()))
+    g_plain = (gs.copy(), ge.copy())
+    O.tikhonov_te(m["m2"], m["m1"], dh, beta, norm, lam, gs, ge)
+    hs = np.float32(np.float32(norm[0]) * np.float32(norm[0])) * frame(hs)
+    he = np.float32(np.float32(norm[1]) * np.float32(norm[1])) * frame(he)
+    dirs_o = []
+    for g, h, gp in ((gs, hs, g_plain[0]), (ge, he, g_plain[1])):
+        O.apply_hess(0.005, h, g)
+        O.apply_hess(0.005, h, gp)
+        d, dp = -frame(g), -frame(gp)
+        assert np.linalg.norm(d - dp) >= 0.1 * np.linalg.norm(d)     # the regulariser changes the direction
+        dirs_o.append(d)
+    pl = fdfd.Plan(NX, NY)
+    ws = fdfd.Workspace(pl, len(freqs), G.FP64)
+    _, d_sig, d_eps = _te_direction_device(pl, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs,
+                                           [dev(p) for p in pobs], norm, beta, lam, ws)
+    for name, dd, do in (("sigma", d_sig, dirs_o[0]), ("eps", d_eps, dirs_o[1])):
+        e = float(np.linalg.norm(dd.cpu().numpy() - do) / np.linalg.norm(do))
+        say("  TE direction", name, "rel err vs oracle sequence", e)
+        assert e <= 1e-5
+
+
 def test_fwi_iteration_te():
-    """Two TE FWI iterations in the order of src/TE/fwi_FD_TE.c:258-345 with
-    steepest descent, every stage on the device: gradient and misfit
-    (grad_obj_TE) with the Tikhonov cost and gradient, Shin pseudo-Hessians
-    and apply_hess per material, cp_grad_frame's PML damping, descent
-    directions (each scaled to a maximum of one on its normalised material),
-    and the normalised step search with the Tikhonov term.  The misfit must
-    fall at every iteration."""
+    """Two TE FWI iterations in the order of src/TE/fwi_FD_TE.c:258-345
+    (GRAD_METHOD=3, HESSIAN=2, LAMBDA > 0 so precond is skipped), every stage on
+    the device and every factorisation in one Workspace: the direction of
+    test_te_direction_matches_oracle_sequence, then the normalised step search
+    with the Tikhonov term.  The Tikhonov weights are set from the first
+    iteration's gradients so that the regulariser is a third of each.  The
+    misfit with the Tikhonov term must fall at every iteration."""
     m = models(G.TE)
     dh, freqs = m["dh"], m["freqs"]
     srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (4, 13, 22, 31, 40)]
@@ -554,27 +693,27 @@ def test_fwi_iteration_te():
     eps, sig = dev(m["m1"]), dev(m["m2"])
     eps0 = 8.854187817e-12
     norm, low, up = (0.003, 5.0 * eps0), (0.0, eps0), (1.0, 100.0 * eps0)
-    beta, lam = (1.0, 1.0), (1e-9, 1e-9)
+    beta = (1.0, 1.0)
     nx0, ny0 = NX - 2 * NPML, NY - 2 * NPML
-    misfits = []
+    ws = fdfd.Workspace(pl, len(freqs), G.FP64)
+    # Tikhonov weights: a third of the scaled data gradients' maxima
+    _, (g_sig, g_eps) = fdfd.grad_obj(pl, G.TE, dh, freqs, eps, sig, srcs, recs, pobs, npml=NPML, tol=None, ws=ws)
+    t_sig, t_eps = torch.zeros_like(g_sig), torch.zeros_like(g_eps)
+    fdfd.tikhonov_te(sig, eps, dh, beta, norm, (1.0, 1.0), t_sig, t_eps)
+    lam = tuple(float(np.float32(nm) * g.abs().max() / (3.0 * t.abs().max()))
+                for nm, g, t in ((norm[0], g_sig, t_sig), (norm[1], g_eps, t_eps)))
+    misfits, final = [], None
     for it in range(2):
-        L2, (g_sig, g_eps) = fdfd.grad_obj(pl, G.TE, dh, freqs, eps, sig, srcs, recs, pobs, npml=NPML, tol=None)
-        cs, ce = fdfd.tikhonov_te(sig, eps, dh, beta, norm, lam, g_sig, g_eps)
-        L2 += lam[0] * cs + lam[1] * ce
-        h_sig, h_eps = fdfd.hessian_shin(pl, G.TE, dh, freqs, eps, sig, srcs, npml=NPML, tol=None)
-        fdfd.apply_hess(g_sig, h_sig, 0.005)
-        fdfd.apply_hess(g_eps, h_eps, 0.005)
-        fdfd.grad_frame(g_sig, nx0, ny0, NPML, 0)
-        fdfd.grad_frame(g_eps, nx0, ny0, NPML, 0)
-        d_sig = -g_sig / g_sig.abs().max()
-        d_eps = -g_eps / g_eps.abs().max()
+        L2, d_sig, d_eps = _te_direction_device(pl, dh, freqs, eps, sig, srcs, recs, pobs, norm, beta, lam, ws)
         step, trials = fdfd.line_search_te(pl, dh, freqs, eps, sig, d_eps, d_sig, srcs, recs, pobs, L2, 0.01, norm,
-                                           low, up, tikhonov=(beta, lam), npml=NPML, tol=None)
+                                           low, up, tikhonov=(beta, lam), npml=NPML, tol=None, ws=ws)
         say("TE FWI iteration", it, "misfit", L2, "step", step, "trials", len(trials))
         assert step > 0.0
         sig = fdfd.model_update_norm(step, d_sig, sig, norm[0], low[0], up[0])
         eps = fdfd.model_update_norm(step, d_eps, eps, norm[1], low[1], up[1])
         misfits.append(L2)
-    final = fdfd.objective(pl, G.TE, dh, freqs, eps, sig, srcs, recs, pobs, npml=NPML, tol=None)
-    say("  misfits", misfits, "final (without the Tikhonov term)", final)
-    assert misfits[1] < misfits[0] and final < misfits[1]
+    cs, ce = fdfd.tikhonov_te(sig, eps, dh, beta, norm, lam)
+    final = fdfd.objective(pl, G.TE, dh, freqs, eps, sig, srcs, recs, pobs, npml=NPML, tol=None, ws=ws)
+    final += lam[0] * cs + lam[1] * ce
+    say("  misfits", misfits, "final (with the Tikhonov term)", final, "lam", lam)
+    assert misfits[1] < misfits[0] and final < misfits[1] and ws.allocations == 1

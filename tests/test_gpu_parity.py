@@ -291,6 +291,48 @@ def test_pivoting_solves_match_oracle(sysno, nrhs):
         assert rel(Xs, Xr.T) <= 1e-8, (z, k)
 
 
+@pytest.mark.parametrize("matrix", ["helmholtz", "random"])
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_global_memory_panel_path_matches_oracle(precision, matrix):
+    """The blocked path's global-memory panel kernel, which factors panels too
+    large for LDS (separators above 318 rows in FP64, 637 in FP32): forced on
+    every blocked panel of a small grid by a one-byte LDS cap, against the
+    banded oracle for M and M^T -- on the assembled Helmholtz matrix and on
+    random values whose partial pivoting swaps rows."""
+    nx, ny = 45, 33
+    say("global-memory panel path", precision, matrix)
+    if matrix == "helmholtz":
+        _, _, csc = system(nx, ny)
+    else:
+        csc = random_grid_csc(nx, ny, 31)
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    vals = dev((csc[2] + 1j * csc[3])[None, :])
+    rng = np.random.default_rng(9)
+    B = rng.standard_normal((3, nx * ny)) + 1j * rng.standard_normal((3, nx * ny))
+    fdfd.factor_tuning(1)
+    try:
+        F = pl.factor(vals, precision)
+    finally:
+        fdfd.factor_tuning(0)
+    F0 = pl.factor(vals, precision)
+    for sysno in (0, 2):
+        Xr, _ = O.band_solve(*csc, B.T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+        if precision == G.FP64:
+            X = F.solve(0, dev(B), sys=sysno).cpu().numpy()
+            X0 = F0.solve(0, dev(B), sys=sysno).cpu().numpy()
+            bar = 1e-10 if matrix == "helmholtz" else 1e-8
+        else:
+            X, it, rr = F.solve_refine(0, dev(B), sys=sysno, tol=1e-12, max_iter=30)
+            X0, _, _ = F0.solve_refine(0, dev(B), sys=sysno, tol=1e-12, max_iter=30)
+            X, X0 = X.cpu().numpy(), X0.cpu().numpy()
+            say("  sys", sysno, "iterations", it, "relres", rr)
+            bar = 1e-8
+        say("  sys", sysno, "rel err", rel(X, Xr.T), "vs LDS panels", rel(X, X0))
+        assert rel(X, Xr.T) <= bar
+        # the same factorisation as the LDS panel kernels, to rounding
+        assert rel(X, X0) <= (1e-9 if precision == G.FP64 else 1e-6)
+
+
 @pytest.mark.parametrize("precision", [G.FP64, G.FP32])
 @pytest.mark.parametrize("path", [(64, -1), (16, -1), (32, 0)])
 @pytest.mark.parametrize("sysno", [0, 1, 2])

@@ -161,3 +161,42 @@ def test_grad_frame_oracle_closed_form(free_surf):
         assert (out[0, npml:npml + nx0] == 1.0).all()
     else:
         assert abs(out[0, nx // 2] - 0.4) <= 1e-6 and abs(out[0, 0] - 0.16) <= 1e-6
+
+
+def test_lbfgs_direction_matches_dense_bfgs():
+    """The l-BFGS two-loop recursion (src/LBFGS.c:36-160) equals the dense BFGS
+    inverse-Hessian update H <- (I - r s y^T) H (I - r y s^T) + r s s^T applied
+    to gamma I for the stored pairs oldest first (gamma = s.y / y.y of the
+    newest pair; Nocedal & Wright, Algorithm 7.4): the first direction is
+    -grad, later ones -H grad; the newest pair satisfies the secant condition
+    H y = s; a memory of m keeps the newest m pairs."""
+    import torch
+    from germaine_amd import fdfd
+    rng = np.random.default_rng(4)
+    n = 6
+    Q = rng.standard_normal((n, n))
+    A = Q @ Q.T + n * np.eye(n)
+    grad_of = lambda x: A @ x
+    for memory in (5, 2):
+        lb = fdfd.LBFGS(memory=memory)
+        x = rng.standard_normal(n)
+        xs, gs = [], []
+        for it in range(5):
+            g = grad_of(x)
+            d = lb.direction(torch.as_tensor(g), torch.as_tensor(x)).numpy()
+            xs.append(x.copy())
+            gs.append(g.copy())
+            pairs = [(xs[k + 1] - xs[k], gs[k + 1] - gs[k]) for k in range(len(xs) - 1)][-memory:]
+            if not pairs:
+                H = np.eye(n)
+            else:
+                s_n, y_n = pairs[-1]
+                H = (s_n @ y_n) / (y_n @ y_n) * np.eye(n)
+                for s_, y in pairs:
+                    r = 1.0 / (y @ s_)
+                    V = np.eye(n) - r * np.outer(y, s_)
+                    H = V.T @ H @ V + r * np.outer(s_, s_)
+                assert np.allclose(H @ y_n, s_n, rtol=1e-10)
+            assert np.allclose(d, -H @ g, rtol=1e-9, atol=1e-12 * np.abs(g).max())
+            assert len(lb.pairs) == min(it, memory)
+            x = x + 0.1 * d

@@ -72,3 +72,26 @@ def test_band_vs_superlu_config_grid():
     R = B - M @ X
     den = abs(M) @ np.abs(X) + np.abs(B)
     assert np.max(np.abs(R) / den) <= 1e-13
+
+
+def test_stencil_diagonals_match_oracle_matvec():
+    """The nine-diagonal test operator (tests/_stencil.py) equals the oracle's
+    CSC matvec of M and M^T on a PML grid with S > 0."""
+    from tests import _stencil as ST
+    from tests._grids import smooth_model
+    nx, ny = 23, 17
+    vp, rho = smooth_model(nx, ny, seed=4)
+    px, py = O.pml_profiles(nx, ny, 4, 10.0, 90.0, 6.0, 0.3, 0)
+    tri = O.assemble(O.AC, nx, ny, 10.0, 6.0, 0.3, vp, rho, px, py)
+    csc = O.csc_of_M(nx, ny, *tri)
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((3, nx * ny)) + 1j * rng.standard_normal((3, nx * ny))
+    for sysno, tr in ((0, False), (2, True)):
+        D = ST.diagonals(nx * ny, *tri, transpose=tr)
+        assert len(D) == 9
+        Y = ST.apply(D, X)
+        for c in range(3):
+            ref = O.matvec(*csc, X[c], sysno)
+            assert np.max(np.abs(Y[c] - ref)) <= 1e-12 * np.max(np.abs(ref))
+    B = ST.apply(ST.diagonals(nx * ny, *tri), X)
+    assert np.max(ST.relres(ST.diagonals(nx * ny, *tri), X, B)) <= 1e-15
