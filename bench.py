@@ -143,6 +143,7 @@ def main():
     amps = [fdfd.source_amp(w.physics, f, w.s, w.dh) for f in w.freqs]
     Bs = torch.empty((nf, ns, n), dtype=torch.complex128, device="cuda")
     Xs = torch.empty((nf, ns, n), dtype=torch.complex128, device="cuda")
+    Xs2 = torch.empty((nf, ns, n), dtype=torch.complex128, device="cuda")
     freqs = list(range(nf))
 
     store = {}
@@ -176,23 +177,28 @@ def main():
             L.gfdfd_profile_reset()
         L.gfdfd_profile(1 if on else 0)
 
-    def timed(precision, refine, steps, warmup, profile=False):
-        """Warm-up steps (every kernel class event-timed when profile: the per-class
-        breakdown and the dominant class), then the timed steps, in which only the
-        dominant class records events (two per launch), so the headline time carries
-        no per-launch profiling of the other classes."""
+    def timed(precision, refine, steps, warmup, profile=False, legs=False):
+        """Warm-up steps -- the first one, whose factorisation runs eagerly (the
+        later ones replay the captured graph, which records no events), with
+        every kernel class event-timed when profile: the per-kernel table of one
+        step and the dominant class -- then the timed steps, in which only the
+        dominant class records events (two per launch), so the headline time
+        carries no per-launch profiling of the other classes.  With legs, after
+        the timed steps: the factorisation alone (graph replays) and one
+        batched solve pass alone (no refinement), each the median of 5."""
         kw, dom = {}, None
-        if profile and warmup:
-            profiling(True)
-        for _ in range(warmup):
+        for i in range(warmup):
+            if profile and i == 0:
+                profiling(True)
             step(precision, refine)
+            if profile and i == 0:
+                torch.cuda.synchronize()
+                profiling(False)
+                kw = kernel_stats(L, ctypes)
+                dom = max(kw, key=lambda k: kw[k]["ms"]) if kw else None
         if warmup and not args.no_graph:
             store[precision].refactor_graph(vbuf)   # capture the factorisation graph before the timed steps
         torch.cuda.synchronize()
-        if profile and warmup:
-            profiling(False)
-            kw = kernel_stats(L, ctypes)
-            dom = max(kw, key=lambda k: kw[k]["ms"]) if kw else None
         if profile:
             profiling(True, dom)
         t0 = time.perf_counter()
@@ -205,8 +211,25 @@ def main():
             profiling(False)
             kt = kernel_stats(L, ctypes)
             G.check(L.gfdfd_profile_select(None))
+        leg = {}
+        if legs and precision in store:
+            F = store[precision]
+            tf, tsv = [], []
+            for _ in range(5):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                F.refactor_graph(vbuf) if not args.no_graph else F.refactor(vbuf)
+                torch.cuda.synchronize()
+                tf.append(time.perf_counter() - t1)
+            for _ in range(5):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                F.solve_batch(freqs, Bs, out=Xs2)
+                torch.cuda.synchronize()
+                tsv.append(time.perf_counter() - t1)
+            leg = dict(factor_s=float(np.median(tf)), solve_pass_s=float(np.median(tsv)))
         store.pop(precision, None)                  # release this precision's factors
-        return t, its, (kw or kt), (kt if dom else {})
+        return t, its, kw, (kt if dom else {}), leg
 
     # --- kernel selection: autotuner (before any timed region) ----------------------------
     chosen_by = "flags"
@@ -227,16 +250,17 @@ def main():
     info = pl.info
 
     # --- headline: FP32 fronts + FP64 refinement, batched -----------------------------
-    t32, its32, kst, kdom = timed(G.FP32, True, args.steps, args.warmup, profile=True)
-    log(f"fp32+IR batched: {t32 * 1e3:.1f} ms/step, refinement (iters, relres) per freq {its32}")
+    t32, its32, kst, kdom, leg32 = timed(G.FP32, True, args.steps, max(1, args.warmup), profile=True, legs=True)
+    log(f"fp32+IR batched: {t32 * 1e3:.1f} ms/step, refinement (iters, relres) per freq {its32}, legs {leg32}")
     X32 = Xs.clone() if not args.no_check else None
 
     # --- FP64 fronts, batched, no refinement ---------------------------------------------
     n64 = max(1, args.steps // 2)
     t64, kst64 = None, {}
+    leg64 = {}
     if not args.no_fp64:
-        t64, _, kst64, _ = timed(G.FP64, False, n64, 1, profile=True)
-        log(f"fp64 batched: {t64 * 1e3:.1f} ms/step")
+        t64, _, kst64, _, leg64 = timed(G.FP64, False, n64, 1, profile=True, legs=True)
+        log(f"fp64 batched: {t64 * 1e3:.1f} ms/step, legs {leg64}")
     err = None
     if X32 is not None and t64 is not None:
         err = []
@@ -244,24 +268,31 @@ def main():
             d = torch.linalg.vector_norm(X32[f] - Xs[f], dim=1) / torch.linalg.vector_norm(Xs[f], dim=1)
             err.append(float(d.max()))
         log(f"fp32+IR vs fp64 rel wavefield error per freq: {err}")
-        del X32
+    X32_last = X32[nf - 1].clone() if X32 is not None else None
+    del X32
 
     # --- plain FP64: what the drop-in executes (per-frequency factor, one solve per shot) ---
-    tplain = None
+    tplain, t1solve = None, None
     if not args.no_plain:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        tsolves = 0.0
         for f in range(nf):
             vals = pl.assemble(w.physics, w.dh, [w.freqs[f]], m1, m2, s=w.s, npml=w.npml, a0_pml=w.a0_pml)
             F = pl.factor(vals, G.FP64)
             G.check(L.gfdfd_spike_panel(n, ns, ctypes.c_void_p(ksrc.data_ptr()), amps[f].real, amps[f].imag,
                                         ctypes.c_void_p(Bs[f].data_ptr()), ctypes.c_void_p(st.cuda_stream)))
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
             for s in range(ns):
                 F.solve(0, Bs[f][s], out=Xs[f][s])
+            torch.cuda.synchronize()
+            tsolves += time.perf_counter() - t1
             del F
         torch.cuda.synchronize()
         tplain = time.perf_counter() - t0
-        log(f"plain fp64 per-shot: {tplain * 1e3:.1f} ms")
+        t1solve = tsolves / (nf * ns)
+        log(f"plain fp64 per-shot: {tplain * 1e3:.1f} ms, {t1solve * 1e3:.3f} ms per single-RHS solve")
 
     if args.copy_probe:
         # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
@@ -274,8 +305,7 @@ def main():
         del src, dst
 
     # --- roofline of the dominant kernel, event-timed over the timed steps ------------------
-    # (kst: every class over the warm-up steps; kdom: the dominant class over the timed steps)
-    kst_steps = args.warmup if args.warmup else args.steps
+    # (kst: every class over the first warm-up step; kdom: the dominant class over the timed steps)
     src = kdom if kdom else kst
     dom = max(src, key=lambda k: src[k]["ms"]) if src else None
     roof = None
@@ -291,15 +321,36 @@ def main():
             roof["traffic_source"] = tr["source"]
 
     # --- CPU baseline ---------------------------------------------------------------------------
-    cpu = None
+    cpu, err_oracle = None, None
     if not args.no_cpu:
         try:
             from oracle import cpu_baseline
-            cpu = cpu_baseline.measure(w, threads=args.cpu_threads or None)
+            cpu, idx, Xo = cpu_baseline.measure(w, threads=args.cpu_threads or None)
+            if X32_last is not None:
+                # the headline path's wavefields against the oracle's FP64 solutions of the same shots
+                Xd = X32_last[idx].cpu().numpy()
+                err_oracle = max(float(np.linalg.norm(Xd[r] - Xo[r]) / np.linalg.norm(Xo[r]))
+                                 for r in range(len(idx)))
+                log(f"fp32+IR vs oracle ({len(idx)} shots at {w.freqs[-1]:g} Hz): {err_oracle:.2e}")
         except Exception as e:                       # reported, never fatal
             log(f"cpu baseline unavailable: {e}")
 
     value = nf * ns / t32
+    es32, es64 = 8.0, 16.0
+    fac_flops = nf * info.factor_flops
+    sweep_flops = 8.0 * info.nnz_lu * ns * nf
+    def legs_of(leg, es, peak):
+        if not leg:
+            return None
+        by = (info.nnz_lu * es + 2.0 * n * ns * es) * nf
+        return {"factor_ms": round(leg["factor_s"] * 1e3, 3),
+                "factor_tflops": round(fac_flops / leg["factor_s"] / 1e12, 2),
+                "factor_mfma_frac": round(fac_flops / leg["factor_s"] / peak, 4),
+                "solve_pass_ms": round(leg["solve_pass_s"] * 1e3, 3),
+                "solve_tflops": round(sweep_flops / leg["solve_pass_s"] / 1e12, 2),
+                "solve_mfma_frac": round(sweep_flops / leg["solve_pass_s"] / peak, 4),
+                "solve_gbps": round(by / leg["solve_pass_s"] / 1e9, 1),
+                "solve_hbm_frac": round(by / leg["solve_pass_s"] / PEAK["hbm"], 4)}
     res = {
         "metric": "shot-solves/sec (factor+solve) FP32+IR vs FP64 LU at rel err <=1e-8; MFMA %",
         "value": round(value, 2),
@@ -318,17 +369,26 @@ def main():
                    "n": n, "nnz": pl.nnz, "fronts": info.fronts, "factor_gflop_per_freq": info.factor_flops / 1e9,
                    "nnz_lu": info.nnz_lu, "refine_tol": args.tol,
                    "tuning": dict(cfg, chosen_by=chosen_by)},
+        "legs": legs_of(leg32, es32, PEAK["fp32_mfma"]),
+        "legs_note": "factor_*: one batched factorisation of all frequencies (graph replay), against the dense "
+                     "MFMA peak of the factor precision; solve_*: one batched forward+backward solve pass of all "
+                     "panels without refinement (incl. the caller-layout transposes), flops 8 nnz(L+U) per RHS, "
+                     "bytes nnz(L+U) + 2 n nrhs entries",
         "fp64_batched": None if t64 is None else {
             "ms_per_step": round(t64 * 1e3, 3), "value": round(nf * ns / t64, 2),
+            "legs": legs_of(leg64, es64, PEAK["fp64_mfma"]),
             "kernels_ms_per_step": {k: round(v["ms"], 3) for k, v in kst64.items()}},
-        "plain_fp64_per_shot": None if tplain is None else {"ms": round(tplain * 1e3, 3),
-                                                             "value": round(nf * ns / tplain, 2)},
+        "plain_fp64_per_shot": None if tplain is None else {
+            "ms": round(tplain * 1e3, 3), "value": round(nf * ns / tplain, 2),
+            "ms_per_single_rhs_solve": round(t1solve * 1e3, 4),
+            "solve_hbm_frac": round(info.nnz_lu * es64 / t1solve / PEAK["hbm"], 4)},
         "speedup_vs_plain_fp64": None if tplain is None else round(tplain / t32, 3),
         "refinement": [{"iters": a, "relres": b} for a, b in its32],
         "max_rel_err_vs_fp64": None if err is None else max(err),
+        "max_rel_err_vs_oracle": err_oracle,
         "roofline": roof,
-        "kernels_note": "every kernel class event-timed over the warm-up step(s), per step",
-        "kernels": {k: {"ms_per_step": round(v["ms"] / kst_steps, 3), "launches_per_step": v["launches"] / kst_steps,
+        "kernels_note": "every kernel class event-timed over the first warm-up step (eager factorisation)",
+        "kernels": {k: {"ms_per_step": round(v["ms"], 3), "launches_per_step": v["launches"],
                         "tflops": round(v["flops"] / max(v["ms"], 1e-9) * 1e-9, 3),
                         "gbps": round(v["bytes"] / max(v["ms"], 1e-9) * 1e-6, 1)} for k, v in kst.items()},
         "cpu_baseline": cpu,

@@ -10,24 +10,35 @@
 namespace gfdfd {
 
 namespace {
-std::mutex mem_mu;
-std::unordered_map<void *, size_t> mem_live;
-size_t mem_now = 0, mem_peak = 0;
+// Allocated once and never destroyed: device buffers held by other static
+// objects (the plan cache) are freed during static destruction at exit, in
+// an order across translation units that C++ leaves open.
+struct MemBook {
+    std::mutex mu;
+    std::unordered_map<void *, size_t> live;
+    size_t now = 0, peak = 0;
+};
+MemBook &mem_book() {
+    static MemBook *b = new MemBook();
+    return *b;
+}
 }  // namespace
 
 void mem_track_alloc(void *p, size_t bytes) {
-    std::lock_guard<std::mutex> g(mem_mu);
-    mem_live[p] = bytes;
-    mem_now += bytes;
-    if (mem_now > mem_peak) mem_peak = mem_now;
+    MemBook &b = mem_book();
+    std::lock_guard<std::mutex> g(b.mu);
+    b.live[p] = bytes;
+    b.now += bytes;
+    if (b.now > b.peak) b.peak = b.now;
 }
 
 void mem_track_free(void *p) {
-    std::lock_guard<std::mutex> g(mem_mu);
-    auto it = mem_live.find(p);
-    if (it == mem_live.end()) return;
-    mem_now -= it->second;
-    mem_live.erase(it);
+    MemBook &b = mem_book();
+    std::lock_guard<std::mutex> g(b.mu);
+    auto it = b.live.find(p);
+    if (it == b.live.end()) return;
+    b.now -= it->second;
+    b.live.erase(it);
 }
 
 const char *const kClassNames[K_NCLASS] = {
@@ -114,10 +125,11 @@ using namespace gfdfd;
 extern "C" {
 
 int gfdfd_device_memory(long long *current_bytes, long long *peak_bytes, int reset_peak) {
-    std::lock_guard<std::mutex> g(mem_mu);
-    if (current_bytes) *current_bytes = (long long)mem_now;
-    if (peak_bytes) *peak_bytes = (long long)mem_peak;
-    if (reset_peak) mem_peak = mem_now;
+    MemBook &b = mem_book();
+    std::lock_guard<std::mutex> g(b.mu);
+    if (current_bytes) *current_bytes = (long long)b.now;
+    if (peak_bytes) *peak_bytes = (long long)b.peak;
+    if (reset_peak) b.peak = b.now;
     return GFDFD_OK;
 }
 

@@ -85,12 +85,18 @@ __global__ void k_adjoint(int n, int nshots, int ntr, const int *__restrict__ kr
     if (threadIdx.x == 0) {
         cf32 w = {1.0f, 0.0f};
         if (stf_inv) {
+            // conj() is the double-precision function, so each product is a
+            // complex double (float x float products are exact in double; the
+            // sums round once) and `wiennom += ...` adds in double before
+            // rounding back to the complex float accumulator (:35-36)
             cf32 nom = {0.0f, 0.0f}, den = {0.0f, 0.0f};
             for (int t = 0; t < ntr; t++) {
                 const cx<float> o = pobs[(int64_t)s * ntr + t], c = pcal[(int64_t)s * ntr + t];
-                const cf32 a = cf_mul({c.x, -c.y}, {o.x, o.y}), d = cf_mul({o.x, -o.y}, {o.x, o.y});
-                nom = {nom.r + a.r, nom.i + a.i};
-                den = {den.r + d.r, den.i + d.i};
+                const double cr = c.x, ci = -(double)c.y, orr = o.x, oi = o.y;
+                nom.r = (float)((double)nom.r + (cr * orr - ci * oi));
+                nom.i = (float)((double)nom.i + (cr * oi + ci * orr));
+                den.r = (float)((double)den.r + (orr * orr - (-oi) * oi));
+                den.i = (float)((double)den.i + (orr * oi + (-oi) * orr));
             }
             w = cf_div(nom, den);
         }

@@ -558,11 +558,14 @@ def parabolic_line_search(obj, L2, alpha, scalefac=2.0, stepmax=6):
 
 
 def line_search(plan, dh, freqs, vp, rho, direction, ksrc, krec, pobs, L2, eps_scale, lo, hi, scalefac=2.0,
-                stepmax=6, ws=None, **kw):
+                stepmax=6, ws=None, alpha0=None, **kw):
     """One FWI iteration's step search for the acoustic model (src/AC/fwi_FD_AC.c
     calling parabolicls_AC): each trial steps vp along direction on the device
     (model_update into one reused buffer) and evaluates objective() on the same
     plan and factor storage (ws, or one workspace for the whole search).
+    The first trial step is EPS_SCALE * max(vp) / max(direction) with signed
+    maxima (src/AC/parabolicls_AC.c:35-37), or alpha0 when given (the signed
+    rule degenerates for a direction whose entries are almost all negative).
     Returns (step, trials); the caller applies model_update(step, ...)."""
     if ws is None:
         ws = Workspace(plan, len(freqs), kw.get("precision", FP64))
@@ -570,7 +573,8 @@ def line_search(plan, dh, freqs, vp, rho, direction, ksrc, krec, pobs, L2, eps_s
     def obj(alpha):
         model_update(alpha, direction, vp, lo, hi, out=trial)
         return objective(plan, AC, dh, freqs, trial, rho, ksrc, krec, pobs, ws=ws, **kw)
-    return parabolic_line_search(obj, L2, initial_step(direction, vp, eps_scale), scalefac, stepmax)
+    a0 = initial_step(direction, vp, eps_scale) if alpha0 is None else alpha0
+    return parabolic_line_search(obj, L2, a0, scalefac, stepmax)
 
 
 class LBFGS:
@@ -635,7 +639,7 @@ def tikhonov_te(sigma, eps, dh, beta, norm, lam, grad_sigma=None, grad_eps=None,
 
 
 def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, pobs, L2, eps_scale, norm,
-                   low, up, scalefac=2.0, stepmax=6, tikhonov=None, ws=None, **kw):
+                   low, up, scalefac=2.0, stepmax=6, tikhonov=None, ws=None, alpha0=None, **kw):
     """One FWI iteration's step search for the TE model (parabolicls_TE,
     src/TE/parabolicls_TE.c:28-160): both materials step by the same alpha on
     their normalised scales (norm, low, up: (MAT1 for sigma, MAT2 for epsilon)
@@ -644,7 +648,8 @@ def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, 
     each trial's misfit comes from objective() on the same plan, plus the
     Tikhonov term (Tikhonov_cost_TE) when tikhonov = (beta, lam) pairs are
     given and a lam is positive.  L2 is the start model's misfit with that
-    term.  Returns (step, trials)."""
+    term.  alpha0, when given, replaces the signed-maximum first step.
+    Returns (step, trials)."""
     f32 = np.float32
     if not (norm[0] > 0 and norm[1] > 0):
         raise ValueError("line_search_te: the normalisations must be positive")
@@ -661,7 +666,8 @@ def line_search_te(plan, dh, freqs, eps, sigma, dir_eps, dir_sigma, ksrc, krec, 
             l2 += tikhonov[1][0] * cs + tikhonov[1][1] * ce
         return l2
     # max(a * sigma) = a * max(sigma) for a > 0: float multiplication is monotone
-    alpha0 = f32(eps_scale) * (f32(1.0 / norm[0]) * f32(float(sigma.max()))) / f32(float(dir_sigma.max()))
+    if alpha0 is None:
+        alpha0 = f32(eps_scale) * (f32(1.0 / norm[0]) * f32(float(sigma.max()))) / f32(float(dir_sigma.max()))
     return parabolic_line_search(obj, L2, alpha0, scalefac, stepmax)
 
 

@@ -378,7 +378,9 @@ void orc_source_amp(int physics, float freq, float s, float dh, double *re, doub
 /* src/AC/calc_res_AC.c:10-95 in full, for one shot.  misfit 1 (L2), 2
  * (logarithmic, :66), 3 (phase only, :70); rtm != 0 is INVMAT=2 (res = pobs,
  * :77); stf_inv != 0 estimates the source-time function by the Wiener quotient
- * sum conj(pcal) pobs / sum conj(pobs) pobs in complex float (:21-40), else it
+ * sum conj(pcal) pobs / sum conj(pobs) pobs (:21-40; conj() is the double
+ * function, so each product is complex double and each += adds in double
+ * before rounding to the complex float sums), else it
  * is 1 (:43-45).  The estimate is returned in stf[0..1] (waveAC.stfr/stfi,
  * :47-48, which ass_grad_AC/TE read).  Residuals are muted where
  * pobsr<1e-20 && pobsi<1e-20 (:82-85, signed); the returned sum |res|^2 runs

@@ -252,8 +252,12 @@ def test_ac_801x201_fwi_gradient_and_lbfgs(precision):
         fdfd.grad_frame(grad, nx0, ny0, w.npml, 0)
         direction = lb.direction(grad, vp)
         fdfd.grad_frame(direction, nx0, ny0, w.npml, 0)
+        # first trial: a change of 1% of max(vp) at the largest |direction| (the reference's
+        # signed-maximum rule gives no usable step for a direction that is almost all negative)
+        a0 = 0.01 * float(vp.max()) / float(direction.abs().max())
         step, trials = fdfd.line_search(pl, w.dh, w.freqs, vp, rho, direction, w.src, w.rec, pobs, L2, 0.01,
-                                        1500.0, 7000.0, npml=w.npml, precision=precision, tol=tol, ws=ws)
+                                        1500.0, 7000.0, npml=w.npml, precision=precision, tol=tol, ws=ws,
+                                        alpha0=a0)
         say(f"  l-BFGS iteration {it + 1}: misfit {L2:.6e}, step {step:.4g}, {len(trials)} trials")
         assert step > 0.0
         vp = fdfd.model_update(step, direction, vp, 1500.0, 7000.0)
