@@ -33,9 +33,9 @@ PEAK = {"fp64_mfma": 78.6e12, "fp32_mfma": 157.3e12, "hbm": 8.0e12}
 KERNELS = ["assemble", "front_assemble", "scatter_orig", "small_lu", "panel", "panel_apply", "trsm_u", "gemm_update", "diag_inv",
            "fwd_gather", "fwd_leaf", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk", "bwd_small1", "bwd_front1", "bwd_front",
            "bwd_contrib", "bwd_blk",
-           "piv_unswap", "fwd_vec", "bwd_vec", "permute_in", "permute_out", "residual", "norm_reduce", "axpy"]
+           "piv_unswap", "fwd_vec", "bwd_vec", "permute_in", "permute_out", "residual", "norm_reduce", "axpy", "subtree"]
 # kernels whose arithmetic runs on the matrix cores (at the factor precision)
-MFMA_KERNELS = {"small_lu", "panel", "trsm_u", "gemm_update", "fwd_leaf", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk",
+MFMA_KERNELS = {"small_lu", "subtree", "panel", "trsm_u", "gemm_update", "fwd_leaf", "fwd_small1", "fwd_front1", "fwd_front", "fwd_blk",
                 "bwd_small1", "bwd_front1", "bwd_front", "bwd_contrib", "bwd_blk"}
 
 

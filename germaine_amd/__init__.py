@@ -63,7 +63,7 @@ SIGNATURES = {
     "gfdfd_refactor": (_I, [_P, _P, _P]),
     "gfdfd_refactor_graph": (_I, [_P, _P, _P]),
     "gfdfd_solve_tuning": (_I, [_I, _I, _I]),
-    "gfdfd_factor_tuning": (_I, [_I]),
+    "gfdfd_factor_tuning": (_I, [_I, _I]),
     "gfdfd_factors_free": (_V, [C.POINTER(_P)]),
     "gfdfd_factors_singular": (_I, [_P, _P]),
     "gfdfd_solve": (_I, [_P, _I, _I, _I, _P, _P, _P]),
@@ -92,6 +92,7 @@ SIGNATURES = {
     "gfdfd_grad_te_stf": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_profile": (_I, [_I]),
     "gfdfd_device_memory": (_I, [_LLP, _LLP, _I]),
+    "gfdfd_release_workspace": (_I, []),
     "gfdfd_profile_select": (_I, [C.c_char_p]),
     "gfdfd_profile_reset": (_V, []),
     "gfdfd_profile_read": (_I, [C.c_char_p, _DP, _LLP, _DP, _DP]),

@@ -88,6 +88,7 @@ std::unique_ptr<Factors<T>> make_factors(std::shared_ptr<DevPlan> dp, int nfreq,
     F->rowperm = dev_alloc<int>((size_t)pl.n * nfreq);
     F->sing = dev_alloc<int>(nfreq);
     F->gsrc = dev_alloc<int>((size_t)pl.fv_rows * 3 * nfreq);
+    if (dp->nroots > 0) F->sched = dev_alloc<int>(1);
     if (copy_vals) {
         F->vals = dev_alloc<cx<double>>((size_t)pl.nnz * nfreq);
         F->own_vals = true;
@@ -308,9 +309,17 @@ int gfdfd_factor(gfdfd_plan plan, int nfreq, const double *d_vals, int precision
     GUARD_END
 }
 
-int gfdfd_factor_tuning(int panel_lds_cap) {
+int gfdfd_release_workspace(void) {
+    GUARD_BEGIN
+    GFDFD_HIP(hipDeviceSynchronize());
+    release_solve_workspaces();
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_factor_tuning(int panel_lds_cap, int persistent) {
     if (panel_lds_cap < 0) return report(GFDFD_ERR_ARG, "gfdfd_factor_tuning: negative panel LDS cap");
-    set_factor_tuning(panel_lds_cap);
+    set_factor_tuning(panel_lds_cap, persistent);
     return GFDFD_OK;
 }
 

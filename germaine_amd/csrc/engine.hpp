@@ -13,6 +13,8 @@ namespace gfdfd {
 constexpr int kSingleMmax = 300;   // largest front order of a single-block level (its panels fit in LDS)
 constexpr int kTile = 64;       // output tile of the trailing-update GEMM
 constexpr int kMaxBatch = 64;   // frequencies per batched solve
+constexpr int kSubMaxM = 64;    // largest front order of the persistent subtree launch
+constexpr int kAsmMaxChildren = 8;   // children a front may have (assembly row maps in LDS)
 
 // Plan arrays uploaded once per grid size and shared by every factorisation.
 struct DevPlan {
@@ -46,7 +48,27 @@ struct DevPlan {
     std::vector<FrontDev *> lv_fr;                  // per level: the level's fronts, contiguous
     std::vector<int> lv_children;                   // per level: 1 if any front has children
 
+    // Small subtrees (fronts whose whole subtree has m <= small_max), factored
+    // by one persistent launch: a task is one maximal such subtree (a
+    // contiguous postorder range of fronts) for one frequency, taken from a
+    // global counter by a workgroup that factors its fronts leaves to root.
+    // The level path takes the remaining fronts.
+    int nsub = 0, nroots = 0, sub_mmax = 0;
+    int *root_first = nullptr, *root_last = nullptr;   // [nroots] front range of each subtree
+    int *slot_of = nullptr;                         // [nfronts] subtree-front slot, -1 outside the subtrees
+    int64_t *sub_optr = nullptr;                    // [nsub + 1] ranges of the slots' original entries
+    int64_t *sub_oq = nullptr;                      // CSC index of each such entry
+    int *sub_ooff = nullptr;                        // its local offset r + c m in the front
+    double sub_flops = 0, sub_elems = 0;            // algorithmic counts of the launch (per frequency)
+    std::vector<int *> lv_asm_rest, lv_small_rest;  // per level: fronts outside the subtrees
+    std::vector<int> n_asm_rest, n_small_rest, mmax_asm_rest, small_mmax_rest;
+    int64_t *orig_q_rest = nullptr, *orig_dst_rest = nullptr;
+    std::vector<std::pair<int64_t, int64_t>> orig_rest_range;   // per level [begin, end)
+    std::vector<double> asm_rest_elems;             // per level: entries the level path assembles
+    std::vector<double> small_rest_flops, small_rest_elems;   // per level: the level path's small fronts
+
     explicit DevPlan(std::shared_ptr<const Plan> p);
+    void build_subtrees(const Plan &pl);
     ~DevPlan();
     DevPlan(const DevPlan &) = delete;
     DevPlan &operator=(const DevPlan &) = delete;
@@ -65,6 +87,7 @@ struct Factors {
     int *gsrc = nullptr;            // nfreq * fv_rows * 3: gather sources after pivoting (forward sweep of M)
     cx<double> *vals = nullptr;     // nfreq * nnz FP64 values of M (CSC order) kept for refinement
     bool own_vals = false;
+    int *sched = nullptr;           // task counter of the subtree launch
     hipGraphExec_t graph = nullptr; // captured refactorisation (gfdfd_refactor_graph)
     const void *graph_src = nullptr;   // the values buffer it reads
     ~Factors();
@@ -94,8 +117,11 @@ void cast_c64_to_c128(int64_t count, const cx<float> *a, cx<double> *b, hipStrea
 // (0: elimination order, 1: user layout with a transpose per correction)
 void set_solve_tuning(int rw_cap, int fused_mmax, int refine_layout);
 // factorisation kernel selection: panels needing more LDS than panel_lds_cap
-// bytes (0: the hardware limit) take the global-memory panel kernel
-void set_factor_tuning(int panel_lds_cap);
+// bytes (0: the hardware limit) take the global-memory panel kernel; the small
+// subtrees take the persistent launch when persistent != 0, else the level path
+void set_factor_tuning(int panel_lds_cap, int persistent);
+// frees the calling thread's cached solve work buffers
+void release_solve_workspaces();
 
 struct RefineStats {
     int iterations = 0;

@@ -27,6 +27,123 @@ namespace gfdfd {
 // ---------------------------------------------------------------------------
 // DevPlan
 // ---------------------------------------------------------------------------
+// Small subtrees: a front belongs when its order is at most the plan's
+// small_max and all its children belong (leaves included), so a whole subtree
+// is factored by the persistent launch and the level path never waits on it.
+// Fronts are in postorder, so each maximal subtree is a contiguous range
+// ending at its root.
+void DevPlan::build_subtrees(const Plan &pl) {
+    const int nfr = (int)pl.fronts.size();
+    std::vector<char> in(nfr, 0);
+    for (int f = 0; f < nfr; f++) {   // postorder: children first
+        const Front &F = pl.fronts[f];
+        bool ok = pl.grid && F.m <= pl.small_max && F.m <= kSubMaxM && F.nchild <= kAsmMaxChildren;
+        for (int c = 0; c < F.nchild && ok; c++) ok = in[pl.children[F.child_off + c]] != 0;
+        in[f] = ok;
+    }
+    std::vector<int> list, slot(nfr, -1), first, last, lo(nfr);
+    for (int f = 0; f < nfr; f++) {   // first (lowest) front of f's subtree
+        const Front &F = pl.fronts[f];
+        lo[f] = f;
+        for (int c = 0; c < F.nchild; c++) lo[f] = std::min(lo[f], lo[pl.children[F.child_off + c]]);
+    }
+    for (int f = 0; f < nfr; f++)
+        if (in[f]) {
+            slot[f] = (int)list.size();
+            list.push_back(f);
+            const int par = pl.fronts[f].parent;
+            if (par < 0 || !in[par]) {
+                for (int g = lo[f]; g <= f; g++)
+                    if (!in[g]) throw std::logic_error("small subtree is not a contiguous postorder range");
+                first.push_back(lo[f]);
+                last.push_back(f);
+            }
+        }
+    nsub = (int)list.size();
+    nroots = (int)first.size();
+    // original entries: the subtrees' by slot (local offsets), the rest by level
+    std::vector<std::vector<std::pair<int64_t, int>>> per_slot(nsub);
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> rest(pl.nlevels);
+    std::vector<std::pair<int64_t, int>> starts;   // buf_off -> front
+    for (int f = 0; f < nfr; f++) starts.push_back({pl.fronts[f].buf_off, f});
+    std::sort(starts.begin(), starts.end());
+    for (size_t e = 0; e < pl.orig_q.size(); e++) {
+        const int64_t d = pl.orig_dst[e];
+        auto it = std::upper_bound(starts.begin(), starts.end(), std::make_pair(d, INT32_MAX));
+        if (it == starts.begin()) throw std::logic_error("original entry before the first front");
+        const int f = std::prev(it)->second;
+        const Front &F = pl.fronts[f];
+        if (d - F.buf_off >= (int64_t)F.m * F.m) throw std::logic_error("original entry outside its front");
+        if (slot[f] >= 0) per_slot[slot[f]].push_back({pl.orig_q[e], (int)(d - F.buf_off)});
+        else rest[F.level].push_back({pl.orig_q[e], d});
+    }
+    std::vector<int64_t> optr(nsub + 1, 0), oq;
+    std::vector<int> ooff;
+    for (int k = 0; k < nsub; k++) {
+        for (auto &e : per_slot[k]) { oq.push_back(e.first); ooff.push_back(e.second); }
+        optr[k + 1] = (int64_t)oq.size();
+    }
+    std::vector<int64_t> rq, rd;
+    for (int l = 0; l < pl.nlevels; l++) {
+        const int64_t b = (int64_t)rq.size();
+        for (auto &e : rest[l]) { rq.push_back(e.first); rd.push_back(e.second); }
+        orig_rest_range.push_back({b, (int64_t)rq.size()});
+    }
+    root_first = dev_upload(first);
+    root_last = dev_upload(last);
+    slot_of = dev_upload(slot);
+    sub_optr = dev_upload(optr);
+    sub_oq = dev_upload(oq);
+    sub_ooff = dev_upload(ooff);
+    orig_q_rest = dev_upload(rq);
+    orig_dst_rest = dev_upload(rd);
+    bytes += (2 * first.size() + slot.size() + ooff.size()) * 4 + (optr.size() + oq.size() + rq.size() + rd.size()) * 8;
+    for (int f : list) {
+        const Front &F = pl.fronts[f];
+        sub_mmax = std::max(sub_mmax, F.m);
+        for (int k = 0; k < F.p; k++) {
+            const double t = F.m - k - 1;
+            sub_flops += 4.0 * (2.0 * t * t + t);
+        }
+        sub_elems += (double)F.m * F.m;
+    }
+    // the level path's lists of the remaining fronts
+    for (int l = 0; l < pl.nlevels; l++) {
+        const LevelPlan &L = pl.levels[l];
+        std::vector<int> a, sm;
+        int mm = 0, smm = 0;
+        double el = 0, sfl = 0, sel = 0;
+        for (int f : L.small)
+            if (slot[f] < 0) {
+                const Front &F = pl.fronts[f];
+                a.push_back(f);
+                sm.push_back(f);
+                mm = std::max(mm, F.m);
+                smm = std::max(smm, F.m);
+                el += (double)F.m * F.m;
+                sel += 2.0 * F.m * F.m;
+                for (int k = 0; k < F.p; k++) {
+                    const double t = F.m - k - 1;
+                    sfl += 4.0 * (2.0 * t * t + t);
+                }
+            }
+        for (int f : L.large) {
+            a.push_back(f);
+            mm = std::max(mm, pl.fronts[f].m);
+            el += (double)pl.fronts[f].m * pl.fronts[f].m;
+        }
+        lv_asm_rest.push_back(dev_upload(a));
+        lv_small_rest.push_back(dev_upload(sm));
+        n_asm_rest.push_back((int)a.size());
+        n_small_rest.push_back((int)sm.size());
+        mmax_asm_rest.push_back(mm);
+        small_mmax_rest.push_back(smm);
+        asm_rest_elems.push_back(el);
+        small_rest_flops.push_back(sfl);
+        small_rest_elems.push_back(sel);
+    }
+}
+
 DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
     const Plan &pl = *host;
     std::vector<FrontDev> fd(pl.fronts.size());
@@ -122,6 +239,7 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
     for (size_t l = 0; l < pl.levels.size(); l++)
         single.push_back(pmax_all[l] <= 32 && mmax_all[l] <= kSingleMmax && max_children <= 2 ? 1 : 0);
     lv_single = dev_upload(single);
+    build_subtrees(pl);
     counts.resize(pl.levels.size());
     for (size_t l = 0; l < pl.levels.size(); l++) {
         const LevelPlan &L = pl.levels[l];
@@ -184,6 +302,10 @@ DevPlan::~DevPlan() {
     dev_free(orig_q); dev_free(orig_dst); dev_free(colptr); dev_free(rowind); dev_free(tq); dev_free(tiles);
     dev_free(lv_single);
     dev_free(gsrc_id);
+    dev_free(root_first); dev_free(root_last); dev_free(slot_of); dev_free(sub_optr); dev_free(sub_oq);
+    dev_free(sub_ooff); dev_free(orig_q_rest); dev_free(orig_dst_rest);
+    for (auto p : lv_asm_rest) dev_free(p);
+    for (auto p : lv_small_rest) dev_free(p);
     for (auto p : lv_fr) dev_free(p);
     for (auto p : lv_small) dev_free(p);
     for (auto p : lv_large) dev_free(p);
@@ -198,6 +320,7 @@ Factors<T>::~Factors() {
     dev_free(rowperm);
     dev_free(sing);
     dev_free(gsrc);
+    dev_free(sched);
     if (own_vals) dev_free(vals);
 }
 template struct Factors<double>;
@@ -206,7 +329,6 @@ template struct Factors<float>;
 // ---------------------------------------------------------------------------
 // 1+2: front assembly
 // ---------------------------------------------------------------------------
-constexpr int kAsmMaxChildren = 8;
 template <typename T>
 __global__ void __launch_bounds__(256) k_front_assemble(cx<T> *__restrict__ arena, int64_t fstride,
                                                         const FrontDev *__restrict__ fronts,
@@ -290,21 +412,13 @@ __device__ int block_argmax(T best, int bi, T *sb, int *si) {
 // ---------------------------------------------------------------------------
 // 3a: small fronts, one workgroup, front in LDS
 // ---------------------------------------------------------------------------
+// Partial LU of an m x m front held column-major in LDS (p pivots, row
+// pivoting within the p fully-summed rows); the swap of step j goes to piv[j].
+// All threads of the workgroup call it.
 template <typename T>
-__global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int64_t fstride,
-                                                  const FrontDev *__restrict__ fronts, const int *__restrict__ list,
-                                                  int *__restrict__ piv, int n, int *__restrict__ sing) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cx<T> *S = reinterpret_cast<cx<T> *>(smem_raw);
-    __shared__ int si[1];
-    const FrontDev F = fronts[list[blockIdx.x]];
-    const int fq = blockIdx.y;
-    cx<T> *G = arena + fq * fstride + F.buf_off;
-    const int m = F.m, p = F.p, mm = m * m, tid = threadIdx.x, bs = blockDim.x;
-    __shared__ cx<T> pv[2];   // pivot entry and the entry it swaps with, published with the pivot row
-#pragma unroll 4
-    for (int e = tid; e < mm; e += bs) S[e] = G[e];
-    __syncthreads();
+__device__ void small_lu_lds(cx<T> *S, int m, int p, int *__restrict__ piv, int *__restrict__ sing, cx<T> *pv,
+                             int *si) {
+    const int tid = threadIdx.x, bs = blockDim.x;
     for (int j = 0; j < p; j++) {
         // pivot search by wave 0 alone (p <= 64 candidates: one per lane)
         if (tid < 64) {
@@ -320,8 +434,8 @@ __global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int
                 si[0] = bi;
                 pv[0] = S[bi + j * m];
                 pv[1] = S[j + j * m];
-                piv[(int64_t)fq * n + F.pos0 + j] = bi;
-                if (best == T(0)) atomicOr(sing + fq, 1);
+                piv[j] = bi;
+                if (best == T(0)) atomicOr(sing, 1);
             }
         }
         __syncthreads();
@@ -348,7 +462,106 @@ __global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int
         }
         __syncthreads();
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int64_t fstride,
+                                                  const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                  int *__restrict__ piv, int n, int *__restrict__ sing) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *S = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ int si[1];
+    __shared__ cx<T> pv[2];   // pivot entry and the entry it swaps with, published with the pivot row
+    const FrontDev F = fronts[list[blockIdx.x]];
+    const int fq = blockIdx.y;
+    cx<T> *G = arena + fq * fstride + F.buf_off;
+    const int m = F.m, mm = m * m, tid = threadIdx.x, bs = blockDim.x;
+#pragma unroll 4
+    for (int e = tid; e < mm; e += bs) S[e] = G[e];
+    __syncthreads();
+    small_lu_lds<T>(S, m, F.p, piv + (int64_t)fq * n + F.pos0, sing + fq, pv, si);
     for (int e = tid; e < mm; e += bs) G[e] = S[e];
+}
+
+// ---------------------------------------------------------------------------
+// 3a': the small subtrees in one persistent launch.  A task is one maximal
+// small subtree (fronts first..last, children before parents) for one
+// frequency; workgroups take tasks from a global counter until none is left.
+// A workgroup factors its subtree's fronts in order: each front is assembled
+// in LDS from its children's contribution blocks -- written by this same
+// workgroup earlier in the task, so a workgroup barrier is all the
+// synchronisation there is -- and its original entries, factored there and
+// written to the arena.  No workgroup ever waits for another.  Same
+// arithmetic, in the same order, as k_front_assemble + k_scatter_orig +
+// k_small_lu.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_subtree_factor(cx<T> *__restrict__ arena, int64_t fstride,
+                                                        const FrontDev *__restrict__ fronts,
+                                                        const int *__restrict__ root_first,
+                                                        const int *__restrict__ root_last,
+                                                        const int *__restrict__ slot_of,
+                                                        const int *__restrict__ children, const int *__restrict__ inv,
+                                                        const int64_t *__restrict__ optr,
+                                                        const int64_t *__restrict__ oq, const int *__restrict__ ooff,
+                                                        const cx<double> *__restrict__ vals, int64_t vstride,
+                                                        int nroots, int nf, int *__restrict__ piv, int n,
+                                                        int *__restrict__ sing, int *__restrict__ counter) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *S = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ int cmap[kAsmMaxChildren][kSubMaxM];
+    __shared__ int64_t cbase[kAsmMaxChildren];
+    __shared__ int cstride[kAsmMaxChildren];
+    __shared__ int task;
+    __shared__ int si[1];
+    __shared__ cx<T> pv[2];
+    const int tid = threadIdx.x, ntask = nroots * nf;
+    for (;;) {
+        if (tid == 0) task = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = task;
+        if (t >= ntask) return;
+        const int r = t / nf, fq = t - r * nf;
+        cx<T> *A = arena + (int64_t)fq * fstride;
+        const cx<double> *V = vals + (int64_t)fq * vstride;
+        const int f0 = root_first[r], f1 = root_last[r];
+        for (int f = f0; f <= f1; f++) {
+            const FrontDev F = fronts[f];
+            const int m = F.m, nc = F.nchild, mm = m * m, slot = slot_of[f];
+            if (tid < nc) {
+                const FrontDev C = fronts[children[F.child_off + tid]];
+                cbase[tid] = C.buf_off + C.p + (int64_t)C.p * C.m;   // contribution block origin
+                cstride[tid] = C.m;
+            }
+            for (int c = 0; c < nc; c++) {
+                const int io = fronts[children[F.child_off + c]].inv_off;
+                for (int i = tid; i < m; i += blockDim.x) cmap[c][i] = inv[io + i];
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int e = tid; e < mm; e += blockDim.x) {
+                const int c2 = e / m, i = e - c2 * m;
+                cx<T> v = {T(0), T(0)};
+                for (int c = 0; c < nc; c++) {
+                    const int a = cmap[c][i], b = cmap[c][c2];
+                    if (a >= 0 && b >= 0) v = cadd(v, A[cbase[c] + a + (int64_t)b * cstride[c]]);
+                }
+                S[e] = v;
+            }
+            __syncthreads();
+            for (int64_t e = optr[slot] + tid; e < optr[slot + 1]; e += blockDim.x) {
+                const cx<double> v = V[oq[e]];
+                cx<T> &d = S[ooff[e]];
+                d.x += (T)v.x;
+                d.y += (T)v.y;
+            }
+            __syncthreads();
+            small_lu_lds<T>(S, m, F.p, piv + (int64_t)fq * n + F.pos0, sing + fq, pv, si);
+            cx<T> *G = A + F.buf_off;
+            for (int e = tid; e < mm; e += blockDim.x) G[e] = S[e];
+            __syncthreads();   // the parent, later in this loop, reads this block
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -823,8 +1036,12 @@ __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int
 // ---------------------------------------------------------------------------
 namespace {
 size_t g_panel_lds_cap = 0;   // 0: the hardware limit
+int g_persistent = 1;         // small subtrees in one persistent launch
 }
-void set_factor_tuning(int panel_lds_cap) { g_panel_lds_cap = panel_lds_cap > 0 ? (size_t)panel_lds_cap : 0; }
+void set_factor_tuning(int panel_lds_cap, int persistent) {
+    g_panel_lds_cap = panel_lds_cap > 0 ? (size_t)panel_lds_cap : 0;
+    g_persistent = persistent != 0;
+}
 
 template <typename T>
 void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, int64_t vstride, hipStream_t st) {
@@ -846,36 +1063,62 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)panel_lds_hw));
         GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_panel_lu<float>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)panel_lds_hw));
+        GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_subtree_factor<double>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+        GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_subtree_factor<float>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
         attr_set = true;
+    }
+    const bool sub = g_persistent && dp.nroots > 0 && F.sched;
+    if (sub) {
+        const int ntask = dp.nroots * nf;
+        const size_t lds = (size_t)dp.sub_mmax * dp.sub_mmax * sizeof(cx<T>);
+        if (lds > 64 * 1024) throw std::logic_error("subtree front exceeds its LDS");
+        hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, st, F.sched, 1);
+        ProfScope ps(K_SUBTREE, st, dp.sub_flops * nf, 3.0 * dp.sub_elems * sizeof(cx<T>) * nf);
+        // every workgroup takes tasks until none is left; a few per CU keep the CUs busy
+        const int grid = std::min(ntask, 256 * 8);
+        hipLaunchKernelGGL(k_subtree_factor<T>, dim3(grid), dim3(256), lds, st, F.arena, F.fstride, dp.fronts,
+                           dp.root_first, dp.root_last, dp.slot_of, dp.children, dp.inv, dp.sub_optr, dp.sub_oq,
+                           dp.sub_ooff, vals, vstride, dp.nroots, nf, F.piv, pl.n, F.sing, F.sched);
     }
     for (int l = 0; l < pl.nlevels; l++) {
         const LevelPlan &L = pl.levels[l];
         const DevPlan::LevelCounts &cnt = dp.counts[l];
         const double es = sizeof(cx<T>) * (double)nf;
-        if (dp.n_all[l] > 0) {
-            ProfScope ps(K_FRONT_ASSEMBLE, st, 0.0, cnt.asm_elems * es);
-            int64_t mm = (int64_t)dp.mmax_all[l] * dp.mmax_all[l];
+        // the level path: every front of the level, or those outside the small subtrees
+        const int n_asm = sub ? dp.n_asm_rest[l] : dp.n_all[l];
+        const int mmax_asm = sub ? dp.mmax_asm_rest[l] : dp.mmax_all[l];
+        const int *lv_asm = sub ? dp.lv_asm_rest[l] : dp.lv_all[l];
+        if (n_asm > 0) {
+            ProfScope ps(K_FRONT_ASSEMBLE, st, 0.0, (sub ? dp.asm_rest_elems[l] : cnt.asm_elems) * es);
+            int64_t mm = (int64_t)mmax_asm * mmax_asm;
             // about 16 entries per thread (measured faster than 4 or 8)
             int chunks = (int)std::min<int64_t>(std::max<int64_t>(1, mm / (256 * 16)), 256);
-            const size_t lds = sizeof(int) * (size_t)dp.max_children * dp.mmax_all[l];
+            const size_t lds = sizeof(int) * (size_t)dp.max_children * mmax_asm;
             if (dp.max_children > kAsmMaxChildren || lds > 64 * 1024)
                 throw std::logic_error("front assembly: too many children or too large a front for the row maps");
-            hipLaunchKernelGGL(k_front_assemble<T>, dim3(dp.n_all[l], chunks, nf), dim3(256), lds, st, F.arena,
-                               F.fstride, dp.fronts, dp.lv_all[l], dp.children, dp.inv);
+            hipLaunchKernelGGL(k_front_assemble<T>, dim3(n_asm, chunks, nf), dim3(256), lds, st, F.arena,
+                               F.fstride, dp.fronts, lv_asm, dp.children, dp.inv);
         }
-        const int64_t ocnt = L.orig_end - L.orig_begin;
+        const int64_t obeg = sub ? dp.orig_rest_range[l].first : L.orig_begin;
+        const int64_t ocnt = sub ? dp.orig_rest_range[l].second - obeg : L.orig_end - L.orig_begin;
         if (ocnt > 0) {
             ProfScope ps(K_SCATTER, st, 0.0, (double)ocnt * (16.0 + 16.0 + 2 * sizeof(cx<T>)) * nf);
             int blocks = (int)std::min<int64_t>((ocnt + 255) / 256, 4096);
             hipLaunchKernelGGL(k_scatter_orig<T>, dim3(blocks, nf), dim3(256), 0, st, F.arena, F.fstride, vals,
-                               vstride, dp.orig_q + L.orig_begin, dp.orig_dst + L.orig_begin, ocnt);
+                               vstride, (sub ? dp.orig_q_rest : dp.orig_q) + obeg,
+                               (sub ? dp.orig_dst_rest : dp.orig_dst) + obeg, ocnt);
         }
-        if (dp.n_small[l] > 0) {
-            size_t lds = (size_t)L.small_mmax * L.small_mmax * sizeof(cx<T>);
+        const int n_small = sub ? dp.n_small_rest[l] : dp.n_small[l];
+        if (n_small > 0) {
+            const int smm = sub ? dp.small_mmax_rest[l] : L.small_mmax;
+            size_t lds = (size_t)smm * smm * sizeof(cx<T>);
             if (lds > lds_small_max) throw std::logic_error("small front exceeds LDS");
-            ProfScope ps(K_SMALL_LU, st, cnt.small_flops * nf, cnt.small_elems * es);
-            hipLaunchKernelGGL(k_small_lu<T>, dim3(dp.n_small[l], nf), dim3(256), lds, st, F.arena, F.fstride,
-                               dp.fronts, dp.lv_small[l], F.piv, pl.n, F.sing);
+            ProfScope ps(K_SMALL_LU, st, (sub ? dp.small_rest_flops[l] : cnt.small_flops) * nf,
+                         (sub ? dp.small_rest_elems[l] : cnt.small_elems) * es);
+            hipLaunchKernelGGL(k_small_lu<T>, dim3(n_small, nf), dim3(256), lds, st, F.arena, F.fstride,
+                               dp.fronts, sub ? dp.lv_small_rest[l] : dp.lv_small[l], F.piv, pl.n, F.sing);
         }
         if (dp.n_large[l] > 0) {
             int si = 0;

@@ -45,7 +45,7 @@ const char *const kClassNames[K_NCLASS] = {
     "assemble",   "front_assemble", "scatter_orig", "small_lu",   "panel",       "panel_apply", "trsm_u",  "gemm_update",
     "diag_inv",   "fwd_gather",     "fwd_leaf",     "fwd_small1",   "fwd_front1", "fwd_front",   "fwd_blk", "bwd_small1",
     "bwd_front1", "bwd_front",      "bwd_contrib",  "bwd_blk",    "piv_unswap",  "fwd_vec", "bwd_vec",
-    "permute_in", "permute_out",    "residual",     "norm_reduce", "axpy"};
+    "permute_in", "permute_out",    "residual",     "norm_reduce", "axpy",    "subtree"};
 static_assert(sizeof(kClassNames) / sizeof(kClassNames[0]) == K_NCLASS, "one name per kernel class");
 
 namespace {

@@ -10,7 +10,7 @@ enum KClass {
     K_ASSEMBLE, K_FRONT_ASSEMBLE, K_SCATTER, K_SMALL_LU, K_PANEL, K_PANEL_APPLY, K_TRSM, K_GEMM, K_DIAG_INV,
     K_FWD_GATHER, K_FWD_LEAF, K_FWD_SMALL1, K_FWD_FRONT1, K_FWD_FRONT, K_FWD_BLK,
     K_BWD_SMALL1, K_BWD_FRONT1, K_BWD_FRONT, K_BWD_CONTRIB, K_BWD_BLK, K_PIV_UNSWAP,
-    K_FWD_VEC, K_BWD_VEC, K_PERMUTE_IN, K_PERMUTE_OUT, K_RESIDUAL, K_NORM_REDUCE, K_AXPY, K_NCLASS
+    K_FWD_VEC, K_BWD_VEC, K_PERMUTE_IN, K_PERMUTE_OUT, K_RESIDUAL, K_NORM_REDUCE, K_AXPY, K_SUBTREE, K_NCLASS
 };
 
 extern const char *const kClassNames[K_NCLASS];

@@ -1521,6 +1521,147 @@ __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena
     for (int i = tid; i < p; i += 256) X[F.pos0 + (trans ? rowperm[F.pos0 + i] : i)] = sw[i];
 }
 
+// ---- single-vector sweeps of large fronts, split across workgroups ------------
+// For levels of few, large multi-block fronts, k_fwd_vec/k_bwd_vec would leave
+// the device nearly idle (one workgroup per front).  The triangular part over
+// the p pivot rows stays with one workgroup per front; the rectangular part
+// -- forward: contribution rows w2 -= L21 y1, backward: pivot rows
+// z = y1 - U12 x2 -- runs as chunks of 256 rows on many workgroups.  Each row
+// accumulates over t (or s) in the same order as the one-workgroup kernels,
+// so the results are the same to the bit.
+constexpr int kVecSplitRows = 256;
+
+// forward, triangular part: gather the front's rows, solve its p pivot rows
+// block by block (updating only pivot rows), write y1 to X and the gathered
+// contribution rows to W for k_fwd_vec_rect.  grid: fronts
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_vec_tri(const cx<T> *__restrict__ arena,
+                                                     const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                     const int *__restrict__ children, const int *__restrict__ inv,
+                                                     const int *__restrict__ rowperm, cx<T> *__restrict__ X,
+                                                     cx<T> *__restrict__ W, int trans, int cj) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *sw = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ cx<T> Dsh[32][33];
+    const FrontDev F = fronts[list[blockIdx.x]];
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    const cx<T> *G = arena + F.buf_off;
+    auto Lop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
+    for (int i = tid; i < m; i += 256) {
+        int src = (!trans && i < p) ? rowperm[F.pos0 + i] : i;
+        sw[i] = front_row(F, src, 0, 1, X, W, fronts, children, inv);
+    }
+    __syncthreads();
+    for (int tb = 0; tb < p; tb += 32) {
+        const int nb = min(32, p - tb);
+        for (int e = tid; e < nb * nb; e += 256) Dsh[e % nb][e / nb] = dop(G, m, tb, e % nb, e / nb, true, trans, cj);
+        __syncthreads();
+        cx<T> y = {T(0), T(0)};
+        if (tid < nb)
+            for (int t = 0; t <= tid; t++) y = cadd(y, cmul(Dsh[tid][t], sw[tb + t]));
+        __syncthreads();
+        if (tid < nb) sw[tb + tid] = y;
+        __syncthreads();
+        for (int i = tb + nb + tid; i < p; i += 256) {
+            cx<T> acc = sw[i];
+#pragma unroll 8
+            for (int t = 0; t < nb; t++) acc = cmsub(acc, Lop(i, tb + t), sw[tb + t]);
+            sw[i] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < m; i += 256) {
+        if (i < p) X[F.pos0 + i] = sw[i];
+        else W[F.fv_off + i] = sw[i];
+    }
+}
+
+// forward, rectangular part: contribution rows i of chunk blockIdx.x,
+// w_i -= sum_t L(i, t) y_t over t = 0..p-1.  grid: (row chunks, fronts)
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_vec_rect(const cx<T> *__restrict__ arena,
+                                                      const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                      const cx<T> *__restrict__ X, cx<T> *__restrict__ W, int trans,
+                                                      int cj) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *y = reinterpret_cast<cx<T> *>(smem_raw);
+    const FrontDev F = fronts[list[blockIdx.y]];
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    const int i = p + blockIdx.x * kVecSplitRows + tid;
+    if (p + (int)blockIdx.x * kVecSplitRows >= m) return;
+    const cx<T> *G = arena + F.buf_off;
+    for (int t = tid; t < p; t += 256) y[t] = X[F.pos0 + t];
+    __syncthreads();
+    if (i >= m) return;
+    cx<T> acc = W[F.fv_off + i];
+#pragma unroll 8
+    for (int t = 0; t < p; t++)
+        acc = cmsub(acc, trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj), y[t]);
+    W[F.fv_off + i] = acc;
+}
+
+// backward, rectangular part: pivot rows i of chunk blockIdx.x,
+// z_i = x_i - sum_{s >= p} U(i, s) x2_s, written over x_i.  grid: (row chunks, fronts)
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_vec_rect(const cx<T> *__restrict__ arena,
+                                                      const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                      const int *__restrict__ idx, cx<T> *__restrict__ X, int trans,
+                                                      int cj) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *x2 = reinterpret_cast<cx<T> *>(smem_raw);
+    const FrontDev F = fronts[list[blockIdx.y]];
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    const int i = blockIdx.x * kVecSplitRows + tid;
+    if ((int)blockIdx.x * kVecSplitRows >= p) return;
+    const cx<T> *G = arena + F.buf_off;
+    for (int s2 = tid; s2 < m - p; s2 += 256) x2[s2] = X[idx[F.idx_off + p + s2]];
+    __syncthreads();
+    if (i >= p) return;
+    cx<T> acc = X[F.pos0 + i];
+#pragma unroll 8
+    for (int s2 = p; s2 < m; s2++)
+        acc = cmsub(acc, trans ? ld(G + s2 + (int64_t)i * m, cj) : ld(G + i + (int64_t)s2 * m, cj), x2[s2 - p]);
+    X[F.pos0 + i] = acc;
+}
+
+// backward, triangular part: the front's p pivot rows from z, block by block
+// from the last; transposed solves apply the row permutation on the way out
+// (as k_bwd_vec).  grid: fronts
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_vec_tri(const cx<T> *__restrict__ arena,
+                                                     const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                     const int *__restrict__ rowperm, cx<T> *__restrict__ X,
+                                                     int trans, int cj) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *sw = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ cx<T> Dsh[32][33];
+    const FrontDev F = fronts[list[blockIdx.x]];
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    const cx<T> *G = arena + F.buf_off;
+    auto Uop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
+    for (int i = tid; i < p; i += 256) sw[i] = X[F.pos0 + i];
+    __syncthreads();
+    for (int tb = ((p - 1) / 32) * 32; tb >= 0; tb -= 32) {
+        const int nb = min(32, p - tb);
+        for (int e = tid; e < nb * nb; e += 256) Dsh[e % nb][e / nb] = dop(G, m, tb, e % nb, e / nb, false, trans, cj);
+        __syncthreads();
+        cx<T> x = {T(0), T(0)};
+        if (tid < nb)
+            for (int t = tid; t < nb; t++) x = cadd(x, cmul(Dsh[tid][t], sw[tb + t]));
+        __syncthreads();
+        if (tid < nb) sw[tb + tid] = x;
+        __syncthreads();
+        for (int i = tid; i < tb; i += 256) {
+            cx<T> acc = sw[i];
+#pragma unroll 8
+            for (int t = 0; t < nb; t++) acc = cmsub(acc, Uop(i, tb + t), sw[tb + t]);
+            sw[i] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < p; i += 256) X[F.pos0 + (trans ? rowperm[F.pos0 + i] : i)] = sw[i];
+}
+
 // ---- FP64 residuals ------------------------------------------------------------
 // r = b - op(M) x, column-major n x nrhs panels; one thread per row keeps the
 // row's (at most nine) entries in registers and walks the right-hand sides.
@@ -1883,6 +2024,11 @@ struct Workspace {
         }
         return p;
     }
+    void release() {
+        dev_free(p);
+        p = nullptr;
+        bytes = 0;
+    }
     ~Workspace() { dev_free(p); }
 };
 thread_local Workspace ws_x, ws_w, ws_r;
@@ -2165,6 +2311,81 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
     }
 }
 
+// the single-vector sweeps split a level's fronts across workgroups when the
+// level is multi-block (some front has more than one 32-pivot block) and has
+// too few fronts to fill the device with one workgroup each
+static bool vec_split(const DevPlan &dp, int l) {
+    static const bool off = std::getenv("GFDFD_VEC_NOSPLIT") != nullptr;
+    return !off && !dp.single[l] && dp.n_all[l] < 128;
+}
+
+// Up to kVecMaxRhs right-hand sides: one pass per frequency and column through
+// the single-vector kernels, in the work buffers Xp (n entries, elimination
+// order) and Wp (fv_rows entries: every front's contribution rows, each
+// written by its front before the parent reads it, so neither needs clearing).
+template <typename T, typename TIO>
+static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, cx<T> *Xp,
+                      cx<T> *Wp, hipStream_t st, bool accumulate) {
+    const DevPlan &dp = *F.plan;
+    const Plan &pl = *dp.host;
+    const int n = pl.n;
+    const bool trans = sys != 0, cj = sys == 1;
+    const double es = sizeof(cx<T>);
+    for (int z = 0; z < sel.n; z++) {
+        const int f = sel.f[z];
+        const cx<T> *arena = F.arena + f * F.fstride;
+        const int *rowperm = F.rowperm + (int64_t)f * n;
+        for (int r = 0; r < nrhs; r++) {
+            const int64_t off = ((int64_t)sel.s[z] * nrhs + r) * n;
+            {
+                ProfScope ps(K_PERMUTE_IN, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
+                hipLaunchKernelGGL((k_vec_in<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, B + off, Xp, dp.perm, n);
+            }
+            for (int l = 0; l < pl.nlevels; l++) {
+                if (dp.n_all[l] == 0) continue;
+                const DevPlan::LevelCounts &c = dp.counts[l];
+                ProfScope ps(K_FWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
+                size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
+                if (vec_split(dp, l)) {
+                    hipLaunchKernelGGL(k_fwd_vec_tri<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
+                                       dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj);
+                    const int chunks = (dp.mmax_all[l] + kVecSplitRows - 1) / kVecSplitRows;
+                    hipLaunchKernelGGL(k_fwd_vec_rect<T>, dim3(chunks, dp.n_all[l]), dim3(256),
+                                       sizeof(cx<T>) * (size_t)dp.pmax_all[l], st, arena, dp.fronts, dp.lv_all[l], Xp,
+                                       Wp, (int)trans, (int)cj);
+                } else {
+                    hipLaunchKernelGGL(k_fwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
+                                       dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj,
+                                       dp.single[l]);
+                }
+            }
+            for (int l = pl.nlevels - 1; l >= 0; l--) {
+                if (dp.n_all[l] == 0) continue;
+                const DevPlan::LevelCounts &c = dp.counts[l];
+                ProfScope ps(K_BWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
+                size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
+                if (vec_split(dp, l)) {
+                    const int chunks = (dp.pmax_all[l] + kVecSplitRows - 1) / kVecSplitRows;
+                    hipLaunchKernelGGL(k_bwd_vec_rect<T>, dim3(chunks, dp.n_all[l]), dim3(256), lds, st, arena,
+                                       dp.fronts, dp.lv_all[l], dp.idx, Xp, (int)trans, (int)cj);
+                    hipLaunchKernelGGL(k_bwd_vec_tri<T>, dim3(dp.n_all[l]), dim3(256),
+                                       sizeof(cx<T>) * (size_t)dp.pmax_all[l], st, arena, dp.fronts, dp.lv_all[l],
+                                       rowperm, Xp, (int)trans, (int)cj);
+                } else {
+                    hipLaunchKernelGGL(k_bwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
+                                       dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj, dp.single[l]);
+                }
+            }
+            {
+                ProfScope ps(K_PERMUTE_OUT, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
+                hipLaunchKernelGGL((k_vec_out<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, Xp, X + off, dp.iperm, n,
+                                   (int)accumulate);
+            }
+        }
+    }
+    GFDFD_HIP(hipGetLastError());
+}
+
 template <typename T, typename TIO>
 void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::vector<int> &slots, int nslot_buf,
                  int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, hipStream_t st, bool accumulate) {
@@ -2185,42 +2406,7 @@ void solve_batch(const Factors<T> &F, const std::vector<int> &freqs, const std::
         // few right-hand sides: one pass per frequency and column through the single-vector kernels
         cx<T> *Xp = (cx<T> *)ws_x.get(sizeof(cx<T>) * (size_t)n);
         cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)pl.fv_rows);
-        for (int z = 0; z < ns; z++) {
-            const int f = sel.f[z];
-            const cx<T> *arena = F.arena + f * F.fstride;
-            const int *rowperm = F.rowperm + (int64_t)f * n;
-            for (int r = 0; r < nrhs; r++) {
-                const int64_t off = ((int64_t)sel.s[z] * nrhs + r) * n;
-                {
-                    ProfScope ps(K_PERMUTE_IN, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
-                    hipLaunchKernelGGL((k_vec_in<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, B + off, Xp, dp.perm,
-                                       n);
-                }
-                for (int l = 0; l < pl.nlevels; l++) {
-                    if (dp.n_all[l] == 0) continue;
-                    const DevPlan::LevelCounts &c = dp.counts[l];
-                    ProfScope ps(K_FWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
-                    size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-                    hipLaunchKernelGGL(k_fwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
-                                       dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj,
-                                       dp.single[l]);
-                }
-                for (int l = pl.nlevels - 1; l >= 0; l--) {
-                    if (dp.n_all[l] == 0) continue;
-                    const DevPlan::LevelCounts &c = dp.counts[l];
-                    ProfScope ps(K_BWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
-                    size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-                    hipLaunchKernelGGL(k_bwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
-                                       dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj, dp.single[l]);
-                }
-                {
-                    ProfScope ps(K_PERMUTE_OUT, st, 0.0, (double)n * (sizeof(cx<TIO>) + es + 4));
-                    hipLaunchKernelGGL((k_vec_out<T, TIO>), dim3(grid_for(n)), dim3(256), 0, st, Xp, X + off,
-                                       iout + f * mstride, n, (int)accumulate);
-                }
-            }
-        }
-        GFDFD_HIP(hipGetLastError());
+        solve_vec<T, TIO>(F, sel, sys, nrhs, B, X, Xp, Wp, st, accumulate);
         return;
     }
     // internal panels are indexed by launch slot: slot z of Xp/Wp serves frequency sel.f[z]
@@ -2506,5 +2692,11 @@ template std::vector<RefineStats> solve_refined_batch<double>(const Factors<doub
 template std::vector<RefineStats> solve_refined_batch<float>(const Factors<float> &, const std::vector<int> &, int, int,
                                                              const cx<double> *, cx<double> *, double, int,
                                                              hipStream_t);
+
+// the calling thread's cached solve buffers back to the device (they grow to
+// the largest solve and are otherwise kept for the next one)
+void release_solve_workspaces() {
+    for (Workspace *w : {&ws_x, &ws_w, &ws_r, &ws_n, &ws_p, &ws_b64, &ws_x64a, &ws_x64b, &ws_p2}) w->release();
+}
 
 }  // namespace gfdfd

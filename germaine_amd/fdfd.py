@@ -89,11 +89,12 @@ def solve_tuning(rw_cap=64, fused_mmax=0, refine_layout=0):
     check(lib().gfdfd_solve_tuning(rw_cap, fused_mmax, refine_layout), "gfdfd_solve_tuning")
 
 
-def factor_tuning(panel_lds_cap=0):
+def factor_tuning(panel_lds_cap=0, persistent=1):
     """Factorisation kernel selection (process-wide): panels needing more than
     panel_lds_cap bytes of LDS take the global-memory panel kernel (0: the
-    hardware limit)."""
-    check(lib().gfdfd_factor_tuning(int(panel_lds_cap)), "gfdfd_factor_tuning")
+    hardware limit); persistent selects the one-launch factorisation of the
+    small subtrees (else level by level)."""
+    check(lib().gfdfd_factor_tuning(int(panel_lds_cap), int(persistent)), "gfdfd_factor_tuning")
 
 
 class Plan:

@@ -162,8 +162,11 @@ int gfdfd_solve_tuning(int rw_cap, int fused_mmax, int refine_layout);
  * graph captures): a 32-column panel whose candidate rows need more than
  * panel_lds_cap bytes of LDS is factored by the global-memory panel kernel
  * instead of in LDS (0: the hardware limit, 159 KB -- separators above 318
- * rows in FP64, 637 in FP32).  Lets small systems exercise that path.        */
-int gfdfd_factor_tuning(int panel_lds_cap);
+ * rows in FP64, 637 in FP32), which lets small systems exercise that path;
+ * persistent != 0 (the default) factors the small subtrees -- fronts of order
+ * <= 64 whose descendants are all that small -- in one persistent launch over
+ * (front, frequency) tasks with per-task done flags, 0 level by level.      */
+int gfdfd_factor_tuning(int panel_lds_cap, int persistent);
 
 /* ---- shot kernels --------------------------------------------------------------
  * RHS_source_AC / RHS_source_TE for all shots (src/AC/RHS_source_AC.c:46-53):
@@ -304,7 +307,8 @@ int gfdfd_grad_te_stf(int n, int nshots, const float *omega, const float *omega2
  * scatter_orig, small_lu, panel, panel_apply, trsm_u, gemm_update, diag_inv,
  * fwd_gather, fwd_leaf, fwd_small1, fwd_front1, fwd_front, fwd_blk,
  * bwd_small1, bwd_front1, bwd_front, bwd_contrib, bwd_blk, piv_unswap,
- * fwd_vec, bwd_vec, permute_in, permute_out, residual, norm_reduce, axpy.
+ * fwd_vec, bwd_vec, permute_in, permute_out, residual, norm_reduce, axpy,
+ * subtree.
  * No counterpart in the reference (it times whole phases with MPI_Wtime,
  * src/AC/forward_shot_AC.c:54-65).                                          */
 int gfdfd_profile(int enable);
@@ -313,6 +317,10 @@ int gfdfd_profile(int enable);
  * restarts the peak from the current value.  No counterpart in the reference
  * (UMFPACK reports host memory in Info[]).                                   */
 int gfdfd_device_memory(long long *current_bytes, long long *peak_bytes, int reset_peak);
+/* Free the solve work buffers the calling thread has cached (they grow to the
+ * largest solve so far and are otherwise kept for the next call); waits for
+ * the device first.  No counterpart in the reference.                        */
+int gfdfd_release_workspace(void);
 /* record only the named class (NULL or "": every class), e.g. to time one
  * kernel inside a timed region at the cost of two events per launch       */
 int gfdfd_profile_select(const char *kernel);

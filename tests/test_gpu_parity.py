@@ -293,6 +293,45 @@ def test_pivoting_solves_match_oracle(sysno, nrhs):
 
 @pytest.mark.parametrize("matrix", ["helmholtz", "random"])
 @pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_persistent_subtrees_equal_level_path(precision, matrix):
+    """The small subtrees factored by the one persistent launch (a task is a
+    whole subtree of one frequency) against the level-by-level
+    kernels: the same arithmetic in the same order, so the solutions agree
+    bit for bit, for assembled Helmholtz matrices and for random values whose
+    pivoting swaps rows, three frequencies in one batch; and both against the
+    banded oracle."""
+    nx, ny = 61, 41
+    say("persistent subtrees", precision, matrix)
+    if matrix == "helmholtz":
+        m1, m2 = smooth_model(nx, ny, seed=7)
+        freqs = [3.0, 8.0, 13.5]
+        mats = [oracle_system(G.AC, nx, ny, 6, 10.0, f, m1, m2)[1] for f in freqs]
+    else:
+        mats = [random_grid_csc(nx, ny, seed) for seed in (41, 42, 43)]
+    pl = fdfd.Plan(nx, ny)
+    vals = dev(np.stack([c[2] + 1j * c[3] for c in mats]))
+    rng = np.random.default_rng(4)
+    B = rng.standard_normal((3, 2, nx * ny)) + 1j * rng.standard_normal((3, 2, nx * ny))
+    out = {}
+    for persistent in (1, 0):
+        fdfd.factor_tuning(0, persistent)
+        try:
+            F = pl.factor(vals, precision)
+            assert np.all(F.singular() == 0)
+            if precision == G.FP64:
+                out[persistent] = F.solve_batch([0, 1, 2], dev(B)).cpu().numpy()
+            else:
+                out[persistent] = F.solve_refine_batch([0, 1, 2], dev(B), tol=1e-12, max_iter=30)[0].cpu().numpy()
+        finally:
+            fdfd.factor_tuning(0, 1)
+    assert np.array_equal(out[0], out[1])
+    for z in range(3):
+        Xr, _ = O.band_solve(*mats[z], B[z].T, sys=0, perm=O.grid_transpose_perm(nx, ny))
+        assert rel(out[1][z], Xr.T) <= (1e-10 if matrix == "helmholtz" and precision == G.FP64 else 1e-8)
+
+
+@pytest.mark.parametrize("matrix", ["helmholtz", "random"])
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
 def test_global_memory_panel_path_matches_oracle(precision, matrix):
     """The blocked path's global-memory panel kernel, which factors panels too
     large for LDS (separators above 318 rows in FP64, 637 in FP32): forced on
@@ -313,7 +352,7 @@ def test_global_memory_panel_path_matches_oracle(precision, matrix):
     try:
         F = pl.factor(vals, precision)
     finally:
-        fdfd.factor_tuning(0)
+        fdfd.factor_tuning(0, 1)
     F0 = pl.factor(vals, precision)
     for sysno in (0, 2):
         Xr, _ = O.band_solve(*csc, B.T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
