@@ -68,6 +68,8 @@ SIGNATURES = {
     "gfdfd_factors_singular": (_I, [_P, _P]),
     "gfdfd_solve": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     "gfdfd_solve_refine": (_I, [_P, _I, _I, _I, _P, _P, _D, _I, _IP, _DP, _P]),
+    "gfdfd_solve_noalloc": (_I, [_P, _I, _I, _I, _P, _P, _P]),
+    "gfdfd_solve_noalloc_limit": (C.c_longlong, []),
     "gfdfd_solve_batch": (_I, [_P, _I, _P, _I, _I, _P, _P, _P]),
     "gfdfd_solve_refine_batch": (_I, [_P, _I, _P, _I, _I, _P, _P, _D, _I, _P, _P, _P]),
     "gfdfd_residual": (_I, [_P, _P, _I, _I, _P, _P, _P, _P]),

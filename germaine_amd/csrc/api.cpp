@@ -381,6 +381,22 @@ int gfdfd_solve(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b
     GUARD_END
 }
 
+int gfdfd_solve_noalloc(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x,
+                        void *stream) {
+    if (!f || !d_b || !d_x) return report(GFDFD_ERR_ARG, "gfdfd_solve_noalloc: missing argument");
+    GUARD_BEGIN
+    auto B = reinterpret_cast<const cx<double> *>(d_b);
+    auto X = reinterpret_cast<cx<double> *>(d_x);
+    if (f->f64)
+        solve_noalloc<double>(*f->f64, ifreq, sys, nrhs, B, X, S(stream));
+    else
+        solve_noalloc<float>(*f->f32, ifreq, sys, nrhs, B, X, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+long long gfdfd_solve_noalloc_limit(void) { return (long long)kNoAllocScratchBytes; }
+
 int gfdfd_solve_refine(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x, double tol,
                        int max_iter, int *iters, double *relres, void *stream) {
     if (!f || !d_b || !d_x || nrhs < 1) return report(GFDFD_ERR_ARG, "gfdfd_solve_refine: missing argument");

@@ -123,6 +123,15 @@ void set_factor_tuning(int panel_lds_cap, int persistent);
 // frees the calling thread's cached solve work buffers
 void release_solve_workspaces();
 
+// Solve of 1..4 right-hand sides (user layout, FP64) whose work buffers live in
+// a file-scope device array of kNoAllocScratchBytes: no device allocation.
+// The grid must fit: noalloc_scratch_bytes(plan, sizeof(cx<T>)) <= the array.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+size_t noalloc_scratch_bytes(const Plan &pl, size_t entry_bytes);
+template <typename T>
+void solve_noalloc(const Factors<T> &F, int ifreq, int sys, int nrhs, const cx<double> *B, cx<double> *X,
+                   hipStream_t st);
+
 struct RefineStats {
     int iterations = 0;
     double max_rel_residual = 0;   // max over columns of ||b - M x|| / ||b||

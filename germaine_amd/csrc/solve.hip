@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "engine.hpp"
 #include "mfma.hpp"
@@ -2698,5 +2699,62 @@ template std::vector<RefineStats> solve_refined_batch<float>(const Factors<float
 void release_solve_workspaces() {
     for (Workspace *w : {&ws_x, &ws_w, &ws_r, &ws_n, &ws_p, &ws_b64, &ws_x64a, &ws_x64b, &ws_p2}) w->release();
 }
+
+// ---------------------------------------------------------------------------
+// Solve without allocation: the work buffers of the few-right-hand-side path
+// live in one file-scope device array.  Host code reaches it only through
+// the address hipGetSymbolAddress returns for the current device (each GPU
+// holds its own copy), looked up once per device.  Every call on a GPU
+// shares that GPU's array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static void *scratch_on_device() {
+    static std::mutex mu;
+    static void *ptr[64] = {};
+    int dev = 0;
+    GFDFD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw std::invalid_argument("solve_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw hip_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        ptr[dev] = p;
+    }
+    return ptr[dev];
+}
+
+size_t noalloc_scratch_bytes(const Plan &pl, size_t entry_bytes) {
+    const size_t xb = ((size_t)pl.n * entry_bytes + 255) / 256 * 256;
+    return xb + (size_t)pl.fv_rows * entry_bytes;
+}
+
+template <typename T>
+void solve_noalloc(const Factors<T> &F, int ifreq, int sys, int nrhs, const cx<double> *B, cx<double> *X,
+                   hipStream_t st) {
+    const Plan &pl = *F.plan->host;
+    if (ifreq < 0 || ifreq >= F.nfreq) throw std::invalid_argument("frequency index out of range");
+    if (sys < 0 || sys > 2) throw std::invalid_argument("invalid system");
+    if (nrhs < 1 || nrhs > kVecMaxRhs)
+        throw std::invalid_argument("gfdfd_solve_noalloc solves 1 to " + std::to_string(kVecMaxRhs) +
+                                    " right-hand sides per call");
+    const size_t need = noalloc_scratch_bytes(pl, sizeof(cx<T>));
+    if (need > kNoAllocScratchBytes)
+        throw std::invalid_argument("gfdfd_solve_noalloc: the grid needs " + std::to_string(need) +
+                                    " bytes of scratch (n + work rows entries), the static array holds " +
+                                    std::to_string(kNoAllocScratchBytes));
+    set_smem_attrs<T>();
+    ProfPause no_events;   // profiling would create events
+    unsigned char *base = static_cast<unsigned char *>(scratch_on_device());
+    cx<T> *Xp = reinterpret_cast<cx<T> *>(base);
+    cx<T> *Wp = reinterpret_cast<cx<T> *>(base + ((size_t)pl.n * sizeof(cx<T>) + 255) / 256 * 256);
+    solve_vec<T, double>(F, make_sel({ifreq}, {0}), sys, nrhs, B, X, Xp, Wp, st, false);
+}
+template void solve_noalloc<double>(const Factors<double> &, int, int, int, const cx<double> *, cx<double> *,
+                                    hipStream_t);
+template void solve_noalloc<float>(const Factors<float> &, int, int, int, const cx<double> *, cx<double> *,
+                                   hipStream_t);
 
 }  // namespace gfdfd

@@ -131,6 +131,18 @@ int gfdfd_factors_singular(gfdfd_factors f, int *flags);
  * sys: 0 M x = b, 1 M^H x = b, 2 M^T x = b.  d_b and d_x: n x nrhs complex
  * doubles; gfdfd_solve allows d_x == d_b.                                   */
 int gfdfd_solve(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x, void *stream);
+/* gfdfd_solve for 1 to 4 right-hand sides with no device allocation (no
+ * counterpart in the reference: for callers that cannot allocate -- code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * work buffers (n + work_rows entries of the factor precision, aligned) live
+ * in a file-scope device array of gfdfd_solve_noalloc_limit() bytes (64 MiB)
+ * that every GPU the library runs on holds until the process ends; a grid
+ * needing more is rejected with GFDFD_ERR_ARG (FP64 factors fit up to
+ * 1601 x 401).  All calls on one GPU share that GPU's array, so calls that
+ * overlap on one GPU race; d_x may equal d_b.                               */
+int gfdfd_solve_noalloc(gfdfd_factors f, int ifreq, int sys, int nrhs, const double *d_b, double *d_x,
+                        void *stream);
+long long gfdfd_solve_noalloc_limit(void);
 /* solve with FP64 iterative refinement (residual b - op(M) x from the FP64
  * values) until max over columns of ||r||/||b|| <= tol or max_iter
  * corrections; d_x must not alias d_b.  iters/relres may be NULL.           */
