@@ -27,7 +27,8 @@ struct TileDev {
 };
 
 struct hip_error : std::runtime_error {
-    explicit hip_error(const std::string &s) : std::runtime_error(s) {}
+    hipError_t code;   // what the failing HIP call returned (hipErrorUnknown when no call failed)
+    explicit hip_error(const std::string &s, hipError_t c = hipErrorUnknown) : std::runtime_error(s), code(c) {}
 };
 
 #define GFDFD_HIP(call)                                                                              \
@@ -35,7 +36,7 @@ struct hip_error : std::runtime_error {
         hipError_t e_ = (call);                                                                      \
         if (e_ != hipSuccess)                                                                        \
             throw ::gfdfd::hip_error(std::string(#call) + ": " + hipGetErrorString(e_) + " at " +    \
-                                     __FILE__ + ":" + std::to_string(__LINE__));                     \
+                                     __FILE__ + ":" + std::to_string(__LINE__), e_);                 \
     } while (0)
 
 // device bytes the library holds now and at most since the last reset

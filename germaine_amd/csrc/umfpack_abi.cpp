@@ -47,6 +47,12 @@ int fail(int code, const char *fn, const char *msg) {
     return code;
 }
 
+// a failed device allocation is UMFPACK's out-of-memory; any other device
+// failure (no GPU, a launch or copy error) is an internal error
+int hip_status(const hip_error &e) {
+    return e.code == hipErrorOutOfMemory ? UMFPACK_ERROR_out_of_memory : UMFPACK_ERROR_internal_error;
+}
+
 void set_info(double Info[], int status) {
     if (Info) Info[UMFPACK_STATUS] = status;
 }
@@ -157,8 +163,9 @@ int umfpack_zi_symbolic(int n_row, int n_col, const int Ap[], const int Ai[], co
         set_info(Info, UMFPACK_OK);
         return UMFPACK_OK;
     } catch (const hip_error &e) {
-        set_info(Info, UMFPACK_ERROR_out_of_memory);
-        return fail(UMFPACK_ERROR_out_of_memory, fn, e.what());
+        const int st = hip_status(e);
+        set_info(Info, st);
+        return fail(st, fn, e.what());
     } catch (const std::exception &e) {
         set_info(Info, UMFPACK_ERROR_internal_error);
         return fail(UMFPACK_ERROR_internal_error, fn, e.what());
@@ -202,8 +209,9 @@ int umfpack_zi_numeric(const int Ap[], const int Ai[], const double Ax[], const 
         set_info(Info, UMFPACK_OK);
         return UMFPACK_OK;
     } catch (const hip_error &e) {
-        set_info(Info, UMFPACK_ERROR_out_of_memory);
-        return fail(UMFPACK_ERROR_out_of_memory, fn, e.what());
+        const int st = hip_status(e);
+        set_info(Info, st);
+        return fail(st, fn, e.what());
     } catch (const std::exception &e) {
         set_info(Info, UMFPACK_ERROR_internal_error);
         return fail(UMFPACK_ERROR_internal_error, fn, e.what());
