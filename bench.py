@@ -108,6 +108,11 @@ def main():
                          "(0: none, -1: as far as LDS allows)")
     ap.add_argument("--refine-layout", type=int, default=0,
                     help="panel refinement layout: 0 elimination order (default), 1 caller's layout")
+    ap.add_argument("--persistent", type=int, default=1,
+                    help="small subtrees factored in one persistent launch (1) or level by level (0)")
+    ap.add_argument("--deep-min-m", type=int, default=0,
+                    help="blocked levels with a front of order >= this update the trailing block once per 64 "
+                         "pivots (0: every level; 1073741824: none)")
     ap.add_argument("--no-graph", action="store_true",
                     help="refactor with eager launches instead of replaying the captured hipGraph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -135,7 +140,7 @@ def main():
     m2 = torch.as_tensor(w.m2, device="cuda")
     ksrc = torch.as_tensor(np.asarray(w.src, np.int32), device="cuda")
     cfg = dict(small_max=args.small_max, rw_cap=args.fused_rw, fused_mmax=args.fused_mmax,
-               refine_layout=args.refine_layout)
+               refine_layout=args.refine_layout, persistent=args.persistent, deep_min_m=args.deep_min_m)
     fdfd.solve_tuning(cfg["rw_cap"], cfg["fused_mmax"], cfg["refine_layout"])
     pl = fdfd.Plan(w.nx, w.ny, small_max=cfg["small_max"])
     n = pl.n
@@ -247,6 +252,28 @@ def main():
         log(f"autotune ({chosen_by}): {cfg}")
         fdfd.solve_tuning(cfg["rw_cap"], cfg["fused_mmax"], cfg["refine_layout"])
         pl = fdfd.Plan(w.nx, w.ny, small_max=cfg["small_max"])
+
+        # second stage: the factorisation schedule, timed on eager refactorisations
+        def measure_factor(c, precision=G.FP32):
+            fdfd.factor_tuning(0, c["persistent"], c["deep_min_m"])
+            vals = pl.assemble(w.physics, w.dh, w.freqs, m1, m2, s=w.s, npml=w.npml, a0_pml=w.a0_pml, out=vbuf)
+            F = pl.factor(vals, precision)
+            ts = []
+            for _ in range(3):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                F.refactor(vals)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            del F
+            return float(np.median(ts))
+
+        fcfg, fchosen, _ = autotune.tune(k + "|factor", measure_factor, cands=autotune.factor_candidates(), log=log,
+                                         use_cache=args.tune == "auto")
+        log(f"autotune factor schedule ({fchosen}): {fcfg}")
+        cfg.update(fcfg)
+        chosen_by = chosen_by if chosen_by == fchosen else f"{chosen_by}+{fchosen}"
+    fdfd.factor_tuning(0, cfg["persistent"], cfg["deep_min_m"])
     info = pl.info
 
     # --- headline: FP32 fronts + FP64 refinement, batched -----------------------------
@@ -258,8 +285,18 @@ def main():
     n64 = max(1, args.steps // 2)
     t64, kst64 = None, {}
     leg64 = {}
+    cfg64 = dict(persistent=cfg["persistent"], deep_min_m=cfg["deep_min_m"], chosen_by="as fp32+IR")
     if not args.no_fp64:
+        if args.tune != "off":
+            # the FP64 factorisation's own schedule (its MFMA rate and LDS budget differ)
+            k64 = autotune.key(w.nx, w.ny, nf, ns, G.FP64, torch.cuda.get_device_name(0)) + "|factor"
+            c64, how, _ = autotune.tune(k64, lambda c: measure_factor(c, G.FP64), cands=autotune.factor_candidates(),
+                                        log=log, use_cache=args.tune == "auto")
+            cfg64 = dict(c64, chosen_by=how)
+            log(f"autotune fp64 factor schedule ({how}): {c64}")
+        fdfd.factor_tuning(0, cfg64["persistent"], cfg64["deep_min_m"])
         t64, _, kst64, _, leg64 = timed(G.FP64, False, n64, 1, profile=True, legs=True)
+        fdfd.factor_tuning(0, cfg["persistent"], cfg["deep_min_m"])
         log(f"fp64 batched: {t64 * 1e3:.1f} ms/step, legs {leg64}")
     err = None
     if X32 is not None and t64 is not None:
@@ -274,6 +311,7 @@ def main():
     # --- plain FP64: what the drop-in executes (per-frequency factor, one solve per shot) ---
     tplain, t1solve = None, None
     if not args.no_plain:
+        fdfd.factor_tuning(0, cfg64["persistent"], cfg64["deep_min_m"])   # FP64 factors: the FP64 schedule
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         tsolves = 0.0
@@ -377,6 +415,7 @@ def main():
         "fp64_batched": None if t64 is None else {
             "ms_per_step": round(t64 * 1e3, 3), "value": round(nf * ns / t64, 2),
             "legs": legs_of(leg64, es64, PEAK["fp64_mfma"]),
+            "tuning": cfg64,
             "kernels_ms_per_step": {k: round(v["ms"], 3) for k, v in kst64.items()}},
         "plain_fp64_per_shot": None if tplain is None else {
             "ms": round(tplain * 1e3, 3), "value": round(nf * ns / tplain, 2),

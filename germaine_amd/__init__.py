@@ -63,7 +63,7 @@ SIGNATURES = {
     "gfdfd_refactor": (_I, [_P, _P, _P]),
     "gfdfd_refactor_graph": (_I, [_P, _P, _P]),
     "gfdfd_solve_tuning": (_I, [_I, _I, _I]),
-    "gfdfd_factor_tuning": (_I, [_I, _I]),
+    "gfdfd_factor_tuning": (_I, [_I, _I, _I]),
     "gfdfd_factors_free": (_V, [C.POINTER(_P)]),
     "gfdfd_factors_singular": (_I, [_P, _P]),
     "gfdfd_solve": (_I, [_P, _I, _I, _I, _P, _P, _P]),

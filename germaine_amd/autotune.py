@@ -8,7 +8,14 @@ Switches it chooses between (all correct, their speed depends on the workload):
 * ``fused_mmax``: largest front order of a multi-block level that takes the fused per-front
   solve kernels instead of the per-block ones (0: none, -1: as far as LDS allows);
 * ``refine_layout``: the refinement in row-major grid order with one fused residual-and-update
-  pass per step (0), or in the caller's layout with a transpose per correction (1).
+  pass per step (0), or in the caller's layout with a transpose per correction (1);
+
+and, in a second stage with the solve switches fixed, the factorisation's schedule:
+
+* ``persistent``: the small subtrees in one persistent launch (1) or level by level (0);
+* ``deep_min_m``: the front-size bucket boundary of the blocked path -- levels whose largest
+  blocked front has at least this order factor two 32-column panels per step and update the
+  trailing block once per 64 pivots (product depth 64), smaller ones once per panel (depth 32).
 
 ``tune`` times a caller-supplied measurement (seconds per step of the real workload for one
 configuration) over the candidates, keeps the fastest and records it in a JSON cache keyed by the
@@ -21,6 +28,8 @@ CACHE = os.environ.get("GFDFD_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "gfdfd", "tuning.json"))
 
 DEFAULT = dict(small_max=64, rw_cap=64, fused_mmax=0, refine_layout=0)
+NO_DEEP = 1 << 30
+FACTOR_DEFAULT = dict(persistent=1, deep_min_m=0)
 
 
 def candidates():
@@ -33,6 +42,12 @@ def candidates():
                     out.append(dict(small_max=small_max, rw_cap=rw_cap, fused_mmax=fused_mmax,
                                     refine_layout=refine_layout))
     return out
+
+
+def factor_candidates():
+    """The factorisation schedules (8 configurations): product depth 64 on every blocked level,
+    on levels with fronts of order >= 384 or >= 1024 only, or on none; subtrees persistent or not."""
+    return [dict(persistent=p, deep_min_m=d) for p in (1, 0) for d in (0, 384, 1024, NO_DEEP)]
 
 
 def key(nx, ny, nfreq, nrhs, precision, device):
@@ -83,7 +98,7 @@ def tune(k, measure, cands=None, log=None, path=CACHE, use_cache=True):
         if log:
             log(f"autotune: {cfg} -> {t * 1e3:.1f} ms")
     if not table:
-        return dict(DEFAULT), "default", table
+        return dict((cands or candidates())[0]), "default", table
     best = min(table, key=lambda e: e[1])[0]
     store(k, best, [dict(cfg=c, seconds=t) for c, t in table], path)
     return dict(best), "search", table

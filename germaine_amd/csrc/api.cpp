@@ -317,9 +317,10 @@ int gfdfd_release_workspace(void) {
     GUARD_END
 }
 
-int gfdfd_factor_tuning(int panel_lds_cap, int persistent) {
+int gfdfd_factor_tuning(int panel_lds_cap, int persistent, int deep_min_m) {
     if (panel_lds_cap < 0) return report(GFDFD_ERR_ARG, "gfdfd_factor_tuning: negative panel LDS cap");
-    set_factor_tuning(panel_lds_cap, persistent);
+    if (deep_min_m < 0) return report(GFDFD_ERR_ARG, "gfdfd_factor_tuning: negative deep_min_m");
+    set_factor_tuning(panel_lds_cap, persistent, deep_min_m);
     return GFDFD_OK;
 }
 

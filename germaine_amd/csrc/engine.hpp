@@ -28,6 +28,16 @@ struct DevPlan {
     std::vector<int> n_small, n_large, n_all, mmax_all, pmax_all;
     TileDev *tiles = nullptr;
     std::vector<std::vector<std::pair<int64_t, int>>> steps;   // [level][step] -> (offset, count)
+    // the two-panel schedule of the blocked path: per level and 64-column block,
+    // the thin update of the second panel's columns by the first panel and the
+    // depth-64 trailing update, as tile ranges of `tiles`, with their counts
+    struct DeepStep {
+        int64_t thin_off = 0, deep_off = 0;
+        int thin_cnt = 0, deep_cnt = 0;
+        double thin_flops = 0, thin_elems = 0, deep_flops = 0, deep_elems = 0;
+        double panel1_flops = 0, panel2_flops = 0, trsm_thin_flops = 0, trsm_deep_flops = 0;
+    };
+    std::vector<std::vector<DeepStep>> deep_steps;
     size_t bytes = 0;
     // algorithmic counts per level (complex entries or real flops), for measurement
     struct LevelCounts {
@@ -119,7 +129,11 @@ void set_solve_tuning(int rw_cap, int fused_mmax, int refine_layout);
 // factorisation kernel selection: panels needing more LDS than panel_lds_cap
 // bytes (0: the hardware limit) take the global-memory panel kernel; the small
 // subtrees take the persistent launch when persistent != 0, else the level path
-void set_factor_tuning(int panel_lds_cap, int persistent);
+// panel_lds_cap: panels needing more LDS take the global-memory panel kernel
+// (0: the hardware limit); persistent: the small subtrees in one launch;
+// deep_min_m: levels whose largest blocked front has order >= deep_min_m take
+// the two-panel (depth-64) schedule, the others one panel at a time
+void set_factor_tuning(int panel_lds_cap, int persistent, int deep_min_m);
 // frees the calling thread's cached solve work buffers
 void release_solve_workspaces();
 

@@ -231,6 +231,48 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
             for (const Tile &t : tmp) tl.push_back({t.front, t.ti, t.tj});
         }
     }
+    // the two-panel schedule: per 64-column block b0 of a level, the thin update
+    // (rows from b0 + 32, the second panel's columns: fronts with p > b0 + 32)
+    // and the depth-min(64, p - b0) trailing update
+    if (pl.nb != 32) throw std::logic_error("the blocked path assumes 32-column panels");
+    deep_steps.resize(pl.levels.size());
+    for (size_t l = 0; l < pl.levels.size(); l++) {
+        const LevelPlan &L = pl.levels[l];
+        for (int b0 = 0; b0 < L.large_pmax; b0 += 64) {
+            DeepStep d;
+            d.thin_off = (int64_t)tl.size();
+            for (int li = 0; li < (int)L.large.size(); li++) {
+                const Front &F = pl.fronts[L.large[li]];
+                if (F.p <= b0 + 32) continue;
+                const int kb2 = std::min(32, F.p - b0 - 32), t = F.m - b0 - 32;
+                for (int ti = 0; ti < (t + kTile - 1) / kTile; ti++) tl.push_back({li, ti, 0});
+                d.thin_flops += 8.0 * t * kb2 * 32;
+                d.thin_elems += 2.0 * t * kb2 + t * 32.0 + 32.0 * kb2;
+                d.trsm_thin_flops += 8.0 * 0.5 * 32 * 32 * kb2;
+            }
+            d.thin_cnt = (int)(tl.size() - d.thin_off);
+            d.deep_off = (int64_t)tl.size();
+            for (int li = 0; li < (int)L.large.size(); li++) {
+                const Front &F = pl.fronts[L.large[li]];
+                if (F.p <= b0) continue;
+                const int kd = std::min(64, F.p - b0), t = F.m - b0 - kd;
+                for (int j = b0; j < b0 + kd; j++) {
+                    const int pe = j < b0 + 32 ? std::min(b0 + 32, b0 + kd) : b0 + kd;   // the panel's end
+                    (j < b0 + 32 ? d.panel1_flops : d.panel2_flops) +=
+                        8.0 * (F.m - j - 1) * (pe - j - 1) + 6.0 * (F.m - j - 1);
+                }
+                if (t <= 0) continue;
+                const int nt = (t + kTile - 1) / kTile;
+                for (int tj = 0; tj < nt; tj++)
+                    for (int ti = 0; ti < nt; ti++) tl.push_back({li, ti, tj});
+                d.deep_flops += 8.0 * t * t * kd;
+                d.deep_elems += 2.0 * t * kd + 2.0 * t * t;
+                d.trsm_deep_flops += 8.0 * 0.5 * kd * kd * t;
+            }
+            d.deep_cnt = (int)(tl.size() - d.deep_off);
+            deep_steps[l].push_back(d);
+        }
+    }
     tiles = dev_upload(tl);
     bytes += tl.size() * sizeof(TileDev);
     for (const Front &F : pl.fronts) max_children = std::max(max_children, F.nchild);
@@ -606,7 +648,6 @@ __global__ void __launch_bounds__(256) k_panel(cx<T> *__restrict__ arena, int64_
     }
 }
 
-// U12 block row of the panel: rows [k0,k0+kb), columns [k0+kb, m): L_kb^-1 * F
 // ---------------------------------------------------------------------------
 // 3c: blocked path, panel held in LDS.  The pivot candidates of a 32-column
 // panel are the front's remaining fully-summed rows [k0, p); their panel
@@ -733,17 +774,25 @@ __global__ void __launch_bounds__(64) k_panel_apply(cx<T> *__restrict__ arena, i
         if (t < kb) G[i + (int64_t)(k0 + t) * m] = x[t];
 }
 
-template <typename T>
+// U12 rows of the pivot block: rows [k0, k0+kb) of the columns beyond it,
+// solved with the unit lower triangle L[k0:k0+kb, k0:k0+kb].  KB = 32: one
+// panel, kb = min(32, p - k0), XW = 64 columns per workgroup; KB = 64: two
+// panels at once, kb = min(64, p - k0), 32 columns per workgroup, triangle and
+// columns in dynamic LDS.  ccap > 0 ends the columns at min(k0 + ccap, p) (the
+// second panel's columns only), else at m.  grid (column chunks, fronts, nfreq)
+template <typename T, int KB>
 __global__ void __launch_bounds__(256) k_trsm_u(cx<T> *__restrict__ arena, int64_t fstride,
                                                const FrontDev *__restrict__ fronts, const int *__restrict__ list,
-                                               int k0, int nb) {
-    __shared__ cx<T> Ls[32][33];
-    __shared__ cx<T> Xs[32][65];
+                                               int k0, int ccap) {
+    constexpr int XW = KB == 32 ? 64 : 32, LPC = 256 / XW;   // columns per workgroup, lanes per column
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>(*Ls)[KB + 1] = reinterpret_cast<cx<T>(*)[KB + 1]>(smem_raw);
+    cx<T>(*Xs)[XW + 1] = reinterpret_cast<cx<T>(*)[XW + 1]>(smem_raw + sizeof(cx<T>) * KB * (KB + 1));
     const FrontDev F = fronts[list[blockIdx.y]];
     if (F.p <= k0) return;
-    const int m = F.m, kb = min(nb, F.p - k0);
-    const int c0 = k0 + kb + blockIdx.x * 64;
-    if (c0 >= m) return;
+    const int m = F.m, kb = min(KB, F.p - k0), cend = ccap > 0 ? min(k0 + ccap, F.p) : m;
+    const int c0 = k0 + kb + blockIdx.x * XW;
+    if (c0 >= cend) return;
     cx<T> *G = arena + blockIdx.z * fstride + F.buf_off;
     const int tid = threadIdx.x;
     // unrolled so that a batch of loads is in flight at once
@@ -753,78 +802,98 @@ __global__ void __launch_bounds__(256) k_trsm_u(cx<T> *__restrict__ arena, int64
         Ls[i][t] = G[(k0 + i) + (int64_t)(k0 + t) * m];
     }
 #pragma unroll 8
-    for (int e = tid; e < kb * 64; e += 256) {
+    for (int e = tid; e < kb * XW; e += 256) {
         int i = e % kb, c = e / kb, col = c0 + c;
-        Xs[i][c] = col < m ? G[(k0 + i) + (int64_t)col * m] : cx<T>{T(0), T(0)};
+        Xs[i][c] = col < cend ? G[(k0 + i) + (int64_t)col * m] : cx<T>{T(0), T(0)};
     }
     __syncthreads();
     {
-        // four adjacent lanes (one wave) per column: each sums every fourth term
-        // of row i's dot product, two xor-shuffles add them, lane 0 of the four
+        // LPC adjacent lanes (one wave) per column: each sums every LPC-th term
+        // of row i's dot product, xor-shuffles add them, lane 0 of the group
         // stores the entry the next rows read
-        const int c = tid >> 2, part = tid & 3;
+        const int c = tid / LPC, part = tid % LPC;
         for (int i = 1; i < kb; i++) {
             cx<T> acc = {T(0), T(0)};
-            for (int t = part; t < i; t += 4) acc = cmsub(acc, Ls[i][t], Xs[t][c]);
-            acc.x += __shfl_xor(acc.x, 1);
-            acc.y += __shfl_xor(acc.y, 1);
-            acc.x += __shfl_xor(acc.x, 2);
-            acc.y += __shfl_xor(acc.y, 2);
+            for (int t = part; t < i; t += LPC) acc = cmsub(acc, Ls[i][t], Xs[t][c]);
+#pragma unroll
+            for (int o = 1; o < LPC; o <<= 1) {
+                acc.x += __shfl_xor(acc.x, o);
+                acc.y += __shfl_xor(acc.y, o);
+            }
             if (part == 0) Xs[i][c] = cadd(Xs[i][c], acc);
         }
     }
     __syncthreads();
-    for (int e = tid; e < kb * 64; e += 256) {
+    for (int e = tid; e < kb * XW; e += 256) {
         int i = e % kb, c = e / kb, col = c0 + c;
-        if (col < m) G[(k0 + i) + (int64_t)col * m] = Xs[i][c];
+        if (col < cend) G[(k0 + i) + (int64_t)col * m] = Xs[i][c];
     }
+}
+template <typename T, int KB>
+constexpr size_t trsm_lds() {
+    return sizeof(cx<T>) * KB * (KB + 1) + sizeof(cx<T>) * KB * ((KB == 32 ? 64 : 32) + 1);
 }
 
 // ---------------------------------------------------------------------------
-// trailing update C -= A*B on MFMA (complex = 3 real products in FP32, 4 in FP64), 64x64 tiles,
-// K <= 32.  4 waves in 2x2, each wave 32x32 = 2x2 blocks of 16x16.
+// trailing update C -= A*B on MFMA (complex = 3 real products in FP32, 4 in FP64), 64x64 tiles.
+// The product's depth is the front's next min(depth, p - k0) pivot columns
+// (depth 32: one panel; 64: two panels), taken 32 at a time through the same
+// LDS while the C tile stays in the accumulators, so C is read and written once
+// per `depth` pivots.  The updated block starts at row and column k0 + that
+// depth; ccap > 0 ends its columns at min(k0 + ccap, p) (the thin update of the
+// second panel's columns), else at m.
+// DEEP = false compiles the one-chunk form (depth <= 32) without the second
+// chunk's prefetch registers, which would cost occupancy.
+// 4 waves in 2x2, each wave 32x32 = 2x2 blocks of 16x16.
 // MFMA 16x16x4 operand layout: A lane l -> (row l%16, k l/16), B lane l ->
 // (k l/16, col l%16).  Accumulator register v of lane l holds column l%16 and
 // row 4*(l/16)+v for f32, but row 4*v+l/16 for f64.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool DEEP>
 __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, int64_t fstride,
                                                      const FrontDev *__restrict__ fronts, const int *__restrict__ list,
-                                                     const TileDev *__restrict__ tiles, int k0, int nb) {
+                                                     const TileDev *__restrict__ tiles, int k0, int depth, int ccap) {
     constexpr int KB = 32;
     __shared__ T Ar[KB][64], Ai[KB][64], Br[KB][64], Bi[KB][64];
     const TileDev tl = tiles[blockIdx.x];
     const FrontDev F = fronts[list[tl.front]];
-    const int m = F.m, kb = min(nb, F.p - k0);
-    const int base = k0 + kb;
+    const int m = F.m, kd = min(depth, F.p - k0);
+    const int base = k0 + kd, cend = ccap > 0 ? min(k0 + ccap, F.p) : m;
     const int r0 = base + tl.ti * 64, c0 = base + tl.tj * 64;
     cx<T> *G = arena + blockIdx.y * fstride + F.buf_off;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6, wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
-    // every load up front: the operand panels and the lane's 16 entries of C
+    // the lane's 16 entries of C: loaded up front with the first operand chunk
+    // (one chunk), or after the products (two chunks: registers for occupancy)
     cx<T> c[2][2][4];
+    auto load_c = [&]() {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+        for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+            for (int b = 0; b < 2; b++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = r0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = c0 + wc + b * 16 + lr;
-                c[a][b][v] = (row < m && col < m) ? G[row + (int64_t)col * m] : cx<T>{T(0), T(0)};
-            }
-    {
-        cx<T> av[KB * 64 / 256], bv[KB * 64 / 256];
+                for (int v = 0; v < 4; v++) {
+                    const int row = r0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = c0 + wc + b * 16 + lr;
+                    c[a][b][v] = (row < m && col < cend) ? G[row + (int64_t)col * m] : cx<T>{T(0), T(0)};
+                }
+    };
+    if (!DEEP) load_c();
+    cx<T> av[KB * 64 / 256], bv[KB * 64 / 256];
+    auto load_chunk = [&](int kc) {   // operand columns/rows k0+kc .. k0+kc+31 into registers
+        const int kb = min(KB, kd - kc);
 #pragma unroll
         for (int u = 0; u < KB * 64 / 256; u++) {
             const int e = tid + 256 * u, i = e & 63, k = e >> 6;
-            av[u] = (k < kb && r0 + i < m) ? G[(r0 + i) + (int64_t)(k0 + k) * m] : cx<T>{T(0), T(0)};
+            av[u] = (k < kb && r0 + i < m) ? G[(r0 + i) + (int64_t)(k0 + kc + k) * m] : cx<T>{T(0), T(0)};
         }
 #pragma unroll
         for (int u = 0; u < KB * 64 / 256; u++) {
             const int e = tid + 256 * u, k = e % KB, j = e / KB;
-            bv[u] = (k < kb && c0 + j < m) ? G[(k0 + k) + (int64_t)(c0 + j) * m] : cx<T>{T(0), T(0)};
+            bv[u] = (k < kb && c0 + j < cend) ? G[(k0 + kc + k) + (int64_t)(c0 + j) * m] : cx<T>{T(0), T(0)};
         }
+    };
+    auto stage_chunk = [&]() {
 #pragma unroll
         for (int u = 0; u < KB * 64 / 256; u++) {
             const int e = tid + 256 * u;
@@ -833,28 +902,44 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
             Br[e % KB][e / KB] = bv[u].x;
             Bi[e % KB][e / KB] = bv[u].y;
         }
-    }
-    __syncthreads();
+    };
     cacc3<T> acc[2][2];   // three real products per complex product
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b].zero();
-    const int kend = (kb + 3) & ~3;
-    for (int kk = 0; kk < kend; kk += 4) {
-        const int k = kk + lk;
-        T ar[2], ai[2], br[2], bi[2];
-        for (int a = 0; a < 2; a++) {
-            ar[a] = Ar[k][wr + a * 16 + lr];
-            ai[a] = Ai[k][wr + a * 16 + lr];
+    load_chunk(0);
+    stage_chunk();
+    __syncthreads();
+    // FP32: the next chunk's loads in flight during this chunk's products; FP64
+    // loads it after them (its prefetch registers would halve the occupancy)
+    constexpr bool kPrefetch = sizeof(T) == 4;
+    for (int kc = 0; kc < kd; kc += KB) {
+        const bool more = DEEP && kc + KB < kd;
+        if (kPrefetch && more) load_chunk(kc + KB);
+        const int kend = (min(KB, kd - kc) + 3) & ~3;
+        for (int kk = 0; kk < kend; kk += 4) {
+            const int k = kk + lk;
+            T ar[2], ai[2], br[2], bi[2];
+            for (int a = 0; a < 2; a++) {
+                ar[a] = Ar[k][wr + a * 16 + lr];
+                ai[a] = Ai[k][wr + a * 16 + lr];
+            }
+            for (int b = 0; b < 2; b++) {
+                br[b] = Br[k][wc + b * 16 + lr];
+                bi[b] = Bi[k][wc + b * 16 + lr];
+            }
+            for (int b = 0; b < 2; b++) {
+                const T bs = br[b] + bi[b];
+                for (int a = 0; a < 2; a++) acc[a][b].mma(ar[a], ai[a], br[b], bi[b], bs);
+            }
         }
-        for (int b = 0; b < 2; b++) {
-            br[b] = Br[k][wc + b * 16 + lr];
-            bi[b] = Bi[k][wc + b * 16 + lr];
-        }
-        for (int b = 0; b < 2; b++) {
-            const T bs = br[b] + bi[b];
-            for (int a = 0; a < 2; a++) acc[a][b].mma(ar[a], ai[a], br[b], bi[b], bs);
+        if (more) {
+            __syncthreads();   // every wave is done with this chunk
+            if (!kPrefetch) load_chunk(kc + KB);
+            stage_chunk();
+            __syncthreads();
         }
     }
+    if (DEEP) load_c();
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -862,7 +947,7 @@ __global__ void __launch_bounds__(256) k_gemm_update(cx<T> *__restrict__ arena, 
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = r0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = c0 + wc + b * 16 + lr;
-                if (row < m && col < m)
+                if (row < m && col < cend)
                     G[row + (int64_t)col * m] = {c[a][b][v].x - acc[a][b].re(v), c[a][b][v].y - acc[a][b].im(v)};
             }
 }
@@ -1037,10 +1122,12 @@ __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int
 namespace {
 size_t g_panel_lds_cap = 0;   // 0: the hardware limit
 int g_persistent = 1;         // small subtrees in one persistent launch
+int g_deep_min_m = 0;         // blocked levels with a front of order >= this take the two-panel schedule
 }
-void set_factor_tuning(int panel_lds_cap, int persistent) {
+void set_factor_tuning(int panel_lds_cap, int persistent, int deep_min_m) {
     g_panel_lds_cap = panel_lds_cap > 0 ? (size_t)panel_lds_cap : 0;
     g_persistent = persistent != 0;
+    g_deep_min_m = deep_min_m;
 }
 
 template <typename T>
@@ -1067,6 +1154,10 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
         GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_subtree_factor<float>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+        GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_trsm_u<double, 64>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)trsm_lds<double, 64>()));
+        GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_trsm_u<float, 64>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)trsm_lds<float, 64>()));
         attr_set = true;
     }
     const bool sub = g_persistent && dp.nroots > 0 && F.sched;
@@ -1121,12 +1212,15 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
                                dp.fronts, sub ? dp.lv_small_rest[l] : dp.lv_small[l], F.piv, pl.n, F.sing);
         }
         if (dp.n_large[l] > 0) {
-            int si = 0;
-            for (int k0 = 0; k0 < L.large_pmax; k0 += pl.nb, si++) {
+            const size_t lds32 = trsm_lds<T, 32>(), lds64 = trsm_lds<T, 64>();
+            // one 32-column panel of every blocked front of the level: its LU
+            // (in LDS, or in global memory when the candidate rows do not fit)
+            // and the row swaps on the other columns with the contribution rows
+            auto panel = [&](int k0, double flops) {
                 const size_t plds = (size_t)(L.large_pmax - k0) * pl.nb * sizeof(cx<T>);
                 if (plds <= panel_lds_max) {
                     {
-                        ProfScope ps(K_PANEL, st, cnt.panel_flops[si] * nf, 0.0);
+                        ProfScope ps(K_PANEL, st, flops * nf, 0.0);
                         hipLaunchKernelGGL(k_panel_lu<T>, dim3(dp.n_large[l], nf), dim3(256), plds, st, F.arena,
                                            F.fstride, dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, F.sing);
                     }
@@ -1136,21 +1230,64 @@ void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, in
                     hipLaunchKernelGGL(k_panel_apply<T>, dim3(ccols + crows, dp.n_large[l], nf), dim3(64), 0, st,
                                        F.arena, F.fstride, dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, ccols);
                 } else {
-                    ProfScope ps(K_PANEL, st, cnt.panel_flops[si] * nf, 0.0);
+                    ProfScope ps(K_PANEL, st, flops * nf, 0.0);
                     hipLaunchKernelGGL(k_panel<T>, dim3(dp.n_large[l], nf), dim3(256), 0, st, F.arena, F.fstride,
                                        dp.fronts, dp.lv_large[l], F.piv, pl.n, k0, pl.nb, F.sing);
                 }
-                {
-                ProfScope ps(K_TRSM, st, cnt.trsm_flops[si] * nf, 0.0);
-                int colchunks = (L.large_mmax - k0 + 63) / 64;
-                hipLaunchKernelGGL(k_trsm_u<T>, dim3(colchunks, dp.n_large[l], nf), dim3(256), 0, st, F.arena,
-                                   F.fstride, dp.fronts, dp.lv_large[l], k0, pl.nb);
+            };
+            auto gemm = [&](int64_t off, int cnt, int k0, int depth, int ccap, double flops, double elems) {
+                if (cnt <= 0) return;
+                ProfScope ps(K_GEMM, st, flops * nf, elems * es);
+                if (depth > 32)
+                    hipLaunchKernelGGL((k_gemm_update<T, true>), dim3(cnt, nf), dim3(256), 0, st, F.arena, F.fstride,
+                                       dp.fronts, dp.lv_large[l], dp.tiles + off, k0, depth, ccap);
+                else
+                    hipLaunchKernelGGL((k_gemm_update<T, false>), dim3(cnt, nf), dim3(256), 0, st, F.arena,
+                                       F.fstride, dp.fronts, dp.lv_large[l], dp.tiles + off, k0, depth, ccap);
+            };
+            if (L.large_mmax >= g_deep_min_m) {
+                // two panels per block: panel 1; its U rows and update on panel 2's
+                // columns only; panel 2; the U rows of both panels (one 64-row
+                // solve) and one depth-64 trailing update, so the trailing block
+                // is read and written once per 64 pivots.  Panel 2's row swaps
+                // reach the not yet updated columns and panel 1's L columns alike,
+                // which keeps the deferred update consistent.
+                int bi = 0;
+                for (int b0 = 0; b0 < L.large_pmax; b0 += 64, bi++) {
+                    const auto &d = dp.deep_steps[l][bi];
+                    panel(b0, d.panel1_flops);
+                    if (d.thin_cnt > 0) {
+                        {
+                            ProfScope ps(K_TRSM, st, d.trsm_thin_flops * nf, 0.0);
+                            hipLaunchKernelGGL((k_trsm_u<T, 32>), dim3(1, dp.n_large[l], nf), dim3(256),
+                                               lds32, st, F.arena, F.fstride, dp.fronts, dp.lv_large[l],
+                                               b0, 64);
+                        }
+                        gemm(d.thin_off, d.thin_cnt, b0, 32, 64, d.thin_flops, d.thin_elems);
+                        panel(b0 + 32, d.panel2_flops);
+                    }
+                    {
+                        ProfScope ps(K_TRSM, st, d.trsm_deep_flops * nf, 0.0);
+                        const int colchunks = (L.large_mmax - b0 + 31) / 32;
+                        hipLaunchKernelGGL((k_trsm_u<T, 64>), dim3(colchunks, dp.n_large[l], nf), dim3(256),
+                                           lds64, st, F.arena, F.fstride, dp.fronts, dp.lv_large[l], b0,
+                                           0);
+                    }
+                    gemm(d.deep_off, d.deep_cnt, b0, 64, 0, d.deep_flops, d.deep_elems);
                 }
-                const auto &sp = dp.steps[l][si];
-                if (sp.second > 0) {
-                    ProfScope ps(K_GEMM, st, cnt.gemm_flops[si] * nf, cnt.gemm_elems[si] * es);
-                    hipLaunchKernelGGL(k_gemm_update<T>, dim3(sp.second, nf), dim3(256), 0, st, F.arena, F.fstride,
-                                       dp.fronts, dp.lv_large[l], dp.tiles + sp.first, k0, pl.nb);
+            } else {
+                int si = 0;
+                for (int k0 = 0; k0 < L.large_pmax; k0 += pl.nb, si++) {
+                    panel(k0, cnt.panel_flops[si]);
+                    {
+                        ProfScope ps(K_TRSM, st, cnt.trsm_flops[si] * nf, 0.0);
+                        const int colchunks = (L.large_mmax - k0 + 63) / 64;
+                        hipLaunchKernelGGL((k_trsm_u<T, 32>), dim3(colchunks, dp.n_large[l], nf), dim3(256),
+                                           lds32, st, F.arena, F.fstride, dp.fronts, dp.lv_large[l], k0,
+                                           0);
+                    }
+                    const auto &sp = dp.steps[l][si];
+                    gemm(sp.first, sp.second, k0, 32, 0, cnt.gemm_flops[si], cnt.gemm_elems[si]);
                 }
             }
         }

@@ -1417,21 +1417,34 @@ __global__ void __launch_bounds__(256) k_fwd_vec(const cx<T> *__restrict__ arena
     }
     __syncthreads();
     if (single) {
-        // one product over all rows: out = [Dop; Lop21'] w1 (m <= 512 rows, two per thread)
-        cx<T> acc[2] = {{T(0), T(0)}, {T(0), T(0)}};
-        for (int u = 0; u < 2; u++) {
-            const int i = tid + 256 * u;
+        // one product over all rows: out = [Dop; Lop21'] w1.  Four lanes per row
+        // (each sums every fourth pivot column, two xor-shuffles add them), 64
+        // rows per pass, so every wave works and each lane waits on at most
+        // eight loads (p <= 32)
+        static_assert(kSingleMmax <= 5 * 64, "rows of a single-block front: five passes of 64");
+        const int ri = tid >> 2, part = tid & 3;
+        cx<T> res[5];
+#pragma unroll
+        for (int u = 0; u < 5; u++) {
+            const int i = ri + 64 * u;
+            cx<T> acc = {T(0), T(0)};
             if (i < m)
 #pragma unroll 8
-                for (int t = 0; t < p; t++) {
+                for (int t = part; t < p; t += 4) {
                     const cx<T> a = i < p ? dop(G, m, 0, i, t, true, trans, cj) : Lop(i, t);
-                    acc[u] = cadd(acc[u], cmul(a, sw[t]));
+                    acc = cadd(acc, cmul(a, sw[t]));
                 }
+            acc.x += __shfl_xor(acc.x, 1);
+            acc.y += __shfl_xor(acc.y, 1);
+            acc.x += __shfl_xor(acc.x, 2);
+            acc.y += __shfl_xor(acc.y, 2);
+            res[u] = acc;
         }
         __syncthreads();
-        for (int u = 0; u < 2; u++) {
-            const int i = tid + 256 * u;
-            if (i < m) sw[i] = i < p ? acc[u] : csub(sw[i], acc[u]);
+#pragma unroll
+        for (int u = 0; u < 5; u++) {
+            const int i = ri + 64 * u;
+            if (part == 0 && i < m) sw[i] = i < p ? res[u] : csub(sw[i], res[u]);
         }
         __syncthreads();
     }

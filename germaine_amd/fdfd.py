@@ -89,12 +89,17 @@ def solve_tuning(rw_cap=64, fused_mmax=0, refine_layout=0):
     check(lib().gfdfd_solve_tuning(rw_cap, fused_mmax, refine_layout), "gfdfd_solve_tuning")
 
 
-def factor_tuning(panel_lds_cap=0, persistent=1):
+NO_DEEP = 1 << 30   # deep_min_m that keeps every level on one panel per step
+
+
+def factor_tuning(panel_lds_cap=0, persistent=1, deep_min_m=0):
     """Factorisation kernel selection (process-wide): panels needing more than
     panel_lds_cap bytes of LDS take the global-memory panel kernel (0: the
     hardware limit); persistent selects the one-launch factorisation of the
-    small subtrees (else level by level)."""
-    check(lib().gfdfd_factor_tuning(int(panel_lds_cap), int(persistent)), "gfdfd_factor_tuning")
+    small subtrees (else level by level); levels whose largest blocked front
+    has order >= deep_min_m take the two-panel schedule (trailing update once
+    per 64 pivots), the others one panel at a time (NO_DEEP: none)."""
+    check(lib().gfdfd_factor_tuning(int(panel_lds_cap), int(persistent), int(deep_min_m)), "gfdfd_factor_tuning")
 
 
 class Plan:

@@ -176,9 +176,12 @@ int gfdfd_solve_tuning(int rw_cap, int fused_mmax, int refine_layout);
  * instead of in LDS (0: the hardware limit, 159 KB -- separators above 318
  * rows in FP64, 637 in FP32), which lets small systems exercise that path;
  * persistent != 0 (the default) factors the small subtrees -- fronts of order
- * <= 64 whose descendants are all that small -- in one persistent launch over
- * (front, frequency) tasks with per-task done flags, 0 level by level.      */
-int gfdfd_factor_tuning(int panel_lds_cap, int persistent);
+ * <= 64 whose descendants are all that small -- in one persistent launch in
+ * which a workgroup factors one whole subtree of one frequency per task, 0
+ * level by level; levels whose largest blocked front has order >= deep_min_m
+ * (default 0: every level) factor two 32-column panels per step and update
+ * the trailing block once per 64 pivots, the others once per panel.        */
+int gfdfd_factor_tuning(int panel_lds_cap, int persistent, int deep_min_m);
 
 /* ---- shot kernels --------------------------------------------------------------
  * RHS_source_AC / RHS_source_TE for all shots (src/AC/RHS_source_AC.c:46-53):

@@ -332,6 +332,52 @@ def test_persistent_subtrees_equal_level_path(precision, matrix):
 
 @pytest.mark.parametrize("matrix", ["helmholtz", "random"])
 @pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_two_panel_schedule_matches_oracle(precision, matrix):
+    """The blocked path's two schedules -- two 32-column panels per step with
+    one depth-64 trailing update (every level), and one panel per step with a
+    depth-32 update (no level) -- each against the banded oracle for M and
+    M^T, three frequencies in one batch, on a grid whose separators have more
+    than 64 pivots (several 64-column blocks, a last block of one panel), on the
+    assembled Helmholtz matrices and on random values whose partial pivoting
+    swaps rows (so the second panel's swaps reach the deferred update)."""
+    nx, ny = 97, 81
+    say("two-panel schedule", precision, matrix)
+    if matrix == "helmholtz":
+        m1, m2 = smooth_model(nx, ny, seed=12)
+        mats = [oracle_system(G.AC, nx, ny, 8, 10.0, f, m1, m2)[1] for f in (3.0, 9.0, 15.0)]
+    else:
+        mats = [random_grid_csc(nx, ny, seed) for seed in (51, 52, 53)]
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    assert pl.info.max_pivots > 64
+    vals = dev(np.stack([c[2] + 1j * c[3] for c in mats]))
+    rng = np.random.default_rng(21)
+    B = rng.standard_normal((3, 2, nx * ny)) + 1j * rng.standard_normal((3, 2, nx * ny))
+    out = {}
+    for deep in (0, fdfd.NO_DEEP):
+        fdfd.factor_tuning(0, 1, deep)
+        try:
+            F = pl.factor(vals, precision)
+        finally:
+            fdfd.factor_tuning()
+        assert np.all(F.singular() == 0)
+        for sysno in (0, 2):
+            if precision == G.FP64:
+                X = F.solve_batch([0, 1, 2], dev(B), sys=sysno).cpu().numpy()
+            else:
+                X = F.solve_refine_batch([0, 1, 2], dev(B), sys=sysno, tol=1e-12, max_iter=30)[0].cpu().numpy()
+            out[deep, sysno] = X
+            for z in range(3):
+                Xr, _ = O.band_solve(*mats[z], B[z].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+                err = rel(X[z], Xr.T)
+                say("  deep_min_m", deep, "sys", sysno, "freq", z, "rel err", err)
+                assert err <= (1e-10 if matrix == "helmholtz" and precision == G.FP64 else 1e-8)
+    # the two schedules give the same factors to rounding
+    for sysno in (0, 2):
+        assert rel(out[0, sysno], out[fdfd.NO_DEEP, sysno]) <= (1e-9 if precision == G.FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("matrix", ["helmholtz", "random"])
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
 def test_global_memory_panel_path_matches_oracle(precision, matrix):
     """The blocked path's global-memory panel kernel, which factors panels too
     large for LDS (separators above 318 rows in FP64, 637 in FP32): forced on
