@@ -27,6 +27,14 @@ class GfdfdError(RuntimeError):
     pass
 
 
+class ShotTaperC(C.Structure):
+    """gfdfd_shot_taper (include/gfdfd.h)."""
+    _fields_ = [("mode", C.c_int), ("nx", C.c_int), ("dh", C.c_float), ("srtradius", C.c_float),
+                ("filtsize", C.c_int), ("d_srcx", C.c_void_p), ("d_srcy", C.c_void_p), ("nsrc", C.c_int),
+                ("d_src_i", C.c_void_p), ("d_src_j", C.c_void_p), ("nrec", C.c_int), ("d_rec_i", C.c_void_p),
+                ("d_rec_j", C.c_void_p)]
+
+
 class PlanInfo(C.Structure):
     _fields_ = [("n", C.c_longlong), ("nnz", C.c_longlong), ("fronts", C.c_int), ("levels", C.c_int),
                 ("max_front", C.c_int), ("max_pivots", C.c_int), ("arena_elems", C.c_longlong),
@@ -79,6 +87,9 @@ SIGNATURES = {
     "gfdfd_residual_sources": (_I, [_I, _I, _I, _P, _P, _P, _F, _I, _I, _I, _P, _P, _P, _P]),
     "gfdfd_grad_ac": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_grad_ac_stf": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "gfdfd_grad_ac_taper": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "gfdfd_wavelet_amp": (None, [_I, _F, _F, _P, _I, _I, _P]),
+    "gfdfd_shot_taper_exponent": (_F, []),
     "gfdfd_hess_shin_ac": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
     "gfdfd_hess_shin_te": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
     "gfdfd_hess_gn_ac": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
@@ -92,6 +103,7 @@ SIGNATURES = {
     "gfdfd_tikhonov_te": (_I, [_I, _I, _F, _F, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P]),
     "gfdfd_grad_te": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_grad_te_stf": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "gfdfd_grad_te_taper": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gfdfd_profile": (_I, [_I]),
     "gfdfd_device_memory": (_I, [_LLP, _LLP, _I]),
     "gfdfd_release_workspace": (_I, []),

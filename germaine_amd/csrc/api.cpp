@@ -527,7 +527,7 @@ int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, con
         return report(GFDFD_ERR_ARG, "gfdfd_grad_ac: bad argument");
     GUARD_BEGIN
     grad_ac(n, nshots, omega2, reinterpret_cast<const cx<double> *>(d_u), reinterpret_cast<const cx<double> *>(d_lam),
-            d_vp, d_rho, nullptr, d_grad, S(stream));
+            d_vp, d_rho, nullptr, ShotTaperDev{}, d_grad, S(stream));
     return GFDFD_OK;
     GUARD_END
 }
@@ -538,7 +538,52 @@ int gfdfd_grad_ac_stf(int n, int nshots, const float *omega2, const double *d_u,
         return report(GFDFD_ERR_ARG, "gfdfd_grad_ac_stf: bad argument");
     GUARD_BEGIN
     grad_ac(n, nshots, omega2, reinterpret_cast<const cx<double> *>(d_u), reinterpret_cast<const cx<double> *>(d_lam),
-            d_vp, d_rho, reinterpret_cast<const cx<float> *>(d_stf), d_grad, S(stream));
+            d_vp, d_rho, reinterpret_cast<const cx<float> *>(d_stf), ShotTaperDev{}, d_grad, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+namespace {
+// the device form of a gfdfd_shot_taper, with the taper's float exponent
+bool taper_dev(const gfdfd_shot_taper *t, int nshots, ShotTaperDev &d) {
+    d = ShotTaperDev{};
+    if (!t || t->mode == 0) return true;
+    if (t->mode != 1 && t->mode != 2) return false;
+    if (t->nx < 1 || !(t->dh > 0.0f) || t->nrec < 0 || (t->nrec > 0 && (!t->d_rec_i || !t->d_rec_j))) return false;
+    if (t->mode == 1 && (!(t->srtradius > 0.0f) || !t->d_srcx || !t->d_srcy)) return false;
+    if (t->mode == 2 && (t->filtsize < 0 || t->nsrc < 0 || (t->nsrc > 0 && (!t->d_src_i || !t->d_src_j))))
+        return false;
+    (void)nshots;
+    d.mode = t->mode;
+    d.nx = t->nx;
+    d.dh = t->dh;
+    d.radius = t->srtradius;
+    d.a = gfdfd_shot_taper_exponent();
+    d.f0 = 1e-3f;
+    d.filtsize = t->filtsize;
+    d.srcx = t->d_srcx;
+    d.srcy = t->d_srcy;
+    d.nsrc = t->nsrc;
+    d.src_i = t->d_src_i;
+    d.src_j = t->d_src_j;
+    d.nrec = t->nrec;
+    d.rec_i = t->d_rec_i;
+    d.rec_j = t->d_rec_j;
+    return true;
+}
+}  // namespace
+
+int gfdfd_grad_ac_taper(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam,
+                        const float *d_vp, const float *d_rho, const float *d_stf, const gfdfd_shot_taper *taper,
+                        double *d_grad, void *stream) {
+    ShotTaperDev tp;
+    if (n < 1 || nshots < 1 || !omega2 || !d_u || !d_lam || !d_vp || !d_rho || !d_grad || !taper_dev(taper, nshots, tp))
+        return report(GFDFD_ERR_ARG, "gfdfd_grad_ac_taper: bad argument");
+    if (tp.mode && (int64_t)tp.nx * ((n + tp.nx - 1) / tp.nx) != n)
+        return report(GFDFD_ERR_ARG, "gfdfd_grad_ac_taper: n = %d is not a multiple of the grid width %d", n, tp.nx);
+    GUARD_BEGIN
+    grad_ac(n, nshots, omega2, reinterpret_cast<const cx<double> *>(d_u), reinterpret_cast<const cx<double> *>(d_lam),
+            d_vp, d_rho, reinterpret_cast<const cx<float> *>(d_stf), tp, d_grad, S(stream));
     return GFDFD_OK;
     GUARD_END
 }
@@ -689,7 +734,7 @@ int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, co
         return report(GFDFD_ERR_ARG, "gfdfd_grad_te: bad argument");
     GUARD_BEGIN
     grad_te(n, nshots, omega, omega2, reinterpret_cast<const cx<double> *>(d_u),
-            reinterpret_cast<const cx<double> *>(d_lam), nullptr, d_grad_sigma, d_grad_eps, S(stream));
+            reinterpret_cast<const cx<double> *>(d_lam), nullptr, ShotTaperDev{}, d_grad_sigma, d_grad_eps, S(stream));
     return GFDFD_OK;
     GUARD_END
 }
@@ -701,7 +746,24 @@ int gfdfd_grad_te_stf(int n, int nshots, const float *omega, const float *omega2
         return report(GFDFD_ERR_ARG, "gfdfd_grad_te_stf: bad argument");
     GUARD_BEGIN
     grad_te(n, nshots, omega, omega2, reinterpret_cast<const cx<double> *>(d_u),
-            reinterpret_cast<const cx<double> *>(d_lam), reinterpret_cast<const cx<float> *>(d_stf), d_grad_sigma,
+            reinterpret_cast<const cx<double> *>(d_lam), reinterpret_cast<const cx<float> *>(d_stf), ShotTaperDev{},
+            d_grad_sigma, d_grad_eps, S(stream));
+    return GFDFD_OK;
+    GUARD_END
+}
+
+int gfdfd_grad_te_taper(int n, int nshots, const float *omega, const float *omega2, const double *d_u,
+                        const double *d_lam, const float *d_stf, const gfdfd_shot_taper *taper, double *d_grad_sigma,
+                        double *d_grad_eps, void *stream) {
+    ShotTaperDev tp;
+    if (n < 1 || nshots < 1 || !omega || !omega2 || !d_u || !d_lam || !d_grad_sigma || !d_grad_eps ||
+        !taper_dev(taper, nshots, tp))
+        return report(GFDFD_ERR_ARG, "gfdfd_grad_te_taper: bad argument");
+    if (tp.mode && (int64_t)tp.nx * ((n + tp.nx - 1) / tp.nx) != n)
+        return report(GFDFD_ERR_ARG, "gfdfd_grad_te_taper: n = %d is not a multiple of the grid width %d", n, tp.nx);
+    GUARD_BEGIN
+    grad_te(n, nshots, omega, omega2, reinterpret_cast<const cx<double> *>(d_u),
+            reinterpret_cast<const cx<double> *>(d_lam), reinterpret_cast<const cx<float> *>(d_stf), tp, d_grad_sigma,
             d_grad_eps, S(stream));
     return GFDFD_OK;
     GUARD_END

@@ -7,6 +7,8 @@
  *   Omega^2, Omega          src/AC/init_A_AC_9p_pml.c:50, src/SH/init_A_SH_9p_pml.c:50,
  *                           src/TE/init_A_TE_9p_pml.c:54-58
  *   point-source amplitude  src/AC/RHS_source_AC.c:37-53, src/TE/RHS_source_TE.c:22-56
+ *   wavelet source amplitude src/read_stf_dft.c:34-70 with RHS_source_AC.c:41-53
+ *   per-shot taper exponent src/taper_grad_shot.c:37-40
  *   gradient Omega^2        src/AC/ass_grad_AC.c:22
  */
 #include <complex.h>
@@ -109,4 +111,40 @@ void gfdfd_source_amp(int physics, float freq, float s, float dh, double *amp)
     }
     amp[0] = re / (dh * dh);
     amp[1] = im / (dh * dh);
+}
+
+/* READ_WAVELET == 1: the source amplitude is the DFT of the time-domain
+ * wavelet at the frequency (src/read_stf_dft.c:64-70), divided by DH^2 as the
+ * spike is (src/AC/RHS_source_AC.c:52-53, src/TE/RHS_source_TE.c:55-56; the
+ * DFT replaces the TE spike's -i omega mu0 factor).  samples: the ns values
+ * of the SU trace; ndt: its header's sample interval, converted as the
+ * reference does (dt = ndt / 1e6 for AC/SH, ndt / 1e12 for TE).  The reference reads the ns samples into data[0..ns-1] but
+ * sums data[1..ns] at t = (j-1) dt, j = 1..ns: sample k >= 1 enters at time
+ * (k-1) dt, sample 0 is skipped, and the last term takes data[ns], an element
+ * the read never filled (undefined contents); here that term is zero.  The
+ * sum is rounded to float after every term (amp[] is float).               */
+void gfdfd_wavelet_amp(int physics, float freq, float dh, const float *samples, int ns, int ndt, double *amp)
+{
+    float dt;
+    if (physics == 3) dt = ndt / 1e12;
+    else dt = ndt / 1e6;
+    float re = 0.0, im = 0.0;
+    for (int j = 1; j <= ns; j++) {
+        float t = (j - 1) * dt;
+        float w = j < ns ? samples[j] : 0.0f;
+        re += w * cos(2.0 * t * freq * M_PI) * dt;
+        im += w * sin(2.0 * t * freq * M_PI) * dt;
+    }
+    amp[0] = re / (dh * dh);
+    amp[1] = im / (dh * dh);
+}
+
+/* taper_grad_shot's exponent a = -(log(frh - f0) - log(1 - f0)) / log(2) with
+ * frh = 0.1, f0 = 1e-3 in float (src/taper_grad_shot.c:36-39): the taper
+ * f0 + (1 - f0) (0.5 - 0.5 cos(pi r / R))^a is 0.1 at r = R/2.            */
+float gfdfd_shot_taper_exponent(void)
+{
+    float frh = 0.1, f0 = 1e-3, a;
+    a = -(log(frh - f0) - log(1.0 - f0)) / log(2.0);
+    return a;
 }

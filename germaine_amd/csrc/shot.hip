@@ -159,6 +159,42 @@ __device__ __forceinline__ void stage_stf_weights(const cx<float> *__restrict__ 
     }
 }
 
+// ---- per-shot gradient tapers (ass_grad_AC/TE with SWS_TAPER_CIRCULAR_PER_SHOT) ----
+// taper_grad_shot (mode 1, src/taper_grad_shot.c:34-87): the shot's gradient
+// is multiplied by f0 + (1 - f0) (0.5 - 0.5 cos(pi r/R))^a within r <= R of
+// its source and of every receiver, r measured from (ix DH, iy DH) of cell
+// (ix, iy) -- the PML-extended grid's 1-based indices -- to srcpos (m) and to
+// recpos * DH, in float as the reference rounds them.  taper_grad_shot1
+// (mode 2, src/taper_grad_shot1.c:32-66): the shot's gradient is 1e-5 on the
+// (2 FILTSIZE + 1)^2 boxes around iround(srcpos / DH) of every source of the
+// survey and around every receiver.  The products here accumulate in double,
+// as the rest of the gradient does.
+__device__ double taper_at(float x, float y, float cx, float cy, const ShotTaperDev &t) {
+    const float dx = x - cx, dy = y - cy;
+    const float r = (float)sqrt((double)dx * dx + (double)dy * dy);
+    if (r > t.radius) return 1.0;
+    const float v = t.f0 + (1.0 - t.f0) * pow(0.5 - 0.5 * cos(3.141592653589793 * r / t.radius), (double)t.a);
+    return v;
+}
+
+// the taper factors of cell q that do not depend on the shot: the receivers'
+// product (mode 1), or whether the cell is in a source or receiver box (mode 2)
+__device__ void taper_cell(int q, const ShotTaperDev &t, double &trec, bool &boxed) {
+    trec = 1.0;
+    boxed = false;
+    const int ix = q % t.nx + 1, iy = q / t.nx + 1;
+    if (t.mode == 1) {
+        const float x = ix * t.dh, y = iy * t.dh;
+        for (int r = 0; r < t.nrec; r++) trec *= taper_at(x, y, t.rec_i[r] * t.dh, t.rec_j[r] * t.dh, t);
+    } else if (t.mode == 2) {
+        for (int k = 0; k < t.nsrc && !boxed; k++)
+            boxed = abs(ix - t.src_i[k]) <= t.filtsize && abs(iy - t.src_j[k]) <= t.filtsize;
+        for (int r = 0; r < t.nrec && !boxed; r++)
+            boxed = abs(ix - t.rec_i[r]) <= t.filtsize && abs(iy - t.rec_j[r]) <= t.filtsize;
+    }
+}
+
+
 // src/AC/ass_grad_AC.c:30-31 summed over shots in FP64: cell q uses the
 // solutions at q+1 (src/vec2mat.c:19-24); the last cell reads past the
 // solution in the reference and gets 0 here.  Shot s is weighted by
@@ -167,27 +203,34 @@ __device__ __forceinline__ void stage_stf_weights(const cx<float> *__restrict__ 
 __global__ void __launch_bounds__(256) k_grad_ac(int n, int nshots, double o2r, double o2i,
                                                  const cx<double> *__restrict__ U, const cx<double> *__restrict__ L,
                                                  const float *__restrict__ vp, const float *__restrict__ rho,
-                                                 const cx<float> *__restrict__ stf, double *__restrict__ grad) {
+                                                 const cx<float> *__restrict__ stf, ShotTaperDev tp,
+                                                 double *__restrict__ grad) {
     __shared__ double wr[kGradShotChunk], wi[kGradShotChunk];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = q + 1 < n;
+    double trec = 1.0;
+    bool boxed = false;
+    if (on && tp.mode) taper_cell(q, tp, trec, boxed);
+    const float x = (q % max(tp.nx, 1) + 1) * tp.dh, y = (q / max(tp.nx, 1) + 1) * tp.dh;
     double ar = 0.0, ai = 0.0;
     for (int s0 = 0; s0 < nshots; s0 += kGradShotChunk) {
         stage_stf_weights(stf, s0, nshots, wr, wi);
         __syncthreads();
         const int cnt = min(kGradShotChunk, nshots - s0);
-        if (on)
+        if (on && !boxed)
             for (int t = 0; t < cnt; t++) {
                 const cx<double> u = U[q + 1 + (int64_t)(s0 + t) * n], l = L[q + 1 + (int64_t)(s0 + t) * n];
                 const double pr = u.x * l.x - u.y * l.y, pi = u.x * l.y + u.y * l.x;
-                ar += wr[t] * pr - wi[t] * pi;
-                ai += wr[t] * pi + wi[t] * pr;
+                const double ts = tp.mode == 1 ? taper_at(x, y, tp.srcx[s0 + t], tp.srcy[s0 + t], tp) : 1.0;
+                ar += ts * (wr[t] * pr - wi[t] * pi);
+                ai += ts * (wr[t] * pi + wi[t] * pr);
             }
         __syncthreads();
     }
     if (on) {
         const double v = vp[q];
-        grad[q] += -2.0 * (o2r * ar - o2i * ai) / ((double)rho[q] * v * v * v);
+        if (boxed) grad[q] += nshots * (double)1e-5f;   // taper_grad_shot1's value, once per shot
+        else grad[q] += trec * (-2.0 * (o2r * ar - o2i * ai) / ((double)rho[q] * v * v * v));
     }
 }
 
@@ -499,29 +542,39 @@ __global__ void k_grad_frame(int nx, int ny, int nx0, int ny0, int npml, int fss
 // solutions at q+1; the last cell is left unchanged.  One thread per cell.
 __global__ void __launch_bounds__(256) k_grad_te(int n, int nshots, double omr, double omi, double o2r, double o2i,
                                                  const cx<double> *__restrict__ U, const cx<double> *__restrict__ L,
-                                                 const cx<float> *__restrict__ stf, double *__restrict__ gsig,
-                                                 double *__restrict__ geps) {
+                                                 const cx<float> *__restrict__ stf, ShotTaperDev tp,
+                                                 double *__restrict__ gsig, double *__restrict__ geps) {
     __shared__ double wr[kGradShotChunk], wi[kGradShotChunk];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = q + 1 < n;
+    double trec = 1.0;
+    bool boxed = false;
+    if (on && tp.mode) taper_cell(q, tp, trec, boxed);
+    const float x = (q % max(tp.nx, 1) + 1) * tp.dh, y = (q / max(tp.nx, 1) + 1) * tp.dh;
     double ar = 0.0, ai = 0.0;
     for (int s0 = 0; s0 < nshots; s0 += kGradShotChunk) {
         stage_stf_weights(stf, s0, nshots, wr, wi);
         __syncthreads();
         const int cnt = min(kGradShotChunk, nshots - s0);
-        if (on)
+        if (on && !boxed)
             for (int t = 0; t < cnt; t++) {
                 const cx<double> u = U[q + 1 + (int64_t)(s0 + t) * n], l = L[q + 1 + (int64_t)(s0 + t) * n];
                 const double pr = u.x * l.x - u.y * l.y, pi = u.x * l.y + u.y * l.x;
-                ar += wr[t] * pr - wi[t] * pi;
-                ai += wr[t] * pi + wi[t] * pr;
+                const double ts = tp.mode == 1 ? taper_at(x, y, tp.srcx[s0 + t], tp.srcy[s0 + t], tp) : 1.0;
+                ar += ts * (wr[t] * pr - wi[t] * pi);
+                ai += ts * (wr[t] * pi + wi[t] * pr);
             }
         __syncthreads();
     }
     if (on) {
         // Re((i Omega) z) = -(omr*ai + omi*ar);  Re(Omega^2 z) = o2r*ar - o2i*ai
-        gsig[q] += -(omr * ai + omi * ar);
-        geps[q] += o2r * ar - o2i * ai;
+        if (boxed) {
+            gsig[q] += nshots * (double)1e-5f;
+            geps[q] += nshots * (double)1e-5f;
+        } else {
+            gsig[q] += trec * -(omr * ai + omi * ar);
+            geps[q] += trec * (o2r * ar - o2i * ai);
+        }
     }
 }
 
@@ -597,9 +650,9 @@ void adjoint_sources(int n, int nshots, int ntr, const int *krec, const cx<float
 }
 
 void grad_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const cx<double> *L, const float *vp,
-             const float *rho, const cx<float> *stf, double *grad, hipStream_t st) {
+             const float *rho, const cx<float> *stf, const ShotTaperDev &tp, double *grad, hipStream_t st) {
     hipLaunchKernelGGL(k_grad_ac, dim3((n + 255) / 256), dim3(256), 0, st, n, nshots, (double)omega2[0],
-                       (double)omega2[1], U, L, vp, rho, stf, grad);
+                       (double)omega2[1], U, L, vp, rho, stf, tp, grad);
     GFDFD_HIP(hipGetLastError());
 }
 
@@ -641,9 +694,10 @@ void hess_shin_te(int n, int nshots, const float omega[2], const float omega2[2]
 }
 
 void grad_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U,
-             const cx<double> *L, const cx<float> *stf, double *gsig, double *geps, hipStream_t st) {
+             const cx<double> *L, const cx<float> *stf, const ShotTaperDev &tp, double *gsig, double *geps,
+             hipStream_t st) {
     hipLaunchKernelGGL(k_grad_te, dim3((n + 255) / 256), dim3(256), 0, st, n, nshots, (double)omega[0],
-                       (double)omega[1], (double)omega2[0], (double)omega2[1], U, L, stf, gsig, geps);
+                       (double)omega[1], (double)omega2[0], (double)omega2[1], U, L, stf, tp, gsig, geps);
     GFDFD_HIP(hipGetLastError());
 }
 

@@ -4,6 +4,19 @@
 
 namespace gfdfd {
 
+// ass_grad_AC/TE's per-shot taper (SWS_TAPER_CIRCULAR_PER_SHOT): mode 0 none,
+// 1 taper_grad_shot, 2 taper_grad_shot1; device pointers
+struct ShotTaperDev {
+    int mode = 0, nx = 1;
+    float dh = 1.0f, radius = 0.0f, a = 0.0f, f0 = 1e-3f;
+    int filtsize = 0;
+    const float *srcx = nullptr, *srcy = nullptr;   // mode 1: the panel's shots' sources (m)
+    int nsrc = 0;
+    const int *src_i = nullptr, *src_j = nullptr;   // mode 2: iround(srcpos / DH) of every source
+    int nrec = 0;
+    const int *rec_i = nullptr, *rec_j = nullptr;   // recpos (grid indices)
+};
+
 void spike_panel(int n, int nshots, const int *k, cx<double> amp, cx<double> *B, hipStream_t st);
 void sample_receivers(int n, int nshots, int ntr, const int *krec, const cx<double> *X, cx<float> *pcal,
                       hipStream_t st);
@@ -11,7 +24,7 @@ void adjoint_sources(int n, int nshots, int ntr, const int *krec, const cx<float
                      float dh, int misfit, int stf_inv, int rtm, cx<double> *B, double *l2, cx<float> *stf,
                      hipStream_t st);
 void grad_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const cx<double> *L, const float *vp,
-             const float *rho, const cx<float> *stf, double *grad, hipStream_t st);
+             const float *rho, const cx<float> *stf, const ShotTaperDev &tp, double *grad, hipStream_t st);
 void hess_shin_ac(int n, int nshots, const float omega2[2], const cx<double> *U, const float *vp, const float *rho,
                   double *hess, hipStream_t st);
 void apply_hess(int n, float eps, const double *hess, double *grad, hipStream_t st);
@@ -35,6 +48,7 @@ void hess_gn_te(int n, int nshots, int nrec, const float omega[2], const float o
 void hess_shin_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U, double *hsig,
                   double *heps, hipStream_t st);
 void grad_te(int n, int nshots, const float omega[2], const float omega2[2], const cx<double> *U,
-             const cx<double> *L, const cx<float> *stf, double *gsig, double *geps, hipStream_t st);
+             const cx<double> *L, const cx<float> *stf, const ShotTaperDev &tp, double *gsig, double *geps,
+             hipStream_t st);
 
 }  // namespace gfdfd

@@ -23,7 +23,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import AC, FP32, FP64, SH, TE, FreqParams, PlanInfo, check, lib
+from . import AC, FP32, FP64, SH, TE, FreqParams, PlanInfo, ShotTaperC, check, lib
 
 
 def _ptr(t):
@@ -64,9 +64,17 @@ def omega2_grad_ac(freq, s=0.0):
     return o2
 
 
-def source_amp(physics, freq, s, dh):
+def source_amp(physics, freq, s, dh, wavelet=None):
+    """Point-source amplitude of the right-hand side: the spike (src/AC/RHS_source_AC.c:37-53,
+    src/TE/RHS_source_TE.c:22-56), or with wavelet = (samples, ndt) -- an SU trace as
+    formats.read_su_wavelet returns it (READ_WAVELET = 1) -- its DFT at freq over DH^2
+    (src/read_stf_dft.c:64-70)."""
     a = np.zeros(2, np.float64)
-    lib().gfdfd_source_amp(physics, freq, s, dh, a.ctypes.data)
+    if wavelet is None:
+        lib().gfdfd_source_amp(physics, freq, s, dh, a.ctypes.data)
+    else:
+        w = np.ascontiguousarray(wavelet[0], np.float32)
+        lib().gfdfd_wavelet_amp(physics, freq, dh, w.ctypes.data, len(w), int(wavelet[1]), a.ctypes.data)
     return complex(a[0], a[1])
 
 
@@ -341,7 +349,7 @@ def residual_sources(n, krec, pcal, pobs, dh, misfit=1, stf_inv=0, rtm=False, st
 
 
 def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-                  precision=FP64, tol=1e-10, max_iter=10, keep_fields=False, ws=None):
+                  precision=FP64, tol=1e-10, max_iter=10, keep_fields=False, ws=None, wavelet=None):
     """All shots x all frequencies (src/AC/forward_shot_AC.c:38-101, batched).
 
     Returns dict(data=[nfreq, nshots, ntr] complex64 receiver samples or None,
@@ -351,7 +359,7 @@ def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_
                        free_surf=free_surf)
     data, fields, its, res = [], [], [], []
     for f, fr in enumerate(freqs):
-        B = spike_panel(plan.n, ksrc, source_amp(physics, fr, s, dh))
+        B = spike_panel(plan.n, ksrc, source_amp(physics, fr, s, dh, wavelet))
         if precision == FP64 and tol is None:
             X = F.solve(f, B)
             it, rr = 0, float("nan")
@@ -367,7 +375,7 @@ def forward_shots(plan, physics, dh, freqs, m1, m2, ksrc, krec=None, npml=0, a0_
 
 
 def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=0.0, free_surf=0, precision=FP64,
-                 tol=1e-10, max_iter=10, ws=None):
+                 tol=1e-10, max_iter=10, ws=None, wavelet=None):
     """Shin's pseudo-Hessian diagonal summed over frequencies and shots
     (src/AC/hessian_shin_AC.c:50-125, src/TE/hessian_shin_TE.c:53-137, batched): per frequency
     one factorisation and one panel solve for all shots, then the |J|^2 sums on the device.
@@ -383,7 +391,7 @@ def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=
     h1 = torch.zeros(n, dtype=torch.float64, device="cuda")
     h2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     for f, fr in enumerate(freqs):
-        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
+        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh, wavelet))
         if precision == FP64 and tol is None:
             U = F.solve(f, B)
         else:
@@ -400,7 +408,7 @@ def hessian_shin(plan, physics, dh, freqs, m1, m2, ksrc, npml=0, a0_pml=90.0, s=
 
 
 def hessian_gn(plan, physics, dh, freqs, m1, m2, ksrc, krec, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-               precision=FP64, tol=1e-10, max_iter=10, ws=None):
+               precision=FP64, tol=1e-10, max_iter=10, ws=None, wavelet=None):
     """Gauss-Newton Hessian diagonal summed over frequencies, shots and receivers
     (src/AC/hessian_AC.c:50-160, src/TE/hessian_TE.c:53-177, batched): per frequency one
     factorisation, one panel solve for the shots and one for the receivers' Green's functions (a
@@ -417,7 +425,7 @@ def hessian_gn(plan, physics, dh, freqs, m1, m2, ksrc, krec, npml=0, a0_pml=90.0
     hess = torch.zeros(n, dtype=torch.float64, device="cuda")
     h2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     for f, fr in enumerate(freqs):
-        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
+        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh, wavelet))
         BG = spike_panel(n, krec, complex(np.float32(1.0) / np.float32(np.float32(dh) * np.float32(dh))))
         if precision == FP64 and tol is None:
             U, GR = F.solve(f, B), F.solve(f, BG)
@@ -475,7 +483,7 @@ def model_update_norm(alpha, direction, m_old, norm, lo, hi, out=None, stream=No
 
 
 def objective(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-              precision=FP64, tol=1e-10, max_iter=10, misfit=1, stf_inv=0, ws=None):
+              precision=FP64, tol=1e-10, max_iter=10, misfit=1, stf_inv=0, ws=None, wavelet=None):
     """Misfit only (obj_AC, src/AC/obj_AC.c:46-135): forward panel per frequency,
     receiver samples, residual norms (calc_res_AC with misfit / stf_inv);
     0.5 * sum |res|^2 over frequencies and shots.  The plan (ordering, symbolic
@@ -486,7 +494,7 @@ def objective(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml
                     free_surf=free_surf)
     l2 = 0.0
     for f, fr in enumerate(freqs):
-        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
+        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh, wavelet))
         if precision == FP64 and tol is None:
             U = F.solve(f, B)
         else:
@@ -720,8 +728,49 @@ def grad_frame(grad, nx0, ny0, npml, free_surf, stream=None):
     return grad
 
 
+class ShotTaper:
+    """ass_grad_AC/TE's per-shot gradient taper, SWS_TAPER_CIRCULAR_PER_SHOT
+    (src/AC/ass_grad_AC.c:37-43, src/TE/ass_grad_TE.c:40-49).
+
+    mode 1 (taper_grad_shot, src/taper_grad_shot.c:34-87): every shot's gradient is
+    multiplied by f0 + (1 - f0)(0.5 - 0.5 cos(pi r/SRTRADIUS))^a within SRTRADIUS of the
+    shot's source and of every receiver; mode 2 (taper_grad_shot1, src/taper_grad_shot1.c):
+    it is 1e-5 on the (2 FILTSIZE + 1)^2 boxes around every source and receiver.
+    Distances and boxes use the reference's coordinates: cell (ix, iy) of the
+    PML-extended grid at (ix DH, iy DH) against srcpos in metres and recpos in grid
+    indices, as sources() and receiver() return them (par.read_sources /
+    par.read_receivers), without the PML offset the solves add.
+    srcpos: [nsrc, 2] metres; recpos: [ntr, 2] grid indices; shots: the rows of srcpos
+    that the gradient's shot panel holds, in panel order (default: all)."""
+
+    def __init__(self, mode, nx, dh, srcpos, recpos, srtradius=0.0, filtsize=0, shots=None):
+        import torch
+        if mode not in (1, 2):
+            raise ValueError("SWS_TAPER_CIRCULAR_PER_SHOT must be 1 or 2")
+        srcpos = np.asarray(srcpos, np.float32).reshape(-1, 2)
+        recpos = np.asarray(recpos, np.int32).reshape(-1, 2)
+        shots = np.arange(len(srcpos)) if shots is None else np.asarray(shots)
+        dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+        self.mode, self.nx, self.dh = int(mode), int(nx), float(dh)
+        self.srtradius, self.filtsize = float(srtradius), int(filtsize)
+        self.nshots = len(shots)
+        self._srcx, self._srcy = dev(srcpos[shots, 0]), dev(srcpos[shots, 1])
+        # iround(srcpos / DH): the float quotient, floor(x + 0.5) in double
+        q = srcpos / np.float32(dh)
+        self._si = dev(np.floor(q[:, 0].astype(np.float64) + 0.5).astype(np.int32))
+        self._sj = dev(np.floor(q[:, 1].astype(np.float64) + 0.5).astype(np.int32))
+        self._ri, self._rj = dev(recpos[:, 0]), dev(recpos[:, 1])
+
+    def device_struct(self, nshots):
+        if nshots != self.nshots:
+            raise ValueError(f"the taper holds {self.nshots} shots, the panel {nshots}")
+        return ShotTaperC(self.mode, self.nx, self.dh, self.srtradius, self.filtsize, self._srcx.data_ptr(),
+                          self._srcy.data_ptr(), len(self._si), self._si.data_ptr(), self._sj.data_ptr(),
+                          len(self._ri), self._ri.data_ptr(), self._rj.data_ptr())
+
+
 def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=90.0, s=0.0, free_surf=0,
-             precision=FP64, tol=1e-10, max_iter=10, rtm=False, misfit=1, stf_inv=0, ws=None):
+             precision=FP64, tol=1e-10, max_iter=10, rtm=False, misfit=1, stf_inv=0, ws=None, wavelet=None, taper=None):
     """Misfit and gradient summed over frequencies and shots (src/AC/grad_obj_AC.c:50-168,
     src/TE/grad_obj_TE.c:53-172).  pobs: list per frequency of complex64 cuda [nshots, ntr].
     Returns (L2 = 0.5 * sum |res|^2, grad (AC: [n] float64; TE: (grad_sigma, grad_eps))).
@@ -732,18 +781,21 @@ def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=
     rtm=True is the reverse-time-migration pass (RTM_FD_AC, src/AC/RTM_FD_AC.c:190):
     the residual is the observed data itself (src/AC/calc_res_AC.c:77), with the
     same mute, so the returned gradient is the image before precond().
-    ws: a Workspace whose factor storage is reused (None: fresh storage)."""
+    ws: a Workspace whose factor storage is reused (None: fresh storage).
+    wavelet: (samples, ndt) of an SU source wavelet (READ_WAVELET = 1), see source_amp.
+    taper: ass_grad's per-shot taper (SWS_TAPER_CIRCULAR_PER_SHOT), a ShotTaper."""
     import torch
     n = plan.n
     F, _ = _factors(plan, ws, physics, dh, freqs, m1, m2, precision, s=s, npml=npml, a0_pml=a0_pml,
                     free_surf=free_surf)
+    tp = None if taper is None else taper.device_struct(len(ksrc))
     g1 = torch.zeros(n, dtype=torch.float64, device="cuda")
     g2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     l2 = 0.0
     adj_sys = 0 if physics == AC else 2      # AC solves with M (grad_obj_AC.c:147), TE with M^T (grad_obj_TE.c:149)
     k = torch.as_tensor(np.asarray(krec, np.int32), device="cuda")
     for f, fr in enumerate(freqs):
-        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh))
+        B = spike_panel(n, ksrc, source_amp(physics, fr, s, dh, wavelet))
         if precision == FP64 and tol is None:
             U = F.solve(f, B)
         else:
@@ -756,15 +808,16 @@ def grad_obj(plan, physics, dh, freqs, m1, m2, ksrc, krec, pobs, npml=0, a0_pml=
         else:
             LAM, _, _ = F.solve_refine(f, BA, sys=adj_sys, tol=tol, max_iter=max_iter)
         nshots = len(ksrc)
+        tpp = None if tp is None else C.byref(tp)
         if physics == AC:
             o2 = omega2_grad_ac(fr, s)
-            check(lib().gfdfd_grad_ac_stf(n, nshots, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(m1.contiguous()),
-                                          _ptr(m2.contiguous()), _ptr(stf), _ptr(g1), _stream(None)),
-                  "gfdfd_grad_ac_stf")
+            check(lib().gfdfd_grad_ac_taper(n, nshots, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(m1.contiguous()),
+                                            _ptr(m2.contiguous()), _ptr(stf), tpp, _ptr(g1), _stream(None)),
+                  "gfdfd_grad_ac_taper")
         elif physics == TE:
             o2, om = omega(TE, fr, s)
-            check(lib().gfdfd_grad_te_stf(n, nshots, om.ctypes.data, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(stf),
-                                          _ptr(g1), _ptr(g2), _stream(None)), "gfdfd_grad_te_stf")
+            check(lib().gfdfd_grad_te_taper(n, nshots, om.ctypes.data, o2.ctypes.data, _ptr(U), _ptr(LAM), _ptr(stf),
+                                            tpp, _ptr(g1), _ptr(g2), _stream(None)), "gfdfd_grad_te_taper")
         else:
             raise ValueError("SH has no gradient path in the reference (src/germaine.c:112-120)")
     grad = g1 if physics == AC else (g1, g2)

@@ -83,3 +83,22 @@ def write_model(path, m, npml=0, free_surf=1, interior=False):
         ny, nx = m.shape
         m = m[fsshift:ny - npml, npml:nx - npml]
     np.ascontiguousarray(m.T, "<f4").tofile(path)
+
+
+def read_su_wavelet(path):
+    """The time-domain source wavelet of READ_WAVELET = 1 (src/read_stf_dft.c:25-58):
+    the first trace of an SU file -- a 240-byte trace header whose bytes 114-115
+    hold ns and 116-117 the sample interval (native-endian unsigned 16-bit
+    fields of the SU header), then ns float32 samples.  Returns (samples, ndt);
+    fdfd.source_amp(..., wavelet=(samples, ndt)) turns it into the per-frequency
+    source amplitude."""
+    with open(path, "rb") as f:
+        head = f.read(240)
+        if len(head) < 240:
+            raise ValueError(f"{path}: shorter than one SU trace header")
+        ns = int(np.frombuffer(head, np.uint16, 1, 114)[0])
+        ndt = int(np.frombuffer(head, np.uint16, 1, 116)[0])
+        data = np.fromfile(f, np.float32, ns)
+    if len(data) < ns:
+        raise ValueError(f"{path}: the header announces {ns} samples, the file holds {len(data)}")
+    return data, ndt

@@ -90,6 +90,15 @@ void gfdfd_omega(int physics, float freq, float s, float *omega2, float *omega);
 void gfdfd_omega2_grad_ac(float freq, float s, float *omega2);
 /* src/AC/RHS_source_AC.c:37-53, src/TE/RHS_source_TE.c:22-56 */
 void gfdfd_source_amp(int physics, float freq, float s, float dh, double *amp);
+/* READ_WAVELET = 1 (src/AC/RHS_source_AC.c:41-43, src/TE/RHS_source_TE.c:44-46):
+ * the DFT of the SU wavelet at freq (src/read_stf_dft.c:34-70) over DH^2.
+ * samples: the trace's ns values; ndt: its header's sample interval
+ * (dt = ndt/1e6 for AC/SH, ndt/1e12 for TE).  The reference sums data[1..ns]
+ * at t = (j-1) dt after reading data[0..ns-1]; its last term reads an element
+ * the read never filled, which counts as zero here.                         */
+void gfdfd_wavelet_amp(int physics, float freq, float dh, const float *samples, int ns, int ndt, double *amp);
+/* taper_grad_shot's exponent a (src/taper_grad_shot.c:36-39), in float     */
+float gfdfd_shot_taper_exponent(void);
 
 /* ---- device assembly --------------------------------------------------------
  * replaces init_mat_AC + init_A_AC_9p_pml (src/AC/init_A_AC_9p_pml.c:17-220),
@@ -220,6 +229,30 @@ int gfdfd_grad_ac(int n, int nshots, const float *omega2, const double *d_u, con
  * d_stf[2s+1] from gfdfd_residual_sources (NULL: 1).                       */
 int gfdfd_grad_ac_stf(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam,
                       const float *d_vp, const float *d_rho, const float *d_stf, double *d_grad, void *stream);
+/* ass_grad_AC's per-shot taper (SWS_TAPER_CIRCULAR_PER_SHOT, src/AC/ass_grad_AC.c:37-43,
+ * src/TE/ass_grad_TE.c:40-49).  mode 1, taper_grad_shot (src/taper_grad_shot.c:34-87):
+ * each shot's gradient times f0 + (1-f0)(0.5 - 0.5 cos(pi r/R))^a within r <= R of the
+ * shot's source and of every receiver, r from (ix DH, iy DH) of the PML-extended grid's
+ * 1-based cell (ix, iy) to srcpos (m) and to recpos * DH, as the reference measures it;
+ * mode 2, taper_grad_shot1 (src/taper_grad_shot1.c:32-66): each shot's gradient is 1e-5
+ * on the (2 FILTSIZE + 1)^2 cell boxes around iround(srcpos/DH) of every source and
+ * around every receiver.  Device arrays.                                    */
+typedef struct {
+    int mode;                       /* 0 none, 1 taper_grad_shot, 2 taper_grad_shot1 */
+    int nx;                         /* NX (cells per grid row, PML included); n must be a multiple */
+    float dh;                       /* DH */
+    float srtradius;                /* SRTRADIUS, mode 1 */
+    int filtsize;                   /* FILTSIZE, mode 2 */
+    const float *d_srcx, *d_srcy;   /* mode 1: srcpos[1..2] of the panel's nshots shots, metres */
+    int nsrc;                       /* mode 2: every source of the survey, */
+    const int *d_src_i, *d_src_j;   /*   iround(srcpos[1..2] / DH) */
+    int nrec;                       /* receivers: recpos[1..2] (grid indices, receiver()) */
+    const int *d_rec_i, *d_rec_j;
+} gfdfd_shot_taper;
+/* gfdfd_grad_ac_stf with the per-shot taper (taper NULL or mode 0: none)   */
+int gfdfd_grad_ac_taper(int n, int nshots, const float *omega2, const double *d_u, const double *d_lam,
+                        const float *d_vp, const float *d_rho, const float *d_stf, const gfdfd_shot_taper *taper,
+                        double *d_grad, void *stream);
 /* hessian_shin_AC (src/AC/hessian_shin_AC.c:117-124): Shin's pseudo-Hessian
  * diagonal, hess += |J|^2 with J = 2 Omega^2 u / (rho vp^3), summed over the
  * nshots forward fields of d_u in FP64 and added to d_hess (n doubles).
@@ -313,6 +346,11 @@ int gfdfd_grad_te(int n, int nshots, const float *omega, const float *omega2, co
 int gfdfd_grad_te_stf(int n, int nshots, const float *omega, const float *omega2, const double *d_u,
                       const double *d_lam, const float *d_stf, double *d_grad_sigma, double *d_grad_eps,
                       void *stream);
+/* gfdfd_grad_te_stf with the per-shot taper, applied to both gradients as
+ * src/TE/ass_grad_TE.c:40-49 does                                           */
+int gfdfd_grad_te_taper(int n, int nshots, const float *omega, const float *omega2, const double *d_u,
+                        const double *d_lam, const float *d_stf, const gfdfd_shot_taper *taper, double *d_grad_sigma,
+                        double *d_grad_eps, void *stream);
 
 /* ---- measurement ------------------------------------------------------------
  * Per-kernel-class timing with HIP events on the launch stream, plus the

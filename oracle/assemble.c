@@ -738,3 +738,83 @@ void orc_grad_te_accumulate(int nx, int ny, float freq, float s,
         grad_eps[q] += ge;
     }
 }
+
+/* src/read_stf_dft.c:34-70 with src/AC/RHS_source_AC.c:41-53: the wavelet
+ * source's amplitude.  The trace was read into data[0..ns-1] and the sum runs
+ * over data[1..ns] at t = (j-1) dt; data[ns] was never read by the reference
+ * (its value is whatever the stack held) and counts as zero here.  amp[1..2]
+ * are float, so the sum is rounded to float after every term.               */
+void orc_wavelet_amp(int physics_te, float freq, float dh, const float *data, int ns, int ndt, double *re_out,
+                     double *im_out)
+{
+    float dt;
+    if (!physics_te) dt = ndt / 1e6;
+    else dt = ndt / 1e12;
+    float amp1 = 0.0, amp2 = 0.0;
+    for (int j = 1; j <= ns; j++) {
+        float t = (j - 1) * dt;
+        float wj = (j < ns) ? data[j] : 0.0f;
+        amp1 += wj * cos(2.0 * t * freq * M_PI) * dt;
+        amp2 += wj * sin(2.0 * t * freq * M_PI) * dt;
+    }
+    *re_out = amp1 / (dh * dh);
+    *im_out = amp2 / (dh * dh);
+}
+
+#define ORC_PI (3.141592653589793)
+
+/* src/taper_grad_shot.c:34-87, applied to one shot's gradient g[(iy-1)*nx +
+ * ix-1] in place: the source taper of shot (srcx, srcy) (metres) and then
+ * every receiver's (recpos[1..2][n] grid indices) in receiver order, each a
+ * float multiplication, r from (ix*DH, iy*DH).                               */
+void orc_taper_grad_shot(int nx, int ny, float dh, float srtradius, float srcx, float srcy, int ntr,
+                         const int *rec_i, const int *rec_j, float *g)
+{
+    float R = srtradius, frh = 0.1, f0 = 1e-3, a, x, y, r, taper;
+    a = -(log(frh - f0) - log(1.0 - f0)) / log(2.0);
+    for (int iy = 1; iy <= ny; iy++)
+        for (int ix = 1; ix <= nx; ix++) {
+            x = ix * dh;
+            y = iy * dh;
+            r = sqrt(pow((x - srcx), 2.0) + pow((y - srcy), 2.0));
+            taper = 1.0;
+            if (r <= R) taper = f0 + (1.0 - f0) * pow((0.5 - 0.5 * cos(ORC_PI * r / R)), a);
+            g[(size_t)(iy - 1) * nx + ix - 1] *= taper;
+        }
+    for (int n = 0; n < ntr; n++)
+        for (int iy = 1; iy <= ny; iy++)
+            for (int ix = 1; ix <= nx; ix++) {
+                x = ix * dh;
+                y = iy * dh;
+                r = sqrt(pow((x - (rec_i[n] * dh)), 2.0) + pow((y - (rec_j[n] * dh)), 2.0));
+                taper = 1.0;
+                if (r <= R) taper = f0 + (1.0 - f0) * pow((0.5 - 0.5 * cos(ORC_PI * r / R)), a);
+                g[(size_t)(iy - 1) * nx + ix - 1] *= taper;
+            }
+}
+
+/* iround of the reference: floor(x + 0.5) in double                         */
+static int orc_iround(double x) { return (int)floor(x + 0.5); }
+
+/* src/taper_grad_shot1.c:32-66, one shot's gradient in place: 1e-5 on the
+ * (2 filtsize + 1)^2 boxes around iround(srcpos/DH) of every source and
+ * around every receiver; cells outside the grid (an out-of-bounds write in
+ * the reference) are skipped.                                               */
+void orc_taper_grad_shot1(int nx, int ny, float dh, int filtsize, int nsrc, const float *srcx, const float *srcy,
+                          int ntr, const int *rec_i, const int *rec_j, float *g)
+{
+    float damp_src_rec = 1e-5;
+    for (int n = 0; n < nsrc + ntr; n++) {
+        int i1, j1;
+        if (n < nsrc) {
+            i1 = orc_iround(srcx[n] / dh);
+            j1 = orc_iround(srcy[n] / dh);
+        } else {
+            i1 = rec_i[n - nsrc];
+            j1 = rec_j[n - nsrc];
+        }
+        for (int i = i1 - filtsize; i <= i1 + filtsize; i++)
+            for (int j = j1 - filtsize; j <= j1 + filtsize; j++)
+                if (i >= 1 && i <= nx && j >= 1 && j <= ny) g[(size_t)(j - 1) * nx + i - 1] = damp_src_rec;
+    }
+}

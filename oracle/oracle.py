@@ -65,6 +65,15 @@ def lib():
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
                                              f(np.float64), f(np.float64), C.c_double, C.c_double,
                                              f(np.float32), f(np.float32), C.c_float, C.c_float, f(np.float32)]
+        L.orc_wavelet_amp.restype = None
+        L.orc_wavelet_amp.argtypes = [C.c_int, C.c_float, C.c_float, f(np.float32), C.c_int, C.c_int,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_taper_grad_shot.restype = None
+        L.orc_taper_grad_shot.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                          f(np.int32), f(np.int32), f(np.float32)]
+        L.orc_taper_grad_shot1.restype = None
+        L.orc_taper_grad_shot1.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, f(np.float32),
+                                           f(np.float32), C.c_int, f(np.int32), f(np.int32), f(np.float32)]
         L.orc_hess_shin_ac_accumulate.restype = None
         L.orc_hess_shin_ac_accumulate.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float,
                                                   f(np.float64), f(np.float64), C.c_double, C.c_double,
@@ -359,3 +368,28 @@ def grad_te_accumulate(nx, ny, freq, s, u, u_last, lam, lam_last, gsig, geps, st
                                  np.ascontiguousarray(lam.real), np.ascontiguousarray(lam.imag),
                                  lam_last.real, lam_last.imag, stf.real, stf.imag, gsig, geps)
     return gsig, geps
+
+
+def wavelet_amp(physics, freq, dh, data, ndt):
+    """Source amplitude of READ_WAVELET = 1: the DFT of the SU trace `data`
+    (float32, ns samples; header interval ndt) at freq, over DH^2."""
+    re, im = C.c_double(), C.c_double()
+    d = np.ascontiguousarray(data, np.float32)
+    lib().orc_wavelet_amp(1 if physics == TE else 0, freq, dh, d, len(d), int(ndt), C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def taper_grad_shot(g, nx, ny, dh, srtradius, srcx, srcy, rec_i, rec_j):
+    """taper_grad_shot on one shot's float32 gradient [ny*nx], in place."""
+    lib().orc_taper_grad_shot(nx, ny, dh, srtradius, srcx, srcy, len(rec_i), np.ascontiguousarray(rec_i, np.int32),
+                              np.ascontiguousarray(rec_j, np.int32), g)
+    return g
+
+
+def taper_grad_shot1(g, nx, ny, dh, filtsize, srcx, srcy, rec_i, rec_j):
+    """taper_grad_shot1 on one shot's float32 gradient [ny*nx], in place; srcx/srcy:
+    every source of the survey (metres)."""
+    lib().orc_taper_grad_shot1(nx, ny, dh, filtsize, len(srcx), np.ascontiguousarray(srcx, np.float32),
+                               np.ascontiguousarray(srcy, np.float32), len(rec_i),
+                               np.ascontiguousarray(rec_i, np.int32), np.ascontiguousarray(rec_j, np.int32), g)
+    return g

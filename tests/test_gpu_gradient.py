@@ -723,3 +723,97 @@ def test_fwi_iteration_te():
     final += lam[0] * cs + lam[1] * ce
     say("  misfits", misfits, "final (with the Tikhonov term)", final, "lam", lam)
     assert misfits[1] < misfits[0] and final < misfits[1] and ws.allocations == 1
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("phys", [G.AC, G.TE])
+def test_gradient_shot_taper_matches_oracle(phys, mode):
+    """ass_grad's per-shot taper (SWS_TAPER_CIRCULAR_PER_SHOT, src/AC/ass_grad_AC.c:37-43,
+    src/TE/ass_grad_TE.c:40-49): the oracle tapers every shot's float gradient with
+    taper_grad_shot (mode 1: SRTRADIUS 30 m around the shot's source and every receiver) or
+    taper_grad_shot1 (mode 2: FILTSIZE 2 boxes around every source and receiver) before
+    adding it, in the reference's coordinates (cell (ix, iy) at (ix DH, iy DH) against
+    srcpos in metres and recpos in grid indices); the device applies the same factors
+    inside its shot sum."""
+    say("gradient shot taper", phys, mode)
+    m = models(phys)
+    dh, freqs, n = m["dh"], m["freqs"], NX * NY
+    xs = (4, 13, 22, 31, 40)
+    srcpos = np.array([[x * dh, 12 * dh] for x in xs], np.float32)
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in xs]
+    recpos = np.array([[x, 10] for x in range(1, 44)], np.int32)
+    recs = np.array([O.source_index(x * dh, 10 * dh, dh, NPML, 0, NX) for x in range(1, 44)], np.int32)
+    pobs = []
+    for f in freqs:
+        _, Ut = oracle_fields(phys, dh, f, m["t1"], m["t2"], srcs)
+        pobs.append(Ut[recs, :].T.astype(np.complex64))
+    radius, filtsize = 3.0 * dh, 2
+    taper = fdfd.ShotTaper(mode, NX, dh, srcpos, recpos, srtradius=radius, filtsize=filtsize)
+    pl = fdfd.Plan(NX, NY)
+    _, grad = fdfd.grad_obj(pl, phys, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs, [dev(p) for p in pobs],
+                            npml=NPML, precision=G.FP64, tol=None, taper=taper)
+    torch.cuda.synchronize()
+    g1 = np.zeros(n, np.float32)
+    g2 = np.zeros(n, np.float32)
+    for fi, f in enumerate(freqs):
+        csc, U = oracle_fields(phys, dh, f, m["m1"], m["m2"], srcs)
+        rhs = np.zeros((n, len(srcs)), complex)
+        for s in range(len(srcs)):
+            res, _ = O.residual(U[recs, s].astype(np.complex64), pobs[fi][s])
+            rhs[:, s] = O.adjoint_rhs(n, recs, res, dh)
+        _, LAM = oracle_fields(phys, dh, f, m["m1"], m["m2"], srcs, rhs, m["adj"])
+        for s in range(len(srcs)):
+            shot = [np.zeros(n, np.float32), np.zeros(n, np.float32)]
+            if phys == G.AC:
+                O.grad_ac_accumulate(NX, NY, f, 0.0, U[:, s], 0j, LAM[:, s], 0j, m["m1"], m["m2"], shot[0])
+            else:
+                O.grad_te_accumulate(NX, NY, f, 0.0, U[:, s], 0j, LAM[:, s], 0j, shot[0], shot[1])
+            for g in shot:
+                if mode == 1:
+                    O.taper_grad_shot(g, NX, NY, dh, radius, srcpos[s, 0], srcpos[s, 1], recpos[:, 0], recpos[:, 1])
+                else:
+                    O.taper_grad_shot1(g, NX, NY, dh, filtsize, srcpos[:, 0], srcpos[:, 1], recpos[:, 0],
+                                       recpos[:, 1])
+            g1 += shot[0]
+            g2 += shot[1]
+    got = [grad.cpu().numpy()] if phys == G.AC else [grad[0].cpu().numpy(), grad[1].cpu().numpy()]
+    plain = fdfd.grad_obj(pl, phys, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, recs, [dev(p) for p in pobs],
+                          npml=NPML, precision=G.FP64, tol=None)[1]
+    plain = [plain.cpu().numpy()] if phys == G.AC else [plain[0].cpu().numpy(), plain[1].cpu().numpy()]
+    for d, r, u in zip(got, [g1, g2], plain):
+        e = float(np.linalg.norm(d - r) / np.linalg.norm(r))
+        changed = int(np.sum(np.abs(u - d) > 1e-3 * np.abs(u)))
+        say("  gradient rel err", e, "cells the taper changed", changed)
+        assert e <= 1e-5
+        assert changed > 0
+        if mode == 2:
+            # a box cell holds 1e-5 per shot and frequency, summed in float by the oracle
+            boxed = np.abs(r - 1e-5 * len(freqs) * len(srcs)) <= 1e-9
+            assert boxed.sum() >= len(srcs) * 25
+            assert np.allclose(d[boxed], r[boxed], rtol=1e-6, atol=0)
+
+
+def test_wavelet_source_matches_oracle():
+    """READ_WAVELET = 1 (src/AC/RHS_source_AC.c:41-43): the forward panel with the
+    wavelet's DFT as source amplitude, against the oracle's band solve with the
+    oracle's DFT of the same trace."""
+    say("wavelet source")
+    m = models(G.AC)
+    dh, freqs = m["dh"], m["freqs"]
+    t = np.arange(400, dtype=np.float32) * np.float32(1e-3)
+    w = ((1 - 2 * (np.pi * 8.0 * (t - 0.1)) ** 2) * np.exp(-(np.pi * 8.0 * (t - 0.1)) ** 2)).astype(np.float32)
+    srcs = [O.source_index(x * dh, 12 * dh, dh, NPML, 0, NX) for x in (6, 25, 44)]
+    pl = fdfd.Plan(NX, NY)
+    out = fdfd.forward_shots(pl, G.AC, dh, freqs, dev(m["m1"]), dev(m["m2"]), srcs, npml=NPML, precision=G.FP64,
+                             tol=None, keep_fields=True, wavelet=(w, 1000))
+    for fi, f in enumerate(freqs):
+        amp = O.wavelet_amp(G.AC, f, dh, w, 1000)
+        assert amp == fdfd.source_amp(G.AC, f, 0.0, dh, (w, 1000))
+        rhs = np.zeros((NX * NY, len(srcs)), complex)
+        for s, k in enumerate(srcs):
+            rhs[k, s] = amp
+        _, X = oracle_fields(G.AC, dh, f, m["m1"], m["m2"], srcs, rhs)
+        got = out["fields"][fi].cpu().numpy().T
+        e = float(np.max(np.abs(got - X)) / np.max(np.abs(X)))
+        say("  freq", f, "amp", amp, "rel err", e)
+        assert e <= 1e-10
