@@ -138,6 +138,7 @@ void refactor_graph(Factors<T> &F, const cx<double> *vals, hipStream_t st) {
         GFDFD_HIP(e);
         F.graph_src = vals;
     }
+    F.vinv_ready = false;   // the replay changes the factors
     GFDFD_HIP(hipGraphLaunch(F.graph, st));
 }
 

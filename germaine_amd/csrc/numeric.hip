@@ -281,6 +281,35 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
     for (size_t l = 0; l < pl.levels.size(); l++)
         single.push_back(pmax_all[l] <= 32 && mmax_all[l] <= kSingleMmax && max_children <= 2 ? 1 : 0);
     lv_single = dev_upload(single);
+    // inverse squares for the levels of fewer than kVecSplitFronts multi-block
+    // fronts whose pivot blocks fit the inversion kernel's LDS
+    {
+        std::vector<int> lst;
+        std::vector<int64_t> loff;
+        vinv.assign(pl.levels.size(), 0);
+        vinv_off.assign(pl.levels.size(), nullptr);
+        for (size_t l = 0; l < pl.levels.size(); l++) {
+            if (single[l] || n_all[l] >= kVecSplitFronts || pmax_all[l] > kVinvPmax || n_all[l] == 0) continue;
+            vinv[l] = 1;
+            const LevelPlan &L = pl.levels[l];
+            std::vector<int> all(L.small);
+            all.insert(all.end(), L.large.begin(), L.large.end());
+            std::vector<int64_t> off;
+            for (int f : all) {
+                const int p = pl.fronts[f].p;
+                off.push_back(vinv_elems);
+                lst.push_back(f);
+                loff.push_back(vinv_elems);
+                vinv_elems += (int64_t)p * p;
+                vinv_pmax = std::max(vinv_pmax, p);
+            }
+            vinv_off[l] = dev_upload(off);
+        }
+        vinv_n = (int)lst.size();
+        vinv_list = dev_upload(lst);
+        vinv_list_off = dev_upload(loff);
+        bytes += lst.size() * 12;
+    }
     build_subtrees(pl);
     counts.resize(pl.levels.size());
     for (size_t l = 0; l < pl.levels.size(); l++) {
@@ -344,6 +373,9 @@ DevPlan::~DevPlan() {
     dev_free(orig_q); dev_free(orig_dst); dev_free(colptr); dev_free(rowind); dev_free(tq); dev_free(tiles);
     dev_free(lv_single);
     dev_free(gsrc_id);
+    for (auto p : vinv_off) dev_free(p);
+    dev_free(vinv_list);
+    dev_free(vinv_list_off);
     dev_free(root_first); dev_free(root_last); dev_free(slot_of); dev_free(sub_optr); dev_free(sub_oq);
     dev_free(sub_ooff); dev_free(orig_q_rest); dev_free(orig_dst_rest);
     for (auto p : lv_asm_rest) dev_free(p);
@@ -363,6 +395,7 @@ Factors<T>::~Factors() {
     dev_free(sing);
     dev_free(gsrc);
     dev_free(sched);
+    dev_free(vinv);
     if (own_vals) dev_free(vals);
 }
 template struct Factors<double>;
@@ -1134,6 +1167,7 @@ template <typename T>
 void factor_numeric(const DevPlan &dp, Factors<T> &F, const cx<double> *vals, int64_t vstride, hipStream_t st) {
     const Plan &pl = *dp.host;
     const int nf = F.nfreq;
+    F.vinv_ready = false;   // the single-vector path's inverse squares follow the new factors
     hipLaunchKernelGGL(k_clear_flags, dim3((nf + 63) / 64), dim3(64), 0, st, F.sing, nf);
     const size_t lds_small_max = 160 * 1024 - 256;
     const size_t panel_lds_hw = 160 * 1024 - 1024;    // panels of up to 640 c64 / 320 c128 candidate rows

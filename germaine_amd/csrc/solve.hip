@@ -50,6 +50,11 @@ __device__ __forceinline__ cx<T> ld(const cx<T> *p, bool cj) {
     return v;
 }
 
+template <typename T>
+__device__ __forceinline__ cx<T> cmac(cx<T> c, cx<T> a, cx<T> b) {   // c + a b
+    return cadd(c, cmul(a, b));
+}
+
 // ---- elimination order <-> user order --------------------------------------
 // User panels are column-major n x nrhs (one shot per column); internal panels
 // are row-major in elimination order.  The panel kernels move 64 x 32 tiles
@@ -1398,15 +1403,15 @@ __global__ void __launch_bounds__(256) k_piv_unswap(const FrontDev *__restrict__
 // M solves), blocked lower solve (each 32-row diagonal block applied as a
 // product with its inverse staged in LDS, block updates coalesced over rows),
 // pivot rows to X and contribution rows to W for the parent
-template <typename T>
+// (SINGLE: the level's fronts are in the single-block format, one product each;
+// compiled apart so that this form carries no registers or LDS of the block loop)
+template <typename T, bool SINGLE>
 __global__ void __launch_bounds__(256) k_fwd_vec(const cx<T> *__restrict__ arena, const FrontDev *__restrict__ fronts,
                                                  const int *__restrict__ list, const int *__restrict__ children,
                                                  const int *__restrict__ inv, const int *__restrict__ rowperm,
-                                                 cx<T> *__restrict__ X, cx<T> *__restrict__ W, int trans, int cj,
-                                                 int single) {
+                                                 cx<T> *__restrict__ X, cx<T> *__restrict__ W, int trans, int cj) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *sw = reinterpret_cast<cx<T> *>(smem_raw);
-    __shared__ cx<T> Dsh[32][33];
     const FrontDev F = fronts[list[blockIdx.x]];
     const int m = F.m, p = F.p, tid = threadIdx.x;
     const cx<T> *G = arena + F.buf_off;
@@ -1416,20 +1421,20 @@ __global__ void __launch_bounds__(256) k_fwd_vec(const cx<T> *__restrict__ arena
         sw[i] = front_row(F, src, 0, 1, X, W, fronts, children, inv);
     }
     __syncthreads();
-    if (single) {
+    if constexpr (SINGLE) {
         // one product over all rows: out = [Dop; Lop21'] w1.  Four lanes per row
         // (each sums every fourth pivot column, two xor-shuffles add them), 64
         // rows per pass, so every wave works and each lane waits on at most
-        // eight loads (p <= 32)
-        static_assert(kSingleMmax <= 5 * 64, "rows of a single-block front: five passes of 64");
+        // eight loads (p <= 32); the results go to the second half of the
+        // dynamic LDS (2 m entries) until every pass has read w1
+        cx<T> *so = sw + m;
         const int ri = tid >> 2, part = tid & 3;
-        cx<T> res[5];
-#pragma unroll
-        for (int u = 0; u < 5; u++) {
-            const int i = ri + 64 * u;
+#pragma unroll 1
+        for (int i0 = 0; i0 < m; i0 += 64) {
+            const int i = i0 + ri;
             cx<T> acc = {T(0), T(0)};
             if (i < m)
-#pragma unroll 8
+#pragma unroll 4
                 for (int t = part; t < p; t += 4) {
                     const cx<T> a = i < p ? dop(G, m, 0, i, t, true, trans, cj) : Lop(i, t);
                     acc = cadd(acc, cmul(a, sw[t]));
@@ -1438,34 +1443,33 @@ __global__ void __launch_bounds__(256) k_fwd_vec(const cx<T> *__restrict__ arena
             acc.y += __shfl_xor(acc.y, 1);
             acc.x += __shfl_xor(acc.x, 2);
             acc.y += __shfl_xor(acc.y, 2);
-            res[u] = acc;
+            if (part == 0 && i < m) so[i] = acc;
         }
         __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 5; u++) {
-            const int i = ri + 64 * u;
-            if (part == 0 && i < m) sw[i] = i < p ? res[u] : csub(sw[i], res[u]);
-        }
+        for (int i = tid; i < m; i += 256) sw[i] = i < p ? so[i] : csub(sw[i], so[i]);
         __syncthreads();
-    }
-    for (int tb = 0; tb < (single ? 0 : p); tb += 32) {
-        const int nb = min(32, p - tb);
-        for (int e = tid; e < nb * nb; e += 256) Dsh[e % nb][e / nb] = dop(G, m, tb, e % nb, e / nb, true, trans, cj);
-        __syncthreads();
-        cx<T> y = {T(0), T(0)};
-        if (tid < nb)
-            for (int t = 0; t <= tid; t++) y = cadd(y, cmul(Dsh[tid][t], sw[tb + t]));
-        __syncthreads();
-        if (tid < nb) sw[tb + tid] = y;
-        __syncthreads();
-        // the factor loads of a row are unrolled so that several are in flight at once
-        for (int i = tb + nb + tid; i < m; i += 256) {
-            cx<T> acc = sw[i];
+    } else {
+        __shared__ cx<T> Dsh[32][33];
+        for (int tb = 0; tb < p; tb += 32) {
+            const int nb = min(32, p - tb);
+            for (int e = tid; e < nb * nb; e += 256)
+                Dsh[e % nb][e / nb] = dop(G, m, tb, e % nb, e / nb, true, trans, cj);
+            __syncthreads();
+            cx<T> y = {T(0), T(0)};
+            if (tid < nb)
+                for (int t = 0; t <= tid; t++) y = cadd(y, cmul(Dsh[tid][t], sw[tb + t]));
+            __syncthreads();
+            if (tid < nb) sw[tb + tid] = y;
+            __syncthreads();
+            // the factor loads of a row are unrolled so that several are in flight at once
+            for (int i = tb + nb + tid; i < m; i += 256) {
+                cx<T> acc = sw[i];
 #pragma unroll 8
-            for (int t = 0; t < nb; t++) acc = cmsub(acc, Lop(i, tb + t), sw[tb + t]);
-            sw[i] = acc;
+                for (int t = 0; t < nb; t++) acc = cmsub(acc, Lop(i, tb + t), sw[tb + t]);
+                sw[i] = acc;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     for (int i = tid; i < m; i += 256) {
         if (i < p) X[F.pos0 + i] = sw[i];
@@ -1543,7 +1547,7 @@ __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena
 // z = y1 - U12 x2 -- runs as chunks of 256 rows on many workgroups.  Each row
 // accumulates over t (or s) in the same order as the one-workgroup kernels,
 // so the results are the same to the bit.
-constexpr int kVecSplitRows = 256;
+constexpr int kVecSplitRows = 256;   // rows per workgroup of the rectangular parts
 
 // forward, triangular part: gather the front's rows, solve its p pivot rows
 // block by block (updating only pivot rows), write y1 to X and the gathered
@@ -1674,6 +1678,208 @@ __global__ void __launch_bounds__(256) k_bwd_vec_tri(const cx<T> *__restrict__ a
         __syncthreads();
     }
     for (int i = tid; i < p; i += 256) X[F.pos0 + (trans ? rowperm[F.pos0 + i] : i)] = sw[i];
+}
+
+// ---- single-vector sweeps through explicit inverses of the pivot triangles ----
+// The levels of few multi-block fronts spend the one-workgroup kernels above
+// in a chain of 32-row block steps per front.  With the pivot block's
+// triangles inverted once (after a factorisation, on the first single-vector
+// solve), every part of such a front's sweep is a product with a known matrix,
+// split over many workgroups by rows.
+
+// inverse squares: per front, column block of kVinvCb columns, lower (x <
+// ncb) or upper triangle; the front's diagonal 32-blocks already hold their
+// inverses (strictly lower: inv(L_bb) without its unit diagonal; upper:
+// inv(U_bb)) and the blocks off the diagonal hold L11 and U11.  The column
+// block of the inverse is solved block by block in LDS, W[p][kVinvCb + 1].
+// grid (2 ncb, fronts with squares, nfreq)
+constexpr int kVinvCb = 16;
+template <typename T>
+__global__ void __launch_bounds__(256) k_vinv(const cx<T> *__restrict__ arena, int64_t fstride,
+                                              const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                              const int64_t *__restrict__ off, cx<T> *__restrict__ sq,
+                                              int64_t sqstride, int ncb) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>(*Wl)[kVinvCb + 1] = reinterpret_cast<cx<T>(*)[kVinvCb + 1]>(smem_raw);
+    const FrontDev F = fronts[list[blockIdx.y]];
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    const bool upper = (int)blockIdx.x >= ncb;
+    const int c0 = ((int)blockIdx.x - (upper ? ncb : 0)) * kVinvCb;
+    if (c0 >= p) return;
+    const int nc = min(kVinvCb, p - c0);
+    const cx<T> *G = arena + blockIdx.z * fstride + F.buf_off;
+    cx<T> *S = sq + blockIdx.z * sqstride + off[blockIdx.y];
+    const cx<T> one = {T(1), T(0)}, zero = {T(0), T(0)};
+    // rows that the column block's inverse can reach: lower [b00, p), upper [0, c0 + nc)
+    const int b00 = (c0 / 32) * 32, rlo = upper ? 0 : b00, rhi = upper ? c0 + nc : p;
+    for (int e = tid; e < (rhi - rlo) * kVinvCb; e += 256) {
+        const int r = rlo + e / kVinvCb, cc = e % kVinvCb;
+        Wl[r][cc] = (cc < nc && r == c0 + cc) ? one : zero;
+    }
+    __syncthreads();
+    if (!upper) {
+        for (int b0 = b00; b0 < p; b0 += 32) {
+            const int kb = min(32, p - b0);
+            // y = inv(L_bb) w over the block's rows (unit diagonal), two outputs per thread
+            cx<T> y[2];
+            for (int u = 0; u < 2; u++) {
+                const int e = tid + 256 * u, i = e / kVinvCb, cc = e % kVinvCb;
+                y[u] = zero;
+                if (i < kb) {
+                    cx<T> a = Wl[b0 + i][cc];
+                    for (int t = 0; t < i; t++) a = cmac(a, G[(b0 + i) + (int64_t)(b0 + t) * m], Wl[b0 + t][cc]);
+                    y[u] = a;
+                }
+            }
+            __syncthreads();
+            for (int u = 0; u < 2; u++) {
+                const int e = tid + 256 * u, i = e / kVinvCb, cc = e % kVinvCb;
+                if (i < kb) Wl[b0 + i][cc] = y[u];
+            }
+            __syncthreads();
+            // rows below the block: w_r -= L(r, block) y
+            for (int e = tid; e < (p - b0 - kb) * kVinvCb; e += 256) {
+                const int r = b0 + kb + e % (p - b0 - kb), cc = e / (p - b0 - kb);
+                cx<T> a = Wl[r][cc];
+                for (int t = 0; t < kb; t++) a = cmsub(a, G[r + (int64_t)(b0 + t) * m], Wl[b0 + t][cc]);
+                Wl[r][cc] = a;
+            }
+            __syncthreads();
+        }
+        // strictly lower part of the square's columns c0.. (the diagonal stays inv(U)'s)
+        for (int e = tid; e < (p - b00) * nc; e += 256) {
+            const int r = b00 + e % (p - b00), cc = e / (p - b00), c = c0 + cc;
+            if (r > c) S[r + (int64_t)c * p] = Wl[r][cc];
+        }
+    } else {
+        for (int b0 = ((c0 + nc - 1) / 32) * 32; b0 >= 0; b0 -= 32) {
+            const int kb = min(32, p - b0);
+            // x = inv(U_bb) w over the block's rows (diagonal included)
+            cx<T> y[2];
+            for (int u = 0; u < 2; u++) {
+                const int e = tid + 256 * u, i = e / kVinvCb, cc = e % kVinvCb;
+                y[u] = zero;
+                if (i < kb && b0 + i < rhi) {
+                    cx<T> a = zero;
+                    for (int t = i; t < kb && b0 + t < rhi; t++)
+                        a = cmac(a, G[(b0 + i) + (int64_t)(b0 + t) * m], Wl[b0 + t][cc]);
+                    y[u] = a;
+                }
+            }
+            __syncthreads();
+            for (int u = 0; u < 2; u++) {
+                const int e = tid + 256 * u, i = e / kVinvCb, cc = e % kVinvCb;
+                if (i < kb && b0 + i < rhi) Wl[b0 + i][cc] = y[u];
+            }
+            __syncthreads();
+            // rows above the block: w_r -= U(r, block) x
+            for (int e = tid; e < b0 * kVinvCb; e += 256) {
+                const int r = e % b0, cc = e / b0;
+                cx<T> a = Wl[r][cc];
+                for (int t = 0; t < kb && b0 + t < rhi; t++)
+                    a = cmsub(a, G[r + (int64_t)(b0 + t) * m], Wl[b0 + t][cc]);
+                Wl[r][cc] = a;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < rhi * nc; e += 256) {
+            const int r = e % rhi, cc = e / rhi, c = c0 + cc;
+            if (r <= c) S[r + (int64_t)c * p] = Wl[r][cc];
+        }
+    }
+}
+
+// forward gather of a split level: every row of the front (pivot rows
+// permuted for M solves) into its W rows.  grid: fronts
+template <typename T>
+__global__ void __launch_bounds__(256) k_vgather(const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                 const int *__restrict__ children, const int *__restrict__ inv,
+                                                 const int *__restrict__ rowperm, const cx<T> *__restrict__ X,
+                                                 cx<T> *__restrict__ W, int trans) {
+    const FrontDev F = fronts[list[blockIdx.x]];
+    for (int i = threadIdx.x; i < F.m; i += 256) {
+        const int src = (!trans && i < F.p) ? rowperm[F.pos0 + i] : i;
+        W[F.fv_off + i] = front_row(F, src, 0, 1, X, W, fronts, children, inv);
+    }
+}
+
+// one part of a split level's single-vector sweep as a product, per front:
+//   VG_FT  y1 = inv(T1) w1          rows, columns [0, p); w1 = W rows, y1 -> X
+//   VG_FR  w2 -= T21 y1             rows [p, m), columns [0, p); y1 = X, w2 = W rows
+//   VG_BR  z1 = x1 - T12 x2         rows [0, p), columns [p, m); x2 = X via idx, z1 -> W rows
+//   VG_BT  x1 = inv(T1') z1         rows, columns [0, p); z1 = W rows, x1 -> X (permuted for M^T, M^H)
+// T1/T1' are the pivot block's lower/upper triangles of op(M): for M, L11
+// (unit) forward and U11 backward; for M^T and M^H, U11^T forward and L11^T
+// (unit) backward, read transposed (and conjugated for M^H).  A workgroup
+// takes kGvRows rows and every column: 16 column lanes per row, which for M
+// read 16 consecutive rows of one column (the storage is column-major) and
+// for the transposed operators 16 consecutive columns of one row; the lanes'
+// sums meet in LDS and are added in lane order.  No workgroup depends on
+// another.  grid (row chunks, fronts)
+enum { VG_FT = 0, VG_FR = 1, VG_BR = 2, VG_BT = 3 };
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_vgemv(const cx<T> *__restrict__ arena, const cx<T> *__restrict__ sq,
+                                               const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                               const int64_t *__restrict__ off, const int *__restrict__ idx,
+                                               const int *__restrict__ rowperm, cx<T> *__restrict__ X,
+                                               cx<T> *__restrict__ W, int trans, int cj) {
+    __shared__ cx<T> red[16][kGvRows + 1];
+    const int fl = blockIdx.y;
+    const FrontDev F = fronts[list[fl]];
+    const int m = F.m, p = F.p, tid = threadIdx.x;
+    const int nrow = MODE == VG_FR ? m - p : p, ncol = MODE == VG_BR ? m - p : p;
+    const int r0 = blockIdx.x * kGvRows;
+    if (r0 >= nrow) return;
+    const cx<T> *G = arena + F.buf_off;
+    const cx<T> *S = (MODE == VG_FT || MODE == VG_BT) ? sq + off[fl] : nullptr;
+    // lane -> (row, column lane): rows fastest for M, columns fastest for the transposed operators
+    const int lr = trans ? tid >> 4 : tid & 15, lc = trans ? tid & 15 : tid >> 4, r = r0 + lr;
+    auto a_at = [&](int t) -> cx<T> {
+        if (MODE == VG_FR) {   // T21: L21 (M) or U12^T (M^T, M^H), row p + r
+            const int i = p + r;
+            return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj);
+        } else if (MODE == VG_BR) {   // T12: U12 (M) or L21^T, column p + t
+            const int s = p + t;
+            return trans ? ld(G + s + (int64_t)r * m, cj) : ld(G + r + (int64_t)s * m, cj);
+        } else if (MODE == VG_FT) {   // inv(L11) strictly lower (M) or inv(U11)^T lower incl. diagonal
+            if (!trans) return t < r ? S[r + (int64_t)t * p] : cx<T>{T(0), T(0)};
+            return t <= r ? ld(S + t + (int64_t)r * p, cj) : cx<T>{T(0), T(0)};
+        } else {   // inv(U11) upper incl. diagonal (M) or inv(L11)^T strictly upper
+            if (!trans) return t >= r ? S[r + (int64_t)t * p] : cx<T>{T(0), T(0)};
+            return t > r ? ld(S + t + (int64_t)r * p, cj) : cx<T>{T(0), T(0)};
+        }
+    };
+    auto v_at = [&](int t) -> cx<T> {
+        if (MODE == VG_FR) return X[F.pos0 + t];
+        if (MODE == VG_BR) return X[idx[F.idx_off + p + t]];
+        return W[F.fv_off + t];
+    };
+    // the triangles' rows need only the columns on their side of the diagonal
+    int t0 = 0, t1 = ncol;
+    if (MODE == VG_FT) t1 = min(ncol, r0 + kGvRows);
+    if (MODE == VG_BT) t0 = r0;
+    cx<T> acc = {T(0), T(0)};
+    if (r < nrow)
+#pragma unroll 8
+        for (int t = t0 + lc; t < t1; t += 16) acc = cmac(acc, a_at(t), v_at(t));
+    red[lc][lr] = acc;
+    __syncthreads();
+    if (tid < kGvRows) {
+        const int rr = r0 + tid;
+        if (rr < nrow) {
+            cx<T> s = red[0][tid];
+            for (int k = 1; k < 16; k++) s = cadd(s, red[k][tid]);
+            if (MODE == VG_FT) {
+                X[F.pos0 + rr] = trans ? s : cadd(W[F.fv_off + rr], s);
+            } else if (MODE == VG_FR) {
+                W[F.fv_off + p + rr] = csub(W[F.fv_off + p + rr], s);
+            } else if (MODE == VG_BR) {
+                W[F.fv_off + rr] = csub(X[F.pos0 + rr], s);
+            } else {
+                X[F.pos0 + (trans ? rowperm[F.pos0 + rr] : rr)] = trans ? cadd(W[F.fv_off + rr], s) : s;
+            }
+        }
+    }
 }
 
 // ---- FP64 residuals ------------------------------------------------------------
@@ -2127,7 +2333,9 @@ void set_smem_attrs() {
     if (done) return;
     // the vector kernels also hold a static 32 x 33 diagonal block
     const int big = 160 * 1024 - (int)(32 * 33 * sizeof(cx<T>));
-    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_vec<T>),
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_vec<T, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_vec<T, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, big));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_vec<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, big));
@@ -2330,7 +2538,39 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
 // too few fronts to fill the device with one workgroup each
 static bool vec_split(const DevPlan &dp, int l) {
     static const bool off = std::getenv("GFDFD_VEC_NOSPLIT") != nullptr;
-    return !off && !dp.single[l] && dp.n_all[l] < 128;
+    return !off && !dp.single[l] && dp.n_all[l] < kVecSplitFronts;
+}
+
+// the inverse squares of the split levels, made once after each factorisation
+// (allocated on first use)
+template <typename T>
+static bool ensure_vinv(const Factors<T> &F, hipStream_t st) {
+    static const bool off = std::getenv("GFDFD_VEC_NOINV") != nullptr;
+    const DevPlan &dp = *F.plan;
+    if (off || dp.vinv_n == 0) return false;
+    if (!F.vinv) {
+        F.vinv = dev_alloc<cx<T>>((size_t)dp.vinv_elems * F.nfreq);
+        F.vinv_ready = false;
+    }
+    if (!F.vinv_ready) {
+        const size_t lds = sizeof(cx<T>) * (size_t)dp.vinv_pmax * (kVinvCb + 1);
+        static bool attr = false;
+        if (!attr) {
+            GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vinv<double>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(sizeof(cx<double>) * kVinvPmax * (kVinvCb + 1))));
+            GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vinv<float>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(sizeof(cx<float>) * kVinvPmax * (kVinvCb + 1))));
+            attr = true;
+        }
+        const int ncb = (dp.vinv_pmax + kVinvCb - 1) / kVinvCb;
+        hipLaunchKernelGGL(k_vinv<T>, dim3(2 * ncb, dp.vinv_n, F.nfreq), dim3(256), lds, st, F.arena, F.fstride,
+                           dp.fronts, dp.vinv_list, dp.vinv_list_off, F.vinv, dp.vinv_elems, ncb);
+        GFDFD_HIP(hipGetLastError());
+        F.vinv_ready = true;
+    }
+    return true;
 }
 
 // Up to kVecMaxRhs right-hand sides: one pass per frequency and column through
@@ -2339,15 +2579,17 @@ static bool vec_split(const DevPlan &dp, int l) {
 // written by its front before the parent reads it, so neither needs clearing).
 template <typename T, typename TIO>
 static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, const cx<TIO> *B, cx<TIO> *X, cx<T> *Xp,
-                      cx<T> *Wp, hipStream_t st, bool accumulate) {
+                      cx<T> *Wp, hipStream_t st, bool accumulate, bool inverses = true) {
     const DevPlan &dp = *F.plan;
     const Plan &pl = *dp.host;
     const int n = pl.n;
     const bool trans = sys != 0, cj = sys == 1;
     const double es = sizeof(cx<T>);
+    const bool vi = inverses && ensure_vinv(F, st);
     for (int z = 0; z < sel.n; z++) {
         const int f = sel.f[z];
         const cx<T> *arena = F.arena + f * F.fstride;
+        const cx<T> *sq = vi ? F.vinv + f * dp.vinv_elems : nullptr;
         const int *rowperm = F.rowperm + (int64_t)f * n;
         for (int r = 0; r < nrhs; r++) {
             const int64_t off = ((int64_t)sel.s[z] * nrhs + r) * n;
@@ -2360,7 +2602,17 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                 const DevPlan::LevelCounts &c = dp.counts[l];
                 ProfScope ps(K_FWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
                 size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-                if (vec_split(dp, l)) {
+                const int maxrc = (dp.mmax_all[l] + kGvRows - 1) / kGvRows;
+                if (vec_split(dp, l) && vi && dp.vinv[l]) {
+                    hipLaunchKernelGGL(k_vgather<T>, dim3(dp.n_all[l]), dim3(256), 0, st, dp.fronts, dp.lv_all[l],
+                                       dp.children, dp.inv, rowperm, Xp, Wp, (int)trans);
+                    hipLaunchKernelGGL((k_vgemv<T, VG_FT>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
+                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, rowperm, Xp, Wp,
+                                       (int)trans, (int)cj);
+                    hipLaunchKernelGGL((k_vgemv<T, VG_FR>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
+                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, rowperm, Xp, Wp,
+                                       (int)trans, (int)cj);
+                } else if (vec_split(dp, l)) {
                     hipLaunchKernelGGL(k_fwd_vec_tri<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
                                        dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj);
                     const int chunks = (dp.mmax_all[l] + kVecSplitRows - 1) / kVecSplitRows;
@@ -2368,9 +2620,14 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                                        sizeof(cx<T>) * (size_t)dp.pmax_all[l], st, arena, dp.fronts, dp.lv_all[l], Xp,
                                        Wp, (int)trans, (int)cj);
                 } else {
-                    hipLaunchKernelGGL(k_fwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
-                                       dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj,
-                                       dp.single[l]);
+                    if (dp.single[l])
+                        hipLaunchKernelGGL((k_fwd_vec<T, true>), dim3(dp.n_all[l]), dim3(256), 2 * lds, st, arena,
+                                           dp.fronts, dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans,
+                                           (int)cj);
+                    else
+                        hipLaunchKernelGGL((k_fwd_vec<T, false>), dim3(dp.n_all[l]), dim3(256), lds, st, arena,
+                                           dp.fronts, dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans,
+                                           (int)cj);
                 }
             }
             for (int l = pl.nlevels - 1; l >= 0; l--) {
@@ -2378,7 +2635,15 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                 const DevPlan::LevelCounts &c = dp.counts[l];
                 ProfScope ps(K_BWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
                 size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-                if (vec_split(dp, l)) {
+                const int maxrc = (dp.mmax_all[l] + kGvRows - 1) / kGvRows;
+                if (vec_split(dp, l) && vi && dp.vinv[l]) {
+                    hipLaunchKernelGGL((k_vgemv<T, VG_BR>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
+                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, rowperm, Xp, Wp,
+                                       (int)trans, (int)cj);
+                    hipLaunchKernelGGL((k_vgemv<T, VG_BT>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
+                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, rowperm, Xp, Wp,
+                                       (int)trans, (int)cj);
+                } else if (vec_split(dp, l)) {
                     const int chunks = (dp.pmax_all[l] + kVecSplitRows - 1) / kVecSplitRows;
                     hipLaunchKernelGGL(k_bwd_vec_rect<T>, dim3(chunks, dp.n_all[l]), dim3(256), lds, st, arena,
                                        dp.fronts, dp.lv_all[l], dp.idx, Xp, (int)trans, (int)cj);
@@ -2763,7 +3028,7 @@ void solve_noalloc(const Factors<T> &F, int ifreq, int sys, int nrhs, const cx<d
     unsigned char *base = static_cast<unsigned char *>(scratch_on_device());
     cx<T> *Xp = reinterpret_cast<cx<T> *>(base);
     cx<T> *Wp = reinterpret_cast<cx<T> *>(base + ((size_t)pl.n * sizeof(cx<T>) + 255) / 256 * 256);
-    solve_vec<T, double>(F, make_sel({ifreq}, {0}), sys, nrhs, B, X, Xp, Wp, st, false);
+    solve_vec<T, double>(F, make_sel({ifreq}, {0}), sys, nrhs, B, X, Xp, Wp, st, false, false);
 }
 template void solve_noalloc<double>(const Factors<double> &, int, int, int, const cx<double> *, cx<double> *,
                                     hipStream_t);

@@ -532,3 +532,40 @@ def test_refactor_graph_replays_match_eager(precision):
             _, csc = oracle_system(G.AC, nx, ny, npml, dh, f, m1, m2)
             Xr, _ = O.band_solve(*csc, B[z].cpu().numpy().T, perm=O.grid_transpose_perm(nx, ny))
             assert rel(X[z], Xr.T) <= 1e-10
+
+
+@pytest.mark.parametrize("matrix", ["helmholtz", "random"])
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_single_vector_inverse_path_matches_oracle(precision, matrix):
+    """One right-hand side at a time on a grid whose top levels hold a few
+    multi-block fronts (pivot blocks of up to 81 rows): those levels run on
+    the inverse pivot triangles (made once after the factorisation) and the
+    split products, for M, M^H and M^T, against the banded oracle; the same
+    factors solved again after a refactorisation with other values (the
+    inverses are remade)."""
+    nx, ny = 97, 81
+    say("single-vector inverse path", precision, matrix)
+    if matrix == "helmholtz":
+        m1, m2 = smooth_model(nx, ny, seed=14)
+        mats = [oracle_system(G.AC, nx, ny, 8, 10.0, f, m1, m2)[1] for f in (4.0, 13.0)]
+    else:
+        mats = [random_grid_csc(nx, ny, seed) for seed in (61, 62)]
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    assert pl.info.max_pivots > 64
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((1, nx * ny)) + 1j * rng.standard_normal((1, nx * ny))
+    F = None
+    for z, csc in enumerate(mats):
+        vals = dev((csc[2] + 1j * csc[3])[None, :])
+        F = pl.factor(vals, precision) if F is None else F.refactor(vals)
+        for sysno in (0, 1, 2):
+            Xr, _ = O.band_solve(*csc, B.T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+            if precision == G.FP64:
+                X = F.solve(0, dev(B), sys=sysno).cpu().numpy()
+                bar = 1e-10 if matrix == "helmholtz" else 1e-8
+            else:
+                X = F.solve_refine(0, dev(B), sys=sysno, tol=1e-12, max_iter=30)[0].cpu().numpy()
+                bar = 1e-8
+            e = rel(X, Xr.T)
+            say("  matrix", z, "sys", sysno, "rel err", e)
+            assert e <= bar

@@ -1,9 +1,11 @@
 """gfdfd_solve_noalloc, the solve that allocates no device memory, against
 gfdfd_solve through the C-ABI.
 
-Both run the same single-vector sweep kernels; the no-allocation entry point
-keeps their work buffers in a file-scope device array instead of cached
-hipMalloc'd buffers, so its results must equal gfdfd_solve's bit for bit.
+The no-allocation entry point runs the single-vector sweep kernels with its
+work buffers in a file-scope device array; gfdfd_solve runs the same sweeps
+but, on the levels of few large fronts, multiplies by inverse pivot triangles
+that it allocates once per factorisation.  Both solve exactly, so their
+results agree to rounding: 1e-12 relative with FP64 factors, 1e-5 with FP32.
 Two calls in a row -- a 61 x 41 grid with three right-hand sides, then a
 smaller 45 x 33 grid with one (the array still holds the first call's
 contents) -- for FP64 and for FP32 factors; neither call may change the
@@ -33,6 +35,10 @@ def say(*a):
 
 def ptr(t):
     return C.c_void_p(t.data_ptr())
+
+
+def rel(x, ref):
+    return float((x - ref).abs().max() / ref.abs().max())
 
 
 def library_bytes():
@@ -72,14 +78,15 @@ def test_noalloc_solve_equals_solve(precision):
                 "gfdfd_solve_noalloc")
     torch.cuda.synchronize()
     assert library_bytes() == lib0 and torch.cuda.memory_allocated() == torch0
+    bar = 1e-12 if precision == G.FP64 else 1e-5
     for X, R in zip(outs, refs):
-        assert torch.equal(X, R)
+        assert rel(X, R) <= bar
     # in place (d_x == d_b), as gfdfd_solve allows
     (pl, F), B = systems[1], rhs[1].clone()
     G.check(L.gfdfd_solve_noalloc(F.h, 0, cases[1][3], cases[1][2], ptr(B), ptr(B), C.c_void_p(st.cuda_stream)),
             "gfdfd_solve_noalloc")
     torch.cuda.synchronize()
-    assert torch.equal(B, refs[1])
+    assert rel(B, refs[1]) <= bar
 
 
 def test_noalloc_solve_rejects_what_does_not_fit():
