@@ -1789,40 +1789,35 @@ __global__ void __launch_bounds__(256) k_vinv(const cx<T> *__restrict__ arena, i
     }
 }
 
-// forward gather of a split level: every row of the front (pivot rows
-// permuted for M solves) into its W rows.  grid: fronts
-template <typename T>
-__global__ void __launch_bounds__(256) k_vgather(const FrontDev *__restrict__ fronts, const int *__restrict__ list,
-                                                 const int *__restrict__ children, const int *__restrict__ inv,
-                                                 const int *__restrict__ rowperm, const cx<T> *__restrict__ X,
-                                                 cx<T> *__restrict__ W, int trans) {
-    const FrontDev F = fronts[list[blockIdx.x]];
-    for (int i = threadIdx.x; i < F.m; i += 256) {
-        const int src = (!trans && i < F.p) ? rowperm[F.pos0 + i] : i;
-        W[F.fv_off + i] = front_row(F, src, 0, 1, X, W, fronts, children, inv);
-    }
-}
-
 // one part of a split level's single-vector sweep as a product, per front:
-//   VG_FT  y1 = inv(T1) w1          rows, columns [0, p); w1 = W rows, y1 -> X
-//   VG_FR  w2 -= T21 y1             rows [p, m), columns [0, p); y1 = X, w2 = W rows
-//   VG_BR  z1 = x1 - T12 x2         rows [0, p), columns [p, m); x2 = X via idx, z1 -> W rows
+//   VG_FT  y1 = inv(T1) w1          rows, columns [0, p); w1 gathered, y1 -> W rows [0, p)
+//   VG_FR  w2 = w2 - T21 y1         rows [p, m), columns [0, p); y1 = W rows, w2 gathered -> W rows
+//   VG_BR  z1 = y1 - T12 x2         rows [0, p), columns [p, m); x2 = X via idx, z1 -> W rows
 //   VG_BT  x1 = inv(T1') z1         rows, columns [0, p); z1 = W rows, x1 -> X (permuted for M^T, M^H)
+// y1 and z1 stay in the front's W rows [0, p) (unused otherwise) because the
+// workgroups of VG_FT read X's right-hand-side rows while others write;
+// VG_BR reads and writes only its own rows of them.
 // T1/T1' are the pivot block's lower/upper triangles of op(M): for M, L11
 // (unit) forward and U11 backward; for M^T and M^H, U11^T forward and L11^T
 // (unit) backward, read transposed (and conjugated for M^H).  A workgroup
 // takes kGvRows rows and every column: 16 column lanes per row, which for M
 // read 16 consecutive rows of one column (the storage is column-major) and
 // for the transposed operators 16 consecutive columns of one row; the lanes'
-// sums meet in LDS and are added in lane order.  No workgroup depends on
-// another.  grid (row chunks, fronts)
+// sums meet in LDS and are added in lane order.  The forward parts gather the
+// right-hand-side rows they need themselves (the pivot rows permuted for M
+// solves, plus the children's contribution rows): VG_FT every row up to its
+// chunk's last into LDS (dynamic, p entries), VG_FR its own rows.  No
+// workgroup depends on another.  grid (row chunks, fronts)
 enum { VG_FT = 0, VG_FR = 1, VG_BR = 2, VG_BT = 3 };
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_vgemv(const cx<T> *__restrict__ arena, const cx<T> *__restrict__ sq,
                                                const FrontDev *__restrict__ fronts, const int *__restrict__ list,
                                                const int64_t *__restrict__ off, const int *__restrict__ idx,
+                                               const int *__restrict__ children, const int *__restrict__ inv,
                                                const int *__restrict__ rowperm, cx<T> *__restrict__ X,
                                                cx<T> *__restrict__ W, int trans, int cj) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *wl = reinterpret_cast<cx<T> *>(smem_raw);   // VG_FT: the gathered w1; VG_BR: x2
     __shared__ cx<T> red[16][kGvRows + 1];
     const int fl = blockIdx.y;
     const FrontDev F = fronts[list[fl]];
@@ -1850,14 +1845,21 @@ __global__ void __launch_bounds__(256) k_vgemv(const cx<T> *__restrict__ arena, 
         }
     };
     auto v_at = [&](int t) -> cx<T> {
-        if (MODE == VG_FR) return X[F.pos0 + t];
-        if (MODE == VG_BR) return X[idx[F.idx_off + p + t]];
+        if (MODE == VG_BR || MODE == VG_FT) return wl[t];
         return W[F.fv_off + t];
     };
     // the triangles' rows need only the columns on their side of the diagonal
     int t0 = 0, t1 = ncol;
     if (MODE == VG_FT) t1 = min(ncol, r0 + kGvRows);
     if (MODE == VG_BT) t0 = r0;
+    if (MODE == VG_FT) {
+        for (int t = tid; t < t1; t += 256)
+            wl[t] = front_row(F, trans ? t : rowperm[F.pos0 + t], 0, 1, X, W, fronts, children, inv);
+        __syncthreads();
+    } else if (MODE == VG_BR) {   // the ancestors' solution rows, staged once
+        for (int t = tid; t < ncol; t += 256) wl[t] = X[idx[F.idx_off + p + t]];
+        __syncthreads();
+    }
     cx<T> acc = {T(0), T(0)};
     if (r < nrow)
 #pragma unroll 8
@@ -1870,11 +1872,11 @@ __global__ void __launch_bounds__(256) k_vgemv(const cx<T> *__restrict__ arena, 
             cx<T> s = red[0][tid];
             for (int k = 1; k < 16; k++) s = cadd(s, red[k][tid]);
             if (MODE == VG_FT) {
-                X[F.pos0 + rr] = trans ? s : cadd(W[F.fv_off + rr], s);
+                W[F.fv_off + rr] = trans ? s : cadd(wl[rr], s);
             } else if (MODE == VG_FR) {
-                W[F.fv_off + p + rr] = csub(W[F.fv_off + p + rr], s);
+                W[F.fv_off + p + rr] = csub(front_row(F, p + rr, 0, 1, X, W, fronts, children, inv), s);
             } else if (MODE == VG_BR) {
-                W[F.fv_off + rr] = csub(X[F.pos0 + rr], s);
+                W[F.fv_off + rr] = csub(W[F.fv_off + rr], s);
             } else {
                 X[F.pos0 + (trans ? rowperm[F.pos0 + rr] : rr)] = trans ? cadd(W[F.fv_off + rr], s) : s;
             }
@@ -2604,13 +2606,14 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                 size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
                 const int maxrc = (dp.mmax_all[l] + kGvRows - 1) / kGvRows;
                 if (vec_split(dp, l) && vi && dp.vinv[l]) {
-                    hipLaunchKernelGGL(k_vgather<T>, dim3(dp.n_all[l]), dim3(256), 0, st, dp.fronts, dp.lv_all[l],
-                                       dp.children, dp.inv, rowperm, Xp, Wp, (int)trans);
-                    hipLaunchKernelGGL((k_vgemv<T, VG_FT>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
-                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, rowperm, Xp, Wp,
+                    hipLaunchKernelGGL((k_vgemv<T, VG_FT>), dim3(maxrc, dp.n_all[l]), dim3(256),
+                                       sizeof(cx<T>) * (size_t)dp.pmax_all[l], st, arena,
+                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, dp.children, dp.inv,
+                                       rowperm, Xp, Wp,
                                        (int)trans, (int)cj);
                     hipLaunchKernelGGL((k_vgemv<T, VG_FR>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
-                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, rowperm, Xp, Wp,
+                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, dp.children, dp.inv,
+                                       rowperm, Xp, Wp,
                                        (int)trans, (int)cj);
                 } else if (vec_split(dp, l)) {
                     hipLaunchKernelGGL(k_fwd_vec_tri<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
@@ -2637,11 +2640,14 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                 size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
                 const int maxrc = (dp.mmax_all[l] + kGvRows - 1) / kGvRows;
                 if (vec_split(dp, l) && vi && dp.vinv[l]) {
-                    hipLaunchKernelGGL((k_vgemv<T, VG_BR>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
-                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, rowperm, Xp, Wp,
+                    hipLaunchKernelGGL((k_vgemv<T, VG_BR>), dim3(maxrc, dp.n_all[l]), dim3(256),
+                                       sizeof(cx<T>) * (size_t)dp.mmax_all[l], st, arena,
+                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, dp.children, dp.inv,
+                                       rowperm, Xp, Wp,
                                        (int)trans, (int)cj);
                     hipLaunchKernelGGL((k_vgemv<T, VG_BT>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
-                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, rowperm, Xp, Wp,
+                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, dp.children, dp.inv,
+                                       rowperm, Xp, Wp,
                                        (int)trans, (int)cj);
                 } else if (vec_split(dp, l)) {
                     const int chunks = (dp.pmax_all[l] + kVecSplitRows - 1) / kVecSplitRows;
