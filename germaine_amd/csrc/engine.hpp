@@ -13,7 +13,6 @@ namespace gfdfd {
 constexpr int kSingleMmax = 300;   // largest front order of a single-block level (its panels fit in LDS)
 constexpr int kVecSplitFronts = 1 << 30;   // single-vector solves split the levels of fewer multi-block fronts (every level)
 constexpr int kVinvPmax = 560;         // largest pivot block the inversion kernel holds in LDS (FP64)
-constexpr int kGvRows = 16;   // rows per workgroup of a split single-vector product (16 column lanes each)
 constexpr int kTile = 64;       // output tile of the trailing-update GEMM
 constexpr int kMaxBatch = 64;   // frequencies per batched solve
 constexpr int kSubMaxM = 64;    // largest front order of the persistent subtree launch
@@ -57,16 +56,19 @@ struct DevPlan {
     int max_children = 0;                           // largest number of children of one front
     std::vector<int> single;                        // per level: 1 if stored in the single-block solve format
     int *lv_single = nullptr;                       // device copy of `single`
-    // single-vector solves of the levels of few multi-block fronts: explicit
-    // inverses of each front's pivot-block triangles (a p x p square per front:
-    // inv(L11) strictly below the diagonal, inv(U11) on and above it), so the
-    // triangular parts become products spread over many workgroups
+    // single-vector solves of the multi-block levels: explicit inverses of each
+    // front's pivot-block triangles (a p x p square per front: inv(L11) strictly
+    // below the diagonal, inv(U11) on and above it) followed by the rectangles
+    // L21 inv(L11) ((m - p) x p) and inv(U11) U12 (p x (m - p)), all
+    // column-major, so each sweep of such a front is one product spread over
+    // many workgroups
     std::vector<char> vinv;                         // per level: its fronts have inverses
     std::vector<int64_t *> vinv_off;                // per level (device): square offset of each front of lv_all
-    int64_t vinv_elems = 0;                         // square entries per frequency
+    int64_t vinv_elems = 0;                         // square and rectangle entries per frequency
     int *vinv_list = nullptr;                       // device: every front with a square
     int64_t *vinv_list_off = nullptr;               // device: its square's offset
     int vinv_n = 0, vinv_pmax = 0;
+    int vinv_rect_tiles = 0;                        // most 64 x 64 rectangle tiles of one front
     int *gsrc_id = nullptr;                         // [fv_rows][3] gather sources of front rows before pivoting
     std::vector<FrontDev *> lv_fr;                  // per level: the level's fronts, contiguous
     std::vector<int> lv_children;                   // per level: 1 if any front has children

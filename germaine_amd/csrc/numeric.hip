@@ -296,12 +296,14 @@ DevPlan::DevPlan(std::shared_ptr<const Plan> p) : host(std::move(p)) {
             all.insert(all.end(), L.large.begin(), L.large.end());
             std::vector<int64_t> off;
             for (int f : all) {
-                const int p = pl.fronts[f].p;
+                const int p = pl.fronts[f].p, c = pl.fronts[f].m - p;
                 off.push_back(vinv_elems);
                 lst.push_back(f);
                 loff.push_back(vinv_elems);
-                vinv_elems += (int64_t)p * p;
+                // the square, then L21 inv(L11) (c x p) and inv(U11) U12 (p x c)
+                vinv_elems += (int64_t)p * p + 2 * (int64_t)p * c;
                 vinv_pmax = std::max(vinv_pmax, p);
+                vinv_rect_tiles = std::max(vinv_rect_tiles, 2 * ((p + 63) / 64) * ((c + 63) / 64));
             }
             vinv_off[l] = dev_upload(off);
         }

@@ -1539,6 +1539,136 @@ __global__ void __launch_bounds__(256) k_bwd_vec(const cx<T> *__restrict__ arena
     for (int i = tid; i < p; i += 256) X[F.pos0 + (trans ? rowperm[F.pos0 + i] : i)] = sw[i];
 }
 
+// value of a front row from its gather-table entry g = (X position, W row,
+// W row), -1 for none: X[a] + W[u] + W[w].  The three loads are unconditional
+// (row 0 stands in for a missing one and is not added), so they are in flight
+// together instead of one after another.
+template <typename T>
+__device__ __forceinline__ cx<T> gather3(const int *__restrict__ g, const cx<T> *__restrict__ X,
+                                         const cx<T> *__restrict__ W) {
+    const int a = g[0], u = g[1], w = g[2];
+    const cx<T> xa = X[max(a, 0)], wu = W[max(u, 0)], ww = W[max(w, 0)];
+    cx<T> v = a >= 0 ? xa : cx<T>{T(0), T(0)};
+    if (u >= 0) v = cadd(v, wu);
+    if (w >= 0) v = cadd(v, ww);
+    return v;
+}
+
+// ---- single right-hand side, single-block levels of M solves: one wave per front ----
+// In the single-block format a front's forward step is one product,
+// [y1; w2 - L21 y1] = [inv(L11); L21 inv(L11)] w1, and its backward step one
+// more, x1 = [inv(U11), -inv(U11) U12] [y1; x2].  A wave owns a front, so a
+// compute unit holds many fronts at once and a front costs about three memory
+// round trips with no workgroup barrier.  Columns of the operators are
+// contiguous over rows (M solves; transposed solves keep k_fwd_vec/k_bwd_vec).
+template <typename T>
+__device__ __forceinline__ cx<T> lane_bcast(cx<T> v, int l) {
+    if constexpr (sizeof(T) == 4) {
+        return {__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), l)),
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), l))};
+    } else {
+        auto rd = [l](double d) {
+            const long long b = __double_as_longlong(d);
+            const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+            const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+            return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        };
+        return {rd(v.x), rd(v.y)};
+    }
+}
+
+// forward: lane i gathers front row i (i + 64, ... on later passes) from the
+// gather table -- X position, up to two children's contribution rows in W,
+// after the front's row permutation -- and sums its row's p products; the
+// pivot values w1[t] (t < p <= 32, held by lane t of the first pass) reach
+// every lane by readlane; PM >= p bounds the operator registers (levels of
+// narrow fronts keep more waves resident).  grid: fronts, 64 threads
+template <typename T, int PM>
+__global__ void __launch_bounds__(64) k_fwd_vec1(const cx<T> *__restrict__ arena, const FrontDev *__restrict__ lvF,
+                                                 const int *__restrict__ gsrc, cx<T> *__restrict__ X,
+                                                 cx<T> *__restrict__ W) {
+    const FrontDev F = lvF[blockIdx.x];
+    const int m = F.m, p = F.p, lane = threadIdx.x;
+    const cx<T> *G = arena + F.buf_off;
+    const int *gs = gsrc + (int64_t)F.fv_off * 3;
+    auto gather = [&](int i) { return i < m ? gather3(gs + 3 * i, X, W) : cx<T>{T(0), T(0)}; };
+    cx<T> a[PM];
+    // rows past m read row m - 1 (in bounds) and are not stored
+    auto fetch = [&](int i) {
+#pragma unroll
+        for (int t = 0; t < PM; t++)
+            if (t < p) a[t] = G[min(i, m - 1) + (int64_t)t * m];
+    };
+    fetch(lane);   // the first pass's operator rows are in flight during the gather
+    const cx<T> w0 = gather(lane);
+#pragma unroll 1
+    for (int i0 = 0; i0 < m; i0 += 64) {
+        const int i = i0 + lane;
+        if (i0 > 0) fetch(i);
+        const cx<T> wi = i0 == 0 ? w0 : gather(i);
+#pragma unroll
+        for (int t = 0; t < PM; t++)   // row i < p of inv(L11): unit diagonal, zero above it
+            if (t < p && i < p && t >= i) a[t] = {T(t == i), T(0)};
+        cx<T> acc = {T(0), T(0)};
+#pragma unroll
+        for (int t = 0; t < PM; t++)
+            if (t < p) acc = cadd(acc, cmul(a[t], lane_bcast(w0, t)));
+        if (i < m) {
+            if (i < p) X[F.pos0 + i] = acc;
+            else W[F.fv_off + i] = csub(wi, acc);
+        }
+    }
+}
+
+// backward: lanes are R = (p rounded up to a power of two) pivot rows times
+// 64 / R column parts, part q summing the columns q, q + 64/R, ... in chunks
+// of CH per lane; the values [y1; x2] (own pivot rows, then the ancestors' rows
+// at the contribution positions) are staged in LDS while the first chunk of the
+// operator is in flight.  grid: fronts, 64 threads; LDS: m entries
+template <typename T, int CH>
+__global__ void __launch_bounds__(64) k_bwd_vec1(const cx<T> *__restrict__ arena, const FrontDev *__restrict__ lvF,
+                                                 const int *__restrict__ idx, cx<T> *__restrict__ X) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
+    const FrontDev F = lvF[blockIdx.x];
+    const int m = F.m, p = F.p, lane = threadIdx.x;
+    if (p == 0) return;
+    const cx<T> *G = arena + F.buf_off;
+    int lr = 0;
+    while ((1 << lr) < p) lr++;
+    const int parts = 64 >> lr, i = lane & ((1 << lr) - 1), q = lane >> lr, step = CH * parts;
+    // rows i >= p (up to the power of two) read row p - 1 in bounds; their sums meet only each other
+    const int ic = min(i, p - 1);
+    cx<T> a[CH];
+    auto fetch = [&](int t0) {
+#pragma unroll
+        for (int u = 0; u < CH; u++) a[u] = G[ic + (int64_t)min(t0 + u * parts, m - 1) * m];
+    };
+    fetch(q);   // the first chunk is in flight while the values are gathered
+    for (int t = lane; t < m; t += 64) sv[t] = X[t < p ? F.pos0 + t : idx[F.idx_off + t]];
+    __syncthreads();
+    cx<T> acc = {T(0), T(0)};
+    for (int t0 = q;;) {
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            // row i of inv(U11) is zero left of the diagonal; -inv(U11) U12 beyond p
+            const int t = t0 + u * parts;
+            if (t < m && t >= i) {
+                const cx<T> v = sv[t];
+                acc = t < p ? cadd(acc, cmul(a[u], v)) : csub(acc, cmul(a[u], v));
+            }
+        }
+        t0 += step;
+        if (t0 >= m) break;
+        fetch(t0);
+    }
+    for (int o = 1 << lr; o < 64; o <<= 1) {
+        acc.x += __shfl_xor(acc.x, o);
+        acc.y += __shfl_xor(acc.y, o);
+    }
+    if (q == 0 && i < p) X[F.pos0 + i] = acc;
+}
+
 // ---- single-vector sweeps of large fronts, split across workgroups ------------
 // For levels of few, large multi-block fronts, k_fwd_vec/k_bwd_vec would leave
 // the device nearly idle (one workgroup per front).  The triangular part over
@@ -1789,96 +1919,204 @@ __global__ void __launch_bounds__(256) k_vinv(const cx<T> *__restrict__ arena, i
     }
 }
 
-// one part of a split level's single-vector sweep as a product, per front:
-//   VG_FT  y1 = inv(T1) w1          rows, columns [0, p); w1 gathered, y1 -> W rows [0, p)
-//   VG_FR  w2 = w2 - T21 y1         rows [p, m), columns [0, p); y1 = W rows, w2 gathered -> W rows
-//   VG_BR  z1 = y1 - T12 x2         rows [0, p), columns [p, m); x2 = X via idx, z1 -> W rows
-//   VG_BT  x1 = inv(T1') z1         rows, columns [0, p); z1 = W rows, x1 -> X (permuted for M^T, M^H)
-// y1 and z1 stay in the front's W rows [0, p) (unused otherwise) because the
-// workgroups of VG_FT read X's right-hand-side rows while others write;
-// VG_BR reads and writes only its own rows of them.
-// T1/T1' are the pivot block's lower/upper triangles of op(M): for M, L11
-// (unit) forward and U11 backward; for M^T and M^H, U11^T forward and L11^T
-// (unit) backward, read transposed (and conjugated for M^H).  A workgroup
-// takes kGvRows rows and every column: 16 column lanes per row, which for M
-// read 16 consecutive rows of one column (the storage is column-major) and
-// for the transposed operators 16 consecutive columns of one row; the lanes'
-// sums meet in LDS and are added in lane order.  The forward parts gather the
-// right-hand-side rows they need themselves (the pivot rows permuted for M
-// solves, plus the children's contribution rows): VG_FT every row up to its
-// chunk's last into LDS (dynamic, p entries), VG_FR its own rows.  No
-// workgroup depends on another.  grid (row chunks, fronts)
-enum { VG_FT = 0, VG_FR = 1, VG_BR = 2, VG_BT = 3 };
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_vgemv(const cx<T> *__restrict__ arena, const cx<T> *__restrict__ sq,
+// the rectangles beside each square: RL = L21 inv(L11) ((m - p) x p) and
+// RU = inv(U11) U12 (p x (m - p)), column-major after the square.  A workgroup
+// makes one 64 x 64 tile, 4 x 4 entries per thread, the inner dimension in
+// steps of 16 through LDS; the triangle's zero part is skipped (RL: from the
+// tile's first column on, RU: from its first row on).  The triangles come from
+// the square (inv(L11) has a unit diagonal), L21 and U12 from the front.
+// grid (RL tiles then RU tiles, fronts with squares, nfreq)
+template <typename T>
+__global__ void __launch_bounds__(256) k_vrect(const cx<T> *__restrict__ arena, int64_t fstride,
                                                const FrontDev *__restrict__ fronts, const int *__restrict__ list,
-                                               const int64_t *__restrict__ off, const int *__restrict__ idx,
-                                               const int *__restrict__ children, const int *__restrict__ inv,
-                                               const int *__restrict__ rowperm, cx<T> *__restrict__ X,
-                                               cx<T> *__restrict__ W, int trans, int cj) {
+                                               const int64_t *__restrict__ off, cx<T> *__restrict__ sq,
+                                               int64_t sqstride) {
+    __shared__ cx<T> As[16][65];   // [k][row]
+    __shared__ cx<T> Bs[16][65];   // [k][column]
+    const FrontDev F = fronts[list[blockIdx.y]];
+    const int m = F.m, p = F.p, c = m - p, tid = threadIdx.x;
+    const int tp = (p + 63) / 64, tc = (c + 63) / 64;
+    int tile = blockIdx.x;
+    const bool rl = tile < tp * tc;
+    if (!rl) tile -= tp * tc;
+    if (c == 0 || tile >= tp * tc) return;
+    const cx<T> *G = arena + blockIdx.z * fstride + F.buf_off;
+    cx<T> *Sq = sq + blockIdx.z * sqstride + off[blockIdx.y];
+    const cx<T> *S = Sq;
+    cx<T> *O = Sq + (int64_t)p * p + (rl ? 0 : (int64_t)c * p);
+    const int I = rl ? c : p, J = rl ? p : c, ldo = I;
+    const int ti = rl ? tile % tc : tile % tp, tj = rl ? tile / tc : tile / tp;
+    const int i0 = ti * 64, j0 = tj * 64;
+    // A(i, k): RL: L21(i, k) = G[p + i + k m]; RU: inv(U11)(i, k), cx<T>{T(0), T(0)} left of the diagonal
+    auto A = [&](int i, int k) -> cx<T> {
+        if (i >= I || k >= p) return cx<T>{T(0), T(0)};
+        if (rl) return G[p + i + (int64_t)k * m];
+        return k >= i ? S[i + (int64_t)k * p] : cx<T>{T(0), T(0)};
+    };
+    // B(k, j): RL: inv(L11)(k, j), unit diagonal, cx<T>{T(0), T(0)} above; RU: U12(k, j) = G[k + (p + j) m]
+    auto B = [&](int k, int j) -> cx<T> {
+        if (j >= J || k >= p) return cx<T>{T(0), T(0)};
+        if (rl) return k > j ? S[k + (int64_t)j * p] : (k == j ? cx<T>{T(1), T(0)} : cx<T>{T(0), T(0)});
+        return G[k + (int64_t)(p + j) * m];
+    };
+    const int ri = tid & 15, cjj = tid >> 4;
+    cx<T> acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = cx<T>{T(0), T(0)};
+    for (int k0 = rl ? j0 : i0; k0 < p; k0 += 16) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = tid + 256 * u;
+            As[e >> 6][e & 63] = A(i0 + (e & 63), k0 + (e >> 6));   // 64 consecutive rows of one column
+            Bs[e & 15][e >> 4] = B(k0 + (e & 15), j0 + (e >> 4));   // 16 consecutive k of one column
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < 16; k++) {
+            cx<T> av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) av[a] = As[k][ri + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; b++) bv[b] = Bs[k][cjj + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] = cmac(acc[a][b], av[a], bv[b]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int i = i0 + ri + 16 * a, j = j0 + cjj + 16 * b;
+            if (i < I && j < J) O[i + (int64_t)j * ldo] = acc[a][b];
+        }
+}
+
+// a split level's single-vector sweep of a front as one product (VS_ROWS rows
+// per workgroup):
+//   forward   rows [0, m):  [y1; w2 - T21 y1] = [inv(T1); T21 inv(T1)] w1, to W rows [0, m)
+//   backward  rows [0, p):  x1 = [inv(T1'), -inv(T1') T12] [y1; x2], to X (permuted for M^T, M^H)
+// T1/T1' are the pivot block's lower/upper triangles of op(M): for M, L11
+// (unit) forward and U11 backward, T21 inv(T1) = RL and inv(T1') T12 = RU; for
+// M^T and M^H, U11^T forward and L11^T (unit) backward, with RU^T and RL^T,
+// read transposed (and conjugated for M^H).  y1 stays in the front's W rows
+// [0, p): other workgroups of the front read X's right-hand-side rows while
+// this one writes.  The forward gathers its right-hand-side rows through the
+// gather table (X position and up to two children's contribution rows): w1 into
+// LDS (p entries), the contribution rows it updates itself; the backward stages
+// [y1; x2] (m entries).  A workgroup takes RW rows -- one 128-byte line (8 rows
+// in FP64, 16 in FP32) or more, for levels with few columns -- and 256 / RW
+// column lanes: for M a load instruction reads whole lines of consecutive rows
+// of a column (the storage is column-major), for the transposed operators runs
+// of consecutive columns of one row.  Each lane's
+// first VS_PRE operator entries are requested before the gather, so they
+// arrive while it waits; the lanes' sums meet in LDS and are added in lane
+// order.  No workgroup depends on another.  grid (row chunks, fronts)
+template <typename T>
+constexpr int VS_ROWS = 128 / (int)sizeof(cx<T>);
+constexpr int VS_PRE = 8;
+template <typename T, bool FWD, int RW, int NT>
+__global__ void __launch_bounds__(NT) k_vsweep(const cx<T> *__restrict__ sq, const FrontDev *__restrict__ lvF,
+                                               const int64_t *__restrict__ off, const int *__restrict__ gsrc,
+                                               const int *__restrict__ idx, const int *__restrict__ rowperm,
+                                               cx<T> *__restrict__ X, cx<T> *__restrict__ W, int trans, int cj) {
+    constexpr int CL = NT / RW;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cx<T> *wl = reinterpret_cast<cx<T> *>(smem_raw);   // VG_FT: the gathered w1; VG_BR: x2
-    __shared__ cx<T> red[16][kGvRows + 1];
+    cx<T> *wl = reinterpret_cast<cx<T> *>(smem_raw);
     const int fl = blockIdx.y;
-    const FrontDev F = fronts[list[fl]];
-    const int m = F.m, p = F.p, tid = threadIdx.x;
-    const int nrow = MODE == VG_FR ? m - p : p, ncol = MODE == VG_BR ? m - p : p;
-    const int r0 = blockIdx.x * kGvRows;
+    const FrontDev F = lvF[fl];
+    const int m = F.m, p = F.p, c = m - p, tid = threadIdx.x;
+    const int nrow = FWD ? m : p;
+    const int r0 = blockIdx.x * RW;
     if (r0 >= nrow) return;
-    const cx<T> *G = arena + F.buf_off;
-    const cx<T> *S = (MODE == VG_FT || MODE == VG_BT) ? sq + off[fl] : nullptr;
+    const cx<T> *S = sq + off[fl], *RL = S + (int64_t)p * p, *RU = RL + (int64_t)c * p;
+    const int *gs = gsrc + (int64_t)F.fv_off * 3;
+    auto gather = [&](int i) { return gather3(gs + 3 * i, X, W); };
     // lane -> (row, column lane): rows fastest for M, columns fastest for the transposed operators
-    const int lr = trans ? tid >> 4 : tid & 15, lc = trans ? tid & 15 : tid >> 4, r = r0 + lr;
+    const int lr = trans ? tid / CL : tid % RW, lc = trans ? tid % CL : tid / RW, r = r0 + lr;
+    const bool live = r < nrow;
     auto a_at = [&](int t) -> cx<T> {
-        if (MODE == VG_FR) {   // T21: L21 (M) or U12^T (M^T, M^H), row p + r
-            const int i = p + r;
-            return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj);
-        } else if (MODE == VG_BR) {   // T12: U12 (M) or L21^T, column p + t
-            const int s = p + t;
-            return trans ? ld(G + s + (int64_t)r * m, cj) : ld(G + r + (int64_t)s * m, cj);
-        } else if (MODE == VG_FT) {   // inv(L11) strictly lower (M) or inv(U11)^T lower incl. diagonal
-            if (!trans) return t < r ? S[r + (int64_t)t * p] : cx<T>{T(0), T(0)};
-            return t <= r ? ld(S + t + (int64_t)r * p, cj) : cx<T>{T(0), T(0)};
-        } else {   // inv(U11) upper incl. diagonal (M) or inv(L11)^T strictly upper
+        if (FWD) {
+            if (r < p) {   // inv(L11) strictly lower (M) or inv(U11)^T lower incl. diagonal
+                if (!trans) return t < r ? S[r + (int64_t)t * p] : cx<T>{T(0), T(0)};
+                return t <= r ? ld(S + t + (int64_t)r * p, cj) : cx<T>{T(0), T(0)};
+            }
+            const int i = r - p;   // RL row (M) or RU^T row
+            return trans ? ld(RU + t + (int64_t)i * p, cj) : RL[i + (int64_t)t * c];
+        }
+        if (t < p) {   // inv(U11) upper incl. diagonal (M) or inv(L11)^T strictly upper
             if (!trans) return t >= r ? S[r + (int64_t)t * p] : cx<T>{T(0), T(0)};
             return t > r ? ld(S + t + (int64_t)r * p, cj) : cx<T>{T(0), T(0)};
         }
+        const int s = t - p;   // -RU (M) or -RL^T
+        const cx<T> v = trans ? ld(RL + s + (int64_t)r * c, cj) : RU[r + (int64_t)s * p];
+        return {-v.x, -v.y};
     };
-    auto v_at = [&](int t) -> cx<T> {
-        if (MODE == VG_BR || MODE == VG_FT) return wl[t];
-        return W[F.fv_off + t];
-    };
-    // the triangles' rows need only the columns on their side of the diagonal
-    int t0 = 0, t1 = ncol;
-    if (MODE == VG_FT) t1 = min(ncol, r0 + kGvRows);
-    if (MODE == VG_BT) t0 = r0;
-    if (MODE == VG_FT) {
-        for (int t = tid; t < t1; t += 256)
-            wl[t] = front_row(F, trans ? t : rowperm[F.pos0 + t], 0, 1, X, W, fronts, children, inv);
-        __syncthreads();
-    } else if (MODE == VG_BR) {   // the ancestors' solution rows, staged once
-        for (int t = tid; t < ncol; t += 256) wl[t] = X[idx[F.idx_off + p + t]];
-        __syncthreads();
+    // columns: the forward's triangle rows need only those up to the chunk's
+    // last row, the backward's only those from its first
+    int t0 = 0, t1 = FWD ? p : m;
+    if (FWD && r0 + RW <= p) t1 = min(p, r0 + RW);
+    if (!FWD) t0 = r0;
+    cx<T> pre[VS_PRE];
+#pragma unroll
+    for (int u = 0; u < VS_PRE; u++) {
+        const int t = t0 + lc + u * CL;
+        pre[u] = (live && t < t1) ? a_at(t) : cx<T>{T(0), T(0)};
     }
-    cx<T> acc = {T(0), T(0)};
-    if (r < nrow)
-#pragma unroll 8
-        for (int t = t0 + lc; t < t1; t += 16) acc = cmac(acc, a_at(t), v_at(t));
-    red[lc][lr] = acc;
+    // the forward's own contribution row (rows >= p), fetched by the lane that stores it
+    cx<T> own = {T(0), T(0)};
+    if (FWD && tid < RW && r0 + tid >= p && r0 + tid < m) own = gather(r0 + tid);
+    if (FWD) {
+        for (int t = tid; t < t1; t += NT) wl[t] = gather(t);
+    } else {
+        for (int t = t0 + tid; t < m; t += NT) wl[t] = t < p ? W[F.fv_off + t] : X[idx[F.idx_off + t]];
+    }
     __syncthreads();
-    if (tid < kGvRows) {
+    cx<T> acc = {T(0), T(0)};
+#pragma unroll
+    for (int u = 0; u < VS_PRE; u++) {
+        const int t = t0 + lc + u * CL;
+        if (t < t1) acc = cmac(acc, pre[u], wl[t]);
+    }
+    if (live)
+#pragma unroll 8
+        for (int t = t0 + lc + VS_PRE * CL; t < t1; t += CL) acc = cmac(acc, a_at(t), wl[t]);
+    cx<T> s;
+    if constexpr (NT == 256) {
+        __shared__ cx<T> red[CL][RW + 1];
+        red[lc][lr] = acc;
+        __syncthreads();
+        if (tid < RW) {
+            s = red[0][tid];
+            for (int k = 1; k < CL; k++) s = cadd(s, red[k][tid]);
+        }
+    } else {
+        // M solves only (rows fastest): a row's column lanes within a wave are
+        // RW apart; the waves' sums meet in LDS
+        for (int o = RW; o < 64; o <<= 1) {
+            acc.x += __shfl_xor(acc.x, o);
+            acc.y += __shfl_xor(acc.y, o);
+        }
+        __shared__ cx<T> redw[NT / 64][RW];
+        if ((tid & 63) < RW) redw[tid >> 6][tid & 63] = acc;
+        __syncthreads();
+        if (tid < RW) {
+            s = redw[0][tid];
+            for (int k = 1; k < NT / 64; k++) s = cadd(s, redw[k][tid]);
+        }
+    }
+    if (tid < RW) {
         const int rr = r0 + tid;
         if (rr < nrow) {
-            cx<T> s = red[0][tid];
-            for (int k = 1; k < 16; k++) s = cadd(s, red[k][tid]);
-            if (MODE == VG_FT) {
-                W[F.fv_off + rr] = trans ? s : cadd(wl[rr], s);
-            } else if (MODE == VG_FR) {
-                W[F.fv_off + p + rr] = csub(front_row(F, p + rr, 0, 1, X, W, fronts, children, inv), s);
-            } else if (MODE == VG_BR) {
-                W[F.fv_off + rr] = csub(W[F.fv_off + rr], s);
+            if (FWD) {
+                if (rr < p) W[F.fv_off + rr] = trans ? s : cadd(wl[rr], s);
+                else W[F.fv_off + rr] = csub(own, s);
             } else {
-                X[F.pos0 + (trans ? rowperm[F.pos0 + rr] : rr)] = trans ? cadd(W[F.fv_off + rr], s) : s;
+                X[F.pos0 + (trans ? rowperm[F.pos0 + rr] : rr)] = trans ? cadd(wl[rr], s) : s;
             }
         }
     }
@@ -2543,8 +2781,8 @@ static bool vec_split(const DevPlan &dp, int l) {
     return !off && !dp.single[l] && dp.n_all[l] < kVecSplitFronts;
 }
 
-// the inverse squares of the split levels, made once after each factorisation
-// (allocated on first use)
+// the inverse squares and rectangles of the split levels, made once after each
+// factorisation (allocated on first use)
 template <typename T>
 static bool ensure_vinv(const Factors<T> &F, hipStream_t st) {
     static const bool off = std::getenv("GFDFD_VEC_NOINV") != nullptr;
@@ -2569,10 +2807,69 @@ static bool ensure_vinv(const Factors<T> &F, hipStream_t st) {
         const int ncb = (dp.vinv_pmax + kVinvCb - 1) / kVinvCb;
         hipLaunchKernelGGL(k_vinv<T>, dim3(2 * ncb, dp.vinv_n, F.nfreq), dim3(256), lds, st, F.arena, F.fstride,
                            dp.fronts, dp.vinv_list, dp.vinv_list_off, F.vinv, dp.vinv_elems, ncb);
+        if (dp.vinv_rect_tiles > 0)
+            hipLaunchKernelGGL(k_vrect<T>, dim3(dp.vinv_rect_tiles, dp.vinv_n, F.nfreq), dim3(256), 0, st, F.arena,
+                               F.fstride, dp.fronts, dp.vinv_list, dp.vinv_list_off, F.vinv, dp.vinv_elems);
         GFDFD_HIP(hipGetLastError());
         F.vinv_ready = true;
     }
     return true;
+}
+
+// per-level widths of the single-vector kernels: operator registers of the
+// one-wave forward (the level's widest pivot block), operator entries per lane
+// and round trip of the one-wave backward (8 when that covers every column of
+// a lane), and rows per workgroup of the split products (a 128-byte line of
+// rows, or more when the columns are few: about VS_PRE columns per lane, so
+// each lane requests all its operator entries at once)
+template <typename T>
+static void launch_fwd_vec1(const DevPlan &dp, int l, const cx<T> *arena, const int *gsrc, cx<T> *Xp, cx<T> *Wp,
+                            hipStream_t st) {
+    const int pm = dp.pmax_all[l];
+    const dim3 g(dp.n_all[l]), b(64);
+    if (pm <= 8)
+        hipLaunchKernelGGL((k_fwd_vec1<T, 8>), g, b, 0, st, arena, dp.lv_fr[l], gsrc, Xp, Wp);
+    else if (pm <= 16)
+        hipLaunchKernelGGL((k_fwd_vec1<T, 16>), g, b, 0, st, arena, dp.lv_fr[l], gsrc, Xp, Wp);
+    else
+        hipLaunchKernelGGL((k_fwd_vec1<T, 32>), g, b, 0, st, arena, dp.lv_fr[l], gsrc, Xp, Wp);
+}
+template <typename T>
+static void launch_bwd_vec1(const DevPlan &dp, int l, const cx<T> *arena, cx<T> *Xp, hipStream_t st) {
+    int R = 1;
+    while (R < dp.pmax_all[l]) R <<= 1;
+    const int per_lane = (dp.mmax_all[l] + 64 / R - 1) / (64 / R);
+    const dim3 g(dp.n_all[l]), b(64);
+    const size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
+    if (per_lane <= 8)
+        hipLaunchKernelGGL((k_bwd_vec1<T, 8>), g, b, lds, st, arena, dp.lv_fr[l], dp.idx, Xp);
+    else
+        hipLaunchKernelGGL((k_bwd_vec1<T, 16>), g, b, lds, st, arena, dp.lv_fr[l], dp.idx, Xp);
+}
+template <typename T, bool FWD>
+static void launch_vsweep(const DevPlan &dp, int l, const cx<T> *sq, const int *gsrc, const int *rowperm, cx<T> *Xp,
+                          cx<T> *Wp, bool trans, bool cj, hipStream_t st) {
+    const int rows = FWD ? dp.mmax_all[l] : dp.pmax_all[l], cols = FWD ? dp.pmax_all[l] : dp.mmax_all[l];
+    int rw = VS_ROWS<T>;
+    while (rw < 32 && 2 * rw * cols <= 256 * VS_PRE) rw *= 2;
+    const int wgs = (rows + rw - 1) / rw * dp.n_all[l];
+    // fewer than two workgroups per compute unit, each lane past its prefetch: 1024 threads (M solves)
+    const bool wide = !trans && wgs < 512 && rw * cols > 256 * VS_PRE;
+    const dim3 g((rows + rw - 1) / rw, dp.n_all[l]);
+    const size_t lds = sizeof(cx<T>) * (size_t)cols;
+#define VSWEEP(RW, NT)                                                                                               \
+    hipLaunchKernelGGL((k_vsweep<T, FWD, RW, NT>), g, dim3(NT), lds, st, sq, dp.lv_fr[l], dp.vinv_off[l], gsrc,     \
+                       dp.idx, rowperm, Xp, Wp, (int)trans, (int)cj)
+    if (wide) {
+        if (rw == 8) VSWEEP(8, 1024);
+        else if (rw == 16) VSWEEP(16, 1024);
+        else VSWEEP(32, 1024);
+    } else {
+        if (rw == 8) VSWEEP(8, 256);
+        else if (rw == 16) VSWEEP(16, 256);
+        else VSWEEP(32, 256);
+    }
+#undef VSWEEP
 }
 
 // Up to kVecMaxRhs right-hand sides: one pass per frequency and column through
@@ -2593,6 +2890,8 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
         const cx<T> *arena = F.arena + f * F.fstride;
         const cx<T> *sq = vi ? F.vinv + f * dp.vinv_elems : nullptr;
         const int *rowperm = F.rowperm + (int64_t)f * n;
+        // gather table of the front rows: after the row permutation for M solves
+        const int *gsrc = trans ? dp.gsrc_id : F.gsrc + (int64_t)f * pl.fv_rows * 3;
         for (int r = 0; r < nrhs; r++) {
             const int64_t off = ((int64_t)sel.s[z] * nrhs + r) * n;
             {
@@ -2604,17 +2903,8 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                 const DevPlan::LevelCounts &c = dp.counts[l];
                 ProfScope ps(K_FWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
                 size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-                const int maxrc = (dp.mmax_all[l] + kGvRows - 1) / kGvRows;
                 if (vec_split(dp, l) && vi && dp.vinv[l]) {
-                    hipLaunchKernelGGL((k_vgemv<T, VG_FT>), dim3(maxrc, dp.n_all[l]), dim3(256),
-                                       sizeof(cx<T>) * (size_t)dp.pmax_all[l], st, arena,
-                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, dp.children, dp.inv,
-                                       rowperm, Xp, Wp,
-                                       (int)trans, (int)cj);
-                    hipLaunchKernelGGL((k_vgemv<T, VG_FR>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
-                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, dp.children, dp.inv,
-                                       rowperm, Xp, Wp,
-                                       (int)trans, (int)cj);
+                    launch_vsweep<T, true>(dp, l, sq, gsrc, rowperm, Xp, Wp, trans, cj, st);
                 } else if (vec_split(dp, l)) {
                     hipLaunchKernelGGL(k_fwd_vec_tri<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
                                        dp.lv_all[l], dp.children, dp.inv, rowperm, Xp, Wp, (int)trans, (int)cj);
@@ -2622,6 +2912,8 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                     hipLaunchKernelGGL(k_fwd_vec_rect<T>, dim3(chunks, dp.n_all[l]), dim3(256),
                                        sizeof(cx<T>) * (size_t)dp.pmax_all[l], st, arena, dp.fronts, dp.lv_all[l], Xp,
                                        Wp, (int)trans, (int)cj);
+                } else if (dp.single[l] && !trans) {
+                    launch_fwd_vec1<T>(dp, l, arena, gsrc, Xp, Wp, st);
                 } else {
                     if (dp.single[l])
                         hipLaunchKernelGGL((k_fwd_vec<T, true>), dim3(dp.n_all[l]), dim3(256), 2 * lds, st, arena,
@@ -2638,17 +2930,8 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                 const DevPlan::LevelCounts &c = dp.counts[l];
                 ProfScope ps(K_BWD_VEC, st, 8.0 * c.tri_elems, (c.tri_elems + 2.0 * c.sum_m) * es);
                 size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-                const int maxrc = (dp.mmax_all[l] + kGvRows - 1) / kGvRows;
                 if (vec_split(dp, l) && vi && dp.vinv[l]) {
-                    hipLaunchKernelGGL((k_vgemv<T, VG_BR>), dim3(maxrc, dp.n_all[l]), dim3(256),
-                                       sizeof(cx<T>) * (size_t)dp.mmax_all[l], st, arena,
-                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, dp.children, dp.inv,
-                                       rowperm, Xp, Wp,
-                                       (int)trans, (int)cj);
-                    hipLaunchKernelGGL((k_vgemv<T, VG_BT>), dim3(maxrc, dp.n_all[l]), dim3(256), 0, st, arena,
-                                       sq, dp.fronts, dp.lv_all[l], dp.vinv_off[l], dp.idx, dp.children, dp.inv,
-                                       rowperm, Xp, Wp,
-                                       (int)trans, (int)cj);
+                    launch_vsweep<T, false>(dp, l, sq, gsrc, rowperm, Xp, Wp, trans, cj, st);
                 } else if (vec_split(dp, l)) {
                     const int chunks = (dp.pmax_all[l] + kVecSplitRows - 1) / kVecSplitRows;
                     hipLaunchKernelGGL(k_bwd_vec_rect<T>, dim3(chunks, dp.n_all[l]), dim3(256), lds, st, arena,
@@ -2656,6 +2939,8 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                     hipLaunchKernelGGL(k_bwd_vec_tri<T>, dim3(dp.n_all[l]), dim3(256),
                                        sizeof(cx<T>) * (size_t)dp.pmax_all[l], st, arena, dp.fronts, dp.lv_all[l],
                                        rowperm, Xp, (int)trans, (int)cj);
+                } else if (dp.single[l] && !trans) {
+                    launch_bwd_vec1<T>(dp, l, arena, Xp, st);
                 } else {
                     hipLaunchKernelGGL(k_bwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
                                        dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj, dp.single[l]);
