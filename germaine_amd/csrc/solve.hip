@@ -1577,66 +1577,64 @@ __device__ __forceinline__ cx<T> lane_bcast(cx<T> v, int l) {
     }
 }
 
-// forward: lane i gathers front row i (i + 64, ... on later passes) from the
-// gather table -- X position, up to two children's contribution rows in W,
-// after the front's row permutation -- and sums its row's p products; the
-// pivot values w1[t] (t < p <= 32, held by lane t of the first pass) reach
-// every lane by readlane; PM >= p bounds the operator registers (levels of
-// narrow fronts keep more waves resident).  grid: fronts, 64 threads
+// forward: a wave takes 64 rows of a front (grid.y: the row passes), lane i
+// gathers front row i from the gather table -- X position, up to two
+// children's contribution rows in W, after the front's row permutation -- and
+// sums its row's p products; the pivot values w1[t] (t < p <= 32, gathered by
+// lane t of every wave) reach every lane by readlane; PM >= p bounds the
+// operator registers (levels of narrow fronts keep more waves resident).
+// grid: (fronts, row passes), 64 threads
 template <typename T, int PM>
 __global__ void __launch_bounds__(64) k_fwd_vec1(const cx<T> *__restrict__ arena, const FrontDev *__restrict__ lvF,
                                                  const int *__restrict__ gsrc, cx<T> *__restrict__ X,
                                                  cx<T> *__restrict__ W) {
     const FrontDev F = lvF[blockIdx.x];
-    const int m = F.m, p = F.p, lane = threadIdx.x;
+    const int m = F.m, p = F.p, lane = threadIdx.x, i = blockIdx.y * 64 + lane;
+    if ((int)blockIdx.y * 64 >= m) return;
     const cx<T> *G = arena + F.buf_off;
     const int *gs = gsrc + (int64_t)F.fv_off * 3;
-    auto gather = [&](int i) { return i < m ? gather3(gs + 3 * i, X, W) : cx<T>{T(0), T(0)}; };
+    auto gather = [&](int r) { return r < m ? gather3(gs + 3 * r, X, W) : cx<T>{T(0), T(0)}; };
+    // the operator rows are in flight during the gathers; rows past m read row
+    // m - 1 (in bounds) and are not stored
     cx<T> a[PM];
-    // rows past m read row m - 1 (in bounds) and are not stored
-    auto fetch = [&](int i) {
 #pragma unroll
-        for (int t = 0; t < PM; t++)
-            if (t < p) a[t] = G[min(i, m - 1) + (int64_t)t * m];
-    };
-    fetch(lane);   // the first pass's operator rows are in flight during the gather
+    for (int t = 0; t < PM; t++)
+        if (t < p) a[t] = G[min(i, m - 1) + (int64_t)t * m];
     const cx<T> w0 = gather(lane);
-#pragma unroll 1
-    for (int i0 = 0; i0 < m; i0 += 64) {
-        const int i = i0 + lane;
-        if (i0 > 0) fetch(i);
-        const cx<T> wi = i0 == 0 ? w0 : gather(i);
+    const cx<T> wi = blockIdx.y == 0 ? w0 : gather(i);
 #pragma unroll
-        for (int t = 0; t < PM; t++)   // row i < p of inv(L11): unit diagonal, zero above it
-            if (t < p && i < p && t >= i) a[t] = {T(t == i), T(0)};
-        cx<T> acc = {T(0), T(0)};
+    for (int t = 0; t < PM; t++)   // row i < p of inv(L11): unit diagonal, zero above it
+        if (t < p && i < p && t >= i) a[t] = {T(t == i), T(0)};
+    cx<T> acc = {T(0), T(0)};
 #pragma unroll
-        for (int t = 0; t < PM; t++)
-            if (t < p) acc = cadd(acc, cmul(a[t], lane_bcast(w0, t)));
-        if (i < m) {
-            if (i < p) X[F.pos0 + i] = acc;
-            else W[F.fv_off + i] = csub(wi, acc);
-        }
+    for (int t = 0; t < PM; t++)
+        if (t < p) acc = cadd(acc, cmul(a[t], lane_bcast(w0, t)));
+    if (i < m) {
+        if (i < p) X[F.pos0 + i] = acc;
+        else W[F.fv_off + i] = csub(wi, acc);
     }
 }
 
-// backward: lanes are R = (p rounded up to a power of two) pivot rows times
-// 64 / R column parts, part q summing the columns q, q + 64/R, ... in chunks
-// of CH per lane; the values [y1; x2] (own pivot rows, then the ancestors' rows
-// at the contribution positions) are staged in LDS while the first chunk of the
-// operator is in flight.  grid: fronts, 64 threads; LDS: m entries
-template <typename T, int CH>
-__global__ void __launch_bounds__(64) k_bwd_vec1(const cx<T> *__restrict__ arena, const FrontDev *__restrict__ lvF,
-                                                 const int *__restrict__ idx, cx<T> *__restrict__ X) {
+// backward: the NW waves of a workgroup share a front; its threads are R = (p
+// rounded up to a power of two) pivot rows times 64 NW / R column parts, part q
+// summing the columns q, q + 64 NW / R, ... in chunks of CH per thread; the
+// values [y1; x2] (own pivot rows, then the ancestors' rows at the contribution
+// positions) are staged in LDS while the first chunk of the operator is in
+// flight.  A row's parts meet by xor-shuffles within a wave, then across the
+// waves in LDS.  grid: fronts; LDS: m entries
+template <typename T, int CH, int NW>
+__global__ void __launch_bounds__(64 * NW) k_bwd_vec1(const cx<T> *__restrict__ arena,
+                                                      const FrontDev *__restrict__ lvF, const int *__restrict__ idx,
+                                                      cx<T> *__restrict__ X) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
     const FrontDev F = lvF[blockIdx.x];
-    const int m = F.m, p = F.p, lane = threadIdx.x;
+    const int m = F.m, p = F.p, tid = threadIdx.x;
     if (p == 0) return;
     const cx<T> *G = arena + F.buf_off;
     int lr = 0;
     while ((1 << lr) < p) lr++;
-    const int parts = 64 >> lr, i = lane & ((1 << lr) - 1), q = lane >> lr, step = CH * parts;
+    const int parts = (64 * NW) >> lr, i = tid & ((1 << lr) - 1), q = tid >> lr, step = CH * parts;
     // rows i >= p (up to the power of two) read row p - 1 in bounds; their sums meet only each other
     const int ic = min(i, p - 1);
     cx<T> a[CH];
@@ -1644,11 +1642,11 @@ __global__ void __launch_bounds__(64) k_bwd_vec1(const cx<T> *__restrict__ arena
 #pragma unroll
         for (int u = 0; u < CH; u++) a[u] = G[ic + (int64_t)min(t0 + u * parts, m - 1) * m];
     };
-    fetch(q);   // the first chunk is in flight while the values are gathered
-    for (int t = lane; t < m; t += 64) sv[t] = X[t < p ? F.pos0 + t : idx[F.idx_off + t]];
+    fetch(q);
+    for (int t = tid; t < m; t += 64 * NW) sv[t] = X[t < p ? F.pos0 + t : idx[F.idx_off + t]];
     __syncthreads();
     cx<T> acc = {T(0), T(0)};
-    for (int t0 = q;;) {
+    for (int t0 = q; t0 < m;) {
 #pragma unroll
         for (int u = 0; u < CH; u++) {
             // row i of inv(U11) is zero left of the diagonal; -inv(U11) U12 beyond p
@@ -1659,14 +1657,24 @@ __global__ void __launch_bounds__(64) k_bwd_vec1(const cx<T> *__restrict__ arena
             }
         }
         t0 += step;
-        if (t0 >= m) break;
-        fetch(t0);
+        if (t0 < m) fetch(t0);
     }
     for (int o = 1 << lr; o < 64; o <<= 1) {
         acc.x += __shfl_xor(acc.x, o);
         acc.y += __shfl_xor(acc.y, o);
     }
-    if (q == 0 && i < p) X[F.pos0 + i] = acc;
+    if constexpr (NW > 1) {
+        __shared__ cx<T> red[NW][32];
+        if ((tid & 63) < (1 << lr)) red[tid >> 6][tid & 63] = acc;
+        __syncthreads();
+        if (tid < p) {
+            cx<T> s = red[0][tid];
+            for (int w = 1; w < NW; w++) s = cadd(s, red[w][tid]);
+            X[F.pos0 + tid] = s;
+        }
+    } else {
+        if (q == 0 && i < p) X[F.pos0 + i] = acc;
+    }
 }
 
 // ---- single-vector sweeps of large fronts, split across workgroups ------------
@@ -2826,7 +2834,7 @@ template <typename T>
 static void launch_fwd_vec1(const DevPlan &dp, int l, const cx<T> *arena, const int *gsrc, cx<T> *Xp, cx<T> *Wp,
                             hipStream_t st) {
     const int pm = dp.pmax_all[l];
-    const dim3 g(dp.n_all[l]), b(64);
+    const dim3 g(dp.n_all[l], (dp.mmax_all[l] + 63) / 64), b(64);
     if (pm <= 8)
         hipLaunchKernelGGL((k_fwd_vec1<T, 8>), g, b, 0, st, arena, dp.lv_fr[l], gsrc, Xp, Wp);
     else if (pm <= 16)
@@ -2838,13 +2846,22 @@ template <typename T>
 static void launch_bwd_vec1(const DevPlan &dp, int l, const cx<T> *arena, cx<T> *Xp, hipStream_t st) {
     int R = 1;
     while (R < dp.pmax_all[l]) R <<= 1;
-    const int per_lane = (dp.mmax_all[l] + 64 / R - 1) / (64 / R);
-    const dim3 g(dp.n_all[l]), b(64);
+    // waves per front: as many as keep a thread's columns within one chunk of 16
+    int nw = 1;
+    while (nw < 4 && (dp.mmax_all[l] + 64 * nw / R - 1) / (64 * nw / R) > 16) nw *= 2;
+    const int per_thread = (dp.mmax_all[l] + 64 * nw / R - 1) / (64 * nw / R);
+    const dim3 g(dp.n_all[l]);
     const size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-    if (per_lane <= 8)
-        hipLaunchKernelGGL((k_bwd_vec1<T, 8>), g, b, lds, st, arena, dp.lv_fr[l], dp.idx, Xp);
-    else
-        hipLaunchKernelGGL((k_bwd_vec1<T, 16>), g, b, lds, st, arena, dp.lv_fr[l], dp.idx, Xp);
+#define BWD1(CH, NW) hipLaunchKernelGGL((k_bwd_vec1<T, CH, NW>), g, dim3(64 * NW), lds, st, arena, dp.lv_fr[l], dp.idx, Xp)
+    if (nw == 1) {
+        if (per_thread <= 8) BWD1(8, 1);
+        else BWD1(16, 1);
+    } else if (nw == 2) {
+        BWD1(16, 2);
+    } else {
+        BWD1(16, 4);
+    }
+#undef BWD1
 }
 template <typename T, bool FWD>
 static void launch_vsweep(const DevPlan &dp, int l, const cx<T> *sq, const int *gsrc, const int *rowperm, cx<T> *Xp,
