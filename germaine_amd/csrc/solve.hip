@@ -1594,17 +1594,17 @@ __global__ void __launch_bounds__(64) k_fwd_vec1(const cx<T> *__restrict__ arena
     const cx<T> *G = arena + F.buf_off;
     const int *gs = gsrc + (int64_t)F.fv_off * 3;
     auto gather = [&](int r) { return r < m ? gather3(gs + 3 * r, X, W) : cx<T>{T(0), T(0)}; };
-    // the operator rows are in flight during the gathers; rows past m read row
-    // m - 1 (in bounds) and are not stored
+    // the operator rows are in flight during the gathers; row i < p of
+    // inv(L11) has a unit diagonal and zeros above it, which are not read;
+    // rows past m read row m - 1 (in bounds) and are not stored
     cx<T> a[PM];
 #pragma unroll
-    for (int t = 0; t < PM; t++)
-        if (t < p) a[t] = G[min(i, m - 1) + (int64_t)t * m];
+    for (int t = 0; t < PM; t++) {
+        a[t] = {T(t == i), T(0)};
+        if (t < p && (i >= p || t < i)) a[t] = G[min(i, m - 1) + (int64_t)t * m];
+    }
     const cx<T> w0 = gather(lane);
     const cx<T> wi = blockIdx.y == 0 ? w0 : gather(i);
-#pragma unroll
-    for (int t = 0; t < PM; t++)   // row i < p of inv(L11): unit diagonal, zero above it
-        if (t < p && i < p && t >= i) a[t] = {T(t == i), T(0)};
     cx<T> acc = {T(0), T(0)};
 #pragma unroll
     for (int t = 0; t < PM; t++)
@@ -1635,12 +1635,17 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_vec1(const cx<T> *__restrict__ 
     int lr = 0;
     while ((1 << lr) < p) lr++;
     const int parts = (64 * NW) >> lr, i = tid & ((1 << lr) - 1), q = tid >> lr, step = CH * parts;
-    // rows i >= p (up to the power of two) read row p - 1 in bounds; their sums meet only each other
+    // rows i >= p (up to the power of two) read row p - 1 in bounds, their sums
+    // meet only each other; inv(U11)'s zeros left of the diagonal are not read
     const int ic = min(i, p - 1);
     cx<T> a[CH];
     auto fetch = [&](int t0) {
 #pragma unroll
-        for (int u = 0; u < CH; u++) a[u] = G[ic + (int64_t)min(t0 + u * parts, m - 1) * m];
+        for (int u = 0; u < CH; u++) {
+            const int t = min(t0 + u * parts, m - 1);
+            a[u] = {T(0), T(0)};
+            if (t >= ic) a[u] = G[ic + (int64_t)t * m];
+        }
     };
     fetch(q);
     for (int t = tid; t < m; t += 64 * NW) sv[t] = X[t < p ? F.pos0 + t : idx[F.idx_off + t]];
