@@ -2269,13 +2269,13 @@ __global__ void __launch_bounds__(256) k_ir_in(const cx<double> *__restrict__ B,
 // stride ld).  Xa/Xb and D/R must be distinct buffers: other workgroups read a
 // point's x while its own workgroup writes.  Row k of M has the entries of CSC
 // column k (the pattern is symmetric), all within the 3x3 neighbourhood.
-// Tiles of kIrTx x kIrTy points (16 consecutive points of two grid rows): per
-// 16-column chunk the x of the 4 x (kIrTx + 2) neighbourhood points is staged
-// in LDS once (each point's x is used by up to nine rows; two rows per tile
-// stage 2.25 points per output instead of 3.4 with one), then one thread per
-// (point, column) applies the stencil from LDS.  grid (blocks, slots),
+// Tiles of kIrTx x kIrTy points (8 consecutive points of four grid rows): per
+// 16-column chunk the x of the (kIrTy + 2) x (kIrTx + 2) neighbourhood points is
+// staged in LDS once (each point's x is used by up to nine rows; an 8 x 4 tile
+// stages 1.9 points per output, 16 x 2 stages 2.25, measured 2% slower), then
+// one thread per (point, column) applies the stencil from LDS.  grid (blocks, slots),
 // grid-stride over tiles.
-constexpr int kIrTx = 16, kIrTy = 2, kIrCw = 16, kIrPts = kIrTx * kIrTy, kIrStage = (kIrTy + 2) * (kIrTx + 2);
+constexpr int kIrTx = 8, kIrTy = 4, kIrCw = 16, kIrPts = kIrTx * kIrTy, kIrStage = (kIrTy + 2) * (kIrTx + 2);
 constexpr int kIrThreads = kIrPts * kIrCw, kIrWaves = kIrThreads / 64;
 constexpr int kIrCoef0 = (kIrStage + 63) / 64 * 64;   // first thread of the stencil-entry gather
 static_assert(kIrStage <= kIrCoef0 && kIrCoef0 + 9 * kIrPts <= kIrThreads,
