@@ -534,6 +534,44 @@ def test_refactor_graph_replays_match_eager(precision):
             assert rel(X[z], Xr.T) <= 1e-10
 
 
+@pytest.mark.parametrize("precision", [G.FP64, G.FP32])
+def test_single_vector_wide_levels_match_panel_path(precision):
+    """A grid whose top separators exceed 256 rows, so the single-vector
+    backward products of its top levels take the 1024-thread workgroups and
+    the lower multi-block levels the 16- and 32-row groups: single right-hand
+    sides (the one-wave and split-product kernels) against the same columns
+    solved as one panel of five (the multi-right-hand-side kernels, checked
+    against the oracle elsewhere) on the same factors, for M, M^H and M^T,
+    and the FP64 backward error of the single-vector solutions.  FP64: 1e-10
+    apart, backward error 1e-11 (max norms; measured 4e-13 to 8e-13); FP32
+    factors with FP64 refinement to 1e-12: 1e-8 apart, backward error 1e-10."""
+    nx, ny = 601, 301
+    say("single-vector wide levels", precision)
+    m1, m2 = smooth_model(nx, ny, seed=21)
+    pl = fdfd.Plan(nx, ny)
+    assert pl.info.max_pivots > 256
+    vals = pl.assemble(G.AC, 10.0, [9.0], dev(m1), dev(m2), npml=10)
+    F = pl.factor(vals, precision)
+    rng = np.random.default_rng(8)
+    B = dev(rng.standard_normal((5, pl.n)) + 1j * rng.standard_normal((5, pl.n)))
+    for sysno in (0, 1, 2):
+        if precision == G.FP64:
+            Xp = F.solve(0, B, sys=sysno)
+            Xv = [F.solve(0, B[c:c + 1], sys=sysno) for c in (0, 3)]
+            bar, bbar = 1e-10, 1e-11
+        else:
+            Xp = F.solve_refine(0, B, sys=sysno, tol=1e-12, max_iter=30)[0]
+            Xv = [F.solve_refine(0, B[c:c + 1], sys=sysno, tol=1e-12, max_iter=30)[0] for c in (0, 3)]
+            bar, bbar = 1e-8, 1e-10
+        for c, X in zip((0, 3), Xv):
+            e = rel(X[0].cpu().numpy(), Xp[c].cpu().numpy())
+            R = pl.residual(vals[0], B[c:c + 1], X, sys=sysno)
+            be = float(R.abs().max() / B[c].abs().max())
+            say("  sys", sysno, "column", c, "vs panel", e, "backward error", be)
+            assert e <= bar
+            assert be <= bbar
+
+
 @pytest.mark.parametrize("matrix", ["helmholtz", "random"])
 @pytest.mark.parametrize("precision", [G.FP64, G.FP32])
 def test_single_vector_inverse_path_matches_oracle(precision, matrix):
