@@ -85,6 +85,17 @@ template <typename T>
 __device__ __forceinline__ cx<T> cmsub(cx<T> c, cx<T> a, cx<T> b) {
     return {c.x - (a.x * b.x - a.y * b.y), c.y - (a.x * b.y + a.y * b.x)};
 }
+// a*b and c - a*b with every fused multiply-add written out, so their rounding
+// does not depend on how the compiler contracts the expression where it is
+// inlined: code that two kernels must run bit-identically uses these
+template <typename T>
+__device__ __forceinline__ cx<T> cmul_f(cx<T> a, cx<T> b) {
+    return {fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cmsub_f(cx<T> c, cx<T> a, cx<T> b) {
+    return {fma(-a.x, b.x, fma(a.y, b.y, c.x)), fma(-a.x, b.y, fma(-a.y, b.x, c.y))};
+}
 template <typename T>
 __device__ __forceinline__ cx<T> cconjif(cx<T> a, bool c) {
     return c ? cx<T>{a.x, -a.y} : a;

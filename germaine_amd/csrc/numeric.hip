@@ -491,51 +491,100 @@ __device__ int block_argmax(T best, int bi, T *sb, int *si) {
 // ---------------------------------------------------------------------------
 // Partial LU of an m x m front held column-major in LDS (p pivots, row
 // pivoting within the p fully-summed rows); the swap of step j goes to piv[j].
-// All threads of the workgroup call it.
+// All threads of the workgroup call it (at least one wave); sp holds kLuNb
+// ints of LDS.  Blocked by kLuNb pivots: wave 0 alone factors a panel of kLuNb
+// columns -- pivot search, row swap within the panel, scaling and the rank-1
+// update of the panel's columns, lanes over rows, with no barrier (a wave's
+// LDS operations complete in order) -- then the workgroup applies the panel's
+// swaps to the other columns, forms the panel's rows of U12 by forward
+// substitution with the panel's unit lower block, and makes one trailing update
+// A22 -= L21 U12 of depth kLuNb with 2 x 2 output tiles per thread, so the
+// trailing block passes through LDS once per panel instead of once per pivot.
+// Complex products use the explicit fused multiply-adds (cmul_f, cmsub_f): the
+// persistent subtree kernel and the level path inline this function and must
+// round alike.
+constexpr int kLuNb = 8;
 template <typename T>
-__device__ void small_lu_lds(cx<T> *S, int m, int p, int *__restrict__ piv, int *__restrict__ sing, cx<T> *pv,
-                             int *si) {
-    const int tid = threadIdx.x, bs = blockDim.x;
-    for (int j = 0; j < p; j++) {
-        // pivot search by wave 0 alone (p <= 64 candidates: one per lane)
+__device__ void small_lu_lds(cx<T> *S, int m, int p, int *__restrict__ piv, int *__restrict__ sing, int *sp) {
+    const int tid = threadIdx.x, bs = blockDim.x, lane = tid & 63;
+    for (int jb = 0; jb < p; jb += kLuNb) {
+        const int je = min(p, jb + kLuNb);   // panel columns [jb, je)
         if (tid < 64) {
-            T best = T(-1);
-            int bi = j;
-            for (int r = j + tid; r < p; r += 64) better(best, bi, cabs1(S[r + j * m]), r);
-            for (int off = 32; off > 0; off >>= 1) {
-                const T ob = __shfl_xor(best, off);
-                const int oi = __shfl_xor(bi, off);
-                better(best, bi, ob, oi);
-            }
-            if (tid == 0) {
-                si[0] = bi;
-                pv[0] = S[bi + j * m];
-                pv[1] = S[j + j * m];
-                piv[j] = bi;
-                if (best == T(0)) atomicOr(sing, 1);
+            for (int j = jb; j < je; j++) {
+                // pivot: largest |.|_1 over rows [j, p), ties to the lower row
+                T best = T(-1);
+                int bi = j;
+                for (int r = j + lane; r < p; r += 64) better(best, bi, cabs1(S[r + j * m]), r);
+                for (int off = 32; off > 0; off >>= 1) {
+                    const T ob = __shfl_xor(best, off);
+                    const int oi = __shfl_xor(bi, off);
+                    better(best, bi, ob, oi);
+                }
+                if (lane == 0) {
+                    sp[j - jb] = bi;
+                    piv[j] = bi;
+                    if (best == T(0)) atomicOr(sing, 1);
+                }
+                // swap rows j and bi in the panel's columns
+                if (bi != j && jb + lane < je) {
+                    const int c = jb + lane;
+                    const cx<T> t = S[j + c * m];
+                    S[j + c * m] = S[bi + c * m];
+                    S[bi + c * m] = t;
+                }
+                const cx<T> d = crecip(S[j + j * m]);
+                for (int r = j + 1 + lane; r < m; r += 64) {
+                    const cx<T> l = cmul_f(S[r + j * m], d);
+                    S[r + j * m] = l;
+                    for (int c = j + 1; c < je; c++) S[r + c * m] = cmsub_f(S[r + c * m], l, S[j + c * m]);
+                }
             }
         }
         __syncthreads();
-        // row swap in the other columns and the scaled column j, in one pass:
-        // column j's swapped pair comes from pv, read before anyone wrote
-        const int pr = si[0];
-        const cx<T> d = crecip(pv[0]);
-        if (pr != j)
-            for (int c = tid; c < m; c += bs) {
-                if (c == j) continue;
-                cx<T> t = S[j + c * m];
-                S[j + c * m] = S[pr + c * m];
-                S[pr + c * m] = t;
+        // the panel's swaps, in order, on the columns outside it
+        for (int c = tid; c < m; c += bs) {
+            if (c >= jb && c < je) continue;
+            for (int j = jb; j < je; j++) {
+                const int r = sp[j - jb];
+                if (r != j) {
+                    const cx<T> t = S[j + c * m];
+                    S[j + c * m] = S[r + c * m];
+                    S[r + c * m] = t;
+                }
             }
-        for (int r = j + 1 + tid; r < m; r += bs) S[r + j * m] = cmul(r == pr ? pv[1] : S[r + j * m], d);
-        if (tid == 0) S[j + j * m] = pv[0];
+        }
         __syncthreads();
-        // rank-1 update: lane -> rows (one per lane up to order 65), waves take
-        // the columns in turn; the row's multiplier stays in a register and the
-        // column's pivot-row entry is one broadcast read
-        for (int r = j + 1 + (tid & 63); r < m; r += 64) {
-            const cx<T> l = S[r + j * m];
-            for (int c = j + 1 + (tid >> 6); c < m; c += bs >> 6) S[r + c * m] = cmsub(S[r + c * m], l, S[j + c * m]);
+        if (je >= m) break;
+        // U12 rows [jb, je) of columns [je, m): one thread per column, rows in order
+        for (int c = je + tid; c < m; c += bs)
+            for (int i = jb + 1; i < je; i++) {
+                cx<T> a = S[i + c * m];
+                for (int t = jb; t < i; t++) a = cmsub_f(a, S[i + t * m], S[t + c * m]);
+                S[i + c * m] = a;
+            }
+        __syncthreads();
+        // trailing update of rows and columns [je, m): 2 x 2 tiles, the tile's
+        // rows fastest across threads
+        const int q = m - je, qt = (q + 1) >> 1;
+        for (int tt = tid; tt < qt * qt; tt += bs) {
+            const int r0 = je + 2 * (tt % qt), c0 = je + 2 * (tt / qt);
+            const bool r1 = r0 + 1 < m, c1 = c0 + 1 < m;
+            cx<T> a00 = S[r0 + c0 * m], a10 = r1 ? S[r0 + 1 + c0 * m] : cx<T>{T(0), T(0)};
+            cx<T> a01 = c1 ? S[r0 + (c0 + 1) * m] : cx<T>{T(0), T(0)};
+            cx<T> a11 = (r1 && c1) ? S[r0 + 1 + (c0 + 1) * m] : cx<T>{T(0), T(0)};
+#pragma unroll 2
+            for (int t = jb; t < je; t++) {
+                const cx<T> l0 = S[r0 + t * m], l1 = r1 ? S[r0 + 1 + t * m] : cx<T>{T(0), T(0)};
+                const cx<T> u0 = S[t + c0 * m], u1 = c1 ? S[t + (c0 + 1) * m] : cx<T>{T(0), T(0)};
+                a00 = cmsub_f(a00, l0, u0);
+                a10 = cmsub_f(a10, l1, u0);
+                a01 = cmsub_f(a01, l0, u1);
+                a11 = cmsub_f(a11, l1, u1);
+            }
+            S[r0 + c0 * m] = a00;
+            if (r1) S[r0 + 1 + c0 * m] = a10;
+            if (c1) S[r0 + (c0 + 1) * m] = a01;
+            if (r1 && c1) S[r0 + 1 + (c0 + 1) * m] = a11;
         }
         __syncthreads();
     }
@@ -547,8 +596,7 @@ __global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int
                                                   int *__restrict__ piv, int n, int *__restrict__ sing) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *S = reinterpret_cast<cx<T> *>(smem_raw);
-    __shared__ int si[1];
-    __shared__ cx<T> pv[2];   // pivot entry and the entry it swaps with, published with the pivot row
+    __shared__ int sp[kLuNb];   // the current panel's pivot rows
     const FrontDev F = fronts[list[blockIdx.x]];
     const int fq = blockIdx.y;
     cx<T> *G = arena + fq * fstride + F.buf_off;
@@ -556,7 +604,7 @@ __global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int
 #pragma unroll 4
     for (int e = tid; e < mm; e += bs) S[e] = G[e];
     __syncthreads();
-    small_lu_lds<T>(S, m, F.p, piv + (int64_t)fq * n + F.pos0, sing + fq, pv, si);
+    small_lu_lds<T>(S, m, F.p, piv + (int64_t)fq * n + F.pos0, sing + fq, sp);
     for (int e = tid; e < mm; e += bs) G[e] = S[e];
 }
 
@@ -590,8 +638,7 @@ __global__ void __launch_bounds__(256) k_subtree_factor(cx<T> *__restrict__ aren
     __shared__ int64_t cbase[kAsmMaxChildren];
     __shared__ int cstride[kAsmMaxChildren];
     __shared__ int task;
-    __shared__ int si[1];
-    __shared__ cx<T> pv[2];
+    __shared__ int sp[kLuNb];   // the current panel's pivot rows
     const int tid = threadIdx.x, ntask = nroots * nf;
     for (;;) {
         if (tid == 0) task = atomicAdd(counter, 1);
@@ -633,7 +680,7 @@ __global__ void __launch_bounds__(256) k_subtree_factor(cx<T> *__restrict__ aren
                 d.y += (T)v.y;
             }
             __syncthreads();
-            small_lu_lds<T>(S, m, F.p, piv + (int64_t)fq * n + F.pos0, sing + fq, pv, si);
+            small_lu_lds<T>(S, m, F.p, piv + (int64_t)fq * n + F.pos0, sing + fq, sp);
             cx<T> *G = A + F.buf_off;
             for (int e = tid; e < mm; e += blockDim.x) G[e] = S[e];
             __syncthreads();   // the parent, later in this loop, reads this block
