@@ -427,6 +427,163 @@ __global__ void __launch_bounds__(4 * kFwdRT) k_fwd_blk(const cx<T> *__restrict_
     sub_tile_from<T>(w, nrhs, row0, m, r0, wold, acc);
 }
 
+// Y1[0:nb1] -= L Y0[0:kb] in LDS (L: nb1 x kb in the diagonal block's layout,
+// row i at L[i * (BK + 1)]): waves 0-3 own 16 rows x 32 right-hand sides each,
+// as in diag_apply; ends with a barrier
+template <typename T>
+__device__ __forceinline__ void blk_couple(const cx<T> *L, const T *Y0r, const T *Y0i, T *Y1r, T *Y1i, int kb, int nb1,
+                                           int ncol) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rw = (w >> 1) * 16, wc = (w & 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    cacc3<T> c[2];
+    for (int b = 0; b < 2; b++) c[b].zero();
+    const bool act = w < 4;
+    const int kend = act ? (kb + 3) & ~3 : 0;
+    for (int kk = 0; kk < kend; kk += 4) {
+        const int k = kk + lk;
+        const cx<T> a = L[(rw + lr) * (BK + 1) + k];
+        for (int b = 0; b < 2; b++)
+            if (wc + b * 16 < ncol) {
+                const T br = Y0r[k * 64 + wc + b * 16 + lr], bi = Y0i[k * 64 + wc + b * 16 + lr];
+                c[b].mma(a.x, a.y, br, bi, br + bi);
+            }
+    }
+    if (act)
+        for (int b = 0; b < 2; b++)
+            for (int v = 0; v < 4; v++) {
+                const int row = rw + mfma_t<T>::row(lk, v), col = wc + b * 16 + lr;
+                if (row < nb1) {
+                    Y1r[row * 64 + col] -= c[b].re(v);
+                    Y1i[row * 64 + col] -= c[b].im(v);
+                }
+            }
+    __syncthreads();
+}
+
+// forward blocks b0 and b0 + 32 of every front of a level in one launch:
+// y_0 = Dop_0 w_0, w_1 -= Lop[b1, b0] y_0, y_1 = Dop_1 w_1 (both written to X
+// by the row-tile-0 workgroup), then rows [b0 + 64, m) of W -= Lop[:, b0:b0+64]
+// [y_0; y_1] as one depth-64 update -- the rows below pass through memory once
+// per two blocks instead of once per block.  Every operand but the second block
+// column of Lop is requested before the first barrier; that one is loaded
+// during the first column's products.  A front with a single block left takes
+// the first half alone.  LDS: [Y1 | Y0 | A tile], the A tile's space holding
+// Dop_0, Lop[b1, b0] and Dop_1 in turn.  grid: (row tiles, fronts, rhs tiles)
+template <typename T>
+constexpr size_t blk2_smem_bytes() {
+    return sizeof(T) * 2 * BK * 64 + blk_smem_bytes<T, kFwdRT>();
+}
+template <typename T>
+__global__ void __launch_bounds__(4 * kFwdRT) k_fwd_blk2(const cx<T> *__restrict__ arena, int64_t fstride,
+                                                  const FrontDev *__restrict__ fronts, const int *__restrict__ list,
+                                                  cx<T> *__restrict__ X, cx<T> *__restrict__ W, int n, int64_t wstride,
+                                                  int nrhs, int rt, int b0, int trans, int cj, Sel sel) {
+    constexpr int RT = kFwdRT, NT = 4 * RT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    BlkSmem<T, RT> s0(smem_raw + sizeof(T) * 2 * BK * 64);   // Y0 and the A tile
+    BlkSmem<T, RT> s1 = s0;                                   // Y1 and the same A tile
+    s1.Yr = reinterpret_cast<T *>(smem_raw);
+    s1.Yi = s1.Yr + BK * 64;
+    const FrontDev F = fronts[list[blockIdx.y]];
+    if (F.p <= b0) return;
+    const int m = F.m, nb0 = min(BK, F.p - b0), nb1 = max(0, min(BK, F.p - b0 - BK)), tid = threadIdx.x;
+    const int row0 = b0 + nb0 + nb1 + blockIdx.x * RT;
+    if (blockIdx.x > 0 && row0 >= m) return;
+    const bool upd = row0 < m;
+    const int z = blockIdx.z / rt, r0 = (blockIdx.z - z * rt) * 64, ncol = min(64, nrhs - r0);
+    const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
+    X += (int64_t)sel.s[z] * n * nrhs;
+    cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
+    auto Lop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
+    cx<T> wold[2][2][4];
+    if (upd) load_tile<T>(w, nrhs, row0, m, r0, wold);
+    cx<T> d0[BK * BK / NT], d1[BK * BK / NT], l10[BK * BK / NT], y0v[BK * 64 / NT], y1v[BK * 64 / NT],
+        av[BK * RT / NT];
+#pragma unroll
+    for (int u = 0; u < BK * BK / NT; u++) {
+        const int e = tid + NT * u, i = e % BK, t = e / BK;
+        d0[u] = (i < nb0 && t < nb0) ? dop(G, m, b0, i, t, true, trans, cj) : cx<T>{T(0), T(0)};
+        d1[u] = (i < nb1 && t < nb1) ? dop(G, m, b0 + BK, i, t, true, trans, cj) : cx<T>{T(0), T(0)};
+        l10[u] = (i < nb1 && t < nb0) ? Lop(b0 + BK + i, b0 + t) : cx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int u = 0; u < BK * 64 / NT; u++) {
+        const int e = tid + NT * u, k = e >> 6, j = e & 63;
+        y0v[u] = (k < nb0 && j < ncol) ? w[(int64_t)(b0 + k) * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+        y1v[u] = (k < nb1 && j < ncol) ? w[(int64_t)(b0 + BK + k) * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+    }
+    auto load_a = [&](int c0, int kb) {   // the A tile of block column c0 (kb columns) into av
+#pragma unroll
+        for (int u = 0; u < BK * RT / NT; u++) {
+            const int e = tid + NT * u;
+            int k, i;
+            if (!trans) { i = e % RT; k = e / RT; }        // column-major L: coalesced along rows
+            else { k = e % BK; i = e / BK; }               // U^T: coalesced along k
+            av[u] = (upd && k < kb && row0 + i < m) ? Lop(row0 + i, c0 + k) : cx<T>{T(0), T(0)};
+        }
+    };
+    auto stage_a = [&]() {
+#pragma unroll
+        for (int u = 0; u < BK * RT / NT; u++) {
+            const int e = tid + NT * u;
+            int k, i;
+            if (!trans) { i = e % RT; k = e / RT; }
+            else { k = e % BK; i = e / BK; }
+            s0.Ar[k * RT + i] = av[u].x;
+            s0.Ai[k * RT + i] = av[u].y;
+        }
+    };
+    auto stage_d = [&](const cx<T>(&v)[BK * BK / NT]) {
+#pragma unroll
+        for (int u = 0; u < BK * BK / NT; u++) {
+            const int e = tid + NT * u;
+            s0.D[(e % BK) * (BK + 1) + e / BK] = v[u];
+        }
+    };
+    load_a(b0, nb0);
+    stage_d(d0);
+#pragma unroll
+    for (int u = 0; u < BK * 64 / NT; u++) {
+        const int e = tid + NT * u;
+        s0.Yr[e] = y0v[u].x;
+        s0.Yi[e] = y0v[u].y;
+        s1.Yr[e] = y1v[u].x;
+        s1.Yi[e] = y1v[u].y;
+    }
+    __syncthreads();
+    diag_apply<T, RT>(s0, nb0, ncol);   // y_0; ends with a barrier
+    if (nb1 > 0) {
+        stage_d(l10);
+        __syncthreads();
+        blk_couple<T>(s0.D, s0.Yr, s0.Yi, s1.Yr, s1.Yi, nb0, nb1, ncol);   // w_1 -= L10 y_0
+        stage_d(d1);
+        __syncthreads();
+        diag_apply<T, RT>(s1, nb1, ncol);   // y_1
+    }
+    if (blockIdx.x == 0)
+        for (int e = tid; e < (nb0 + nb1) * 64; e += NT) {
+            const int k = e >> 6, j = e & 63;
+            if (j < ncol) {
+                const int kk = k < nb0 ? k : k - nb0;
+                const T *yr = k < nb0 ? s0.Yr : s1.Yr, *yi = k < nb0 ? s0.Yi : s1.Yi;
+                X[(int64_t)(F.pos0 + b0 + k) * nrhs + r0 + j] = {yr[kk * 64 + j], yi[kk * 64 + j]};
+            }
+        }
+    if (!upd) return;
+    stage_a();   // the diagonal blocks' space is free (diag_apply ended with a barrier)
+    __syncthreads();
+    if (nb1 > 0) load_a(b0 + BK, nb1);   // in flight during the first column's products
+    cacc3<T> acc[2][2];
+    mma_tile<T, RT>(s0, nb0, ncol, acc);
+    if (nb1 > 0) {
+        __syncthreads();   // the first column's tile is no longer read
+        stage_a();
+        __syncthreads();
+        mma_tile<T, RT>(s1, nb1, ncol, acc, true);
+    }
+    sub_tile_from<T>(w, nrhs, row0, m, r0, wold, acc);
+}
+
 // backward, ancestors: W[f][0:p] = y1 - Uop12 x2, x2 gathered from X at the
 // contribution positions.  The positions are staged in LDS first (after the
 // block tiles, m - p ints), so each 32-column chunk of Uop12 and x2 is one
@@ -2595,6 +2752,8 @@ void set_smem_attrs() {
     const int blk = (int)blk_smem_bytes<T>();
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_blk<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)blk_smem_bytes<T, kFwdRT>()));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_blk2<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)blk2_smem_bytes<T>()));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_blk<T>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, blk));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_contrib<T>),
@@ -2714,14 +2873,24 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
                                wstride, nrhs, in);
         }
         const int pmax = dp.pmax_all[l];
-        for (int b0 = 0; b0 < pmax; b0 += BK) {
-            // factor entries of the block column, and the panel rows below the block read and written
-            const double fe = prof ? c.fwd_blk_elems[b0 / BK] : 0.0;
-            ProfScope ps(K_FWD_BLK, st, nz * 8.0 * nr * fe, nz * (fe + 2.0 * nr * fe / BK) * es);
+        static const bool one_block = std::getenv("GFDFD_FWD_BLK1") != nullptr;
+        for (int b0 = 0; b0 < pmax; b0 += one_block ? BK : 2 * BK) {
+            // factor entries of the block columns, and the panel rows below them read and written
+            const int nbk = one_block || b0 + BK >= pmax ? 1 : 2;
+            double fe = 0.0;
+            if (prof)
+                for (int q = 0; q < nbk; q++) fe += c.fwd_blk_elems[b0 / BK + q];
+            ProfScope ps(K_FWD_BLK, st, nz * 8.0 * nr * fe, nz * (fe + 2.0 * nr * fe / (nbk * BK)) * es);
             const int tiles = std::max(1, (dp.mmax_all[l] - b0 - 1) / kFwdRT + 1);
             const size_t fblk = blk_smem_bytes<T, kFwdRT>();
-            hipLaunchKernelGGL(k_fwd_blk<T>, dim3(tiles, nfr, rt * ns), dim3(4 * kFwdRT), fblk, st, F.arena, F.fstride,
-                               dp.fronts, dp.lv_all[l], Xp, Wp, n, wstride, nrhs, rt, b0, (int)trans, (int)cj, in);
+            if (one_block)
+                hipLaunchKernelGGL(k_fwd_blk<T>, dim3(tiles, nfr, rt * ns), dim3(4 * kFwdRT), fblk, st, F.arena,
+                                   F.fstride, dp.fronts, dp.lv_all[l], Xp, Wp, n, wstride, nrhs, rt, b0, (int)trans,
+                                   (int)cj, in);
+            else
+                hipLaunchKernelGGL(k_fwd_blk2<T>, dim3(tiles, nfr, rt * ns), dim3(4 * kFwdRT), blk2_smem_bytes<T>(), st,
+                                   F.arena, F.fstride, dp.fronts, dp.lv_all[l], Xp, Wp, n, wstride, nrhs, rt, b0,
+                                   (int)trans, (int)cj, in);
         }
     }
     for (int l = pl.nlevels - 1; l >= 0; l--) {
