@@ -1092,108 +1092,260 @@ __global__ void k_gsrc(const FrontDev *__restrict__ fronts, int nfronts, const i
 // forward and backward substitutions -- for M as for M^T -- into one product
 // with no dependent phase:  [y; w2 - L21 y] = [inv(L11); L21 inv(L11)] w1  and
 // x1 = [inv(U11), -inv(U11) U12] [y1; x2].  Computed in FP64 for both factor
-// precisions.  One workgroup per front: waves 0-1 build inv(L_bb) and waves
-// 2-3 inv(U_bb) at the same time, four lanes per column; the products run over
-// all threads, 32 rows or columns at a time.  The block is staged in the
-// factor precision (exact in FP64), the inverses in FP64.  grid (fronts, nfreq).
+// precisions (the block is staged in the factor precision, exact in FP64).
+// One workgroup per front, grid (fronts, nfreq); every global load of the
+// front is issued a phase ahead of its use (the next block, or the first
+// row and column chunk of the rectangles, while a block is inverted).
 // ---------------------------------------------------------------------------
+// every lane takes the value that lane q of its group of four holds (DPP quad permute)
+__device__ __forceinline__ int quad_bcast_i(int v, int q) {
+    switch (q) {
+    case 0: return __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, false);
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, false);
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, false);
+    default: return __builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, false);
+    }
+}
+__device__ __forceinline__ double quad_bcast(double d, int q) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+    const unsigned lo = (unsigned)quad_bcast_i((int)(unsigned)b, q);
+    const unsigned hi = (unsigned)quad_bcast_i((int)(unsigned)(b >> 32), q);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ cx<double> quad_bcast(cx<double> v, int q) {
+    return {quad_bcast(v.x, q), quad_bcast(v.y, q)};
+}
+__device__ __forceinline__ double readlane_d(double d, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T>
+__device__ __forceinline__ cx<double> c64(cx<T> v) {
+    return {(double)v.x, (double)v.y};
+}
+// c + a b with every fused multiply-add written out
+__device__ __forceinline__ cx<double> cmac_f(cx<double> c, cx<double> a, cx<double> b) {
+    return {fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y))};
+}
+
+// Inverses of the NB x NB block D (NB = 8, 16, 24 or 32, the identity beyond the
+// block's order): the unit lower inverse into LI and the upper inverse into UI,
+// each complete with its zeros (LI with its unit diagonal).  Waves 0-1 build
+// LI, waves 2-3 UI at the same time; four lanes per column j, lane part holding
+// rows 4k + part in registers.  Right-looking: at step t the lane holding row
+// t hands x_t to its group of four (DPP), and every lane updates its rows below
+// t (above t for UI) -- no LDS round trip and no barrier inside the chain.
+template <typename T, int NB>
+__device__ __forceinline__ void tri_inverses(const cx<T> (*D)[33], cx<double> (*LI)[33], cx<double> (*UI)[33],
+                                             int tid) {
+    constexpr int NK = NB / 4;
+    const int j = (tid & 127) >> 2, part = tid & 3;
+    cx<double> acc[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) acc[k] = {4 * k + part == j ? 1.0 : 0.0, 0.0};
+    // column t of the block for the lane's rows, read a step ahead; the
+    // scheduling barrier per step keeps the unrolled chain from hoisting every
+    // LDS read at once
+    cx<T> an[NK];
+    if (tid < 128) {
+        // column j of inv(L): x_t final once rows < t are applied; rows i > t -= L[i][t] x_t
+#pragma unroll
+        for (int k = 0; k < NK; k++) an[k] = D[4 * k + part][0];
+#pragma unroll
+        for (int t = 0; t < NB - 1; t++) {
+            const int q = t & 3, kt = t >> 2;
+            cx<T> ac[NK];
+#pragma unroll
+            for (int k = kt; k < NK; k++) ac[k] = an[k];
+            if (t + 1 < NB - 1)
+#pragma unroll
+                for (int k = (t + 1) >> 2; k < NK; k++) an[k] = D[4 * k + part][t + 1];
+            const cx<double> x = quad_bcast(acc[kt], q);
+#pragma unroll
+            for (int k = kt; k < NK; k++) {
+                cx<double> a = c64(ac[k]);
+                if (k == kt && part <= q) a = {0.0, 0.0};
+                acc[k] = cmsub_f(acc[k], a, x);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (j < NB)
+#pragma unroll
+            for (int k = 0; k < NK; k++) LI[4 * k + part][j] = acc[k];
+    } else {
+        // column j of inv(U): x_t = acc_t / U[t][t] once rows > t are applied; rows i < t -= U[i][t] x_t
+        // lane l holds 1 / U[l % 32][l % 32]; step t reads lane t's (readlane)
+        const int dl = tid & 31;
+        const cx<double> rl = crecip(c64(D[dl][dl]));
+#pragma unroll
+        for (int k = 0; k < NK; k++) an[k] = D[4 * k + part][NB - 1];
+#pragma unroll
+        for (int t = NB - 1; t >= 0; t--) {
+            const int q = t & 3, kt = t >> 2;
+            cx<T> ac[NK];
+#pragma unroll
+            for (int k = 0; k <= kt; k++) ac[k] = an[k];
+            if (t > 0)
+#pragma unroll
+                for (int k = 0; k <= (t - 1) >> 2; k++) an[k] = D[4 * k + part][t - 1];
+            const cx<double> x = quad_bcast(cmul(acc[kt], cx<double>{readlane_d(rl.x, t), readlane_d(rl.y, t)}), q);
+            if (part == q) acc[kt] = x;
+#pragma unroll
+            for (int k = 0; k <= kt; k++) {
+                cx<double> a = c64(ac[k]);
+                if (k == kt && part >= q) a = {0.0, 0.0};
+                acc[k] = cmsub_f(acc[k], a, x);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (j < NB)
+#pragma unroll
+            for (int k = 0; k < NK; k++) UI[4 * k + part][j] = acc[k];
+    }
+}
+
+// L21 <- L21 inv(L11) on a chunk of 32 rows staged as V[i][s] (zero beyond the
+// front), on the matrix cores: wave w computes one 16 x 16 tile of the
+// transposed product (rows t: columns of L21, columns i: its rows), so each
+// accumulator register stores 16 consecutive rows of one column of G.  K runs
+// from the tile's first column (inv(L11) is zero above its diagonal).
+template <typename T>
+__device__ __forceinline__ void l21_mfma(const cx<T> (*V)[33], const cx<double> (*LI)[33], cx<T> *G, int m, int p,
+                                         int r0, int wave, int lane) {
+    const int ntt = (p + 15) >> 4, lr = lane & 15, lk = lane >> 4;
+    if (wave >= 2 * ntt) return;
+    const int t0 = (wave % ntt) * 16, i0 = (wave / ntt) * 16;
+    if (r0 + i0 >= m) return;
+    d4v cr = {0.0, 0.0, 0.0, 0.0}, ci = cr;
+    const int kend = (p + 3) & ~3;
+    for (int k0 = t0; k0 < kend; k0 += 4) {
+        const cx<double> a = LI[k0 + lk][t0 + lr];       // A[t][s] = inv(L11)[s][t]
+        const cx<double> b = c64(V[i0 + lr][k0 + lk]);   // B[s][i] = L21[i][s]
+        cmma<double>(a.x, a.y, b.x, b.y, cr, ci);
+    }
+    const int i = i0 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int t = t0 + mfma_t<double>::row(lk, v);
+        if (t < p && r0 + i < m) G[r0 + i + (int64_t)t * m] = {(T)cr[v], (T)ci[v]};
+    }
+}
+
+// U12 <- inv(U11) U12 on a chunk of 32 columns staged as V[c][s]: wave w
+// computes one 16 x 16 tile of the transposed product (rows c: columns of U12,
+// columns i: its rows); K runs from the tile's first row (inv(U11) is zero
+// below its diagonal).
+template <typename T>
+__device__ __forceinline__ void u12_mfma(const cx<T> (*V)[33], const cx<double> (*UI)[33], cx<T> *G, int m, int p,
+                                         int c0, int wave, int lane) {
+    const int nit = (p + 15) >> 4, lr = lane & 15, lk = lane >> 4;
+    if (wave >= 2 * nit) return;
+    const int i0 = (wave % nit) * 16, cc0 = (wave / nit) * 16;
+    if (c0 + cc0 >= m) return;
+    d4v cr = {0.0, 0.0, 0.0, 0.0}, ci = cr;
+    const int kend = (p + 3) & ~3;
+    for (int k0 = i0; k0 < kend; k0 += 4) {
+        const cx<double> a = c64(V[cc0 + lr][k0 + lk]);   // A[c][s] = U12[s][c]
+        const cx<double> b = UI[i0 + lr][k0 + lk];        // B[s][i] = inv(U11)[i][s]
+        cmma<double>(a.x, a.y, b.x, b.y, cr, ci);
+    }
+    const int i = i0 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int c = c0 + cc0 + mfma_t<double>::row(lk, v);
+        if (i < p && c < m) G[i + (int64_t)c * m] = {(T)cr[v], (T)ci[v]};
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_diag_inv(cx<T> *__restrict__ arena, int64_t fstride,
                                                   const FrontDev *__restrict__ fronts,
                                                   const int *__restrict__ lv_single) {
-    __shared__ cx<T> D[32][33];
-    __shared__ cx<double> R[32][33];
-    __shared__ cx<double> rd[32];   // 1 / U[i][i]
+    __shared__ cx<T> D[32][33];   // the block being inverted, then the rectangles' chunks
+    __shared__ cx<double> LI[32][33], UI[32][33];
     const FrontDev F = fronts[blockIdx.x];
     cx<T> *G = arena + blockIdx.y * fstride + F.buf_off;
     const int m = F.m, p = F.p, tid = threadIdx.x;
-    auto up = [](cx<T> v) { return cx<double>{(double)v.x, (double)v.y}; };
+    if (p == 0) return;
+    const bool single = lv_single[F.level] && p <= 32;
+    // chunk loads: four entries per thread, entry e = tid + 256u at (e % 32,
+    // e / 32) of a 32 x 32 tile; out-of-range entries read G[0] and are
+    // replaced when staged (unconditional loads stay in flight together)
+    cx<T> rb[4], rl[4], ru[4];
+    auto load_block = [&](int b0) {
+        const int kb = min(32, p - b0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = tid + 256 * u, i = e & 31, t = e >> 5;
+            rb[u] = G[i < kb && t < kb ? b0 + i + (int64_t)(b0 + t) * m : 0];
+        }
+    };
+    auto load_l21 = [&](int r0) {   // rows r0 + i, columns s < p
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = tid + 256 * u, i = e & 31, s = e >> 5;
+            rl[u] = G[r0 + i < m && s < p ? r0 + i + (int64_t)s * m : 0];
+        }
+    };
+    auto load_u12 = [&](int c0) {   // rows s < p, columns c0 + c
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = tid + 256 * u, s = e & 31, c = e >> 5;
+            ru[u] = G[s < p && c0 + c < m ? s + (int64_t)(c0 + c) * m : 0];
+        }
+    };
+    load_block(0);
+    if (single) {
+        if (p < m) {
+            load_l21(p);
+            load_u12(p);
+        }
+    }
     for (int b0 = 0; b0 < p; b0 += 32) {
         const int kb = min(32, p - b0);
-        for (int e = tid; e < kb * kb; e += 256) {
-            const int i = e % kb, t = e / kb;
-            D[i][t] = G[b0 + i + (int64_t)(b0 + t) * m];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = tid + 256 * u, i = e & 31, t = e >> 5;
+            D[i][t] = i < kb && t < kb ? rb[u] : cx<T>{i == t ? T(1) : T(0), T(0)};
         }
         __syncthreads();
-        if (tid < kb) rd[tid] = crecip(up(D[tid][tid]));
-        __syncthreads();
-        {
-            // four adjacent lanes (one wave) per column j: each sums every fourth
-            // term of a row's dot product, xor-shuffles add the partial sums, and
-            // lane 0 of the four stores the entry the next row reads.  The two
-            // triangles of R are disjoint, so both inverses proceed at once.
-            const bool lower = tid < 128;
-            const int j = (tid & 127) >> 2, part = tid & 3;
-            auto sum4 = [](cx<double> a) {
-                for (int o = 1; o < 4; o <<= 1) {
-                    a.x += __shfl_xor(a.x, o);
-                    a.y += __shfl_xor(a.y, o);
-                }
-                return a;
-            };
-            if (lower) {
-                // column j of inv(L_bb), L_bb unit lower: x_j = 1,
-                // x_i = -L[i][j] - sum_{j<t<i} L[i][t] x_t  (R[j][j] belongs to the U half)
-                for (int i = 1; i < kb; i++) {
-                    cx<double> acc = {0.0, 0.0};
-                    if (j < i) {
-                        if (part == 0) acc = {-(double)D[i][j].x, -(double)D[i][j].y};
-                        for (int t = j + 1 + part; t < i; t += 4) acc = cmsub(acc, up(D[i][t]), R[t][j]);
-                    }
-                    acc = sum4(acc);
-                    if (part == 0 && j < i) R[i][j] = acc;
-                }
-            } else {
-                // column j of inv(U_bb): x_j = 1/U[j][j], x_i = -(sum_{i<t<=j} U[i][t] x_t) / U[i][i]
-                if (part == 0 && j < kb) R[j][j] = rd[j];
-                for (int i = kb - 2; i >= 0; i--) {
-                    cx<double> acc = {0.0, 0.0};
-                    if (i < j && j < kb)
-                        for (int t = i + 1 + part; t <= j; t += 4) acc = cmsub(acc, up(D[i][t]), R[t][j]);
-                    acc = sum4(acc);
-                    if (part == 0 && i < j && j < kb) R[i][j] = cmul(acc, rd[i]);
-                }
-            }
-        }
+        if (b0 + 32 < p) load_block(b0 + 32);
+        // the block's order rounded up to 8 (the padding is the identity)
+        if (kb <= 8) tri_inverses<T, 8>(D, LI, UI, tid);
+        else if (kb <= 16) tri_inverses<T, 16>(D, LI, UI, tid);
+        else if (kb <= 24) tri_inverses<T, 24>(D, LI, UI, tid);
+        else tri_inverses<T, 32>(D, LI, UI, tid);
         __syncthreads();
         for (int e = tid; e < kb * kb; e += 256) {
             const int i = e % kb, t = e / kb;
-            G[b0 + i + (int64_t)(b0 + t) * m] = {(T)R[i][t].x, (T)R[i][t].y};
+            const cx<double> v = i > t ? LI[i][t] : UI[i][t];
+            G[b0 + i + (int64_t)(b0 + t) * m] = {(T)v.x, (T)v.y};
         }
-        __syncthreads();
     }
-    if (!lv_single[F.level] || p > 32) return;
-    cx<T>(*V)[33] = D;   // the block's entries are no longer needed: D stages the products' operands
-    // L21 <- L21 inv(L11), 32 rows at a time staged in V (all reads before any write)
+    if (!single) return;
+    const int wave = tid >> 6, lane = tid & 63;
     for (int r0 = p; r0 < m; r0 += 32) {
-        const int rows = min(32, m - r0);
-        for (int e = tid; e < rows * p; e += 256) {
-            const int i = e % rows, s = e / rows;
-            V[i][s] = G[r0 + i + (int64_t)s * m];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = tid + 256 * u, i = e & 31, s = e >> 5;
+            D[i][s] = r0 + i < m && s < p ? rl[u] : cx<T>{T(0), T(0)};
         }
         __syncthreads();
-        for (int e = tid; e < rows * p; e += 256) {
-            const int i = e % rows, t = e / rows;
-            cx<double> acc = up(V[i][t]);
-            for (int s = t + 1; s < p; s++) acc = cadd(acc, cmul(up(V[i][s]), R[s][t]));
-            G[r0 + i + (int64_t)t * m] = {(T)acc.x, (T)acc.y};
-        }
+        if (r0 + 32 < m) load_l21(r0 + 32);
+        l21_mfma<T>(D, LI, G, m, p, r0, wave, lane);
         __syncthreads();
     }
-    // U12 <- inv(U11) U12, 32 columns at a time
     for (int c0 = p; c0 < m; c0 += 32) {
-        const int cols = min(32, m - c0);
-        for (int e = tid; e < cols * p; e += 256) {
-            const int s = e % p, c = e / p;
-            V[c][s] = G[s + (int64_t)(c0 + c) * m];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = tid + 256 * u, s = e & 31, c = e >> 5;
+            D[c][s] = s < p && c0 + c < m ? ru[u] : cx<T>{T(0), T(0)};
         }
         __syncthreads();
-        for (int e = tid; e < cols * p; e += 256) {
-            const int i = e % p, c = e / p;
-            cx<double> acc = {0.0, 0.0};
-            for (int s = i; s < p; s++) acc = cadd(acc, cmul(R[i][s], up(V[c][s])));
-            G[i + (int64_t)(c0 + c) * m] = {(T)acc.x, (T)acc.y};
-        }
+        if (c0 + 32 < m) load_u12(c0 + 32);
+        u12_mfma<T>(D, UI, G, m, p, c0, wave, lane);
         __syncthreads();
     }
 }
