@@ -73,14 +73,18 @@ __global__ void __launch_bounds__(256) k_permute_in_slots(const cx<TIO> *__restr
     const int k0 = (blockIdx.x % ktiles) * PT_K, r0 = (blockIdx.x / ktiles) * PT_R;
     B += sel.s[blockIdx.y] * (int64_t)n * nrhs;
     Xp += (int64_t)blockIdx.y * n * ld;
-    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
-        int kk = e % PT_K, rr = e / PT_K;
-        cx<T> v = {T(0), T(0)};
-        if (k0 + kk < n && r0 + rr < nrhs) {
-            const cx<TIO> b = B[k0 + kk + (int64_t)(r0 + rr) * n];
-            v = {(T)b.x, (T)b.y};
-        }
-        S[rr][kk] = v;
+    // all the tile's loads in flight at once (out-of-range entries read B[0] and are dropped)
+    constexpr int PER = PT_K * PT_R / 256;
+    cx<TIO> b[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u, kk = e % PT_K, rr = e / PT_K;
+        b[u] = B[k0 + kk < n && r0 + rr < nrhs ? k0 + kk + (int64_t)(r0 + rr) * n : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u, kk = e % PT_K, rr = e / PT_K;
+        S[rr][kk] = k0 + kk < n && r0 + rr < nrhs ? cx<T>{(T)b[u].x, (T)b[u].y} : cx<T>{T(0), T(0)};
     }
     __syncthreads();
     for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
@@ -100,18 +104,38 @@ __global__ void __launch_bounds__(256) k_permute_out_slots(const cx<T> *__restri
     X += sel.s[blockIdx.y] * (int64_t)n * nrhs;
     Xp += (int64_t)blockIdx.y * n * ld;
     iout += sel.f[blockIdx.y] * map_stride;
-    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
-        int rr = e % PT_R, kk = e / PT_R;
-        if (k0 + kk < n && r0 + rr < nrhs) S[rr][kk] = Xp[(int64_t)iout[k0 + kk] * ld + r0 + rr];
+    // the rows' positions, then all the tile's loads in flight at once
+    // (out-of-range entries read Xp[0] and are dropped)
+    constexpr int PER = PT_K * PT_R / 256;
+    int64_t q[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u, rr = e % PT_R, kk = e / PT_R;
+        q[u] = k0 + kk < n && r0 + rr < nrhs ? (int64_t)iout[k0 + kk] * ld + r0 + rr : 0;
+    }
+    cx<T> v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) v[u] = Xp[q[u]];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u;
+        S[e % PT_R][e / PT_R] = v[u];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
-        int kk = e % PT_K, rr = e / PT_K;
+    cx<TIO> xo[PER];
+    if (accumulate)
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + 256 * u, kk = e % PT_K, rr = e / PT_K;
+            xo[u] = X[k0 + kk < n && r0 + rr < nrhs ? k0 + kk + (int64_t)(r0 + rr) * n : 0];
+        }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u, kk = e % PT_K, rr = e / PT_K;
         if (k0 + kk < n && r0 + rr < nrhs) {
-            const cx<T> v = S[rr][kk];
-            cx<TIO> &x = X[k0 + kk + (int64_t)(r0 + rr) * n];
-            if (accumulate) x = {x.x + (TIO)v.x, x.y + (TIO)v.y};
-            else x = {(TIO)v.x, (TIO)v.y};
+            const cx<T> w = S[rr][kk];
+            X[k0 + kk + (int64_t)(r0 + rr) * n] =
+                accumulate ? cx<TIO>{xo[u].x + (TIO)w.x, xo[u].y + (TIO)w.y} : cx<TIO>{(TIO)w.x, (TIO)w.y};
         }
     }
 }
@@ -2368,34 +2392,47 @@ __global__ void __launch_bounds__(256) k_norm_reduce(const double *__restrict__ 
 }
 
 // ---- row-major panel refinement ----------------------------------------------------
-// The panel refinement keeps b and the FP64 iterate row-major in grid order
-// (the right-hand sides of a point contiguous, row stride ld64), next to the
-// factor-precision panels the sweeps work on (row-major in elimination order,
-// row stride ld).  A refinement step is then one pass that reads x, the latest
-// correction d and b and writes x + d and the next right-hand side
-// r = b - op(M)(x + d) straight into the sweeps' layout: every access moves
-// whole row segments, and no panel is transposed between the steps.
+// The panel refinement updates the residual rather than the iterate:
+// r_{k+1} = r_k - op(M) d_k in FP64, row-major in grid order (the right-hand
+// sides of a point contiguous, row stride ld64), with every correction d_k
+// kept in the factor-precision panel the sweeps produced it in (row-major in
+// elimination order, row stride ld).  A refinement step is one pass that reads
+// r_k at each point and d_k over the point's 3x3 neighbourhood and writes
+// r_{k+1} twice: in FP64 for the next step and in the factor precision as the
+// next correction solve's right-hand side.  x = d_0 + d_1 + ... is summed in
+// FP64 once, by the pass that writes the caller's layout.  r_1 = b - op(M) d_0
+// with d_0 exactly as stored, so the sum and the residuals agree to FP64
+// rounding as when x itself is carried.
 
-// B (user layout) -> Bg (FP64, grid order: row k, stride ld64) and P (factor
-// precision, elimination order: row iperm[k], stride ld) on slot sel.s[z];
-// also the tile's sums of |b|^2 per column in bpart[(z * ktiles + ktile) *
-// nrhs + r].  grid (tiles, slots), 64 x 32 tiles.
+// B (user layout) -> P (factor precision, elimination order: row iperm[k],
+// stride ld) on slot sel.s[z]; also the tile's sums of |b|^2 per column in
+// bpart[(z * ktiles + ktile) * nrhs + r].  grid (tiles, slots), 64 x 32 tiles.
 template <typename T>
-__global__ void __launch_bounds__(256) k_ir_in(const cx<double> *__restrict__ B, cx<double> *__restrict__ Bp,
-                                               cx<T> *__restrict__ P, const int *__restrict__ iperm, int n, int nrhs,
-                                               int ld64, int ld, double *__restrict__ bpart, Sel sel) {
+__global__ void __launch_bounds__(256) k_ir_in(const cx<double> *__restrict__ B, cx<T> *__restrict__ P,
+                                               const int *__restrict__ iperm, int n, int nrhs, int ld,
+                                               double *__restrict__ bpart, Sel sel) {
     __shared__ cx<double> S[PT_R][PT_K + 1];
     const int ktiles = (n + PT_K - 1) / PT_K, kt = blockIdx.x % ktiles;
     const int k0 = kt * PT_K, r0 = (blockIdx.x / ktiles) * PT_R;
     const int s = sel.s[blockIdx.y];
     B += (int64_t)s * n * nrhs;
-    Bp += (int64_t)s * n * ld64;
     P += (int64_t)s * n * ld;
-    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
-        const int kk = e % PT_K, rr = e / PT_K;
-        cx<double> v = {0.0, 0.0};
-        if (k0 + kk < n && r0 + rr < nrhs) v = B[k0 + kk + (int64_t)(r0 + rr) * n];
-        S[rr][kk] = v;
+    {
+        // all the tile's loads in flight at once (out-of-range entries read B[0] and are dropped)
+        constexpr int PER = PT_K * PT_R / 256;
+        cx<double> v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + 256 * u, kk = e % PT_K, rr = e / PT_K;
+            const bool ok = k0 + kk < n && r0 + rr < nrhs;
+            v[u] = B[ok ? k0 + kk + (int64_t)(r0 + rr) * n : 0];
+            if (!ok) v[u] = {0.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + 256 * u;
+            S[e / PT_K][e % PT_K] = v[u];
+        }
     }
     __syncthreads();
     {
@@ -2412,26 +2449,26 @@ __global__ void __launch_bounds__(256) k_ir_in(const cx<double> *__restrict__ B,
         const int rr = e % PT_R, kk = e / PT_R;
         if (k0 + kk >= n) continue;
         const cx<double> v = S[rr][kk];
-        if (r0 + rr < ld64) Bp[(int64_t)(k0 + kk) * ld64 + r0 + rr] = v;
         if (r0 + rr < ld) P[(int64_t)iperm[k0 + kk] * ld + r0 + rr] = {(T)v.x, (T)v.y};
     }
 }
 
 // One refinement step of launch slot z (frequency sel.f[z], slot s = sel.s[z]):
-// x = Xa + D (D alone on the first step: Xa null), r = b - op(M) x;  Xb <- x,
+// r <- r - op(M) D, with r read from Bu (the caller's b, user layout: point k
+// of column c at k + c*n) on the first step and from Rg otherwise;  Rg <- r,
 // R <- r in the factor precision (zero padding columns), and the sum of |r|^2
 // over the workgroup's points of every column in
-// partial[(z * gridDim.x + block) * nrhs + c].  b, Xa and Xb are in grid order
-// (row k = i + j*nx, stride ld64), D and R in elimination order (row iperm[k],
-// stride ld).  Xa/Xb and D/R must be distinct buffers: other workgroups read a
-// point's x while its own workgroup writes.  Row k of M has the entries of CSC
-// column k (the pattern is symmetric), all within the 3x3 neighbourhood.
-// Tiles of kIrTx x kIrTy points (8 consecutive points of four grid rows): per
-// 16-column chunk the x of the (kIrTy + 2) x (kIrTx + 2) neighbourhood points is
-// staged in LDS once (each point's x is used by up to nine rows; an 8 x 4 tile
-// stages 1.9 points per output, 16 x 2 stages 2.25, measured 2% slower), then
-// one thread per (point, column) applies the stencil from LDS.  grid (blocks, slots),
-// grid-stride over tiles.
+// partial[(z * gridDim.x + block) * nrhs + c].  Rg is in grid order (row
+// k = i + j*nx, stride ld64; each point read and written by its own thread
+// only), D and R in elimination order (row iperm[k], stride ld); D and R must
+// be distinct buffers (other workgroups read a point's d while its own
+// workgroup writes r).  Row k of M has the entries of CSC column k (the
+// pattern is symmetric), all within the 3x3 neighbourhood.  Tiles of kIrTx x
+// kIrTy points (8 consecutive points of four grid rows): per 16-column chunk
+// the d of the (kIrTy + 2) x (kIrTx + 2) neighbourhood points is staged in LDS
+// once (each point's d is used by up to nine rows; an 8 x 4 tile stages 1.9
+// points per output), then one thread per (point, column) applies the stencil
+// from LDS.  grid (blocks, slots), grid-stride over tiles.
 constexpr int kIrTx = 8, kIrTy = 4, kIrCw = 16, kIrPts = kIrTx * kIrTy, kIrStage = (kIrTy + 2) * (kIrTx + 2);
 constexpr int kIrThreads = kIrPts * kIrCw, kIrWaves = kIrThreads / 64;
 constexpr int kIrCoef0 = (kIrStage + 63) / 64 * 64;   // first thread of the stencil-entry gather
@@ -2443,12 +2480,11 @@ template <typename T>
 __global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4))) k_ir_step(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
                                                  const int *__restrict__ rowind, const int64_t *__restrict__ tq,
                                                  const int *__restrict__ iperm, int nx, int ny, int nrhs, int ld64,
-                                                 int ld, int sys, int64_t nnz, const cx<double> *__restrict__ Bg,
-                                                 const cx<double> *__restrict__ Xa, const cx<T> *__restrict__ D,
-                                                 cx<double> *__restrict__ Xb, cx<T> *__restrict__ R,
-                                                 double *__restrict__ partial, Sel sel) {
+                                                 int ld, int sys, int64_t nnz, const cx<double> *__restrict__ Bu,
+                                                 cx<double> *Rg, const cx<T> *__restrict__ D,
+                                                 cx<T> *__restrict__ R, double *__restrict__ partial, Sel sel) {
     extern __shared__ double wsum[];   // [kIrWaves][nrhs]
-    __shared__ cx<double> sx[kIrStage][kIrCw];
+    __shared__ cx<T> sx[kIrStage][kIrCw];
     __shared__ cx<double> scoef[kIrPts][9];
     __shared__ int sidx[kIrPts][9];    // staged row of each stencil entry
     __shared__ int sg[kIrStage];       // grid row of each staged point (-1: outside the grid)
@@ -2457,10 +2493,9 @@ __global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu
     const int64_t n = (int64_t)nx * ny;
     vals += sel.f[z] * nnz;
     const int64_t o64 = s * n * ld64, o32 = s * n * ld;
-    Bg += o64;
-    Xb += o64;
-    const bool first = Xa == nullptr;
-    if (!first) Xa += o64;
+    Rg += o64;
+    const bool first = Bu != nullptr;
+    if (first) Bu += s * n * nrhs;
     D += o32;
     R += o32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -2508,32 +2543,30 @@ __global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu
         for (int c0 = 0; c0 < ld; c0 += kIrCw) {
             const int c = c0 + cc;
             const bool own = pin && c < nrhs;
-            // every load of the chunk is issued before the first is used: b, then the
-            // neighbourhood's d and x (a fixed number of staging entries per thread)
-            cx<double> acc = own ? Bg[k * ld64 + c] : cx<double>{0.0, 0.0};
+            // every load of the chunk is issued before the first is used: r, then the
+            // neighbourhood's d (a fixed number of staging entries per thread)
+            cx<double> acc = {0.0, 0.0};
+            if (own) acc = first ? Bu[k + (int64_t)c * n] : Rg[k * ld64 + c];
             cx<T> dv[kIrPer];
-            cx<double> xv[kIrPer];
 #pragma unroll
             for (int q = 0; q < kIrPer; q++) {
                 const int e = tid + kIrThreads * q, row = e / kIrCw, cs = c0 + e % kIrCw;
                 dv[q] = {T(0), T(0)};
-                xv[q] = {0.0, 0.0};
-                if (e < kIrStage * kIrCw && sg[row] >= 0 && cs < nrhs) {
-                    dv[q] = D[(int64_t)se[row] * ld + cs];
-                    if (!first) xv[q] = Xa[(int64_t)sg[row] * ld64 + cs];
-                }
+                if (e < kIrStage * kIrCw && sg[row] >= 0 && cs < nrhs) dv[q] = D[(int64_t)se[row] * ld + cs];
             }
 #pragma unroll
             for (int q = 0; q < kIrPer; q++) {
                 const int e = tid + kIrThreads * q;
-                if (e < kIrStage * kIrCw)
-                    sx[e / kIrCw][e % kIrCw] = cadd(xv[q], cx<double>{(double)dv[q].x, (double)dv[q].y});
+                if (e < kIrStage * kIrCw) sx[e / kIrCw][e % kIrCw] = dv[q];
             }
             __syncthreads();
             if (own) {
 #pragma unroll
-                for (int t = 0; t < 9; t++) acc = cmsub(acc, scoef[r][t], sx[sidx[r][t]][cc]);
-                Xb[k * ld64 + c] = sx[own_li][cc];
+                for (int t = 0; t < 9; t++) {
+                    const cx<T> d = sx[sidx[r][t]][cc];
+                    acc = cmsub(acc, scoef[r][t], cx<double>{(double)d.x, (double)d.y});
+                }
+                Rg[k * ld64 + c] = acc;
                 R[dst + c] = {(T)acc.x, (T)acc.y};
             } else if (pin && c < ld) {
                 R[dst + c] = {T(0), T(0)};
@@ -2554,21 +2587,78 @@ __global__ void __launch_bounds__(kIrThreads) __attribute__((amdgpu_waves_per_eu
     }
 }
 
-// Final iterates (FP64, grid order, stride ld64) -> X (user layout) on slot
-// sel.s[z]; the slot's iterate sits in X1 when bit z of `in1` is set, else in
-// X0.  grid (tiles, slots), 64 x 32 tiles.
-__global__ void __launch_bounds__(256) k_ir_out(const cx<double> *__restrict__ X0, const cx<double> *__restrict__ X1,
-                                                uint64_t in1, cx<double> *__restrict__ X, int n, int nrhs, int ld64,
-                                                Sel sel) {
+// corrections kept apart by the panel refinement before they are folded into
+// one FP64 sum: buffer j holds correction j, j + kIrBufs, ... of every slot
+constexpr int kIrBufs = 8;
+struct IrBufs {
+    const void *p[kIrBufs];
+};
+struct IrCounts {
+    int c[kMaxBatch];
+};
+
+// acc (FP64, elimination order, stride ld) <- (bit z of `had`: acc +) the sum
+// of the kIrBufs correction buffers, on the slots of sel.  grid (blocks, slots)
+template <typename T>
+__global__ void __launch_bounds__(256) k_ir_fold(IrBufs d, cx<double> *__restrict__ acc, uint64_t had, int64_t count,
+                                                 Sel sel) {
+    const int z = blockIdx.y;
+    const int64_t o = sel.s[z] * count;
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < count; e += (int64_t)gridDim.x * 256) {
+        cx<double> v = ((had >> z) & 1) ? acc[o + e] : cx<double>{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < kIrBufs; j++) {
+            const cx<T> w = static_cast<const cx<T> *>(d.p[j])[o + e];
+            v.x += (double)w.x;
+            v.y += (double)w.y;
+        }
+        acc[o + e] = v;
+    }
+}
+
+// X (user layout) on slot sel.s[z] <- the FP64 sum of the first cnt.c[z]
+// correction buffers (elimination order, row iperm[k], stride ld), plus acc
+// when bit z of `folded` is set.  grid (tiles, slots), 64 x 32 tiles.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ir_sum_out(IrBufs d, IrCounts cnt, const cx<double> *__restrict__ acc,
+                                                    uint64_t folded, cx<double> *__restrict__ X,
+                                                    const int *__restrict__ iperm, int n, int nrhs, int ld, Sel sel) {
     __shared__ cx<double> S[PT_R][PT_K + 1];
+    constexpr int PER = PT_K * PT_R / 256;
     const int ktiles = (n + PT_K - 1) / PT_K;
     const int k0 = (blockIdx.x % ktiles) * PT_K, r0 = (blockIdx.x / ktiles) * PT_R;
-    const int z = blockIdx.y, s = sel.s[z];
-    const cx<double> *Xp = (((in1 >> z) & 1) ? X1 : X0) + (int64_t)s * n * ld64;
+    const int z = blockIdx.y, s = sel.s[z], nb = cnt.c[z];
+    const bool fa = (folded >> z) & 1;
+    const int64_t o = (int64_t)s * n * ld;
     X += (int64_t)s * n * nrhs;
-    for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
-        const int rr = e % PT_R, kk = e / PT_R;
-        if (k0 + kk < n && r0 + rr < nrhs) S[rr][kk] = Xp[(int64_t)(k0 + kk) * ld64 + r0 + rr];
+    // every load of the tile is issued before the first is used: the rows'
+    // positions, then all buffers' entries (out-of-range entries read entry o
+    // of the slot and are dropped)
+    int64_t q[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u, rr = e % PT_R, kk = e / PT_R;
+        const bool ok = k0 + kk < n && r0 + rr < nrhs;
+        q[u] = ok ? o + (int64_t)iperm[k0 + kk] * ld + r0 + rr : o;
+    }
+    cx<double> v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) v[u] = fa ? acc[q[u]] : cx<double>{0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < kIrBufs; j++)
+        if (j < nb) {
+            const cx<T> *dj = static_cast<const cx<T> *>(d.p[j]);
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const cx<T> w = dj[q[u]];
+                v[u].x += (double)w.x;
+                v[u].y += (double)w.y;
+            }
+        }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u;
+        S[e % PT_R][e / PT_R] = v[u];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < PT_K * PT_R; e += 256) {
@@ -3315,7 +3405,7 @@ static void keep_unconverged(const std::vector<double> &h, int ns, int nrhs, int
     act_s = nsl;
 }
 
-thread_local Workspace ws_b64, ws_x64a, ws_x64b, ws_p2;
+thread_local Workspace ws_r64, ws_acc64, ws_dk[kIrBufs];
 // widest panel of the row-major refinement: its per-wave column sums and the
 // staged neighbourhood share the default 64 KB of LDS
 constexpr int kIrMaxRhs = 1024;
@@ -3324,12 +3414,13 @@ static_assert(kIrDynLdsMax + (kIrStage * kIrCw + 9 * kIrPts) * 16 + (9 * kIrPts 
               "refinement-step LDS");
 
 // Row-major panel refinement (grid plans, kVecMaxRhs < nrhs <= kIrMaxRhs): b
-// is transposed once (FP64 in grid order, and the factor-precision copy in
-// elimination order the first solve runs on), every step is one k_ir_step
-// pass plus, while a slot has not converged, one correction solve in place on
-// the new right-hand side, and the final iterates are transposed back once.
-// Slot s of every panel serves frequency freqs[s]; converged slots leave the
-// batch.
+// is transposed once into the first solve's panel (elimination order, factor
+// precision), every step is one k_ir_step pass (r -= op(M) d_k) plus, while a
+// slot has not converged, one correction solve in place on the step's
+// factor-precision right-hand side, which becomes d_{k+1}.  The corrections
+// stay in kIrBufs panels (folded into an FP64 sum when those are full) and are
+// summed once into the caller's layout.  Slot s of every panel serves
+// frequency freqs[s]; converged slots leave the batch.
 template <typename T>
 std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<int> &freqs, int sys, int nrhs,
                                        const cx<double> *B, cx<double> *X, double tol, int max_iter,
@@ -3341,10 +3432,9 @@ std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<in
     const int cl = (int)(128 / sizeof(cx<T>));
     const int ld = (nrhs + cl - 1) / cl * cl, ld64 = (nrhs + 7) / 8 * 8;
     const size_t e64 = (size_t)n * ld64 * ns, e32 = (size_t)n * ld * ns;
-    cx<double> *Bp = (cx<double> *)ws_b64.get(sizeof(cx<double>) * e64);
-    cx<double> *Xv[2] = {(cx<double> *)ws_x64a.get(sizeof(cx<double>) * e64),
-                         (cx<double> *)ws_x64b.get(sizeof(cx<double>) * e64)};
-    cx<T> *P[2] = {(cx<T> *)ws_x.get(sizeof(cx<T>) * e32), (cx<T> *)ws_p2.get(sizeof(cx<T>) * e32)};
+    cx<double> *Rg = (cx<double> *)ws_r64.get(sizeof(cx<double>) * e64);
+    // correction buffer j (allocated on first use)
+    auto dbuf = [&](int j) { return (cx<T> *)ws_dk[j].get(sizeof(cx<T>) * e32); };
     cx<T> *Wp = (cx<T> *)ws_w.get(sizeof(cx<T>) * (size_t)pl.fv_rows * ld * ns);
     double *nrm = (double *)ws_n.get(sizeof(double) * 2 * nrhs * ns);
     const int ktiles = (n + PT_K - 1) / PT_K;
@@ -3356,46 +3446,75 @@ std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<in
     const Sel all = make_sel(freqs, slots);
     const double nr = nrhs, es = sizeof(cx<T>), nnz = (double)pl.nnz;
     {
-        ProfScope ps(K_PERMUTE_IN, st, 0.0, ns * ((double)n * nr * (16.0 + 16.0 + es) + 4.0 * n));
-        const int tiles = ktiles * ((std::max(ld, ld64) + PT_R - 1) / PT_R);
-        hipLaunchKernelGGL(k_ir_in<T>, dim3(tiles, ns), dim3(256), 0, st, B, Bp, P[0], dp.iperm, n, nrhs, ld64, ld,
-                           partial, all);
+        ProfScope ps(K_PERMUTE_IN, st, 0.0, ns * ((double)n * nr * (16.0 + es) + 4.0 * n));
+        const int tiles = ktiles * ((ld + PT_R - 1) / PT_R);
+        hipLaunchKernelGGL(k_ir_in<T>, dim3(tiles, ns), dim3(256), 0, st, B, dbuf(0), dp.iperm, n, nrhs, ld, partial,
+                           all);
     }
     hipLaunchKernelGGL(k_norm_reduce, dim3(nrhs, ns), dim3(256), 0, st, partial, ktiles, nrhs,
                        nrm + (int64_t)nrhs * ns, all);
-    sweeps<T>(F, all, sys, ld, P[0], Wp, st, nrhs);
+    sweeps<T>(F, all, sys, ld, dbuf(0), Wp, st, nrhs);
     std::vector<RefineStats> rs(ns);
     std::vector<double> h(2 * (size_t)nrhs * ns);
     std::vector<int> act_f = freqs, act_s = slots;
-    uint64_t in1 = 0;   // bit s: slot s's latest iterate is in Xv[1]
+    std::vector<int> base(ns, 0);   // per slot: corrections already folded into acc
+    uint64_t folded = 0;            // bit s: slot s has a folded sum in acc
+    cx<double> *acc = nullptr;
     for (int it = 0;; it++) {
-        const int a = it & 1;
         const Sel sel = make_sel(act_f, act_s);
+        const int jn = (it + 1) % kIrBufs;   // buffer of the next right-hand side
+        if (jn == 0) {
+            // the buffers hold corrections it + 1 - kIrBufs .. it of every active slot:
+            // fold them into acc before the step overwrites buffer 0
+            acc = (cx<double> *)ws_acc64.get(sizeof(cx<double>) * e32);
+            IrBufs d;
+            for (int j = 0; j < kIrBufs; j++) d.p[j] = dbuf(j);
+            uint64_t had = 0;
+            for (int z = 0; z < sel.n; z++)
+                if ((folded >> sel.s[z]) & 1) had |= uint64_t(1) << z;
+            const int64_t count = (int64_t)n * ld;
+            hipLaunchKernelGGL(k_ir_fold<T>, dim3(grid_for(count), sel.n), dim3(256), 0, st, d, acc, had, count, sel);
+            for (int z = 0; z < sel.n; z++) {
+                folded |= uint64_t(1) << sel.s[z];
+                base[sel.s[z]] = it + 1;
+            }
+        }
         {
-            // per slot: b, x (not on the first step), d, the neighbours' x and d
-            // from cache, x + d and r written; the stencil values and indices
-            const double panel = (double)n * nr * (16.0 + (it ? 16.0 : 0.0) + es + 16.0 + es);
+            // per slot: r (b on the first step), d and the neighbours' d from
+            // cache, r in FP64 and in the factor precision written; the stencil
+            // values and indices
+            const double panel = (double)n * nr * (16.0 + es + 16.0 + es);
             ProfScope ps(K_RESIDUAL, st, sel.n * (8.0 * nnz * nr + 2.0 * n * nr), sel.n * (panel + nnz * 28.0));
             hipLaunchKernelGGL(k_ir_step<T>, dim3(nblk, sel.n), dim3(kIrThreads), kIrWaves * sizeof(double) * nrhs, st,
-                               F.vals,
-                               dp.colptr, dp.rowind, dp.tq, dp.iperm, pl.nx, pl.ny, nrhs, ld64, ld, sys, pl.nnz, Bp,
-                               it ? Xv[a] : nullptr, P[a], Xv[1 - a], P[1 - a], partial, sel);
+                               F.vals, dp.colptr, dp.rowind, dp.tq, dp.iperm, pl.nx, pl.ny, nrhs, ld64, ld, sys,
+                               pl.nnz, it == 0 ? B : nullptr, Rg, dbuf(it % kIrBufs), dbuf(jn), partial, sel);
         }
         {
             ProfScope ps(K_NORM_REDUCE, st, sel.n * (double)nblk * nrhs, sel.n * 8.0 * (nblk + 1.0) * nrhs);
             hipLaunchKernelGGL(k_norm_reduce, dim3(nrhs, sel.n), dim3(256), 0, st, partial, nblk, nrhs, nrm, sel);
         }
-        for (int s : act_s) in1 = a ? (in1 & ~(uint64_t(1) << s)) : (in1 | (uint64_t(1) << s));
         GFDFD_HIP(hipMemcpyAsync(h.data(), nrm, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
         GFDFD_HIP(hipStreamSynchronize(st));
         keep_unconverged(h, ns, nrhs, it, tol, max_iter, act_f, act_s, rs);
         if (act_f.empty()) break;
-        sweeps<T>(F, make_sel(act_f, act_s), sys, ld, P[1 - a], Wp, st, nrhs);   // d = op(M)^-1 r in place
+        sweeps<T>(F, make_sel(act_f, act_s), sys, ld, dbuf(jn), Wp, st, nrhs);   // d = op(M)^-1 r in place
     }
     {
-        ProfScope ps(K_PERMUTE_OUT, st, 0.0, ns * ((double)n * nr * 32.0 + 4.0 * n));
+        // slot s holds corrections base[s] .. iterations in buffers 0, 1, ...
+        IrBufs d;
+        IrCounts cnt;
+        int most = 0;
+        for (int z = 0; z < ns; z++) {
+            cnt.c[z] = rs[z].iterations + 1 - base[z];
+            most = std::max(most, cnt.c[z]);
+        }
+        for (int j = 0; j < kIrBufs; j++) d.p[j] = j < most ? dbuf(j) : nullptr;
+        double reads = 0;
+        for (int z = 0; z < ns; z++) reads += cnt.c[z] * es + (((folded >> z) & 1) ? 16.0 : 0.0);
+        ProfScope ps(K_PERMUTE_OUT, st, 0.0, (double)n * nr * (reads + 16.0 * ns) + 4.0 * n * ns);
         const int tiles = ktiles * ((nrhs + PT_R - 1) / PT_R);
-        hipLaunchKernelGGL(k_ir_out, dim3(tiles, ns), dim3(256), 0, st, Xv[0], Xv[1], in1, X, n, nrhs, ld64, all);
+        hipLaunchKernelGGL(k_ir_sum_out<T>, dim3(tiles, ns), dim3(256), 0, st, d, cnt, acc, folded, X, dp.iperm, n,
+                           nrhs, ld, all);
     }
     GFDFD_HIP(hipGetLastError());
     return rs;
@@ -3457,7 +3576,8 @@ template std::vector<RefineStats> solve_refined_batch<float>(const Factors<float
 // the calling thread's cached solve buffers back to the device (they grow to
 // the largest solve and are otherwise kept for the next one)
 void release_solve_workspaces() {
-    for (Workspace *w : {&ws_x, &ws_w, &ws_r, &ws_n, &ws_p, &ws_b64, &ws_x64a, &ws_x64b, &ws_p2}) w->release();
+    for (Workspace *w : {&ws_x, &ws_w, &ws_r, &ws_n, &ws_p, &ws_r64, &ws_acc64}) w->release();
+    for (Workspace &w : ws_dk) w.release();
 }
 
 // ---------------------------------------------------------------------------

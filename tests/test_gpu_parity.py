@@ -224,14 +224,13 @@ def test_batched_solves_match_oracle(precision, sysno):
 @pytest.mark.parametrize("precision", [G.FP32, G.FP64])
 @pytest.mark.parametrize("sysno", [0, 1, 2])
 def test_refinement_layouts_agree(precision, sysno):
-    """The panel refinement in elimination order (one fused residual-and-update
-    pass per step) against the same refinement in the caller's layout (residual
-    there, a transpose before and after every correction): both round x, d and
-    r to the same precisions and sum the stencil in the same order, so with the
-    same number of steps the iterates agree to rounding (the compiler may fuse
-    the stencil products differently in the two kernels): 1e-13; both within
-    1e-8 of the oracle.  66 right-hand sides leave padding columns in both
-    internal row strides."""
+    """The panel refinement in elimination order (the FP64 residual updated by
+    r -= op(M) d per step, the corrections summed once at the end) against the
+    refinement in the caller's layout (r = b - op(M) x from the FP64 iterate, a
+    transpose before and after every correction): the two residuals differ by
+    FP64 rounding only, so with the same number of steps the iterates agree to
+    1e-13; both within 1e-8 of the oracle.  66 right-hand sides leave padding
+    columns in both internal row strides."""
     nx, ny, npml, dh = 41, 30, 6, 10.0
     m1, m2 = smooth_model(nx, ny, seed=4)
     freqs = [2.5, 7.0, 13.0]
@@ -462,6 +461,40 @@ def test_solve_paths_match_oracle(precision, path, sysno):
             e1 = rel(X1[z], Xr.T)
             say("  unrefined FP32 error", z, e1)
             assert e1 <= 1e-3, (z, e1)
+
+
+@pytest.mark.parametrize("sysno", [0, 2])
+def test_refinement_folds_corrections(sysno):
+    """More corrections than the elimination-order refinement keeps apart (8):
+    with tol 0 every frequency runs max_iter = 17 steps, so the corrections are
+    folded into the FP64 sum twice (before steps 7 and 15) and the final pass
+    adds that sum to the two corrections left.  Against the caller's-layout
+    refinement after the same 17 steps (which carries x itself): 1e-12; both
+    within 1e-8 of the oracle."""
+    nx, ny, npml, dh = 41, 30, 6, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=5)
+    freqs = [3.0, 9.0]
+    say("refinement folds", sysno)
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    F = pl.factor(pl.assemble(G.AC, dh, freqs, dev(m1), dev(m2), npml=npml), G.FP32)
+    rng = np.random.default_rng(31 + sysno)
+    B = dev(rng.standard_normal((2, 40, nx * ny)) + 1j * rng.standard_normal((2, 40, nx * ny)))
+    out = {}
+    try:
+        for layout in (0, 1):
+            fdfd.solve_tuning(refine_layout=layout)
+            X, its, rr = F.solve_refine_batch([1, 0], B, sys=sysno, tol=0.0, max_iter=17)
+            out[layout] = X.cpu().numpy()
+            say("  layout", layout, "iterations", list(its), list(rr))
+            assert list(its) == [17, 17] and max(rr) <= 1e-12
+    finally:
+        fdfd.solve_tuning()
+    assert rel(out[0], out[1]) <= 1e-12
+    Bh = B.cpu().numpy()
+    for z, k in enumerate([1, 0]):
+        _, csc = oracle_system(G.AC, nx, ny, npml, dh, freqs[k], m1, m2)
+        Xr, _ = O.band_solve(*csc, Bh[z].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+        assert rel(out[0][z], Xr.T) <= 1e-8 and rel(out[1][z], Xr.T) <= 1e-8, (z, k)
 
 
 def test_refinement_wide_panel():
