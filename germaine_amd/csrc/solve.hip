@@ -1265,10 +1265,13 @@ __host__ __device__ __forceinline__ int small_stride(int rows) {
 }
 
 // Forward.  CH: the level has fronts with children (contribution rows to add);
-// the leaves' level has none and keeps no registers for them.
+// the leaves' level has none and keeps no registers for them.  PS: pivot
+// slots of four rows a lane gathers (the level's pivot blocks have at most 4 PS
+// rows); levels with children run workgroups of kSmallChThreads.
 // grid: (ceil(fronts/8)*8 * tile groups, frequency slots)
-template <typename T, bool CH>
-__global__ void __launch_bounds__(kSmallThreads<T>)
+constexpr int kSmallChThreads = 256;
+template <typename T, bool CH, int PS = 8>
+__global__ void __launch_bounds__(CH ? kSmallChThreads : kSmallThreads<T>)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1(const cx<T> *__restrict__ arena, int64_t fstride,
                                                     const FrontDev *__restrict__ lvF, int nfr,
                                                     const int *__restrict__ gsrc, int64_t gstride,
@@ -1305,22 +1308,26 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1
         }
     }
     // stage the operator [inv(L11); L21 inv(L11)] (or its transposed-solve form) and the sources
-    // m * p <= 2048 entries: one pass at 512 threads or more
+    // m * p <= 2048 entries: one pass at 512 threads or more.  The loads are
+    // unconditional (entries past the operator read G[0] and are dropped), so
+    // they are all in flight at once
     for (int e0 = 0; e0 < m * p; e0 += 4 * nth) {
         cx<T> v[4];
         int col[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int e = e0 + u * nth + threadIdx.x, k = e / m, i = e - k * m;
-            v[u] = {T(0), T(0)};
+            const bool in = e < m * p;
+            v[u] = G[in ? i * ars + k * acs : 0];
             col[u] = k;
-            if (e < m * p) {
-                v[u] = G[i * ars + k * acs];
-                if (!CH) col[u] = gs[3 * k] - F.pos0;
-                if (cj) v[u].y = -v[u].y;
-                if (i < p && k > i) v[u] = {T(0), T(0)};
-                if (i == k && !trans) v[u] = {T(1), T(0)};
-            }
+            if (!CH) col[u] = gs[in ? 3 * k : 0] - F.pos0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = e0 + u * nth + threadIdx.x, k = e / m, i = e - k * m;
+            if (cj) v[u].y = -v[u].y;
+            if (i < p && k > i) v[u] = {T(0), T(0)};
+            if (i == k && !trans) v[u] = {T(1), T(0)};
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -1337,33 +1344,64 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1
     if (r0 >= nrhs) return;
     // pivot rows in operand layout (k = 4s + lk), and the contribution rows the
     // lane's accumulators will meet in the epilogue (rows >= p; m <= 64: 4 strips)
-    if (CH)
+    // The gathers load unconditionally (absent sources read row 0 and are
+    // dropped by a select once everything has arrived), so the up to three
+    // sources of every row are in flight together; only whole strips past the
+    // front (a uniform test) are skipped.  Column r0 + lr is inside the row
+    // stride for every lane.
+    if (CH) {
+        // bit 3s + j: source j of pivot slot s exists; bit 24 + 2t + j: source
+        // j of contribution slot t exists
+        cx<T> xa[PS], wu[PS], ww[PS], cu[16], cw[16];
+        uint64_t ok = 0;
+#pragma unroll
+        for (int s = 0; s < PS; s++) {
+            if (4 * s >= p) break;
+            const int k = 4 * s + lk, kc = k < p ? k : 0;
+            const int a = src[3 * kc], u = src[3 * kc + 1], w = src[3 * kc + 2];
+            ok |= (uint64_t)(a >= 0) << (3 * s) | (uint64_t)(u >= 0) << (3 * s + 1) |
+                  (uint64_t)(w >= 0) << (3 * s + 2);
+            xa[s] = X[(int64_t)max(a, 0) * nrhs + r0 + lr];
+            wu[s] = W[(int64_t)max(u, 0) * nrhs + r0 + lr];
+            ww[s] = W[(int64_t)max(w, 0) * nrhs + r0 + lr];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (16 * q >= m) break;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int t = 4 * q + v, row = 16 * q + mfma_t<T>::row(lk, v), rc = row < m ? row : 0;
+                const int u = src[3 * rc + 1], w = src[3 * rc + 2];
+                ok |= (uint64_t)(u >= 0) << (24 + 2 * t) | (uint64_t)(w >= 0) << (25 + 2 * t);
+                cu[t] = W[(int64_t)max(u, 0) * nrhs + r0 + lr];
+                cw[t] = W[(int64_t)max(w, 0) * nrhs + r0 + lr];
+            }
+        }
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int k = 4 * s + lk;
             b[s] = {T(0), T(0)};
+            if (s >= PS || 4 * s >= p) continue;
             if (k < p && cin) {
-                const int a = src[3 * k];
-                if (a >= 0) b[s] = X[(int64_t)a * nrhs + r0 + lr];
-                const int u = src[3 * k + 1], w = src[3 * k + 2];
-                if (u >= 0) b[s] = cadd(b[s], W[(int64_t)u * nrhs + r0 + lr]);
-                if (w >= 0) b[s] = cadd(b[s], W[(int64_t)w * nrhs + r0 + lr]);
+                if ((ok >> (3 * s)) & 1) b[s] = xa[s];
+                if ((ok >> (3 * s + 1)) & 1) b[s] = cadd(b[s], wu[s]);
+                if ((ok >> (3 * s + 2)) & 1) b[s] = cadd(b[s], ww[s]);
             }
         }
-    if (CH)
 #pragma unroll
         for (int q = 0; q < 4; q++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = 16 * q + mfma_t<T>::row(lk, v);
-                cx<T> &t = w2[CH ? 4 * q + v : 0];
-                t = {T(0), T(0)};
+                const int t = 4 * q + v, row = 16 * q + mfma_t<T>::row(lk, v);
+                cx<T> &o = w2[CH ? t : 0];
+                o = {T(0), T(0)};
+                if (16 * q >= m) continue;
                 if (row >= p && row < m && cin) {
-                    const int u = src[3 * row + 1], w = src[3 * row + 2];
-                    if (u >= 0) t = W[(int64_t)u * nrhs + r0 + lr];
-                    if (w >= 0) t = cadd(t, W[(int64_t)w * nrhs + r0 + lr]);
+                    if ((ok >> (24 + 2 * t)) & 1) o = cu[t];
+                    if ((ok >> (25 + 2 * t)) & 1) o = cadd(o, cw[t]);
                 }
             }
+    }
     T bs[8];   // br + bi of every operand entry, for the three-product complex MMA
 #pragma unroll
     for (int s = 0; s < 8; s++) bs[s] = b[s].x + b[s].y;
@@ -2866,6 +2904,8 @@ void set_smem_attrs() {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, fl));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_small1<T, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, fl));
+    GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_small1<T, true, 3>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, fl));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fwd_small1<T, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, fl));
     GFDFD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bwd_small1<T>),
@@ -2919,13 +2959,18 @@ void sweeps(const Factors<T> &F, const Sel &in, int sys, int nrhs, cx<T> *Xp, cx
         const DevPlan::LevelCounts &c = dp.counts[l];
         const int nfr = dp.n_all[l];
         if (dp.single[l] && dp.mmax_all[l] <= kSmallM) {
+            const bool ch = dp.lv_children[l];
             const WaveLaunch wl =
                 small_launch(nfr, nrhs, ns, 2 * sizeof(T) * dp.pmax_all[l] * small_stride(dp.mmax_all[l]) + 12 * kSmallM,
-                             kSmallThreads<T> / 64);
+                             (ch ? kSmallChThreads : kSmallThreads<T>) / 64);
             // levels without children (the leaves) are a kernel of their own
-            ProfScope ps(dp.lv_children[l] ? K_FWD_SMALL1 : K_FWD_LEAF, st, nz * 8.0 * nr * c.tri_elems,
+            ProfScope ps(ch ? K_FWD_SMALL1 : K_FWD_LEAF, st, nz * 8.0 * nr * c.tri_elems,
                          nz * (c.tri_elems + 2.0 * c.sum_m * nr) * es);
-            if (dp.lv_children[l])
+            if (ch && dp.pmax_all[l] <= 12)
+                hipLaunchKernelGGL((k_fwd_small1<T, true, 3>), wl.grid, wl.block, wl.lds, st, F.arena, F.fstride,
+                                   dp.lv_fr[l], nfr, trans ? dp.gsrc_id : F.gsrc, trans ? (int64_t)0 : pl.fv_rows * 3,
+                                   Xp, Wp, n, wstride, nrhs, wl.rtg, (int)trans, (int)cj, in);
+            else if (ch)
                 hipLaunchKernelGGL((k_fwd_small1<T, true>), wl.grid, wl.block, wl.lds, st, F.arena, F.fstride,
                                    dp.lv_fr[l], nfr, trans ? dp.gsrc_id : F.gsrc, trans ? (int64_t)0 : pl.fv_rows * 3,
                                    Xp, Wp, n, wstride, nrhs, wl.rtg, (int)trans, (int)cj, in);
