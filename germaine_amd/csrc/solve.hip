@@ -792,25 +792,41 @@ struct AOp {
     bool unit, cj;
 };
 
-// the 8 A values of lane (lr, lk) for k = k0 + 4s + lk, s < 8
+// The 8 A values of lane (lr, lk) for k = k0 + 4s + lk, s < 8, in two steps:
+// load_a_raw issues the loads unconditionally (entries outside the operand
+// read A[0], which lies in the same front), so a caller can keep them in
+// flight past other work; finish_a applies the operand's masks once they are
+// needed (the first use waits for the loads)
 template <typename T>
-__device__ __forceinline__ void load_a(const cx<T> *__restrict__ A, const AOp &o, int k0, cx<T> (&av)[8]) {
+__device__ __forceinline__ void load_a_raw(const cx<T> *__restrict__ A, const AOp &o, int k0, cx<T> (&raw)[8]) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    const bool rok = lr < o.arows;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int k = k0 + 4 * s + lk;
+        raw[s] = A[rok && k < o.K ? lr * o.ars + k * o.acs : 0];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void finish_a(const AOp &o, int k0, cx<T> (&av)[8]) {
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4, di = o.ioff + lr;
     const bool rok = lr < o.arows;
-    const cx<T> *arow = A + lr * o.ars;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const int k = k0 + 4 * s + lk, ka = o.koff + k;
-        cx<T> v = {T(0), T(0)};
-        if (rok && k < o.K) {
-            v = arow[k * o.acs];
-            if (o.cj) v.y = -v.y;
-            if ((o.tri == 1 && ka > di) || (o.tri == 2 && ka < di)) v = {T(0), T(0)};
-            if (o.unit && ka == di) v = {T(1), T(0)};
-            if (ka >= o.kneg) v = {-v.x, -v.y};
-        }
+        cx<T> v = av[s];
+        if (o.cj) v.y = -v.y;
+        if ((o.tri == 1 && ka > di) || (o.tri == 2 && ka < di)) v = {T(0), T(0)};
+        if (o.unit && ka == di) v = {T(1), T(0)};
+        if (ka >= o.kneg) v = {-v.x, -v.y};
+        if (!(rok && k < o.K)) v = {T(0), T(0)};
         av[s] = v;
     }
+}
+template <typename T>
+__device__ __forceinline__ void load_a(const cx<T> *__restrict__ A, const AOp &o, int k0, cx<T> (&av)[8]) {
+    load_a_raw<T>(A, o, k0, av);
+    finish_a<T>(o, k0, av);
 }
 
 // acc[c] += A(16 x [k0, k0+32)) * P[prow0 + k][c*16 .. c*16 + 16) with A from av
@@ -843,7 +859,8 @@ __device__ __forceinline__ void strip_mma_from(cx<T> (&av)[8], const cx<T> *__re
                                                typename mfma_t<T>::acc (&cr)[NC], typename mfma_t<T>::acc (&ci)[NC]) {
     cx<T> an[8];
     for (int k0 = 0; k0 < o.K; k0 += 32) {
-        if (k0 + 32 < o.K) load_a<T>(A, o, k0 + 32, an);
+        if (k0 + 32 < o.K) load_a_raw<T>(A, o, k0 + 32, an);
+        if (k0 > 0) finish_a<T>(o, k0, av);
         mma_a<T, NC>(av, k0, o.K, Pr, Pi, prow0, rs, cr, ci);
 #pragma unroll
         for (int s = 0; s < 8; s++) av[s] = an[s];
@@ -1024,14 +1041,17 @@ __global__ void __launch_bounds__(256) k_fwd_front(const cx<T> *__restrict__ are
         int s0 = b0 + kb + sr;
         if (s0 < m) {
             ot.arows = m - s0;
-            load_a<T>(G + s0 * ars + b0 * acs, ot, 0, at);
+            load_a_raw<T>(G + s0 * ars + b0 * acs, ot, 0, at);
         }
         for (; s0 < m; s0 += 64) {
             const int s1 = s0 + 64;
             if (s1 < m) {
-                ot.arows = m - s1;
-                load_a<T>(G + s1 * ars + b0 * acs, ot, 0, an);
+                AOp on = ot;
+                on.arows = m - s1;
+                load_a_raw<T>(G + s1 * ars + b0 * acs, on, 0, an);
             }
+            ot.arows = m - s0;
+            finish_a<T>(ot, 0, at);
             acc_zero<T, NC>(cr, ci);
             mma_a<T, NC>(at, 0, kb, Pr, Pi, b0, RS, cr, ci);
             acc_store<T, NC>(Pr, Pi, RS, s0, m, true, cr, ci);
@@ -1096,55 +1116,84 @@ __global__ void __launch_bounds__(256) k_fwd_front1(const cx<T> *__restrict__ ar
     const AOp op{ars, acs, m, p, 0, 0, 1, 1 << 30, !trans, (bool)cj};
     // strip 0's factor entries and the rows' gather sources in flight together
     cx<T> av[8];
-    load_a<T>(G, op, 0, av);
+    load_a_raw<T>(G, op, 0, av);
     for (int e = lane; e < 3 * m; e += 64) src[e] = gs[e];
     wave_sync();
-    {   // pivot rows (p * 16 <= 512 elements: at most 8 per lane), all loads before the stores
-        cx<T> v[8];
+    {   // pivot rows (p * 16 <= 512 elements: at most 8 per lane): every source
+        // loaded unconditionally (absent ones read row 0, lanes past the
+        // right-hand sides column r0), validity in bits, selects once all arrived
+        cx<T> xa[8], wb[8], wc[8];
+        uint32_t ok = 0;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            const int e = lane + 64 * u, i = e >> 4, j = e & 15;
-            v[u] = {T(0), T(0)};
-            if (i < p && j < ncol) {
-                const int a = src[3 * i], b = src[3 * i + 1], c = src[3 * i + 2];
-                if (a >= 0) v[u] = X[(int64_t)a * nrhs + r0 + j];
-                if (b >= 0) v[u] = cadd(v[u], W[(int64_t)b * nrhs + r0 + j]);
-                if (c >= 0) v[u] = cadd(v[u], W[(int64_t)c * nrhs + r0 + j]);
-            }
+            if (4 * u >= p) break;
+            const int e = lane + 64 * u, i = e >> 4, j = e & 15, ic = i < p ? i : 0;
+            const int64_t jc = r0 + (j < ncol ? j : 0);
+            const int a = src[3 * ic], b = src[3 * ic + 1], c = src[3 * ic + 2];
+            ok |= (uint32_t)((a >= 0) | (b >= 0) << 1 | (c >= 0) << 2) << (3 * u);
+            xa[u] = X[(int64_t)max(a, 0) * nrhs + jc];
+            wb[u] = W[(int64_t)max(b, 0) * nrhs + jc];
+            wc[u] = W[(int64_t)max(c, 0) * nrhs + jc];
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
+            if (4 * u >= p) break;
             const int e = lane + 64 * u, i = e >> 4, j = e & 15;
+            cx<T> v = {T(0), T(0)};
+            if (i < p && j < ncol) {
+                if ((ok >> (3 * u)) & 1) v = xa[u];
+                if ((ok >> (3 * u + 1)) & 1) v = cadd(v, wb[u]);
+                if ((ok >> (3 * u + 2)) & 1) v = cadd(v, wc[u]);
+            }
             if (i < p) {
-                Br[i * WT + j] = v[u].x;
-                Bi[i * WT + j] = v[u].y;
+                Br[i * WT + j] = v.x;
+                Bi[i * WT + j] = v.y;
             }
         }
     }
     wave_sync();
-    // the lane's four accumulator rows of a strip: contribution-row values
-    auto load_w2 = [&](int s0, cx<T> (&w)[4]) {
+    // the lane's four accumulator rows of a strip: both contribution sources,
+    // loaded unconditionally, and their validity bits
+    const int64_t lc = r0 + (lr < ncol ? lr : 0);
+    auto load_w2 = [&](int s0, cx<T> (&wb)[4], cx<T> (&wc)[4], uint32_t &okw) {
+        okw = 0;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int row = s0 + mfma_t<T>::row(lk, v);
-            w[v] = {T(0), T(0)};
-            if (row >= p && row < m && lr < ncol) {
-                const int b = src[3 * row + 1], c = src[3 * row + 2];
-                if (b >= 0) w[v] = W[(int64_t)b * nrhs + r0 + lr];
-                if (c >= 0) w[v] = cadd(w[v], W[(int64_t)c * nrhs + r0 + lr]);
-            }
+            const bool in = row >= p && row < m && lr < ncol;
+            const int rc = in ? row : 0, b = src[3 * rc + 1], c = src[3 * rc + 2];
+            okw |= (uint32_t)((in && b >= 0) | (in && c >= 0) << 1) << (2 * v);
+            wb[v] = W[(int64_t)max(b, 0) * nrhs + lc];
+            wc[v] = W[(int64_t)max(c, 0) * nrhs + lc];
         }
     };
-    cx<T> w2[4];
+    cx<T> wb[4], wc[4], nb[4], nc[4], an[8];
+    uint32_t okw, okn = 0;
+    load_w2(0, wb, wc, okw);
     for (int s0 = 0; s0 < m; s0 += 16) {
-        // (strip 0's factor entries were requested before the gather)
-        if (s0 > 0) {
+        // the next strip's factor entries and contribution rows are requested
+        // before this strip's products (strip 0's before the gather)
+        const int s1 = s0 + 16;
+        if (s1 < m) {
+            AOp o = op;
+            o.arows = m - s1;
+            o.ioff = s1;
+            load_a_raw<T>(G + s1 * ars, o, 0, an);
+            load_w2(s1, nb, nc, okn);
+        }
+        {
             AOp o = op;
             o.arows = m - s0;
             o.ioff = s0;
-            load_a<T>(G + s0 * ars, o, 0, av);
+            finish_a<T>(o, 0, av);
         }
-        load_w2(s0, w2);
+        cx<T> w2[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            w2[v] = {T(0), T(0)};
+            if ((okw >> (2 * v)) & 1) w2[v] = wb[v];
+            if ((okw >> (2 * v + 1)) & 1) w2[v] = cadd(w2[v], wc[v]);
+        }
         cacc3<T> c;
         c.zero();
 #pragma unroll
@@ -1163,6 +1212,14 @@ __global__ void __launch_bounds__(256) k_fwd_front1(const cx<T> *__restrict__ ar
                 else W[(F.fv_off + row) * nrhs + r0 + lr] = {w2[v].x - c.re(v), w2[v].y - c.im(v)};
             }
         }
+#pragma unroll
+        for (int s = 0; s < 8; s++) av[s] = an[s];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            wb[v] = nb[v];
+            wc[v] = nc[v];
+        }
+        okw = okn;
     }
 }
 
@@ -1347,9 +1404,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1
     // The gathers load unconditionally (absent sources read row 0 and are
     // dropped by a select once everything has arrived), so the up to three
     // sources of every row are in flight together; only whole strips past the
-    // front (a uniform test) are skipped.  Column r0 + lr is inside the row
-    // stride for every lane.
+    // front (a uniform test) are skipped.  Lanes past the right-hand sides read
+    // column r0 (the row stride need not be a multiple of 16).
     if (CH) {
+        const int cl = r0 + (cin ? lr : 0);
         // bit 3s + j: source j of pivot slot s exists; bit 24 + 2t + j: source
         // j of contribution slot t exists
         cx<T> xa[PS], wu[PS], ww[PS], cu[16], cw[16];
@@ -1361,9 +1419,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1
             const int a = src[3 * kc], u = src[3 * kc + 1], w = src[3 * kc + 2];
             ok |= (uint64_t)(a >= 0) << (3 * s) | (uint64_t)(u >= 0) << (3 * s + 1) |
                   (uint64_t)(w >= 0) << (3 * s + 2);
-            xa[s] = X[(int64_t)max(a, 0) * nrhs + r0 + lr];
-            wu[s] = W[(int64_t)max(u, 0) * nrhs + r0 + lr];
-            ww[s] = W[(int64_t)max(w, 0) * nrhs + r0 + lr];
+            xa[s] = X[(int64_t)max(a, 0) * nrhs + cl];
+            wu[s] = W[(int64_t)max(u, 0) * nrhs + cl];
+            ww[s] = W[(int64_t)max(w, 0) * nrhs + cl];
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -1373,8 +1431,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !CH ? 8 : 1))) k_fwd_small1
                 const int t = 4 * q + v, row = 16 * q + mfma_t<T>::row(lk, v), rc = row < m ? row : 0;
                 const int u = src[3 * rc + 1], w = src[3 * rc + 2];
                 ok |= (uint64_t)(u >= 0) << (24 + 2 * t) | (uint64_t)(w >= 0) << (25 + 2 * t);
-                cu[t] = W[(int64_t)max(u, 0) * nrhs + r0 + lr];
-                cw[t] = W[(int64_t)max(w, 0) * nrhs + r0 + lr];
+                cu[t] = W[(int64_t)max(u, 0) * nrhs + cl];
+                cw[t] = W[(int64_t)max(w, 0) * nrhs + cl];
             }
         }
 #pragma unroll
@@ -1564,14 +1622,17 @@ __global__ void __launch_bounds__(256) k_bwd_front(const cx<T> *__restrict__ are
         int s0 = sr;
         if (s0 < b0) {
             ot.arows = b0 - s0;
-            load_a<T>(G + s0 * ars + b0 * acs, ot, 0, at);
+            load_a_raw<T>(G + s0 * ars + b0 * acs, ot, 0, at);
         }
         for (; s0 < b0; s0 += 64) {
             const int s1 = s0 + 64;
             if (s1 < b0) {
-                ot.arows = b0 - s1;
-                load_a<T>(G + s1 * ars + b0 * acs, ot, 0, an);
+                AOp on = ot;
+                on.arows = b0 - s1;
+                load_a_raw<T>(G + s1 * ars + b0 * acs, on, 0, an);
             }
+            ot.arows = b0 - s0;
+            finish_a<T>(ot, 0, at);
             acc_zero<T, NC>(cr, ci);
             mma_a<T, NC>(at, 0, kb, Pr, Pi, b0, RS, cr, ci);
             acc_store<T, NC>(Pr, Pi, RS, s0, b0, true, cr, ci);
