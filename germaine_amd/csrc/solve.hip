@@ -254,6 +254,20 @@ __device__ __forceinline__ cx<T> dop(const cx<T> *G, int m, int b0, int i, int t
     if (i == t && fwd != trans) return {T(1), T(0)};
     return trans ? ld(G + b0 + t + (int64_t)(b0 + i) * m, cj) : ld(G + b0 + i + (int64_t)(b0 + t) * m, cj);
 }
+// the same in two steps, so the load can stay in flight: dop_raw loads
+// unconditionally (entries outside the nb x nb block read its (0, 0) entry),
+// dop_fix applies the masks (zero outside the block)
+template <typename T>
+__device__ __forceinline__ cx<T> dop_raw(const cx<T> *G, int m, int b0, int i, int t, int nb, bool trans) {
+    if (!(i < nb && t < nb)) i = t = 0;
+    return G[trans ? b0 + t + (int64_t)(b0 + i) * m : b0 + i + (int64_t)(b0 + t) * m];
+}
+template <typename T>
+__device__ __forceinline__ cx<T> dop_fix(cx<T> v, int i, int t, int nb, bool fwd, bool trans, bool cj) {
+    if (!(i < nb && t < nb) || (fwd ? t > i : t < i)) return {T(0), T(0)};
+    if (i == t && fwd != trans) return {T(1), T(0)};
+    return cj ? cx<T>{v.x, -v.y} : v;
+}
 
 // acc = A[64 x kb] * Y[kb x 64] from LDS (4 waves in 2x2, 32x32 each), or
 // acc += that when accumulate; column sub-tiles at or beyond ncol (right-hand
@@ -314,8 +328,9 @@ __device__ __forceinline__ void load_tile(const cx<T> *w, int nrhs, int row0, in
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
+                // unconditional: entries outside read w[0] and are never used
                 const int row = row0 + wr + a * 16 + mfma_t<T>::row(lk, v), col = r0 + wc + b * 16 + lr;
-                c[a][b][v] = (row < rowend && col < nrhs) ? w[(int64_t)row * nrhs + col] : cx<T>{T(0), T(0)};
+                c[a][b][v] = w[row < rowend && col < nrhs ? (int64_t)row * nrhs + col : 0];
             }
 }
 template <typename T>
@@ -498,7 +513,7 @@ constexpr size_t blk2_smem_bytes() {
     return sizeof(T) * 2 * BK * 64 + blk_smem_bytes<T, kFwdRT>();
 }
 template <typename T>
-__global__ void __launch_bounds__(4 * kFwdRT) k_fwd_blk2(const cx<T> *__restrict__ arena, int64_t fstride,
+__global__ void __launch_bounds__(4 * kFwdRT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) k_fwd_blk2(const cx<T> *__restrict__ arena, int64_t fstride,
                                                   const FrontDev *__restrict__ fronts, const int *__restrict__ list,
                                                   cx<T> *__restrict__ X, cx<T> *__restrict__ W, int n, int64_t wstride,
                                                   int nrhs, int rt, int b0, int trans, int cj, Sel sel) {
@@ -518,32 +533,38 @@ __global__ void __launch_bounds__(4 * kFwdRT) k_fwd_blk2(const cx<T> *__restrict
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
-    auto Lop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
     cx<T> wold[2][2][4];
     if (upd) load_tile<T>(w, nrhs, row0, m, r0, wold);
     cx<T> d0[BK * BK / NT], d1[BK * BK / NT], l10[BK * BK / NT], y0v[BK * 64 / NT], y1v[BK * 64 / NT],
         av[BK * RT / NT];
+    // every load is unconditional (entries outside read a valid address of
+    // the front or panel), and its masks are applied when it is staged, so
+    // the whole set is in flight at once
+    auto lop_at = [&](int i, int t) { return trans ? t + (int64_t)i * m : i + (int64_t)t * m; };
 #pragma unroll
     for (int u = 0; u < BK * BK / NT; u++) {
         const int e = tid + NT * u, i = e % BK, t = e / BK;
-        d0[u] = (i < nb0 && t < nb0) ? dop(G, m, b0, i, t, true, trans, cj) : cx<T>{T(0), T(0)};
-        d1[u] = (i < nb1 && t < nb1) ? dop(G, m, b0 + BK, i, t, true, trans, cj) : cx<T>{T(0), T(0)};
-        l10[u] = (i < nb1 && t < nb0) ? Lop(b0 + BK + i, b0 + t) : cx<T>{T(0), T(0)};
+        d0[u] = dop_raw(G, m, b0, i, t, nb0, trans);
+        d1[u] = dop_raw(G, m, b0 + BK, i, t, nb1, trans);
+        l10[u] = G[i < nb1 && t < nb0 ? lop_at(b0 + BK + i, b0 + t) : 0];
     }
+    const int64_t wc0 = r0;
 #pragma unroll
     for (int u = 0; u < BK * 64 / NT; u++) {
         const int e = tid + NT * u, k = e >> 6, j = e & 63;
-        y0v[u] = (k < nb0 && j < ncol) ? w[(int64_t)(b0 + k) * nrhs + r0 + j] : cx<T>{T(0), T(0)};
-        y1v[u] = (k < nb1 && j < ncol) ? w[(int64_t)(b0 + BK + k) * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+        y0v[u] = w[k < nb0 && j < ncol ? (int64_t)(b0 + k) * nrhs + wc0 + j : 0];
+        y1v[u] = w[k < nb1 && j < ncol ? (int64_t)(b0 + BK + k) * nrhs + wc0 + j : 0];
     }
+    int akb = 0;   // columns of the A tile in av
     auto load_a = [&](int c0, int kb) {   // the A tile of block column c0 (kb columns) into av
+        akb = kb;
 #pragma unroll
         for (int u = 0; u < BK * RT / NT; u++) {
             const int e = tid + NT * u;
             int k, i;
             if (!trans) { i = e % RT; k = e / RT; }        // column-major L: coalesced along rows
             else { k = e % BK; i = e / BK; }               // U^T: coalesced along k
-            av[u] = (upd && k < kb && row0 + i < m) ? Lop(row0 + i, c0 + k) : cx<T>{T(0), T(0)};
+            av[u] = G[upd && k < kb && row0 + i < m ? lop_at(row0 + i, c0 + k) : 0];
         }
     };
     auto stage_a = [&]() {
@@ -553,34 +574,41 @@ __global__ void __launch_bounds__(4 * kFwdRT) k_fwd_blk2(const cx<T> *__restrict
             int k, i;
             if (!trans) { i = e % RT; k = e / RT; }
             else { k = e % BK; i = e / BK; }
-            s0.Ar[k * RT + i] = av[u].x;
-            s0.Ai[k * RT + i] = av[u].y;
+            const bool in = upd && k < akb && row0 + i < m;
+            s0.Ar[k * RT + i] = in ? av[u].x : T(0);
+            s0.Ai[k * RT + i] = in ? (cj ? -av[u].y : av[u].y) : T(0);
         }
     };
-    auto stage_d = [&](const cx<T>(&v)[BK * BK / NT]) {
+    auto stage_d = [&](const cx<T>(&v)[BK * BK / NT], int nb) {   // a diagonal block operator
 #pragma unroll
         for (int u = 0; u < BK * BK / NT; u++) {
             const int e = tid + NT * u;
-            s0.D[(e % BK) * (BK + 1) + e / BK] = v[u];
+            s0.D[(e % BK) * (BK + 1) + e / BK] = dop_fix(v[u], e % BK, e / BK, nb, true, trans, cj);
         }
     };
     load_a(b0, nb0);
-    stage_d(d0);
+    stage_d(d0, nb0);
 #pragma unroll
     for (int u = 0; u < BK * 64 / NT; u++) {
-        const int e = tid + NT * u;
-        s0.Yr[e] = y0v[u].x;
-        s0.Yi[e] = y0v[u].y;
-        s1.Yr[e] = y1v[u].x;
-        s1.Yi[e] = y1v[u].y;
+        const int e = tid + NT * u, k = e >> 6, j = e & 63;
+        const bool in0 = k < nb0 && j < ncol, in1 = k < nb1 && j < ncol;
+        s0.Yr[e] = in0 ? y0v[u].x : T(0);
+        s0.Yi[e] = in0 ? y0v[u].y : T(0);
+        s1.Yr[e] = in1 ? y1v[u].x : T(0);
+        s1.Yi[e] = in1 ? y1v[u].y : T(0);
     }
     __syncthreads();
     diag_apply<T, RT>(s0, nb0, ncol);   // y_0; ends with a barrier
     if (nb1 > 0) {
-        stage_d(l10);
+#pragma unroll
+        for (int u = 0; u < BK * BK / NT; u++) {   // Lop[b1, b0]
+            const int e = tid + NT * u, i = e % BK, t = e / BK;
+            const cx<T> v = l10[u];
+            s0.D[i * (BK + 1) + t] = i < nb1 && t < nb0 ? (cj ? cx<T>{v.x, -v.y} : v) : cx<T>{T(0), T(0)};
+        }
         __syncthreads();
         blk_couple<T>(s0.D, s0.Yr, s0.Yi, s1.Yr, s1.Yi, nb0, nb1, ncol);   // w_1 -= L10 y_0
-        stage_d(d1);
+        stage_d(d1, nb1);
         __syncthreads();
         diag_apply<T, RT>(s1, nb1, ncol);   // y_1
     }
