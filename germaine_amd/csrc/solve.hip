@@ -661,12 +661,15 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
-    auto Uop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
+    auto uop_at = [&](int i, int t) { return trans ? t + (int64_t)i * m : i + (int64_t)t * m; };
     cacc3<T> acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b].zero();
     constexpr int PER = BK * 64 / 256;
     cx<T> av[PER], yv[PER];
+    // a chunk's operands, loaded unconditionally (entries outside read a valid
+    // address and are zeroed when staged), so they stay in flight during the
+    // previous chunk's products
     auto fetch = [&](int s0) {
         const int kb = min(BK, m - s0);
 #pragma unroll
@@ -675,12 +678,13 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
             int k, i;
             if (!trans) { i = e & 63; k = e >> 6; }
             else { k = e % BK; i = e / BK; }
-            av[u] = (k < kb && row0 + i < p) ? Uop(row0 + i, s0 + k) : cx<T>{T(0), T(0)};
+            av[u] = G[k < kb && row0 + i < p ? uop_at(row0 + i, s0 + k) : 0];
         }
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int e = tid + 256 * u, k = e >> 6, j = e & 63;
-            yv[u] = (k < kb && j < ncol) ? X[(int64_t)srow[s0 - p + k] * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+            const bool in = k < kb && j < ncol;
+            yv[u] = X[(int64_t)srow[s0 - p + (in ? k : 0)] * nrhs + (in ? r0 + j : 0)];
         }
     };
     if (p < m) fetch(p);
@@ -693,10 +697,13 @@ __global__ void __launch_bounds__(256) k_bwd_contrib(const cx<T> *__restrict__ a
             int k, i;
             if (!trans) { i = e & 63; k = e >> 6; }
             else { k = e % BK; i = e / BK; }
-            s.Ar[k * 64 + i] = av[u].x;
-            s.Ai[k * 64 + i] = av[u].y;
-            s.Yr[e] = yv[u].x;
-            s.Yi[e] = yv[u].y;
+            const bool ain = k < kb && row0 + i < p;
+            s.Ar[k * 64 + i] = ain ? av[u].x : T(0);
+            s.Ai[k * 64 + i] = ain ? (cj ? -av[u].y : av[u].y) : T(0);
+            const int ky = e >> 6, j = e & 63;
+            const bool yin = ky < kb && j < ncol;
+            s.Yr[e] = yin ? yv[u].x : T(0);
+            s.Yi[e] = yin ? yv[u].y : T(0);
         }
         __syncthreads();
         if (s0 + BK < m) fetch(s0 + BK);   // in flight during this chunk's products
@@ -726,9 +733,10 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
     const cx<T> *G = arena + sel.f[z] * fstride + F.buf_off;
     X += (int64_t)sel.s[z] * n * nrhs;
     cx<T> *w = W + sel.s[z] * wstride + F.fv_off * nrhs;
-    auto Uop = [&](int i, int t) { return trans ? ld(G + t + (int64_t)i * m, cj) : ld(G + i + (int64_t)t * m, cj); };
+    auto uop_at = [&](int i, int t) { return trans ? t + (int64_t)i * m : i + (int64_t)t * m; };
     const bool upd = row0 < b0;
-    // every load before the first barrier (as in k_fwd_blk)
+    // every load before the first barrier, unconditional (entries outside read
+    // a valid address of the front or panel; masks applied when staged)
     cx<T> wold[2][2][4];
     if (upd) load_tile<T>(w, nrhs, row0, b0, r0, wold);
     {
@@ -736,12 +744,12 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
 #pragma unroll
         for (int u = 0; u < BK * BK / 256; u++) {
             const int e = tid + 256 * u, i = e % BK, t = e / BK;
-            dv[u] = (i < nb && t < nb) ? dop(G, m, b0, i, t, false, trans, cj) : cx<T>{T(0), T(0)};
+            dv[u] = dop_raw(G, m, b0, i, t, nb, trans);
         }
 #pragma unroll
         for (int u = 0; u < BK * 64 / 256; u++) {
             const int e = tid + 256 * u, k = e >> 6, j = e & 63;
-            yv[u] = (k < nb && j < ncol) ? w[(int64_t)(b0 + k) * nrhs + r0 + j] : cx<T>{T(0), T(0)};
+            yv[u] = w[k < nb && j < ncol ? (int64_t)(b0 + k) * nrhs + r0 + j : 0];
         }
 #pragma unroll
         for (int u = 0; u < BK * 64 / 256; u++) {
@@ -749,18 +757,19 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
             int k, i;
             if (!trans) { i = e & 63; k = e >> 6; }
             else { k = e % BK; i = e / BK; }
-            av[u] = (upd && k < nb && row0 + i < b0) ? Uop(row0 + i, b0 + k) : cx<T>{T(0), T(0)};
+            av[u] = G[upd && k < nb && row0 + i < b0 ? uop_at(row0 + i, b0 + k) : 0];
         }
 #pragma unroll
         for (int u = 0; u < BK * BK / 256; u++) {
             const int e = tid + 256 * u;
-            s.D[(e % BK) * (BK + 1) + e / BK] = dv[u];
+            s.D[(e % BK) * (BK + 1) + e / BK] = dop_fix(dv[u], e % BK, e / BK, nb, false, trans, cj);
         }
 #pragma unroll
         for (int u = 0; u < BK * 64 / 256; u++) {
-            const int e = tid + 256 * u;
-            s.Yr[e] = yv[u].x;
-            s.Yi[e] = yv[u].y;
+            const int e = tid + 256 * u, k = e >> 6, j = e & 63;
+            const bool in = k < nb && j < ncol;
+            s.Yr[e] = in ? yv[u].x : T(0);
+            s.Yi[e] = in ? yv[u].y : T(0);
         }
         __syncthreads();
         diag_apply<T>(s, nb, ncol);   // ends with a barrier: D's space is free
@@ -770,8 +779,9 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const cx<T> *__restrict__ arena
             int k, i;
             if (!trans) { i = e & 63; k = e >> 6; }
             else { k = e % BK; i = e / BK; }
-            s.Ar[k * 64 + i] = av[u].x;
-            s.Ai[k * 64 + i] = av[u].y;
+            const bool in = upd && k < nb && row0 + i < b0;
+            s.Ar[k * 64 + i] = in ? av[u].x : T(0);
+            s.Ai[k * 64 + i] = in ? (cj ? -av[u].y : av[u].y) : T(0);
         }
     }
     __syncthreads();
