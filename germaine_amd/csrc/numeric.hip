@@ -432,6 +432,41 @@ __global__ void __launch_bounds__(256) k_front_assemble(cx<T> *__restrict__ aren
     __syncthreads();
     const int64_t total = (int64_t)m * m;
     const int64_t step = (int64_t)gridDim.y * blockDim.x;
+    if (nc <= 2 && total < (int64_t(1) << 31)) {
+        // up to two children (the grid plans' fronts): each entry's child
+        // values are loaded unconditionally (an entry a child does not cover
+        // reads the child's block origin and is dropped), four entries at a
+        // time, so eight loads are in flight before the first add
+        // (entry indices in 32 bits)
+        constexpr int U = 4;
+        const int tot = (int)total, stp = (int)step;
+        for (int e0 = blockIdx.y * blockDim.x + threadIdx.x; e0 < tot; e0 += U * stp) {
+            cx<T> w[U][2];
+            bool ok[U][2];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * stp;
+                const int ec = e < tot ? e : 0;
+                const int s = ec / m, r = ec - s * m;
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    const int a = c < nc ? cmap[c * m + r] : -1, b = c < nc ? cmap[c * m + s] : -1;
+                    ok[u][c] = a >= 0 && b >= 0;
+                    w[u][c] = A[c < nc ? cbase[c] + (ok[u][c] ? a + (int64_t)b * cstride[c] : 0) : F.buf_off];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * stp;
+                if (e >= tot) break;
+                cx<T> v = {T(0), T(0)};
+                if (ok[u][0]) v = w[u][0];
+                if (ok[u][1]) v = cadd(v, w[u][1]);
+                G[e] = v;
+            }
+        }
+        return;
+    }
     // unrolled: several entries' value loads in flight at once
 #pragma unroll 4
     for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < total; e += step) {
