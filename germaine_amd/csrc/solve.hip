@@ -1555,30 +1555,37 @@ __global__ void __launch_bounds__(kSmallBwdThreads) k_bwd_small1(const cx<T> *__
     const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
     // stage [inv(U11), -inv(U11) U12] (or its transposed-solve form), rows, output
     // positions; m * p <= 2048 entries: one or two passes at 448 threads
+    // (unconditional loads: entries past the operator read G[0] and are dropped,
+    // so a thread's loads and its row index (m <= 64 rows: one per thread) are
+    // in flight together)
+    const int ti = threadIdx.x;
+    int srow_v = 0;
     for (int e0 = 0; e0 < m * p; e0 += 4 * nth) {
         cx<T> v[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int e = e0 + u * nth + threadIdx.x, k = e / p, i = e - k * p;
-            v[u] = {T(0), T(0)};
-            if (e < m * p) {
-                v[u] = G[i * ars + k * acs];
-                if (cj) v[u].y = -v[u].y;
-                if (k < p && k < i) v[u] = {T(0), T(0)};
-                if (i == k && trans) v[u] = {T(1), T(0)};
-                if (k >= p) v[u] = {-v[u].x, -v[u].y};
-            }
+            // entries past the operator re-read its last one (every lane takes
+            // the same path, so no load sits behind a branch)
+            const int e = min(e0 + u * nth + (int)threadIdx.x, m * p - 1), k = e / p, i = e - k * p;
+            v[u] = G[i * ars + k * acs];
         }
+        if (e0 == 0) srow_v = idx[F.idx_off + (ti < m && ti >= p ? ti : 0)];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int e = e0 + u * nth + threadIdx.x, k = e / p, i = e - k * p;
             if (e < m * p) {
-                Ar[k * as + i] = v[u].x;
-                Ai[k * as + i] = v[u].y;
+                cx<T> w = v[u];
+                if (cj) w.y = -w.y;
+                if (k < p && k < i) w = {T(0), T(0)};
+                if (i == k && trans) w = {T(1), T(0)};
+                if (k >= p) w = {-w.x, -w.y};
+                Ar[k * as + i] = w.x;
+                Ai[k * as + i] = w.y;
             }
         }
     }
-    for (int i = threadIdx.x; i < m; i += nth) srow[i] = i < p ? F.pos0 + i : idx[F.idx_off + i];
+    if (m * p == 0) srow_v = idx[F.idx_off + (ti < m && ti >= p ? ti : 0)];
+    if (ti < m) srow[ti] = ti < p ? F.pos0 + ti : srow_v;
     for (int i = threadIdx.x; i < p; i += nth)
         dstl[i] = F.pos0 + (trans ? rowperm[(int64_t)sel.f[z] * n + F.pos0 + i] : i);
     __syncthreads();
