@@ -1288,33 +1288,57 @@ __global__ void __launch_bounds__(256) k_bwd_front1(const cx<T> *__restrict__ ar
     const AOp o0{ars, acs, p, m, 0, 0, 2, p, (bool)trans, (bool)cj};
     const AOp o1{ars, acs, p - 16, m, 16, 0, 2, p, (bool)trans, (bool)cj};
     // operand row k: pivot row k of the front, or the ancestors' row idx[k]
+    // (the index load is unconditional: rows outside read the front's first)
     auto rows_of = [&](int k0, int (&r)[8]) {
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int k = k0 + 4 * s + lk;
-            r[s] = k >= m ? -1 : k < p ? F.pos0 + k : idx[F.idx_off + k];
+            const int v = idx[F.idx_off + (k < m && k >= p ? k : 0)];
+            // (unpredictable: kept a select, so the load is not moved behind a branch)
+            r[s] = __builtin_unpredictable(k >= p && k < m) ? v : k < p ? F.pos0 + k : -1;
         }
     };
     cx<T> a0[8], a1[8];
     int rows[8];
     rows_of(0, rows);
     int dst[8];   // output positions of the lane's accumulator rows
+    {
+        // (transposed solves: the row permutation, loaded unconditionally)
+        int rp[8];
+        const int *rpf = rowperm + (int64_t)sel.f[z] * n + F.pos0;
 #pragma unroll
-    for (int v = 0; v < 8; v++) {
-        const int row = (v >> 2) * 16 + mfma_t<T>::row(lk, v & 3);
-        dst[v] = row < p ? F.pos0 + (trans ? rowperm[(int64_t)sel.f[z] * n + F.pos0 + row] : row) : -1;
+        for (int v = 0; v < 8; v++) {
+            const int row = (v >> 2) * 16 + mfma_t<T>::row(lk, v & 3);
+            rp[v] = trans ? rpf[row < p ? row : 0] : row;
+        }
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            const int row = (v >> 2) * 16 + mfma_t<T>::row(lk, v & 3);
+            dst[v] = __builtin_unpredictable(row < p) ? F.pos0 + rp[v] : -1;
+        }
     }
     cacc3<T> c0, c1;
     c0.zero();
     c1.zero();
+    const int64_t col = r0 + (cin ? lr : 0);
     for (int k0 = 0; k0 < m; k0 += 32) {
+        // the chunk's factor entries, operand rows and the next chunk's row
+        // indices all requested before the first is used
         cx<T> b[8];
-        load_a<T>(G, o0, k0, a0);
-        if (two) load_a<T>(G + 16 * ars, o1, k0, a1);
+        load_a_raw<T>(G, o0, k0, a0);
+        if (two) load_a_raw<T>(G + 16 * ars, o1, k0, a1);
+        bool bok[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            bok[s] = rows[s] >= 0 && cin;
+            b[s] = X[(int64_t)max(rows[s], 0) * nrhs + col];
+        }
+        if (k0 + 32 < m) rows_of(k0 + 32, rows);
+        finish_a<T>(o0, k0, a0);
+        if (two) finish_a<T>(o1, k0, a1);
 #pragma unroll
         for (int s = 0; s < 8; s++)
-            b[s] = (rows[s] >= 0 && cin) ? X[(int64_t)rows[s] * nrhs + r0 + lr] : cx<T>{T(0), T(0)};
-        if (k0 + 32 < m) rows_of(k0 + 32, rows);
+            if (!bok[s]) b[s] = {T(0), T(0)};
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             if (k0 + 4 * s >= m) break;
