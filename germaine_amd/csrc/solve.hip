@@ -2835,6 +2835,41 @@ __global__ void __launch_bounds__(256) k_ir_sum_out(IrBufs d, IrCounts cnt, cons
     }
 }
 
+// Correction solves on the right-hand sides still above the tolerance only:
+// they are packed into a narrower panel (ldc columns) for the solve and
+// expanded back, the converged columns receiving a zero correction.
+// cmap[z * ldc + j]: source column of packed column j on launch slot z (-1:
+// zero padding); imap[z * ld + c]: packed column of column c (-1: zero).
+// grid (blocks, slots), grid-stride over the slot's panel entries.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ir_pack(const cx<T> *__restrict__ P, cx<T> *__restrict__ C,
+                                                 const int *__restrict__ cmap, int n, int ld, int ldc, Sel sel) {
+    const int z = blockIdx.y, s = sel.s[z];
+    const cx<T> *Ps = P + (int64_t)s * n * ld;
+    cx<T> *Cs = C + (int64_t)s * n * ldc;
+    const int *cm = cmap + z * ldc;
+    const int64_t total = (int64_t)n * ldc;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t row = e / ldc;
+        const int c = cm[e - row * ldc];
+        Cs[e] = c >= 0 ? Ps[row * ld + c] : cx<T>{T(0), T(0)};
+    }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_ir_unpack(const cx<T> *__restrict__ C, cx<T> *__restrict__ P,
+                                                   const int *__restrict__ imap, int n, int ld, int ldc, Sel sel) {
+    const int z = blockIdx.y, s = sel.s[z];
+    const cx<T> *Cs = C + (int64_t)s * n * ldc;
+    cx<T> *Ps = P + (int64_t)s * n * ld;
+    const int *im = imap + z * ld;
+    const int64_t total = (int64_t)n * ld;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t row = e / ld;
+        const int j = im[e - row * ld];
+        Ps[e] = j >= 0 ? Cs[row * ldc + j] : cx<T>{T(0), T(0)};
+    }
+}
+
 // backward error of one column: residual and max_i |r_i| / (|op(M)||x| + |b|)_i
 __global__ void __launch_bounds__(256) k_backerr(const cx<double> *__restrict__ vals, const int *__restrict__ colptr,
                                                  const int *__restrict__ rowind, const int64_t *__restrict__ tq, int n,
@@ -3580,7 +3615,7 @@ static void keep_unconverged(const std::vector<double> &h, int ns, int nrhs, int
     act_s = nsl;
 }
 
-thread_local Workspace ws_r64, ws_acc64, ws_dk[kIrBufs];
+thread_local Workspace ws_r64, ws_acc64, ws_dk[kIrBufs], ws_pack, ws_pmap;
 // widest panel of the row-major refinement: its per-wave column sums and the
 // staged neighbourhood share the default 64 KB of LDS
 constexpr int kIrMaxRhs = 1024;
@@ -3635,6 +3670,7 @@ std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<in
     std::vector<int> base(ns, 0);   // per slot: corrections already folded into acc
     uint64_t folded = 0;            // bit s: slot s has a folded sum in acc
     cx<double> *acc = nullptr;
+    std::vector<int> hmap;          // column maps of a packed correction solve (host side of ws_pmap)
     for (int it = 0;; it++) {
         const Sel sel = make_sel(act_f, act_s);
         const int jn = (it + 1) % kIrBufs;   // buffer of the next right-hand side
@@ -3672,7 +3708,41 @@ std::vector<RefineStats> refine_panels(const Factors<T> &F, const std::vector<in
         GFDFD_HIP(hipStreamSynchronize(st));
         keep_unconverged(h, ns, nrhs, it, tol, max_iter, act_f, act_s, rs);
         if (act_f.empty()) break;
-        sweeps<T>(F, make_sel(act_f, act_s), sys, ld, dbuf(jn), Wp, st, nrhs);   // d = op(M)^-1 r in place
+        // d = op(M)^-1 r in place; when at most half of the columns of every
+        // remaining slot are still above the tolerance, only those are solved
+        // (packed into ldc columns) and the others get a zero correction
+        const Sel nsel = make_sel(act_f, act_s);
+        std::vector<std::vector<int>> cols(nsel.n);
+        int most = 0;
+        for (int z = 0; z < nsel.n; z++) {
+            const int s = nsel.s[z];
+            for (int c = 0; c < nrhs; c++) {
+                const double b = std::sqrt(h[((size_t)ns + s) * nrhs + c]), rr = std::sqrt(h[(size_t)s * nrhs + c]);
+                if ((b > 0 ? rr / b : rr) > tol) cols[z].push_back(c);
+            }
+            most = std::max(most, (int)cols[z].size());
+        }
+        const int ldc = (most + cl - 1) / cl * cl;
+        if (ldc > 0 && 2 * ldc <= ld) {
+            hmap.assign((size_t)nsel.n * (ldc + ld), -1);
+            for (int z = 0; z < nsel.n; z++)
+                for (int j = 0; j < (int)cols[z].size(); j++) {
+                    hmap[(size_t)z * ldc + j] = cols[z][j];
+                    hmap[(size_t)nsel.n * ldc + (size_t)z * ld + cols[z][j]] = j;
+                }
+            int *dmap = (int *)ws_pmap.get(sizeof(int) * hmap.size());
+            // (hmap outlives the copy: the next use follows this step's stream synchronisation)
+            GFDFD_HIP(hipMemcpyAsync(dmap, hmap.data(), sizeof(int) * hmap.size(), hipMemcpyHostToDevice, st));
+            cx<T> *C = (cx<T> *)ws_pack.get(sizeof(cx<T>) * (size_t)n * ldc * ns);
+            const int64_t ce = (int64_t)n * ldc, pe = (int64_t)n * ld;
+            hipLaunchKernelGGL(k_ir_pack<T>, dim3(grid_for(ce), nsel.n), dim3(256), 0, st, dbuf(jn), C, dmap, n, ld, ldc,
+                               nsel);
+            sweeps<T>(F, nsel, sys, ldc, C, Wp, st, most);
+            hipLaunchKernelGGL(k_ir_unpack<T>, dim3(grid_for(pe), nsel.n), dim3(256), 0, st, C, dbuf(jn),
+                               dmap + (size_t)nsel.n * ldc, n, ld, ldc, nsel);
+        } else {
+            sweeps<T>(F, nsel, sys, ld, dbuf(jn), Wp, st, nrhs);
+        }
     }
     {
         // slot s holds corrections base[s] .. iterations in buffers 0, 1, ...
@@ -3751,7 +3821,7 @@ template std::vector<RefineStats> solve_refined_batch<float>(const Factors<float
 // the calling thread's cached solve buffers back to the device (they grow to
 // the largest solve and are otherwise kept for the next one)
 void release_solve_workspaces() {
-    for (Workspace *w : {&ws_x, &ws_w, &ws_r, &ws_n, &ws_p, &ws_r64, &ws_acc64}) w->release();
+    for (Workspace *w : {&ws_x, &ws_w, &ws_r, &ws_n, &ws_p, &ws_r64, &ws_acc64, &ws_pack, &ws_pmap}) w->release();
     for (Workspace &w : ws_dk) w.release();
 }
 

@@ -497,6 +497,47 @@ def test_refinement_folds_corrections(sysno):
         assert rel(out[0][z], Xr.T) <= 1e-8 and rel(out[1][z], Xr.T) <= 1e-8, (z, k)
 
 
+@pytest.mark.parametrize("sysno", [0, 2])
+def test_refinement_packs_unconverged_columns(sysno):
+    """Correction solves on the columns still above the tolerance only: 30 of
+    100 right-hand sides are nonzero (the zero ones have converged from the
+    start), so every correction solve of the elimination-order refinement runs
+    on a packed 32-column panel and the other columns get zero corrections.
+    The zero columns stay exactly zero; against the caller's-layout
+    refinement (which solves every column) the iterates agree to 1e-12 when
+    the step counts agree; both within 1e-8 of the oracle."""
+    nx, ny, npml, dh = 41, 30, 6, 10.0
+    m1, m2 = smooth_model(nx, ny, seed=6)
+    freqs = [4.0, 11.0]
+    say("refinement packs columns", sysno)
+    pl = fdfd.Plan(nx, ny, small_max=16)
+    F = pl.factor(pl.assemble(G.AC, dh, freqs, dev(m1), dev(m2), npml=npml), G.FP32)
+    rng = np.random.default_rng(41 + sysno)
+    live = np.sort(rng.choice(100, 30, replace=False))
+    Bh = np.zeros((2, 100, nx * ny), complex)
+    Bh[:, live] = rng.standard_normal((2, 30, nx * ny)) + 1j * rng.standard_normal((2, 30, nx * ny))
+    B = dev(Bh)
+    out = {}
+    try:
+        for layout in (0, 1):
+            fdfd.solve_tuning(refine_layout=layout)
+            X, its, rr = F.solve_refine_batch([1, 0], B, sys=sysno, tol=1e-12, max_iter=30)
+            out[layout] = (X.cpu().numpy(), list(its))
+            say("  layout", layout, "iterations", its, rr)
+            assert max(rr) <= 1e-12
+    finally:
+        fdfd.solve_tuning()
+    (X0, it0), (X1, it1) = out[0], out[1]
+    dead = np.setdiff1d(np.arange(100), live)
+    assert np.all(X0[:, dead] == 0)
+    if it0 == it1:
+        assert rel(X0, X1) <= 1e-12
+    for z, k in enumerate([1, 0]):
+        _, csc = oracle_system(G.AC, nx, ny, npml, dh, freqs[k], m1, m2)
+        Xr, _ = O.band_solve(*csc, Bh[z][live].T, sys=sysno, perm=O.grid_transpose_perm(nx, ny))
+        assert rel(X0[z][live], Xr.T) <= 1e-8 and rel(X1[z][live], Xr.T) <= 1e-8, (z, k)
+
+
 def test_refinement_wide_panel():
     """The fused elimination-order refinement at 1000 right-hand sides (near its
     1024 limit, where the per-wave column sums need more than the default 64 KB
