@@ -1910,6 +1910,9 @@ __device__ __forceinline__ cx<T> gather3(const int *__restrict__ g, const cx<T> 
 // compute unit holds many fronts at once and a front costs about three memory
 // round trips with no workgroup barrier.  Columns of the operators are
 // contiguous over rows (M solves; transposed solves keep k_fwd_vec/k_bwd_vec).
+// A front of more than 64 rows takes several workgroups (grid y), each of which
+// gathers the pivot rows' right-hand side from X, so y1 goes to the front's own
+// W rows [0, p) (read by no other front's forward) and the backward reads it there.
 template <typename T>
 __device__ __forceinline__ cx<T> lane_bcast(cx<T> v, int l) {
     if constexpr (sizeof(T) == 4) {
@@ -1958,10 +1961,7 @@ __global__ void __launch_bounds__(64) k_fwd_vec1(const cx<T> *__restrict__ arena
 #pragma unroll
     for (int t = 0; t < PM; t++)
         if (t < p) acc = cadd(acc, cmul(a[t], lane_bcast(w0, t)));
-    if (i < m) {
-        if (i < p) X[F.pos0 + i] = acc;
-        else W[F.fv_off + i] = csub(wi, acc);
-    }
+    if (i < m) W[F.fv_off + i] = i < p ? acc : csub(wi, acc);
 }
 
 // backward: the NW waves of a workgroup share a front; its threads are R = (p
@@ -1974,7 +1974,7 @@ __global__ void __launch_bounds__(64) k_fwd_vec1(const cx<T> *__restrict__ arena
 template <typename T, int CH, int NW>
 __global__ void __launch_bounds__(64 * NW) k_bwd_vec1(const cx<T> *__restrict__ arena,
                                                       const FrontDev *__restrict__ lvF, const int *__restrict__ idx,
-                                                      cx<T> *__restrict__ X) {
+                                                      const cx<T> *__restrict__ W, cx<T> *__restrict__ X) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
     const FrontDev F = lvF[blockIdx.x];
@@ -1997,7 +1997,8 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_vec1(const cx<T> *__restrict__ 
         }
     };
     fetch(q);
-    for (int t = tid; t < m; t += 64 * NW) sv[t] = X[t < p ? F.pos0 + t : idx[F.idx_off + t]];
+    // y1 from the front's W rows (k_fwd_vec1), x2 from X at the contribution positions
+    for (int t = tid; t < m; t += 64 * NW) sv[t] = t < p ? W[F.fv_off + t] : X[idx[F.idx_off + t]];
     __syncthreads();
     cx<T> acc = {T(0), T(0)};
     for (int t0 = q; t0 < m;) {
@@ -3317,7 +3318,8 @@ static void launch_fwd_vec1(const DevPlan &dp, int l, const cx<T> *arena, const 
         hipLaunchKernelGGL((k_fwd_vec1<T, 32>), g, b, 0, st, arena, dp.lv_fr[l], gsrc, Xp, Wp);
 }
 template <typename T>
-static void launch_bwd_vec1(const DevPlan &dp, int l, const cx<T> *arena, cx<T> *Xp, hipStream_t st) {
+static void launch_bwd_vec1(const DevPlan &dp, int l, const cx<T> *arena, const cx<T> *Wp, cx<T> *Xp,
+                            hipStream_t st) {
     int R = 1;
     while (R < dp.pmax_all[l]) R <<= 1;
     // waves per front: as many as keep a thread's columns within one chunk of 16
@@ -3326,7 +3328,8 @@ static void launch_bwd_vec1(const DevPlan &dp, int l, const cx<T> *arena, cx<T> 
     const int per_thread = (dp.mmax_all[l] + 64 * nw / R - 1) / (64 * nw / R);
     const dim3 g(dp.n_all[l]);
     const size_t lds = sizeof(cx<T>) * (size_t)dp.mmax_all[l];
-#define BWD1(CH, NW) hipLaunchKernelGGL((k_bwd_vec1<T, CH, NW>), g, dim3(64 * NW), lds, st, arena, dp.lv_fr[l], dp.idx, Xp)
+#define BWD1(CH, NW) \
+    hipLaunchKernelGGL((k_bwd_vec1<T, CH, NW>), g, dim3(64 * NW), lds, st, arena, dp.lv_fr[l], dp.idx, Wp, Xp)
     if (nw == 1) {
         if (per_thread <= 8) BWD1(8, 1);
         else BWD1(16, 1);
@@ -3431,7 +3434,7 @@ static void solve_vec(const Factors<T> &F, const Sel &sel, int sys, int nrhs, co
                                        sizeof(cx<T>) * (size_t)dp.pmax_all[l], st, arena, dp.fronts, dp.lv_all[l],
                                        rowperm, Xp, (int)trans, (int)cj);
                 } else if (dp.single[l] && !trans) {
-                    launch_bwd_vec1<T>(dp, l, arena, Xp, st);
+                    launch_bwd_vec1<T>(dp, l, arena, Wp, Xp, st);
                 } else {
                     hipLaunchKernelGGL(k_bwd_vec<T>, dim3(dp.n_all[l]), dim3(256), lds, st, arena, dp.fronts,
                                        dp.lv_all[l], dp.idx, rowperm, Xp, (int)trans, (int)cj, dp.single[l]);
