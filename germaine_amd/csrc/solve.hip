@@ -1152,10 +1152,25 @@ __global__ void __launch_bounds__(256) k_fwd_front1(const cx<T> *__restrict__ ar
     const int *gs = gsrc + sel.f[z] * gstride + F.fv_off * 3;
     const int64_t ars = trans ? m : 1, acs = trans ? 1 : m;
     const AOp op{ars, acs, m, p, 0, 0, 1, 1 << 30, !trans, (bool)cj};
-    // strip 0's factor entries and the rows' gather sources in flight together
+    // the rows' gather sources (3m <= 900 entries: up to 15 per lane, issued
+    // first, as the memory counter is in order) and strip 0's factor entries in
+    // flight together, then the sources stored
+    constexpr int SRC_PER = (3 * kSingleMmax + 63) / 64;
+    int sv[SRC_PER];
+#pragma unroll
+    for (int u = 0; u < SRC_PER; u++) {
+        if (64 * u >= 3 * m) break;
+        const int e = lane + 64 * u;
+        sv[u] = gs[e < 3 * m ? e : 0];
+    }
     cx<T> av[8];
     load_a_raw<T>(G, op, 0, av);
-    for (int e = lane; e < 3 * m; e += 64) src[e] = gs[e];
+#pragma unroll
+    for (int u = 0; u < SRC_PER; u++) {
+        if (64 * u >= 3 * m) break;
+        const int e = lane + 64 * u;
+        if (e < 3 * m) src[e] = sv[u];
+    }
     wave_sync();
     {   // pivot rows (p * 16 <= 512 elements: at most 8 per lane): every source
         // loaded unconditionally (absent ones read row 0, lanes past the
