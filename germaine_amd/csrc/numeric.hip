@@ -918,10 +918,20 @@ __global__ void __launch_bounds__(256) k_trsm_u(cx<T> *__restrict__ arena, int64
         int i = e % kb, t = e / kb;
         Ls[i][t] = G[(k0 + i) + (int64_t)(k0 + t) * m];
     }
-#pragma unroll 8
-    for (int e = tid; e < kb * XW; e += 256) {
-        int i = e % kb, c = e / kb, col = c0 + c;
-        Xs[i][c] = col < cend ? G[(k0 + i) + (int64_t)col * m] : cx<T>{T(0), T(0)};
+    // (unconditional: columns past the end read column c0 and are zeroed when stored)
+    {
+        constexpr int PER = KB * XW / 256;
+        cx<T> v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = tid + 256 * u, i = e % KB, c = e / KB, col = c0 + c;
+            v[u] = G[(k0 + (i < kb ? i : 0)) + (int64_t)(col < cend ? col : c0) * m];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = tid + 256 * u, i = e % KB, c = e / KB, col = c0 + c;
+            if (i < kb) Xs[i][c] = col < cend ? v[u] : cx<T>{T(0), T(0)};
+        }
     }
     __syncthreads();
     {
