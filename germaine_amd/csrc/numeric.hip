@@ -870,13 +870,28 @@ __global__ void __launch_bounds__(64) k_panel_apply(cx<T> *__restrict__ arena, i
         return;
     }
     // contribution rows: x U_kk = a, U_kk from the factored panel
-    const int i = p + (blockIdx.x - ccols) * 64 + tid;
-    for (int e = tid; e < kb * kb; e += 64) U[e % kb][e / kb] = G[(k0 + e % kb) + (int64_t)(k0 + e / kb) * m];
+    // the block U_kk (16 entries per thread, all in flight before the stores),
+    // then the row's panel entries (unconditional: columns past kb read column
+    // k0 and are not used)
+    const int i = p + (blockIdx.x - ccols) * 64 + tid, ic = i < m ? i : p;
+    {
+        cx<T> uv[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = tid + 64 * u, r = e & 31, c = e >> 5;
+            uv[u] = G[(k0 + (r < kb ? r : 0)) + (int64_t)(k0 + (c < kb ? c : 0)) * m];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = tid + 64 * u, r = e & 31, c = e >> 5;
+            if (r < kb && c < kb) U[r][c] = uv[u];
+        }
+    }
     __syncthreads();
     if (i >= m) return;
     cx<T> x[32];
 #pragma unroll
-    for (int t = 0; t < 32; t++) x[t] = t < kb ? G[i + (int64_t)(k0 + t) * m] : cx<T>{T(0), T(0)};
+    for (int t = 0; t < 32; t++) x[t] = G[ic + (int64_t)(k0 + (t < kb ? t : 0)) * m];
 #pragma unroll
     for (int t = 0; t < 32; t++) {
         if (t < kb) {
