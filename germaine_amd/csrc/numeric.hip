@@ -787,9 +787,20 @@ __global__ void __launch_bounds__(256) k_panel_lu(cx<T> *__restrict__ arena, int
     const int fq = blockIdx.y;
     cx<T> *G = arena + fq * fstride + F.buf_off;
     const int m = F.m, p = F.p, kb = min(nb, p - k0), np = p - k0, tid = threadIdx.x;
-    for (int e = tid; e < np * kb; e += 256) {
-        const int r = e % np, c = e / np;
-        P[c * np + r] = G[(k0 + r) + (int64_t)(k0 + c) * m];
+    // the panel staged eight entries per thread at a time, all eight loads in
+    // flight before the stores (entries past the panel read its first)
+    for (int e0 = 0; e0 < np * kb; e0 += 8 * 256) {
+        cx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + tid + 256 * u, ec = e < np * kb ? e : 0, r = ec % np, c = ec / np;
+            v[u] = G[(k0 + r) + (int64_t)(k0 + c) * m];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + tid + 256 * u;
+            if (e < np * kb) P[(e / np) * np + e % np] = v[u];
+        }
     }
     __syncthreads();
     __shared__ cx<T> pv[2];   // pivot entry and the entry it swaps with, published with the pivot row
