@@ -636,8 +636,28 @@ __global__ void __launch_bounds__(256) k_small_lu(cx<T> *__restrict__ arena, int
     const int fq = blockIdx.y;
     cx<T> *G = arena + fq * fstride + F.buf_off;
     const int m = F.m, mm = m * m, tid = threadIdx.x, bs = blockDim.x;
+    // FP32: the front staged with all of a thread's loads in flight (m <= 64: at
+    // most 16 entries per thread at 256 threads; entries past the front read its
+    // first).  FP64 keeps one round per 4 entries, measured faster at its
+    // register budget.
+    if constexpr (sizeof(T) == 8) {
 #pragma unroll 4
-    for (int e = tid; e < mm; e += bs) S[e] = G[e];
+        for (int e = tid; e < mm; e += bs) S[e] = G[e];
+    } else {
+        for (int e0 = 0; e0 < mm; e0 += 16 * bs) {
+            cx<T> v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int e = e0 + tid + u * bs;
+                v[u] = G[e < mm ? e : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int e = e0 + tid + u * bs;
+                if (e < mm) S[e] = v[u];
+            }
+        }
+    }
     __syncthreads();
     small_lu_lds<T>(S, m, F.p, piv + (int64_t)fq * n + F.pos0, sing + fq, sp);
     for (int e = tid; e < mm; e += bs) G[e] = S[e];
